@@ -1,0 +1,157 @@
+/* dcrf_hip.h - C ABI of the MI355X (gfx950) dense-CRF mean-field library.
+ *
+ * Every entry point stands in for one operation of the reference's DenseCRF /
+ * DenseCRF2D interface (lucasb-eyer/pydensecrf: pydensecrf/densecrf.pyx and
+ * pydensecrf/densecrf/src), cited beside it.  The host front end
+ * (pydensecrf_amd/densecrf.pyx) binds these with `cdef extern`.
+ *
+ * Conventions
+ *  - Plain pointers and sizes; no C++ types cross the boundary.
+ *  - Every function returns an int status: DCRF_OK (0) or a negative code.  The
+ *    message of the last failure on the calling thread is dcrf_last_error().
+ *  - Host arrays use the reference's Python layouts: features (d, N) row-major, unary
+ *    (L, N) row-major, rgb images (H, W, 3) uint8.  Q and the step temporaries use the
+ *    reference's in-memory layout [N][L] (label fastest), which is what the reference's
+ *    MatrixXf buffers expose (pydensecrf/eigen.pyx:68-95).
+ *  - Inputs are copied; the caller keeps ownership.  A model owns its device memory,
+ *    lattices and stream; dcrf_destroy frees them.  Models are not thread-safe.
+ *  - Host-pointer calls are synchronous at return.  *_device calls take device
+ *    pointers and a hipStream_t (as void*), enqueue work on it and return.
+ */
+#ifndef DCRF_HIP_H
+#define DCRF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DCRF_OK 0
+#define DCRF_ERR_VALUE (-1)    /* bad argument: shape, range, unsupported size */
+#define DCRF_ERR_MEMORY (-2)   /* device or host allocation failed */
+#define DCRF_ERR_DEVICE (-3)   /* HIP runtime / kernel failure, no device */
+#define DCRF_ERR_OVERFLOW (-4) /* a lattice key coordinate left the int16 range */
+#define DCRF_ERR_STATE (-5)    /* call not valid in the model's current state */
+
+/* NormalizationType (pydensecrf/densecrf/include/pairwise.h:32-37) */
+#define DCRF_NO_NORMALIZATION 0
+#define DCRF_NORMALIZE_BEFORE 1
+#define DCRF_NORMALIZE_AFTER 2
+#define DCRF_NORMALIZE_SYMMETRIC 3
+/* KernelType (pairwise.h:38-42) */
+#define DCRF_CONST_KERNEL 0
+#define DCRF_DIAG_KERNEL 1
+#define DCRF_FULL_KERNEL 2
+/* LabelCompatibility kind (pydensecrf/densecrf.pyx:11-20: number / 1-D / 2-D) */
+#define DCRF_COMPAT_POTTS 0    /* compat[1]: mu(a,b) = -w [a==b]  (labelcompatibility.cpp:46-48) */
+#define DCRF_COMPAT_DIAGONAL 1 /* compat[L]: out = diag(w) Q          (labelcompatibility.cpp:66-69) */
+#define DCRF_COMPAT_MATRIX 2   /* compat[L*L] row-major m; W = (m+m^T)/2 (labelcompatibility.cpp:79-87) */
+
+/* Largest supported feature dimension, label count and pairwise-term count. */
+#define DCRF_MAX_DIM 15
+#define DCRF_MAX_LABELS 1024
+#define DCRF_MAX_TERMS 8
+
+typedef struct dcrf_model dcrf_model;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dcrf_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int dcrf_device_count(void);
+
+/* ---- construction --------------------------------------------------------------- */
+/* DenseCRF(N, M): densecrf.cpp:38-39, densecrf.pyx:53-64.  Touches no device. */
+int dcrf_create(int N, int L, dcrf_model **out);
+/* DenseCRF2D(W, H, M): densecrf.cpp:47-48, densecrf.pyx:107-118. */
+int dcrf_create_2d(int W, int H, int L, dcrf_model **out);
+/* Batched form: n instances of N[i] variables each, one set of terms applied to every
+ * instance, lattices concatenated with per-instance offset tables.  No reference
+ * counterpart as one call: equivalent to n independent DenseCRF(N[i], L) models. */
+int dcrf_create_batch(int n, const int *N, int L, dcrf_model **out);
+/* Batched DenseCRF2D: instance i is W[i] x H[i]. */
+int dcrf_create_batch_2d(int n, const int *W, const int *H, int L, dcrf_model **out);
+/* ~DenseCRF: densecrf.cpp:40-45. */
+void dcrf_destroy(dcrf_model *m);
+
+/* ---- unary ---------------------------------------------------------------------- */
+/* setUnaryEnergy(u): densecrf.cpp:90-92 (ConstUnaryEnergy, unary.cpp:42-46),
+ * densecrf.pyx:80-89.  U is (L, N) row-major (N = sum of instance sizes). */
+int dcrf_set_unary(dcrf_model *m, const float *U_LxN);
+/* Batched form: the unary of instance i, (L, N[i]) row-major. */
+int dcrf_set_unary_instance(dcrf_model *m, int i, const float *U_LxNi);
+
+/* ---- pairwise terms (lattice built on the GPU at call time) -------------------- */
+/* addPairwiseEnergy(features, compat, kernel, normalization): densecrf.cpp:54-57,
+ * pairwise.cpp:116-122 + 40-62, densecrf.pyx:71-75.  feat is (d, N) row-major. */
+int dcrf_add_pairwise(dcrf_model *m, const float *feat_dxN, int d, int compat_kind,
+                      const float *compat, int ktype, int ntype);
+/* addPairwiseGaussian(sx, sy, ...): densecrf.cpp:61-69, densecrf.pyx:120-124.
+ * Features (x/sx, y/sy) are generated on the device (2-D models only). */
+int dcrf_add_gaussian_2d(dcrf_model *m, float sx, float sy, int compat_kind,
+                         const float *compat, int ktype, int ntype);
+/* addPairwiseBilateral(sx, sy, sr, sg, sb, im, ...): densecrf.cpp:70-81,
+ * densecrf.pyx:126-140.  rgb is (H, W, 3) uint8; for a batch, the images of all
+ * instances concatenated in instance order. */
+int dcrf_add_bilateral_2d(dcrf_model *m, float sx, float sy, float sr, float sg, float sb,
+                          const uint8_t *rgb_hwc, int compat_kind, const float *compat,
+                          int ktype, int ntype);
+
+/* ---- inference (host buffers) -------------------------------------------------- */
+/* inference(n): densecrf.cpp:115-131, densecrf.pyx:91-92.  Q out [N][L]. */
+int dcrf_inference(dcrf_model *m, int n_iterations, float *Q);
+/* startInference(): densecrf.cpp:178-186, densecrf.pyx:94-95.  Q out [N][L]. */
+int dcrf_start_inference(dcrf_model *m, float *Q);
+/* stepInference(Q, tmp1, tmp2): densecrf.cpp:187-201, densecrf.pyx:97-98.  Q is
+ * updated in place; tmp1 receives -U - sum_k pairwise_k(Q), tmp2 the last term's
+ * message (either may be NULL).  All [N][L]. */
+int dcrf_step_inference(dcrf_model *m, float *Q, float *tmp1, float *tmp2);
+/* klDivergence(Q): densecrf.cpp:214-235, densecrf.pyx:100-101. */
+int dcrf_kl_divergence(dcrf_model *m, const float *Q, double *kl);
+/* PairwisePotential::apply / applyTranspose of one term (pairwise.cpp:173-183):
+ * out = compat(K~ Q) or its transpose.  Q, out [N][L]. */
+int dcrf_apply_pairwise(dcrf_model *m, int term, const float *Q, float *out, int transpose);
+
+/* ---- inference (device buffers, stream-ordered) ------------------------------- */
+/* inference(n) on device buffers: U_dev [N][L] (NULL = the model's unary), Q_dev out
+ * [N][L].  Enqueued on `stream` (a hipStream_t; NULL is the null stream).  Returns
+ * once the work is enqueued; the model's internal buffers are in use until it ends. */
+int dcrf_inference_device(dcrf_model *m, int n_iterations, const float *U_dev, float *Q_dev,
+                          void *stream);
+
+/* ---- introspection (parity tests, metrics) ------------------------------------- */
+int dcrf_num_variables(const dcrf_model *m);
+int dcrf_num_labels(const dcrf_model *m);
+int dcrf_num_terms(const dcrf_model *m);
+int dcrf_num_instances(const dcrf_model *m);
+/* Dimension and vertex count of a term's lattice (Permutohedral d_, M_:
+ * permutohedral.h:53-55); for a batch, V is summed over instances. */
+int dcrf_term_info(const dcrf_model *m, int term, int *d, int *V);
+/* Copy a term's lattice to host: offset/rank [N][d+1] int32, bary [N][d+1] float
+ * (Permutohedral offset_/rank_/barycentric_ for the real points, permutohedral.cpp:
+ * 272-274), nbr [(d+1)][V][2] int32 with -1 for an absent neighbour (blur_neighbors_,
+ * :305-318), keys [V][d] int16 (the lattice key of each vertex), norm [N] (DenseKernel
+ * norm_, pairwise.cpp:40-62).  Any pointer may be NULL.  Vertex ids are per instance,
+ * in first-occurrence order like the reference's hash table. */
+int dcrf_get_lattice(const dcrf_model *m, int term, int instance, int *offset, int *rank,
+                     float *bary, int *nbr, int16_t *keys, float *norm);
+/* Vertex count of one instance of a term. */
+int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int *V);
+/* Set the chunk length of the splat's segmented reduction (0 = autotuned default). */
+int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
+/* Kernel timing: when enabled, the library records HIP events around every blur-pass
+ * launch on the model's stream.  dcrf_blur_stats returns the number of timed blur
+ * launches, their total milliseconds and the algorithmic bytes they moved
+ * (sum over launches of V_pass * (2*L*4 + 8)). */
+int dcrf_enable_timing(dcrf_model *m, int on);
+int dcrf_blur_stats(dcrf_model *m, long long *launches, double *ms, double *bytes);
+/* Algorithmic bytes of one mean-field iteration (Q/U streams, lattice entry arrays,
+ * normalisers, vertex rows, neighbour tables). */
+int dcrf_iteration_bytes(const dcrf_model *m, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DCRF_HIP_H */

@@ -1,0 +1,102 @@
+"""In-tree build of the native parts.
+
+    python -m pydensecrf_amd._build
+
+1. libdcrf_hip.so   - the HIP kernels and the C ABI (hipcc, gfx950)
+2. eigen / densecrf - the Cython front end, linked against libdcrf_hip.so
+3. oracle/          - the test-only CPU checkers (make -C oracle [ref])
+
+Object files and generated C++ go to _build/ at the repository root.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+ORACLE = os.path.join(ROOT, "oracle")
+REFERENCE = "/root/reference"
+
+HIP_SOURCES = ["lattice.hip", "filter.hip", "capi.hip"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # keep the compiler from forming packed-FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+CY_MODULES = ["eigen", "densecrf"]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, cwd=None, verbose=False):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), r.stdout))
+    return r.stdout
+
+
+def build_hip(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
+        futs = []
+        for src in HIP_SOURCES:
+            obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+            objs.append(obj)
+            futs.append(ex.submit(_run, [hipcc] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj],
+                                  None, verbose))
+        for f in futs:
+            f.result()
+    out = os.path.join(PKG, "libdcrf_hip.so")
+    tmp = out + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose=verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build_cython(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    ext = sysconfig.get_config_var("EXT_SUFFIX")
+    pyinc = sysconfig.get_paths()["include"]
+    cython = shutil.which("cython") or "cython"
+    for m in CY_MODULES:
+        cpp = os.path.join(BUILD, m + ".cpp")
+        _run([cython, "--cplus", "-3", "-I", ROOT, os.path.join(PKG, m + ".pyx"), "-o", cpp], cwd=ROOT,
+             verbose=verbose)
+        out = os.path.join(PKG, m + ext)
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-fPIC", "-shared", "-w", "-I" + pyinc, "-I" + INCLUDE, cpp, "-L" + PKG, "-ldcrf_hip",
+              "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose=verbose)
+        os.replace(tmp, out)
+
+
+def build_oracle(verbose=False):
+    _run(["make", "-C", ORACLE, "-s"], verbose=verbose)
+    if os.path.isdir(os.path.join(REFERENCE, "pydensecrf", "densecrf", "src")):
+        _run(["make", "-C", ORACLE, "-s", "ref", "REF_ROOT=" + REFERENCE], verbose=verbose)
+
+
+def build(verbose=False):
+    build_hip(verbose)
+    build_cython(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built", flush=True)

@@ -1,0 +1,291 @@
+"""ctypes binding of the C ABI in include/dcrf_hip.h (libdcrf_hip.so, built in-tree).
+
+The Cython module ``pydensecrf_amd.densecrf`` is the drop-in front end; this module
+gives Python direct access to the rest of the ABI: batched models, device-pointer
+entry points (used with torch tensors), lattice introspection and timing.  Loading
+fails loudly if the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcrf_hip.so")
+
+OK, ERR_VALUE, ERR_MEMORY, ERR_DEVICE, ERR_OVERFLOW, ERR_STATE = 0, -1, -2, -3, -4, -5
+NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC = 0, 1, 2, 3
+CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL = 0, 1, 2
+COMPAT_POTTS, COMPAT_DIAGONAL, COMPAT_MATRIX = 0, 1, 2
+MAX_DIM, MAX_LABELS, MAX_TERMS = 15, 1024, 8
+
+_P = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.POINTER(ctypes.c_int)
+_D = ctypes.POINTER(ctypes.c_double)
+_S = ctypes.POINTER(ctypes.c_int16)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_LL = ctypes.POINTER(ctypes.c_longlong)
+_c = ctypes.c_int
+_f = ctypes.c_float
+
+# name -> (restype, argtypes); every symbol declared in include/dcrf_hip.h
+SIGNATURES = {
+    "dcrf_last_error": (ctypes.c_char_p, []),
+    "dcrf_device_count": (_c, []),
+    "dcrf_create": (_c, [_c, _c, ctypes.POINTER(_P)]),
+    "dcrf_create_2d": (_c, [_c, _c, _c, ctypes.POINTER(_P)]),
+    "dcrf_create_batch": (_c, [_c, _I, _c, ctypes.POINTER(_P)]),
+    "dcrf_create_batch_2d": (_c, [_c, _I, _I, _c, ctypes.POINTER(_P)]),
+    "dcrf_destroy": (None, [_P]),
+    "dcrf_set_unary": (_c, [_P, _F]),
+    "dcrf_set_unary_instance": (_c, [_P, _c, _F]),
+    "dcrf_add_pairwise": (_c, [_P, _F, _c, _c, _F, _c, _c]),
+    "dcrf_add_gaussian_2d": (_c, [_P, _f, _f, _c, _F, _c, _c]),
+    "dcrf_add_bilateral_2d": (_c, [_P, _f, _f, _f, _f, _f, _U8, _c, _F, _c, _c]),
+    "dcrf_inference": (_c, [_P, _c, _F]),
+    "dcrf_start_inference": (_c, [_P, _F]),
+    "dcrf_step_inference": (_c, [_P, _F, _F, _F]),
+    "dcrf_kl_divergence": (_c, [_P, _F, _D]),
+    "dcrf_apply_pairwise": (_c, [_P, _c, _F, _F, _c]),
+    "dcrf_inference_device": (_c, [_P, _c, _P, _P, _P]),
+    "dcrf_num_variables": (_c, [_P]),
+    "dcrf_num_labels": (_c, [_P]),
+    "dcrf_num_terms": (_c, [_P]),
+    "dcrf_num_instances": (_c, [_P]),
+    "dcrf_term_info": (_c, [_P, _c, _I, _I]),
+    "dcrf_get_lattice": (_c, [_P, _c, _c, _I, _I, _F, _I, _S, _F]),
+    "dcrf_term_instance_vertices": (_c, [_P, _c, _c, _I]),
+    "dcrf_set_splat_chunk": (_c, [_P, _c]),
+    "dcrf_enable_timing": (_c, [_P, _c]),
+    "dcrf_blur_stats": (_c, [_P, _LL, _D, _D]),
+    "dcrf_iteration_bytes": (_c, [_P, _D]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library with typed entry points (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdcrf_hip.so is not built; run __graft_entry__.build() or "
+                              "python -m pydensecrf_amd._build")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class DCRFError(RuntimeError):
+    pass
+
+
+def check(status):
+    if status == OK:
+        return
+    msg = lib().dcrf_last_error().decode("utf-8", "replace")
+    if status in (ERR_VALUE, ERR_OVERFLOW):
+        raise ValueError(msg)
+    if status == ERR_MEMORY:
+        raise MemoryError(msg)
+    raise DCRFError(msg)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_F)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def compat_args(compat, L):
+    """(kind, float32 array) of a compatibility given like the reference's _labelcomp."""
+    from numbers import Number
+    if isinstance(compat, Number):
+        return COMPAT_POTTS, np.array([compat], dtype=np.float32)
+    c = np.asarray(compat)
+    if c.dtype != np.float32:
+        raise ValueError("Buffer dtype mismatch, expected 'float' but got '%s'" % c.dtype)
+    c = np.ascontiguousarray(c)
+    if c.ndim == 1:
+        return COMPAT_DIAGONAL, c
+    if c.ndim == 2:
+        return COMPAT_MATRIX, c
+    raise ValueError("LabelCompatibility of dimension >2 not meaningful.")
+
+
+class Model:
+    """Owner of a dcrf_model handle (plain, 2-D or batched)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def create(cls, N, L):
+        h = _P()
+        check(lib().dcrf_create(N, L, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create_2d(cls, W, H, L):
+        h = _P()
+        check(lib().dcrf_create_2d(W, H, L, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create_batch(cls, Ns, L):
+        a = np.ascontiguousarray(Ns, dtype=np.int32)
+        h = _P()
+        check(lib().dcrf_create_batch(len(a), a.ctypes.data_as(_I), L, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create_batch_2d(cls, Ws, Hs, L):
+        w = np.ascontiguousarray(Ws, dtype=np.int32)
+        hh = np.ascontiguousarray(Hs, dtype=np.int32)
+        h = _P()
+        check(lib().dcrf_create_batch_2d(len(w), w.ctypes.data_as(_I), hh.ctypes.data_as(_I), L, ctypes.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if self.h:
+            lib().dcrf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- sizes
+    @property
+    def N(self):
+        return lib().dcrf_num_variables(self.h)
+
+    @property
+    def L(self):
+        return lib().dcrf_num_labels(self.h)
+
+    @property
+    def nterms(self):
+        return lib().dcrf_num_terms(self.h)
+
+    @property
+    def ninstances(self):
+        return lib().dcrf_num_instances(self.h)
+
+    # -- setup
+    def set_unary(self, U_LxN):
+        U = f32(U_LxN)
+        if U.shape != (self.L, self.N):
+            raise ValueError("unary must be %s, got %s" % ((self.L, self.N), U.shape))
+        check(lib().dcrf_set_unary(self.h, fptr(U)))
+
+    def set_unary_instance(self, i, U_LxNi):
+        U = f32(U_LxNi)
+        check(lib().dcrf_set_unary_instance(self.h, i, fptr(U)))
+
+    def add_pairwise(self, feat_dxN, compat, ktype=DIAG_KERNEL, ntype=NORMALIZE_SYMMETRIC):
+        f = f32(feat_dxN)
+        if f.ndim != 2 or f.shape[1] != self.N:
+            raise ValueError("features must be (d, %d), got %s" % (self.N, f.shape))
+        kind, c = compat_args(compat, self.L)
+        check(lib().dcrf_add_pairwise(self.h, fptr(f), f.shape[0], kind, fptr(c), ktype, ntype))
+
+    def add_gaussian(self, sx, sy, compat, ktype=DIAG_KERNEL, ntype=NORMALIZE_SYMMETRIC):
+        kind, c = compat_args(compat, self.L)
+        check(lib().dcrf_add_gaussian_2d(self.h, sx, sy, kind, fptr(c), ktype, ntype))
+
+    def add_bilateral(self, sx, sy, sr, sg, sb, rgb, compat, ktype=DIAG_KERNEL, ntype=NORMALIZE_SYMMETRIC):
+        im = np.ascontiguousarray(rgb, dtype=np.uint8)
+        if im.size != self.N * 3:
+            raise ValueError("rgb must hold N*3 bytes")
+        kind, c = compat_args(compat, self.L)
+        check(lib().dcrf_add_bilateral_2d(self.h, sx, sy, sr, sg, sb, im.ctypes.data_as(_U8), kind, fptr(c),
+                                          ktype, ntype))
+
+    # -- inference on host buffers; Q returned as (L, N)
+    def _buf(self):
+        return np.empty((self.N, self.L), dtype=np.float32)
+
+    def inference(self, n):
+        Q = self._buf()
+        check(lib().dcrf_inference(self.h, n, fptr(Q)))
+        return Q.T
+
+    def start(self):
+        Q = self._buf()
+        check(lib().dcrf_start_inference(self.h, fptr(Q)))
+        return Q.T
+
+    def step(self, Q_LxN):
+        Q = np.ascontiguousarray(f32(Q_LxN).T)
+        t1, t2 = self._buf(), self._buf()
+        check(lib().dcrf_step_inference(self.h, fptr(Q), fptr(t1), fptr(t2)))
+        return Q.T, t1.T, t2.T
+
+    def kl(self, Q_LxN):
+        Q = np.ascontiguousarray(f32(Q_LxN).T)
+        out = ctypes.c_double()
+        check(lib().dcrf_kl_divergence(self.h, fptr(Q), ctypes.byref(out)))
+        return out.value
+
+    def apply(self, term, Q_LxN, transpose=False):
+        Q = np.ascontiguousarray(f32(Q_LxN).T)
+        out = self._buf()
+        check(lib().dcrf_apply_pairwise(self.h, term, fptr(Q), fptr(out), int(bool(transpose))))
+        return out.T
+
+    def inference_device(self, n, Q_ptr, U_ptr=None, stream=None):
+        check(lib().dcrf_inference_device(self.h, n, U_ptr, Q_ptr, stream))
+
+    # -- introspection
+    def term_info(self, term):
+        d, V = ctypes.c_int(), ctypes.c_int()
+        check(lib().dcrf_term_info(self.h, term, ctypes.byref(d), ctypes.byref(V)))
+        return d.value, V.value
+
+    def instance_vertices(self, term, inst=0):
+        V = ctypes.c_int()
+        check(lib().dcrf_term_instance_vertices(self.h, term, inst, ctypes.byref(V)))
+        return V.value
+
+    def lattice(self, term, inst=0, Ni=None):
+        """Lattice arrays of one instance: offset, rank, bary (Ni, d+1), nbr (d+1, V, 2),
+        keys (V, d), norm (Ni,)."""
+        d, _ = self.term_info(term)
+        V = self.instance_vertices(term, inst)
+        if Ni is None:
+            Ni = self.N
+        off = np.empty((Ni, d + 1), np.int32)
+        rank = np.empty((Ni, d + 1), np.int32)
+        bary = np.empty((Ni, d + 1), np.float32)
+        nbr = np.empty((d + 1, V, 2), np.int32)
+        keys = np.empty((V, d), np.int16)
+        norm = np.empty(Ni, np.float32)
+        check(lib().dcrf_get_lattice(self.h, term, inst, off.ctypes.data_as(_I), rank.ctypes.data_as(_I),
+                                     fptr(bary), nbr.ctypes.data_as(_I), keys.ctypes.data_as(_S), fptr(norm)))
+        return dict(offset=off, rank=rank, bary=bary, nbr=nbr, keys=keys, norm=norm, V=V, d=d)
+
+    def set_splat_chunk(self, c):
+        check(lib().dcrf_set_splat_chunk(self.h, c))
+
+    def enable_timing(self, on=True):
+        check(lib().dcrf_enable_timing(self.h, int(bool(on))))
+
+    def blur_stats(self):
+        n, ms, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+        check(lib().dcrf_blur_stats(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+        return n.value, ms.value, b.value
+
+    def iteration_bytes(self):
+        b = ctypes.c_double()
+        check(lib().dcrf_iteration_bytes(self.h, ctypes.byref(b)))
+        return b.value

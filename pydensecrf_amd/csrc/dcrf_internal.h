@@ -1,0 +1,212 @@
+// Internal declarations shared by the HIP translation units of libdcrf_hip.so.
+//
+// Device layout of a model (N variables over all instances, L labels):
+//   Q, unary                [N][L] float, label fastest (the reference's Eigen storage)
+// Per pairwise term k with feature dimension d (term-local vertex ids 0..V_k-1):
+//   vtx, bary, rank         [N][d+1]   lattice vertex / barycentric weight / rank per
+//                                      (pixel, remainder)   (Permutohedral offset_ ...)
+//   nbr                     [d+1][V][2] blur neighbours, -1 absent
+//   csr_ptr, csr_e          vertex -> its (pixel, remainder) entries, pixel ascending
+//   chunks                  the splat's segments of at most `chunk` entries
+// Global filter layout (built once the terms are fixed): the vertices of all terms
+// are concatenated in order of decreasing d, followed by one all-zero row (the
+// "missing neighbour" row) and the partial-sum rows of multi-chunk vertices.  Blur
+// pass s then touches a prefix of the vertex space: the terms with d >= s.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dcrf_hip.h"
+
+namespace dcrf {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+    int code;
+    std::string msg;
+};
+void set_error(int code, const std::string& msg);
+#define DCRF_HIP_CHECK(expr)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::dcrf::Error{e_ == hipErrorOutOfMemory ? DCRF_ERR_MEMORY : DCRF_ERR_DEVICE, \
+                                std::string(#expr) + ": " + hipGetErrorString(e_)};      \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // (Re)allocate to hold at least `count` elements; contents are not preserved.
+    void reserve(size_t count) {
+        if (count <= n && p) return;
+        release();
+        size_t bytes = sizeof(T) * (count ? count : 1);
+        DCRF_HIP_CHECK(hipMalloc(&p, bytes));
+        n = count;
+    }
+    T* get() const { return p; }
+};
+
+// ---------------------------------------------------------------- lattice of one term
+struct Chunk {
+    int v;    // vertex (term-local, later global)
+    int beg;  // first CSR entry
+    int end;  // one past the last CSR entry
+    int out;  // output row: >= 0 vertex row; < 0 partial row -(r+1) (term-local)
+};
+struct Combine {
+    int v;      // vertex
+    int first;  // first partial row
+    int count;  // number of partial rows
+    int pad;
+};
+
+struct TermLattice {
+    int d = 0;
+    int V = 0;                    // vertices over all instances
+    std::vector<int> inst_vbase;  // first vertex of each instance (size n_inst+1)
+    int overflow = 0;
+    DevBuf<int> vtx;     // [N][d+1]
+    DevBuf<float> bary;  // [N][d+1]
+    DevBuf<int> rank;    // [N][d+1]
+    DevBuf<int> nbr;     // [(d+1)][V][2]
+    DevBuf<int16_t> keys;  // [V][d]
+    DevBuf<int> csr_ptr;   // [V+1]
+    DevBuf<int> csr_e;     // [N*(d+1)] entry index p*(d+1)+r, sorted by (vertex, entry)
+    DevBuf<Chunk> chunks;  // splat segments
+    int n_chunks = 0;
+    DevBuf<Combine> combines;  // multi-chunk vertices
+    int n_combines = 0;
+    int n_partials = 0;
+    int chunk_len = 0;
+};
+
+struct Term {
+    int ktype = DCRF_DIAG_KERNEL, ntype = DCRF_NORMALIZE_SYMMETRIC, ckind = DCRF_COMPAT_POTTS;
+    std::vector<float> compat;  // Potts [1], Diagonal [L], Matrix [L*L] symmetrised
+    DevBuf<float> d_compat;
+    TermLattice lat;
+    DevBuf<float> norm;  // [N]
+};
+
+// Instance geometry of a model (a plain model is one instance).
+struct Geometry {
+    int n_inst = 1;
+    std::vector<int> N;         // per instance
+    std::vector<int> W, H;      // per instance (2-D models)
+    std::vector<int> pix_base;  // size n+1
+    std::vector<int> pad_base;  // size n+1, padded point bases (ceil4 per instance)
+    bool is2d = false;
+    int N_total() const { return pix_base.back(); }
+    int npad_total() const { return pad_base.back(); }
+};
+
+// Device copies of the geometry tables.
+struct GeometryDev {
+    DevBuf<int> pix_base, pad_base, W;
+};
+
+// ---------------------------------------------------------------- lattice build
+// Builds the lattice of one term from padded point-major features feat [npad][d]
+// (pad points zero).  Fills lat.{vtx,bary,rank,nbr,keys,csr_ptr,csr_e,V,inst_vbase}.
+void build_lattice(const Geometry& g, const GeometryDev& gd, const float* feat, int d,
+                   TermLattice& lat, float hash_load, hipStream_t s);
+// Splat segments of at most `chunk` entries per vertex.
+void build_chunks(TermLattice& lat, int chunk, hipStream_t s);
+// Feature generators into padded point-major layout [npad][d].
+void features_from_dxN(const Geometry& g, const GeometryDev& gd, const float* feat_dxN, int d,
+                       float* out, hipStream_t s);
+void features_gaussian_2d(const Geometry& g, const GeometryDev& gd, float sx, float sy,
+                          float* out, hipStream_t s);
+void features_bilateral_2d(const Geometry& g, const GeometryDev& gd, float sx, float sy, float sr,
+                           float sg, float sb, const uint8_t* rgb, float* out, hipStream_t s);
+
+// ---------------------------------------------------------------- filter kernels
+struct BlurTerm {
+    int vbase;       // first global vertex
+    int V;           // vertices
+    int d;           // feature dimension (axes 0..d)
+    const int2* nbr; // [(d+1)][V] global neighbour rows (zero row for absent)
+};
+struct BlurTable {
+    int nterms;
+    BlurTerm t[DCRF_MAX_TERMS];
+};
+
+// One term as seen by the per-pixel kernels (slice + compatibility).
+struct SliceTerm {
+    const int* vtx;     // [N][d+1] global vertex rows
+    const float* sw;    // [N][d+1] slice weights: bary * alpha * post-normaliser
+    const float* vals;  // vertex rows after the last blur pass
+    int d1;
+    int ckind;
+    float potts;        // Potts weight
+    const float* cw;    // Diagonal [L] or Matrix [L*L] (device)
+    float scale;        // extra weight multiplying the term's message
+};
+struct SliceTable {
+    int nterms;
+    SliceTerm t[DCRF_MAX_TERMS];
+};
+
+struct LaunchShape {
+    int block;  // threads per block
+    int per;    // items per block (block / L)
+};
+LaunchShape shape_for(int L);
+
+void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w,
+                  const float* in, float* vals, int L, hipStream_t s);
+void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s);
+void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step,
+                 int reverse, hipStream_t s);
+// Mean-field update: E = -U - sum_k compat_k(slice_k); Q = softmax(E).  U may be null.
+// tmp1/tmp2 optional outputs (energy, last term's message).
+void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1,
+                      float* tmp2, hipStream_t s);
+// out = compat_k(slice_k) for term index `which` of st.
+void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s);
+// Q0 = softmax(-U).
+void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s);
+// KL divergence partial sums (one double per block).
+int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st, double* partials,
+              hipStream_t s);
+// norm = f(K*1) per the normalisation type, from the sliced values (L = 1).
+void launch_norm_finish(float* norm, int N, int ntype, hipStream_t s);
+void launch_fill(float* p, float v, size_t n, hipStream_t s);
+// out[c][r] = in[r][c] (in is rows x cols row-major).
+void launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s);
+// Global layout remaps.
+void launch_remap_chunks(const Chunk* src, Chunk* dst, int n, int vbase, int ebase, int prow_base,
+                         hipStream_t s);
+void launch_remap_combines(const Combine* src, Combine* dst, int n, int vbase, int prow_base,
+                           hipStream_t s);
+void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);
+void launch_csr_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm,
+                       int use_norm, int* pt_out, float* w_out, hipStream_t s);
+void launch_slice_weights(const int* vtx, const float* bary, const float* norm, int use_norm,
+                          float alpha, int N, int d1, int vbase, int* gvtx, float* sw,
+                          hipStream_t s);
+
+}  // namespace dcrf

@@ -1,0 +1,420 @@
+// Lattice filtering and the fused mean-field update for gfx950.
+//
+// Replaces Permutohedral::compute (pydensecrf/densecrf/src/permutohedral.cpp:476-610),
+// DenseKernel::filter (pairwise.cpp:63-80), LabelCompatibility::apply
+// (labelcompatibility.cpp:46-87) and expAndNormalize (densecrf.cpp:98-106).
+//
+// Thread mapping: a lane group of L consecutive threads handles one item (a splat
+// chunk, a lattice vertex or a pixel), lane l owning label l, so every vertex/pixel
+// row of L floats is read and written as one contiguous segment.  A block holds
+// block/L groups.  All reductions run in a fixed order: results are deterministic
+// from run to run.
+//   splat    each chunk (<= C contributors of one vertex, pixel ascending) is summed
+//            sequentially; vertices with several chunks write partial rows that the
+//            combine kernel adds in chunk order.  No atomics.
+//   blur     one launch per step; step s covers the terms whose lattice has axis s
+//            (a prefix of the vertex space, which is ordered by decreasing d):
+//            new = old + 0.5 * (n1 + n2), the reference's update (:556-569).
+//   meanfield per pixel: slice every term (sum_j w_j*alpha*norm_p * row_j), apply the
+//            compatibility, subtract from -U, softmax over the lane group.
+#include "dcrf_internal.h"
+
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace dcrf {
+
+LaunchShape shape_for(int L) {
+    const int block = L <= 256 ? 256 : (L <= 512 ? 512 : 1024);
+    return LaunchShape{block, block / L};
+}
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
+
+__global__ void k_fill(float* p, float v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_splat(const Chunk* __restrict__ chunks, int n, const int* __restrict__ pt,
+                        const float* __restrict__ w, const float* __restrict__ in,
+                        float* __restrict__ vals, int L, int per) {
+    const int t = threadIdx.x;
+    const int g = t / L, l = t - g * L;
+    const int c = blockIdx.x * per + g;
+    if (g >= per || c >= n) return;
+    const Chunk ch = chunks[c];
+    float acc = 0.0f;
+#pragma unroll 4
+    for (int i = ch.beg; i < ch.end; i++) {
+        const int p = pt[i];
+        const float wi = w[i];
+        const float x = in[(size_t)p * L + l];
+        const float prod = wi * x;
+        acc = acc + prod;
+    }
+    vals[(size_t)ch.out * L + l] = acc;
+}
+
+__global__ void k_combine(const Combine* __restrict__ cb, int n, float* __restrict__ vals, int L,
+                          int per) {
+    const int t = threadIdx.x;
+    const int g = t / L, l = t - g * L;
+    const int c = blockIdx.x * per + g;
+    if (g >= per || c >= n) return;
+    const Combine r = cb[c];
+    float acc = 0.0f;
+    for (int i = 0; i < r.count; i++) acc = acc + vals[(size_t)(r.first + i) * L + l];
+    vals[(size_t)r.v * L + l] = acc;
+}
+
+__global__ void k_blur(const float* __restrict__ src, float* __restrict__ dst, int Vp, int L, int per,
+                       BlurTable bt, int step, int reverse) {
+    const int t = threadIdx.x;
+    const int g = t / L, l = t - g * L;
+    const int v = blockIdx.x * per + g;
+    if (g >= per || v >= Vp) return;
+    // terms are ordered by vbase; pick the last one starting at or before v
+    int vb = bt.t[0].vbase, V = bt.t[0].V, d = bt.t[0].d;
+    const int2* nbr = bt.t[0].nbr;
+#pragma unroll
+    for (int k = 1; k < DCRF_MAX_TERMS; k++) {
+        if (k < bt.nterms && v >= bt.t[k].vbase) {
+            vb = bt.t[k].vbase;
+            V = bt.t[k].V;
+            d = bt.t[k].d;
+            nbr = bt.t[k].nbr;
+        }
+    }
+    const int j = reverse ? d - step : step;
+    const int2 nb = nbr[(size_t)j * V + (v - vb)];
+    const float a = src[(size_t)nb.x * L + l];
+    const float b = src[(size_t)nb.y * L + l];
+    const float o = src[(size_t)v * L + l];
+    const float s2 = a + b;
+    const float h = 0.5f * s2;
+    dst[(size_t)v * L + l] = o + h;
+}
+
+// Slice of term T for pixel p, label l: sum_j sw_j * vals[vtx_j][l] in j order.
+__device__ inline float slice_one(const SliceTerm& T, int p, int l, int L) {
+    float m = 0.0f;
+    const int* vt = T.vtx + (size_t)p * T.d1;
+    const float* sw = T.sw + (size_t)p * T.d1;
+    for (int j = 0; j < T.d1; j++) {
+        const float prod = sw[j] * T.vals[(size_t)vt[j] * L + l];
+        m = m + prod;
+    }
+    return m;
+}
+
+// Compatibility transform of one message value.  Matrix needs the whole label vector
+// of the pixel: it goes through shared memory (sh holds block floats).  Must be
+// called by every thread of the block (it synchronises for the Matrix form).
+__device__ inline float compat_apply(const SliceTerm& T, float m, float* sh, int t, int g, int l,
+                                     int L) {
+    if (T.ckind == DCRF_COMPAT_POTTS) {
+        const float nw = -T.potts;
+        return nw * m;
+    }
+    if (T.ckind == DCRF_COMPAT_DIAGONAL) return T.cw[l] * m;
+    sh[t] = m;
+    __syncthreads();
+    float c = 0.0f;
+    if (l >= 0) {
+        const float* row = T.cw + (size_t)l * L;
+        const float* mv = sh + g * L;
+        for (int b = 0; b < L; b++) {
+            const float prod = row[b] * mv[b];
+            c = c + prod;
+        }
+    }
+    __syncthreads();
+    return c;
+}
+
+__global__ void k_meanfield(const float* __restrict__ U, float* __restrict__ Q, int N, int L, int per,
+                            SliceTable st, float* __restrict__ tmp1, float* __restrict__ tmp2) {
+    extern __shared__ float sh[];
+    const int t = threadIdx.x;
+    int g = t / L, l = t - g * L;
+    const int p = blockIdx.x * per + g;
+    const bool active = g < per && p < N;
+    if (g >= per) { g = 0; l = 0; }  // idle lanes read a valid shared slot
+    const size_t idx = (size_t)p * L + l;
+    float e = 0.0f;
+    if (active && U) e = -U[idx];
+    float last = 0.0f;
+#pragma unroll
+    for (int k = 0; k < DCRF_MAX_TERMS; k++) {
+        if (k < st.nterms) {
+            const SliceTerm& T = st.t[k];
+            float m = active ? slice_one(T, p, l, L) : 0.0f;
+            if (T.scale != 1.0f) m = m * T.scale;
+            const float c = compat_apply(T, m, sh, t, g, l, L);
+            e = e - c;
+            last = c;
+        }
+    }
+    // softmax over the lane group: max, exp, sum in label order
+    sh[t] = e;
+    __syncthreads();
+    float mx = sh[g * L];
+    for (int b = 1; b < L; b++) mx = fmaxf(mx, sh[g * L + b]);
+    const float x = expf(e - mx);
+    __syncthreads();
+    sh[t] = x;
+    __syncthreads();
+    float s = 0.0f;
+    for (int b = 0; b < L; b++) s = s + sh[g * L + b];
+    if (active) {
+        Q[idx] = x / s;
+        if (tmp1) tmp1[idx] = e;
+        if (tmp2) tmp2[idx] = last;
+    }
+}
+
+// out = compat_which(slice_which) (ckind < 0: the plain slice).
+__global__ void k_slice_term(float* __restrict__ out, int N, int L, int per, SliceTable st, int which) {
+    extern __shared__ float sh[];
+    const int t = threadIdx.x;
+    int g = t / L, l = t - g * L;
+    const int p = blockIdx.x * per + g;
+    const bool active = g < per && p < N;
+    if (g >= per) { g = 0; l = 0; }
+    SliceTerm T = st.t[0];
+#pragma unroll
+    for (int k = 1; k < DCRF_MAX_TERMS; k++)
+        if (k == which) T = st.t[k];
+    float m = active ? slice_one(T, p, l, L) : 0.0f;
+    if (T.scale != 1.0f) m = m * T.scale;
+    if (T.ckind >= 0) m = compat_apply(T, m, sh, t, g, l, L);
+    if (active) out[(size_t)p * L + l] = m;
+}
+
+// KL divergence terms per (pixel, label), reduced per block in double in a fixed order
+// (densecrf.cpp:214-235: q log max(q,1e-20) + u q + sum_k q * pairwise_k(Q)).
+__global__ void k_kl(const float* __restrict__ Q, const float* __restrict__ U, int N, int L, int per,
+                     SliceTable st, double* __restrict__ partial) {
+    extern __shared__ float sh[];
+    __shared__ double red[1024];
+    const int t = threadIdx.x;
+    int g = t / L, l = t - g * L;
+    const int p = blockIdx.x * per + g;
+    const bool active = g < per && p < N;
+    if (g >= per) { g = 0; l = 0; }
+    const size_t idx = (size_t)p * L + l;
+    const float q = active ? Q[idx] : 0.0f;
+    double acc = 0.0;
+    if (active) {
+        const float qc = q > 1e-20f ? q : 1e-20f;
+        acc += (double)q * log((double)qc);
+        if (U) acc += (double)(U[idx] * q);
+    }
+#pragma unroll
+    for (int k = 0; k < DCRF_MAX_TERMS; k++) {
+        if (k < st.nterms) {
+            const SliceTerm& T = st.t[k];
+            float m = active ? slice_one(T, p, l, L) : 0.0f;
+            if (T.scale != 1.0f) m = m * T.scale;
+            const float c = compat_apply(T, m, sh, t, g, l, L);
+            if (active) acc += (double)(q * c);
+        }
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) partial[blockIdx.x] = red[0];
+}
+
+// DenseKernel::initLattice normalisers (pairwise.cpp:54-61), in double like the reference.
+__global__ void k_norm_finish(float* norm, int N, int ntype) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double x = (double)norm[i] + 1e-20;
+    if (ntype == DCRF_NORMALIZE_SYMMETRIC) norm[i] = (float)(1.0 / sqrt(x));
+    else if (ntype == DCRF_NORMALIZE_BEFORE || ntype == DCRF_NORMALIZE_AFTER) norm[i] = (float)(1.0 / x);
+}
+
+__global__ void k_remap_chunks(const Chunk* __restrict__ src, Chunk* __restrict__ dst, int n, int vbase,
+                               int ebase, int prow_base) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Chunk c = src[i];
+    c.v += vbase;
+    c.beg += ebase;
+    c.end += ebase;
+    c.out = c.out >= 0 ? c.out + vbase : prow_base + (-c.out - 1);
+    dst[i] = c;
+}
+
+__global__ void k_remap_combines(const Combine* __restrict__ src, Combine* __restrict__ dst, int n,
+                                 int vbase, int prow_base) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Combine c = src[i];
+    c.v += vbase;
+    c.first += prow_base;
+    dst[i] = c;
+}
+
+__global__ void k_remap_nbr(const int* __restrict__ src, int2* __restrict__ dst, int n, int vbase,
+                            int zero_row) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = src[2 * (size_t)i], b = src[2 * (size_t)i + 1];
+    dst[i] = make_int2(a >= 0 ? a + vbase : zero_row, b >= 0 ? b + vbase : zero_row);
+}
+
+__global__ void k_csr_gather(const int* __restrict__ csr_e, int E, int d1, const float* __restrict__ bary,
+                             const float* __restrict__ norm, int use_norm, int* __restrict__ pt,
+                             float* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    const int e = csr_e[i];
+    const int p = e / d1;
+    pt[i] = p;
+    const float b = bary[e];
+    w[i] = use_norm ? b * norm[p] : b;
+}
+
+__global__ void k_slice_weights(const int* __restrict__ vtx, const float* __restrict__ bary,
+                                const float* __restrict__ norm, int use_norm, float alpha, int N, int d1,
+                                int vbase, int* __restrict__ gvtx, float* __restrict__ sw) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * d1) return;
+    const int p = (int)(i / d1);
+    gvtx[i] = vtx[i] + vbase;
+    const float wa = bary[i] * alpha;
+    sw[i] = use_norm ? wa * norm[p] : wa;
+}
+
+// out[c][r] = in[r][c] for an rows x cols row-major input, 32x32 tiles through LDS.
+__global__ void k_transpose(const float* __restrict__ in, float* __restrict__ out, int rows, int cols) {
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    for (int k = 0; k < 32; k += 8) {
+        const int r = by + ty + k, c = bx + tx;
+        if (r < rows && c < cols) tile[ty + k][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = 0; k < 32; k += 8) {
+        const int c = bx + ty + k, r = by + tx;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][ty + k];
+    }
+}
+
+}  // namespace
+
+void launch_fill(float* p, float v, size_t n, hipStream_t s) {
+    if (!n) return;
+    k_fill<<<blocks_for(n), kThreads, 0, s>>>(p, v, n);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s) {
+    if (!rows || !cols) return;
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    k_transpose<<<grid, 256, 0, s>>>(in, out, rows, cols);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w, const float* in,
+                  float* vals, int L, hipStream_t s) {
+    if (!n_chunks) return;
+    const LaunchShape sh = shape_for(L);
+    k_splat<<<blocks_for(n_chunks, sh.per), sh.block, 0, s>>>(chunks, n_chunks, csr_pt, csr_w, in, vals, L, sh.per);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s) {
+    if (!n) return;
+    const LaunchShape sh = shape_for(L);
+    k_combine<<<blocks_for(n, sh.per), sh.block, 0, s>>>(c, n, vals, L, sh.per);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step, int reverse,
+                 hipStream_t s) {
+    if (!Vp) return;
+    const LaunchShape sh = shape_for(L);
+    k_blur<<<blocks_for(Vp, sh.per), sh.block, 0, s>>>(src, dst, Vp, L, sh.per, bt, step, reverse);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1, float* tmp2,
+                      hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    k_meanfield<<<blocks_for(N, sh.per), sh.block, sizeof(float) * sh.block, s>>>(U, Q, N, L, sh.per, st, tmp1,
+                                                                                  tmp2);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
+    SliceTable st{};
+    st.nterms = 0;
+    launch_meanfield(U, Q, N, L, st, nullptr, nullptr, s);
+}
+
+void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    k_slice_term<<<blocks_for(N, sh.per), sh.block, sizeof(float) * sh.block, s>>>(out, N, L, sh.per, st, which);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st, double* partials,
+              hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    const int nb = (int)blocks_for(N, sh.per);
+    k_kl<<<nb, sh.block, sizeof(float) * sh.block, s>>>(Q, U, N, L, sh.per, st, partials);
+    DCRF_HIP_CHECK(hipGetLastError());
+    return nb;
+}
+
+void launch_norm_finish(float* norm, int N, int ntype, hipStream_t s) {
+    k_norm_finish<<<blocks_for(N), kThreads, 0, s>>>(norm, N, ntype);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_remap_chunks(const Chunk* src, Chunk* dst, int n, int vbase, int ebase, int prow_base, hipStream_t s) {
+    if (!n) return;
+    k_remap_chunks<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, ebase, prow_base);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_remap_combines(const Combine* src, Combine* dst, int n, int vbase, int prow_base, hipStream_t s) {
+    if (!n) return;
+    k_remap_combines<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, prow_base);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s) {
+    if (!n) return;
+    k_remap_nbr<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, zero_row);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_csr_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
+                       int* pt_out, float* w_out, hipStream_t s) {
+    if (!E) return;
+    k_csr_gather<<<blocks_for(E), kThreads, 0, s>>>(csr_e, E, d1, bary, norm, use_norm, pt_out, w_out);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_slice_weights(const int* vtx, const float* bary, const float* norm, int use_norm, float alpha, int N,
+                          int d1, int vbase, int* gvtx, float* sw, hipStream_t s) {
+    if (!N) return;
+    k_slice_weights<<<blocks_for((size_t)N * d1), kThreads, 0, s>>>(vtx, bary, norm, use_norm, alpha, N, d1, vbase,
+                                                                    gvtx, sw);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dcrf

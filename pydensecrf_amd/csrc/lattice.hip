@@ -1,0 +1,563 @@
+// Parallel permutohedral-lattice build for gfx950.
+//
+// Replaces Permutohedral::init (SSE2 form, pydensecrf/densecrf/src/permutohedral.cpp:
+// 140-321).  Stages, each one kernel over all instances of a model:
+//   1. features     -> padded point-major [npad][d] (pad lanes of every instance are 0.0,
+//                      like the reference's 4-wide blocks, :191-196)
+//   2. embed        -> per (point, remainder): lattice key (d int16 coordinates plus the
+//                      instance id), barycentric weight and rank; the float op order of
+//                      :199-263 is kept (contraction off) so results are bit-identical
+//   3. hash insert  -> lock-free open addressing; each slot keeps the smallest entry index
+//                      of its key (atomicMin), i.e. the key's first occurrence
+//   4. number       -> scan over "entry is its key's first occurrence" flags: vertex ids in
+//                      first-occurrence order, which is the reference hash table's
+//                      insertion order (:103-111, :272), so offsets match bit for bit
+//   5. neighbours   -> per (vertex, axis) lookups of key +- (d+1)e_j style moves (:305-318)
+//   6. CSR          -> stable radix sort of entries by vertex: each vertex's contributors
+//                      in ascending pixel order, for the atomic-free splat
+#include "dcrf_internal.h"
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace dcrf {
+namespace {
+
+constexpr int kEmpty = 0x7fffffff;
+constexpr int kThreads = 256;
+
+inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
+
+__device__ inline int find_instance(const int* base, int n_inst, int q) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ features
+__global__ void k_feat_dxN(const float* __restrict__ f, int d, int N_total, const int* pix_base,
+                           const int* pad_base, int n_inst, int npad, float* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npad) return;
+    const int i = n_inst > 1 ? find_instance(pad_base, n_inst, q) : 0;
+    const int local = q - pad_base[i];
+    const int Ni = pix_base[i + 1] - pix_base[i];
+    float* o = out + (size_t)q * d;
+    if (local < Ni) {
+        const int p = pix_base[i] + local;
+        for (int j = 0; j < d; j++) o[j] = f[(size_t)j * N_total + p];
+    } else {
+        for (int j = 0; j < d; j++) o[j] = 0.0f;
+    }
+}
+
+// DenseCRF2D::addPairwiseGaussian / addPairwiseBilateral features (densecrf.cpp:61-81):
+// (x/sx, y/sy[, r/sr, g/sg, b/sb]) with IEEE float division.
+__global__ void k_feat_2d(const int* pix_base, const int* pad_base, const int* Wd, int n_inst,
+                          int npad, float sx, float sy, const uint8_t* __restrict__ rgb, float sr,
+                          float sg, float sb, float* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npad) return;
+    const int i = n_inst > 1 ? find_instance(pad_base, n_inst, q) : 0;
+    const int local = q - pad_base[i];
+    const int Ni = pix_base[i + 1] - pix_base[i];
+    const int d = rgb ? 5 : 2;
+    float* o = out + (size_t)q * d;
+    if (local < Ni) {
+        const int W = Wd[i];
+        const int x = local % W, y = local / W;
+        o[0] = (float)x / sx;
+        o[1] = (float)y / sy;
+        if (rgb) {
+            const size_t p = (size_t)pix_base[i] + local;
+            o[2] = (float)rgb[p * 3 + 0] / sr;
+            o[3] = (float)rgb[p * 3 + 1] / sg;
+            o[4] = (float)rgb[p * 3 + 2] / sb;
+        }
+    } else {
+        for (int j = 0; j < d; j++) o[j] = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------ keys
+template <int D>
+struct Key {
+    static constexpr int W = (D + 4) / 4;  // d int16 coordinates + the instance id
+    uint64_t w[W];
+};
+
+template <int D>
+__device__ inline uint32_t key_hash(const Key<D>& k) {
+    uint64_t h = 0x243F6A8885A308D3ull;
+#pragma unroll
+    for (int i = 0; i < Key<D>::W; i++) {
+        h ^= k.w[i];
+        h *= 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31;
+    }
+    return (uint32_t)(h ^ (h >> 17));
+}
+
+template <int D>
+__device__ inline bool key_eq(const Key<D>& a, const Key<D>& b) {
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < Key<D>::W; i++) eq = eq && (a.w[i] == b.w[i]);
+    return eq;
+}
+
+template <int D>
+__device__ inline void key_set(Key<D>& k, int i, int v) {
+    const uint64_t m = (uint64_t)0xffff << ((i & 3) * 16);
+    k.w[i >> 2] = (k.w[i >> 2] & ~m) | ((uint64_t)(uint16_t)v << ((i & 3) * 16));
+}
+
+template <int D>
+__device__ inline int key_get(const Key<D>& k, int i) {
+    return (int)(int16_t)(uint16_t)(k.w[i >> 2] >> ((i & 3) * 16));
+}
+
+struct ScaleArg {
+    float s[DCRF_MAX_DIM];
+    float inv_d1;
+};
+
+// Permutohedral::init per point (permutohedral.cpp:199-276); see the file header.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_embed(const float* __restrict__ feat, int npad,
+                                                    const int* pix_base, const int* pad_base,
+                                                    int n_inst, ScaleArg sc, Key<D>* __restrict__ keys,
+                                                    float* __restrict__ bary, int* __restrict__ rank,
+                                                    int* __restrict__ overflow) {
+    constexpr int D1 = D + 1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npad) return;
+    const int inst = n_inst > 1 ? find_instance(pad_base, n_inst, q) : 0;
+    const int local = q - pad_base[inst];
+    const bool real = local < pix_base[inst + 1] - pix_base[inst];
+    const int p = pix_base[inst] + local;
+
+    float f[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) f[j] = feat[(size_t)q * D + j];
+    // elevate (:199-207)
+    float elev[D1];
+    float sm = 0.0f;
+#pragma unroll
+    for (int j = D; j > 0; j--) {
+        const float cf = f[j - 1] * sc.s[j - 1];
+        const float jc = (float)j * cf;
+        elev[j] = sm - jc;
+        sm = sm + cf;
+    }
+    elev[0] = sm;
+    // closest remainder-0 point, round half to even (:209-221)
+    const float fd1 = (float)D1;
+    float rem0[D1];
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= D; i++) {
+        float v = sc.inv_d1 * elev[i];
+        v = rintf(v);
+        rem0[i] = v * fd1;
+        sum = sum + v;
+    }
+    // rank (:223-233)
+    float rk[D1];
+#pragma unroll
+    for (int i = 0; i <= D; i++) rk[i] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const float di = elev[i] - rem0[i];
+#pragma unroll
+        for (int j = i + 1; j <= D; j++) {
+            const float dj = elev[j] - rem0[j];
+            const float c = di < dj ? 1.0f : 0.0f;
+            rk[i] = rk[i] + c;
+            rk[j] = rk[j] + (1.0f - c);
+        }
+    }
+    // back onto the plane (:235-242)
+#pragma unroll
+    for (int i = 0; i <= D; i++) {
+        rk[i] = rk[i] + sum;
+        const float add = rk[i] < 0.0f ? fd1 : 0.0f;
+        const float sub = rk[i] >= fd1 ? fd1 : 0.0f;
+        rk[i] = rk[i] + (add - sub);
+        rem0[i] = rem0[i] + (add - sub);
+    }
+    // barycentric (:244-263).  Position D-rank_i receives +v_i and position D-rank_i+1
+    // receives -v_i; with ranks a permutation each slot gets at most one of each, and
+    // (0 + a) - b == (0 - b) + a in float, so slot r holds v_{rank=D-r} - v_{rank=D-r+1}.
+    float vr[D1];
+#pragma unroll
+    for (int r = 0; r <= D; r++) vr[r] = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= D; i++) {
+        const float e = elev[i] - rem0[i];
+        const float v = e * sc.inv_d1;
+        const int ri = (int)rk[i];
+#pragma unroll
+        for (int r = 0; r <= D; r++) vr[r] = (ri == r) ? v : vr[r];
+    }
+    float bc[D1];
+#pragma unroll
+    for (int r = 1; r <= D; r++) bc[r] = vr[D - r] - vr[D - r + 1];
+    {
+        const float last = 0.0f - vr[0];  // slot D+1 holds -v_{rank=0}
+        bc[0] = vr[D] + (1.0f + last);
+    }
+    // vertices of the enclosing simplex (:266-276):
+    // key_i(r) = rem0_i + canonical[r][rank_i], canonical[r][k] = r (k <= D-r), r-D-1 (else)
+    bool ovf = false;
+#pragma unroll
+    for (int r = 0; r <= D; r++) {
+        Key<D> k;
+#pragma unroll
+        for (int w = 0; w < Key<D>::W; w++) k.w[w] = 0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const int ri = (int)rk[i];
+            const float c = (float)(ri <= D - r ? r : r - D1);
+            const float kv = rem0[i] + c;
+            const int ki = (int)kv;
+            ovf = ovf || ki > 32767 || ki < -32768;
+            key_set<D>(k, i, ki);
+        }
+        key_set<D>(k, D, inst);
+        keys[(size_t)q * D1 + r] = k;
+        if (real) {
+            bary[(size_t)p * D1 + r] = bc[r];
+            rank[(size_t)p * D1 + r] = (int)rk[r];
+        }
+    }
+    if (ovf) overflow[threadIdx.x & 63] = 1;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_hash_insert(const Key<D>* __restrict__ keys, int E,
+                                                          int* table, uint32_t mask,
+                                                          int* __restrict__ slot) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const Key<D> k = keys[e];
+    uint32_t h = key_hash<D>(k) & mask;
+    for (;;) {
+        int cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == kEmpty) {
+            const int old = atomicCAS(&table[h], kEmpty, e);
+            if (old == kEmpty) {
+                slot[e] = (int)h;
+                return;
+            }
+            cur = old;
+        }
+        if (key_eq<D>(keys[cur], k)) {
+            atomicMin(&table[h], e);
+            slot[e] = (int)h;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+template <int D>
+__device__ inline int hash_find(const Key<D>* __restrict__ keys, const int* __restrict__ table,
+                                uint32_t mask, const Key<D>& k) {
+    uint32_t h = key_hash<D>(k) & mask;
+    for (;;) {
+        const int cur = table[h];
+        if (cur == kEmpty) return -1;
+        if (key_eq<D>(keys[cur], k)) return cur;
+        h = (h + 1) & mask;
+    }
+}
+
+__global__ void k_mark_first(const int* __restrict__ table, const int* __restrict__ slot, int E,
+                             int* __restrict__ flag) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    flag[e] = table[slot[e]] == e ? 1 : 0;
+}
+
+__global__ void k_assign(const int* __restrict__ table, const int* __restrict__ slot,
+                         const int* __restrict__ vid, int E, int d1, const int* pix_base,
+                         const int* pad_base, int n_inst, int* __restrict__ vtx,
+                         int* __restrict__ rep) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int first = table[slot[e]];
+    const int v = vid[first];
+    if (first == e) rep[v] = e;
+    const int q = e / d1, r = e - q * d1;
+    const int inst = n_inst > 1 ? find_instance(pad_base, n_inst, q) : 0;
+    const int local = q - pad_base[inst];
+    if (local < pix_base[inst + 1] - pix_base[inst]) vtx[(size_t)(pix_base[inst] + local) * d1 + r] = v;
+}
+
+// Blur neighbours (permutohedral.cpp:296-318): along axis j, n1 = key - 1 with
+// n1[j] = key[j] + d and n2 = key + 1 with n2[j] = key[j] - d; axis d moves every
+// stored coordinate.  Also exports the vertex keys.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_neighbors(const Key<D>* __restrict__ keys,
+                                                        const int* __restrict__ table, uint32_t mask,
+                                                        const int* __restrict__ vid,
+                                                        const int* __restrict__ rep, int V,
+                                                        int* __restrict__ nbr,
+                                                        int16_t* __restrict__ keys_out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)V * (D + 1)) return;
+    const int j = (int)(t / V), v = (int)(t - (long long)j * V);
+    const Key<D> k = keys[rep[v]];
+    Key<D> n1 = k, n2 = k;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const int c = key_get<D>(k, i);
+        key_set<D>(n1, i, i == j ? c + D : c - 1);
+        key_set<D>(n2, i, i == j ? c - D : c + 1);
+        if (j == 0) keys_out[(size_t)v * D + i] = (int16_t)c;
+    }
+    const int f1 = hash_find<D>(keys, table, mask, n1);
+    const int f2 = hash_find<D>(keys, table, mask, n2);
+    nbr[2 * (size_t)t] = f1 >= 0 ? vid[f1] : -1;
+    nbr[2 * (size_t)t + 1] = f2 >= 0 ? vid[f2] : -1;
+}
+
+__global__ void k_fill_int(int* p, int v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_iota(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_lower_bound(const int* __restrict__ sorted, int n, int V, int* __restrict__ ptr) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sorted[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[v] = lo;
+}
+
+__global__ void k_gather_int(const int* __restrict__ src, const int* __restrict__ idx, int n,
+                             int* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+uint32_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    if (p > (1ull << 31)) throw Error{DCRF_ERR_VALUE, "lattice hash table would exceed 2^31 slots"};
+    return (uint32_t)p;
+}
+
+template <int D>
+void build_lattice_d(const Geometry& g, const GeometryDev& gd, const float* feat, TermLattice& lat,
+                     float hash_load, hipStream_t s) {
+    constexpr int D1 = D + 1;
+    const int npad = g.npad_total();
+    const int N = g.N_total();
+    const long long E_ll = (long long)npad * D1;
+    if (E_ll > 0x7ffffff0ll) throw Error{DCRF_ERR_VALUE, "too many lattice entries for one term"};
+    const int E = (int)E_ll;
+    const int Er = N * D1;
+    lat.d = D;
+
+    // scale factors exactly as permutohedral.cpp:176-179 (double, stored as float)
+    ScaleArg sc{};
+    const float inv_std_dev = (float)(std::sqrt(2.0 / 3.0) * D1);
+    for (int i = 0; i < D; i++) sc.s[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * inv_std_dev);
+    sc.inv_d1 = 1.0f / (float)D1;
+
+    DevBuf<Key<D>> keys;
+    keys.reserve(E);
+    lat.vtx.reserve((size_t)Er);
+    lat.bary.reserve((size_t)Er);
+    lat.rank.reserve((size_t)Er);
+    DevBuf<int> ovf;
+    ovf.reserve(64);
+    k_fill_int<<<1, 64, 0, s>>>(ovf.get(), 0, 64);
+    DCRF_HIP_CHECK(hipGetLastError());
+    k_embed<D><<<blocks_for(npad), kThreads, 0, s>>>(feat, npad, gd.pix_base.get(), gd.pad_base.get(),
+                                                     g.n_inst, sc, keys.get(), lat.bary.get(),
+                                                     lat.rank.get(), ovf.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+
+    if (!(hash_load > 0.0f && hash_load <= 0.5f)) hash_load = 0.5f;
+    const uint32_t cap = next_pow2((uint64_t)std::ceil((double)E / hash_load));
+    DevBuf<int> table, slot, flag, vid;
+    table.reserve(cap);
+    slot.reserve(E);
+    flag.reserve(E);
+    vid.reserve(E);
+    k_fill_int<<<blocks_for(cap), kThreads, 0, s>>>(table.get(), kEmpty, cap);
+    DCRF_HIP_CHECK(hipGetLastError());
+    k_hash_insert<D><<<blocks_for(E), kThreads, 0, s>>>(keys.get(), E, table.get(), cap - 1, slot.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    k_mark_first<<<blocks_for(E), kThreads, 0, s>>>(table.get(), slot.get(), E, flag.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    DevBuf<char> scan_tmp;  // lives until the synchronisation below
+    {
+        size_t tmp_bytes = 0;
+        DCRF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, flag.get(), vid.get(), 0, (size_t)E,
+                                               rocprim::plus<int>(), s));
+        scan_tmp.reserve(tmp_bytes);
+        DCRF_HIP_CHECK(rocprim::exclusive_scan(scan_tmp.get(), tmp_bytes, flag.get(), vid.get(), 0, (size_t)E,
+                                               rocprim::plus<int>(), s));
+    }
+    int last[2];
+    DCRF_HIP_CHECK(hipMemcpyAsync(&last[0], vid.get() + (E - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipMemcpyAsync(&last[1], flag.get() + (E - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    int ovf_h[64];
+    DCRF_HIP_CHECK(hipMemcpyAsync(ovf_h, ovf.get(), sizeof(ovf_h), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    const int V = last[0] + last[1];
+    lat.V = V;
+    lat.overflow = 0;
+    for (int i = 0; i < 64; i++) lat.overflow |= ovf_h[i];
+
+    DevBuf<int> rep;
+    rep.reserve(V);
+    k_assign<<<blocks_for(E), kThreads, 0, s>>>(table.get(), slot.get(), vid.get(), E, D1, gd.pix_base.get(),
+                                                gd.pad_base.get(), g.n_inst, lat.vtx.get(), rep.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    lat.nbr.reserve((size_t)2 * D1 * V);
+    lat.keys.reserve((size_t)V * D);
+    k_neighbors<D><<<blocks_for((size_t)V * D1), kThreads, 0, s>>>(keys.get(), table.get(), cap - 1, vid.get(),
+                                                                   rep.get(), V, lat.nbr.get(), lat.keys.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+
+    // first vertex of every instance: the first entry of an instance is always new
+    {
+        std::vector<int> idx(g.n_inst);
+        for (int i = 0; i < g.n_inst; i++) idx[i] = g.pad_base[i] * D1;
+        DevBuf<int> didx, dout;
+        didx.reserve(g.n_inst);
+        dout.reserve(g.n_inst);
+        DCRF_HIP_CHECK(hipMemcpyAsync(didx.get(), idx.data(), sizeof(int) * g.n_inst, hipMemcpyHostToDevice, s));
+        k_gather_int<<<blocks_for(g.n_inst), kThreads, 0, s>>>(vid.get(), didx.get(), g.n_inst, dout.get());
+        DCRF_HIP_CHECK(hipGetLastError());
+        lat.inst_vbase.assign(g.n_inst + 1, V);
+        DCRF_HIP_CHECK(hipMemcpyAsync(lat.inst_vbase.data(), dout.get(), sizeof(int) * g.n_inst,
+                                      hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // CSR by vertex: stable sort of real entries by vertex id
+    lat.csr_e.reserve((size_t)Er);
+    lat.csr_ptr.reserve((size_t)V + 1);
+    {
+        DevBuf<int> iota, sorted_v;
+        iota.reserve(Er);
+        sorted_v.reserve(Er);
+        k_iota<<<blocks_for(Er), kThreads, 0, s>>>(iota.get(), Er);
+        DCRF_HIP_CHECK(hipGetLastError());
+        int bits = 1;
+        while ((1ll << bits) <= (long long)V) bits++;
+        size_t tmp_bytes = 0;
+        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, lat.vtx.get(), sorted_v.get(), iota.get(),
+                                                 lat.csr_e.get(), (size_t)Er, 0, bits, s));
+        DevBuf<char> tmp;
+        tmp.reserve(tmp_bytes);
+        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, lat.vtx.get(), sorted_v.get(), iota.get(),
+                                                 lat.csr_e.get(), (size_t)Er, 0, bits, s));
+        k_lower_bound<<<blocks_for((size_t)V + 1), kThreads, 0, s>>>(sorted_v.get(), Er, V, lat.csr_ptr.get());
+        DCRF_HIP_CHECK(hipGetLastError());
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace
+
+void build_lattice(const Geometry& g, const GeometryDev& gd, const float* feat, int d,
+                   TermLattice& lat, float hash_load, hipStream_t s) {
+    switch (d) {
+#define DCRF_CASE(D) \
+    case D: build_lattice_d<D>(g, gd, feat, lat, hash_load, s); break;
+        DCRF_CASE(1) DCRF_CASE(2) DCRF_CASE(3) DCRF_CASE(4) DCRF_CASE(5) DCRF_CASE(6) DCRF_CASE(7)
+        DCRF_CASE(8) DCRF_CASE(9) DCRF_CASE(10) DCRF_CASE(11) DCRF_CASE(12) DCRF_CASE(13)
+        DCRF_CASE(14) DCRF_CASE(15)
+#undef DCRF_CASE
+        default:
+            throw Error{DCRF_ERR_VALUE, "feature dimension must be between 1 and " +
+                                            std::to_string(DCRF_MAX_DIM)};
+    }
+}
+
+void build_chunks(TermLattice& lat, int chunk, hipStream_t s) {
+    const int V = lat.V;
+    std::vector<int> ptr(V + 1);
+    DCRF_HIP_CHECK(hipMemcpyAsync(ptr.data(), lat.csr_ptr.get(), sizeof(int) * (V + 1), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<Chunk> ch;
+    std::vector<Combine> cb;
+    ch.reserve(V + V / 4);
+    int prow = 0;
+    for (int v = 0; v < V; v++) {
+        const int b = ptr[v], e = ptr[v + 1];
+        const int deg = e - b;
+        const int n = deg <= chunk ? 1 : (deg + chunk - 1) / chunk;
+        if (n == 1) {
+            ch.push_back(Chunk{v, b, e, v});
+        } else {
+            for (int c = 0; c < n; c++)
+                ch.push_back(Chunk{v, b + c * chunk, std::min(e, b + (c + 1) * chunk), -(prow + c) - 1});
+            cb.push_back(Combine{v, prow, n, 0});
+            prow += n;
+        }
+    }
+    lat.n_chunks = (int)ch.size();
+    lat.n_combines = (int)cb.size();
+    lat.n_partials = prow;
+    lat.chunk_len = chunk;
+    lat.chunks.reserve(ch.size());
+    DCRF_HIP_CHECK(hipMemcpyAsync(lat.chunks.get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, s));
+    lat.combines.reserve(cb.size());
+    if (!cb.empty())
+        DCRF_HIP_CHECK(hipMemcpyAsync(lat.combines.get(), cb.data(), sizeof(Combine) * cb.size(),
+                                      hipMemcpyHostToDevice, s));
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void features_from_dxN(const Geometry& g, const GeometryDev& gd, const float* feat_dxN, int d, float* out,
+                       hipStream_t s) {
+    const int npad = g.npad_total();
+    k_feat_dxN<<<blocks_for(npad), kThreads, 0, s>>>(feat_dxN, d, g.N_total(), gd.pix_base.get(),
+                                                     gd.pad_base.get(), g.n_inst, npad, out);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void features_gaussian_2d(const Geometry& g, const GeometryDev& gd, float sx, float sy, float* out,
+                          hipStream_t s) {
+    const int npad = g.npad_total();
+    k_feat_2d<<<blocks_for(npad), kThreads, 0, s>>>(gd.pix_base.get(), gd.pad_base.get(), gd.W.get(), g.n_inst,
+                                                    npad, sx, sy, nullptr, 1.f, 1.f, 1.f, out);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void features_bilateral_2d(const Geometry& g, const GeometryDev& gd, float sx, float sy, float sr, float sg,
+                           float sb, const uint8_t* rgb, float* out, hipStream_t s) {
+    const int npad = g.npad_total();
+    k_feat_2d<<<blocks_for(npad), kThreads, 0, s>>>(gd.pix_base.get(), gd.pad_base.get(), gd.W.get(), g.n_inst,
+                                                    npad, sx, sy, rgb, sr, sg, sb, out);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dcrf

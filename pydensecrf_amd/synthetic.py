@@ -31,7 +31,7 @@ def voronoi_labels(rng, shape, n_sites):
 def _region_labels(rng, n_sites, L, wrong_frac=0.3):
     true = rng.integers(0, L, n_sites)
     shown = true.copy()
-    n_wrong = int(round(wrong_frac * n_sites))
+    n_wrong = int(round(wrong_frac * n_sites)) if L > 1 else 0
     wrong = rng.choice(n_sites, n_wrong, replace=False) if n_wrong else np.array([], dtype=np.int64)
     for r in wrong:
         alt = rng.integers(0, L - 1)

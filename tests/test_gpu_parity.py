@@ -1,0 +1,155 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same inputs.
+
+Bars: the lattice (vertex offsets, ranks, barycentric weights, blur neighbours, vertex
+keys, V) is bit-exact; Q, step temporaries, pairwise messages and KL within the
+tolerances stated per test.  The oracle is bit-exact with the reference's own C++
+(tests/test_oracle_reference.py), so these are parity checks against the reference.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import assert_lattice_equal, build_both, image_case
+
+pytestmark = pytest.mark.gpu
+
+SYM, BEFORE, AFTER, NONE = 3, 1, 2, 0
+Q_TOL = 1e-5  # absolute, on Q (probabilities)
+
+
+def _std_terms(img, nt=SYM, cg=3.0, cb=10.0):
+    return [("gauss", 3.0, 3.0, cg, nt), ("bilat", 80.0, 80.0, 13.0, 13.0, 13.0, img, cb, nt)]
+
+
+LATTICE_CASES = [
+    # (W, H, L, description)
+    (10, 10, 2, "reference test size"),
+    (13, 7, 3, "N % 4 == 3: pad lanes"),
+    (31, 17, 5, "N % 4 == 3"),
+    (40, 30, 21, "small"),
+    (320, 213, 21, "C0 size"),
+]
+
+
+@pytest.mark.parametrize("W,H,L,desc", LATTICE_CASES)
+def test_lattice_bitexact_2d(W, H, L, desc):
+    img, U, _ = image_case(W, H, L, seed=W * 31 + H)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    for k in range(2):
+        assert_lattice_equal(g.lattice(k), o.lattice(k), W * H)
+
+
+@pytest.mark.parametrize("d,N,scale", [(1, 37, 2.0), (3, 150, 1.5), (4, 401, 1.0), (6, 255, 0.7), (9, 97, 0.5),
+                                       (15, 64, 0.4)])
+def test_lattice_bitexact_generic(d, N, scale):
+    rng = np.random.default_rng(d * 1000 + N)
+    f = (rng.standard_normal((d, N)) * scale * 4).astype(np.float32)
+    L = 3
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    g, o = build_both(N, 1, L, U, [("feat", f, 1.0, SYM)], generic=True)
+    assert_lattice_equal(g.lattice(0), o.lattice(0), N)
+
+
+def test_norm_matches_oracle():
+    W, H, L = 64, 48, 4
+    img, U, _ = image_case(W, H, L, seed=7)
+    for nt in (SYM, BEFORE, AFTER, NONE):
+        g, o = build_both(W, H, L, U, _std_terms(img, nt))
+        for k in range(2):
+            gn = g.lattice(k)["norm"]
+            on = o.norm(k)
+            np.testing.assert_allclose(gn, on, rtol=2e-6, atol=0, err_msg="ntype %d term %d" % (nt, k))
+
+
+@pytest.mark.parametrize("W,H,L,n", [(10, 10, 2, 10), (13, 7, 3, 5), (40, 30, 21, 5), (64, 48, 4, 8)])
+@pytest.mark.parametrize("nt", [SYM, BEFORE, AFTER, NONE])
+def test_inference_potts(W, H, L, n, nt):
+    img, U, _ = image_case(W, H, L, seed=W + H + nt)
+    g, o = build_both(W, H, L, U, _std_terms(img, nt))
+    Qg = g.inference(n)
+    Qo = o.inference(n)
+    assert np.abs(Qg - Qo).max() <= Q_TOL
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 21])
+@pytest.mark.parametrize("ck", ["diag", "matrix"])
+def test_inference_compat_kinds(L, ck):
+    W, H = 37, 23
+    img, U, _ = image_case(W, H, L, seed=L)
+    rng = np.random.default_rng(L + 17)
+    if ck == "diag":
+        c1, c2 = rng.uniform(0.5, 3, L).astype(np.float32), rng.uniform(1, 6, L).astype(np.float32)
+    else:
+        c1 = (-np.eye(L) * 2 + rng.normal(0, 0.3, (L, L))).astype(np.float32)
+        c2 = (-np.eye(L) * 5 + rng.normal(0, 0.5, (L, L))).astype(np.float32)
+    g, o = build_both(W, H, L, U, [("gauss", 3.0, 3.0, c1, SYM), ("bilat", 20.0, 20.0, 13., 13., 13., img, c2, SYM)])
+    assert np.abs(g.inference(6) - o.inference(6)).max() <= Q_TOL
+
+
+def test_inference_c0_size():
+    from pydensecrf_amd import synthetic as syn
+    img, U, _ = syn.make_image(**syn.IMAGE_320x213)
+    g, o = build_both(320, 213, 21, U, _std_terms(img))
+    Qg, Qo = g.inference(5), o.inference(5)
+    err = np.abs(Qg - Qo).max()
+    print("C0 max |Q_gpu - Q_ref| = %.3g" % err)
+    assert err <= Q_TOL
+    assert (Qg.argmax(0) != Qo.argmax(0)).sum() == 0
+
+
+def test_generic_features_inference():
+    rng = np.random.default_rng(3)
+    Z, Y, X = 6, 9, 11
+    N, L = Z * Y * X, 4
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    inten = rng.uniform(0, 1, (Z, Y, X)).astype(np.float32)
+    fg = (np.stack([zz, yy, xx]).reshape(3, -1) / 3.0).astype(np.float32)
+    fb = np.concatenate([np.stack([zz, yy, xx]).reshape(3, -1) / 4.0, inten.reshape(1, -1) / 0.1]).astype(np.float32)
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    w = np.array([1, 2, 2, 3], np.float32)
+    g, o = build_both(N, 1, L, U, [("feat", fg, w, SYM), ("feat", fb, w, SYM)], generic=True)
+    assert np.abs(g.inference(10) - o.inference(10)).max() <= Q_TOL
+
+
+def test_start_step_kl_apply():
+    W, H, L = 41, 29, 6
+    img, U, _ = image_case(W, H, L, seed=11)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    Qg, Qo = g.start(), o.start()
+    assert np.abs(Qg - Qo).max() <= 1e-6
+    for _ in range(3):
+        Qg, t1g, t2g = g.step(Qg)
+        Qo, t1o, t2o = o.step(Qo)
+        assert np.abs(Qg - Qo).max() <= Q_TOL
+        np.testing.assert_allclose(t1g, t1o, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(t2g, t2o, rtol=1e-5, atol=1e-4)
+        klg, klo = g.kl(Qg), o.kl(Qo)
+        assert abs(klg - klo) <= 1e-5 * max(1.0, abs(klo)), (klg, klo)
+    for k in range(2):
+        for tr in (False, True):
+            np.testing.assert_allclose(g.apply(k, Qo, tr), o.apply(k, Qo, tr), rtol=1e-5, atol=1e-5)
+
+
+def test_no_unary_and_no_terms():
+    W, H, L = 9, 7, 3
+    g, o = build_both(W, H, L, None, [])
+    np.testing.assert_allclose(g.inference(3), o.inference(3), atol=1e-7)
+    assert np.all(g.start() == 0)
+
+
+def test_deterministic():
+    W, H, L = 96, 80, 21
+    img, U, _ = image_case(W, H, L, seed=5)
+    g, _ = build_both(W, H, L, U, _std_terms(img))
+    a = g.inference(5)
+    b = g.inference(5)
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+@pytest.mark.parametrize("chunk", [1, 7, 64, 100000])
+def test_splat_chunk_invariance(chunk):
+    W, H, L = 120, 90, 21
+    img, U, _ = image_case(W, H, L, seed=9)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    g.set_splat_chunk(chunk)
+    assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL

@@ -1,0 +1,178 @@
+"""The oracle (oracle/dcrf_oracle.c) against the reference's own C++ outputs.
+
+Golden fixtures in tests/golden/ were written from the reference build by
+tests/golden/make_golden.py.  The bar is bit-exact equality (lattice arrays, Q, step
+temporaries, KL, LogLikelihood gradients) except where the reference's result depends
+on Eigen's blocked GEMM order (MatrixCompatibility), which the oracle restates as a
+plain sequential sum: there the bar is 2e-6 relative.  When oracle/_ref is built (it is
+in the build container), the live reference is compared on further random cases.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.int32)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_lattice_fixtures_bitexact():
+    z = _load("lattice_small.npz")
+    for i in range(int(z["ncases"])):
+        f = z["c%d_feat" % i]
+        lat = po.OracleLattice(f).arrays
+        assert lat.V == int(z["c%d_V" % i])
+        assert np.array_equal(lat.offset, z["c%d_offset" % i])
+        assert np.array_equal(lat.rank, z["c%d_rank" % i])
+        assert np.array_equal(bits(lat.bary), bits(z["c%d_bary" % i]))
+        assert np.array_equal(lat.nbr, z["c%d_nbr" % i])
+
+
+def _crf_case(z, i):
+    W, H, L, ck, nt, n = [int(v) for v in z["c%d_meta" % i]]
+    p = "c%d_" % i
+    c1, c2 = z[p + "c1"], z[p + "c2"]
+    if ck == 0:
+        c1, c2 = float(c1[0]), float(c2[0])
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(z[p + "U"])
+    o.add_gaussian(3.0, 2.0, c1, ntype=nt)
+    o.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], c2, ntype=nt)
+    return o, (W, H, L, ck, nt, n), p
+
+
+@pytest.mark.parametrize("i", range(8))
+def test_crf_fixtures(i):
+    z = _load("crf_small.npz")
+    o, (W, H, L, ck, nt, n), p = _crf_case(z, i)
+    Q = o.inference(n)
+    Qs = o.start()
+    Q1, t1, t2 = o.step(Qs)
+    kl = o.kl(Q)
+    r, ug, cg = o.gradient(n, z[p + "gt"])
+    if ck != 2:
+        assert np.array_equal(bits(Q), bits(z[p + "Q"]))
+        assert np.array_equal(bits(Q1), bits(z[p + "Q1"]))
+        assert np.array_equal(bits(t1), bits(z[p + "tmp1"]))
+        assert np.array_equal(bits(t2), bits(z[p + "tmp2"]))
+        assert kl == float(z[p + "kl"])
+        assert r == float(z[p + "obj"])
+        assert np.array_equal(bits(ug), bits(z[p + "unary_grad"]))
+        assert np.array_equal(bits(cg), bits(z[p + "compat_grad"]))
+    else:
+        assert np.abs(Q - z[p + "Q"]).max() <= 2e-6
+        assert np.abs(Q1 - z[p + "Q1"]).max() <= 2e-6
+        np.testing.assert_allclose(t1, z[p + "tmp1"], rtol=2e-6, atol=2e-6)
+        np.testing.assert_allclose(t2, z[p + "tmp2"], rtol=2e-6, atol=2e-6)
+        assert abs(kl - float(z[p + "kl"])) <= 2e-6 * max(1.0, abs(kl))
+        assert abs(r - float(z[p + "obj"])) <= 2e-6 * max(1.0, abs(r))
+        scale = np.abs(z[p + "unary_grad"]).max()
+        assert np.abs(ug - z[p + "unary_grad"]).max() <= 2e-5 * scale
+        cs = np.abs(z[p + "compat_grad"]).max()
+        assert np.abs(cg - z[p + "compat_grad"]).max() <= 2e-5 * cs
+
+
+def test_reference_suite_fixture_10x10():
+    """pydensecrf/tests/test_dcrf.py:29-83 on the oracle: the 2-D and the generic paths,
+    bit-exact against the reference, and their MAPs agree (test_call_dcrf_eq_dcrf2d)."""
+    from pydensecrf_amd import utils
+    z = _load("simple_10x10.npz")
+    d2 = po.OracleCRF(W=10, H=10, L=2)
+    d2.set_unary(z["U"])
+    d2.add_bilateral(2.0, 2.0, 2.0, 2.0, 2.0, z["img"], 3.0)
+    Q2 = d2.inference(10)
+    d = po.OracleCRF(N=100, L=2)
+    d.set_unary(z["U"])
+    feats = utils.create_pairwise_bilateral(sdims=(2, 2), schan=2, img=z["img"], chdim=2)
+    assert np.array_equal(bits(feats), bits(z["feats"]))
+    d.add_pairwise(feats, 3.0)
+    Qg = d.inference(10)
+    assert np.array_equal(bits(Q2), bits(z["Q2d"]))
+    assert np.array_equal(bits(Qg), bits(z["Qgeneric"]))
+    assert np.array_equal(Q2.argmax(0), Qg.argmax(0))
+
+
+def _image_check(fname, cfg):
+    from pydensecrf_amd import synthetic as syn
+    z = _load(fname)
+    img, U, _ = syn.make_image(**cfg)
+    assert sha(U) == str(z["unary_sha"]) and sha(img) == str(z["image_sha"]), "synthetic generator drifted"
+    W, H, L, n = cfg["W"], cfg["H"], cfg["L"], cfg["iters"]
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(U)
+    o.add_gaussian(3.0, 3.0, 3.0)
+    o.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+    lat = o.lattice(0)
+    assert lat.V == int(z["V_gaussian"])
+    assert sha(lat.offset[:W * H]) == str(z["offset_gaussian_sha"])
+    Q = o.inference(n)
+    assert np.array_equal(bits(Q[:, z["idx"]]), bits(z["Q_sample"]))
+    assert sha(Q.argmax(0).astype(np.int16)) == str(z["map_sha"])
+
+
+def test_image_320x213_fixture():
+    from pydensecrf_amd import synthetic as syn
+    _image_check("image_320x213.npz", syn.IMAGE_320x213)
+
+
+def test_image_1024x768_fixture():
+    from pydensecrf_amd import synthetic as syn
+    _image_check("image_1024x768.npz", syn.IMAGE_1024x768)
+
+
+# ---------------------------------------------------------------- live reference
+ref = pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref (reference build) not present")
+
+
+@ref
+@pytest.mark.parametrize("d,N", [(2, 203), (3, 117), (5, 333), (7, 61), (10, 45), (15, 30)])
+def test_lattice_live_reference(d, N):
+    rng = np.random.default_rng(d * 7 + N)
+    f = (rng.standard_normal((d, N)) * 4).astype(np.float32)
+    a, b = po.OracleLattice(f).arrays, po.RefLattice(f).arrays
+    assert a.V == b.V
+    assert np.array_equal(a.offset, b.offset) and np.array_equal(a.rank, b.rank)
+    assert np.array_equal(bits(a.bary), bits(b.bary)) and np.array_equal(a.nbr, b.nbr)
+
+
+@ref
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 8, 21, 33])
+def test_compute_live_reference(L):
+    rng = np.random.default_rng(L)
+    f = (rng.standard_normal((4, 257)) * 3).astype(np.float32)
+    v = rng.standard_normal((L, 257)).astype(np.float32)
+    a, b = po.OracleLattice(f), po.RefLattice(f)
+    for rev in (False, True):
+        assert np.array_equal(bits(a.compute(v, rev)), bits(b.compute(v, rev)))
+
+
+@ref
+def test_generic_model_live_reference():
+    rng = np.random.default_rng(5)
+    N, L = 500, 4
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    f1 = (rng.standard_normal((3, N)) * 2).astype(np.float32)
+    f2 = (rng.standard_normal((4, N)) * 2).astype(np.float32)
+    w = np.array([1, 2, 2, 3], np.float32)
+    out = []
+    for cls in (po.OracleCRF, po.RefCRF):
+        c = cls(N=N, L=L)
+        c.set_unary(U)
+        c.add_pairwise(f1, w)
+        c.add_pairwise(f2, w, ntype=po.NORMALIZE_AFTER)
+        out.append(c.inference(7))
+    assert np.array_equal(bits(out[0]), bits(out[1]))
