@@ -8,13 +8,20 @@ DIAG_KERNEL, NORMALIZE_SYMMETRIC, on a seeded synthetic Voronoi image and unary
 image with the unary and the lattices resident in HBM; Q is written to a device
 buffer.  value = Mpixel*iterations/s over the timed steps.
 
-Also reported on the same line: the blur-pass roofline (algorithmic bytes of the
-timed blur launches / their HIP-event time, against 8 TB/s), lattice-build time, and
-the reference CPU path (oracle/_ref, the reference's own C++ built from its sources;
-the oracle restatement if that build is absent) timed on the host on the same
-workload.
+Also reported on the same line:
+  roofline   the kernel class with the largest share of the timed region: its
+             algorithmic bytes per launch over its HIP-event launch time, against 8 TB/s
+  kernels    the same figures for every kernel class (splat, combine, blur, meanfield)
+  hipgraph   the same steps replayed as one captured hipGraph
+  batched    64 images of uneven size in one batched model vs 64 plain per-image calls
+  fwd_bwd    512x512x21, 5 iterations, learnable Matrix compatibility: forward and
+             forward+backward through the torch.autograd layer
+  cpu_baseline  the reference CPU path (oracle/_ref, the reference's own C++ built from
+             its sources; the oracle restatement if that build is absent) on the host,
+             on the same 1024x768x21 workload
 
     python bench.py [--steps K] [--warmup W] [--iters I] [--no-cpu-baseline]
+                    [--no-batched] [--no-fwd-bwd]
 """
 import argparse
 import json
@@ -60,6 +67,130 @@ def cpu_baseline(img, U, W, H, L, iters):
     }
 
 
+def bench_batched(torch, capi, syn, n_images, iters, reps, log):
+    """64 images of uneven size: one batched model (one launch per kernel for the whole
+    batch, replayed as a hipGraph) against one plain model per image called in turn.
+    Lattices are built beforehand in both cases; images/s over `reps` passes."""
+    import numpy as np  # noqa: F401
+    from pydensecrf_amd.batched import BatchedDenseCRF2D
+    sizes = syn.batch_image_sizes(n_images)
+    L = 21
+    imgs, Us = [], []
+    for i, (w, h) in enumerate(sizes):
+        img, U, _ = syn.make_image(w, h, L=L, sites=12, seed=3000 + i)
+        imgs.append(img)
+        Us.append(U)
+    total_px = sum(w * h for w, h in sizes)
+    stream = torch.cuda.current_stream()
+    # plain per-image path
+    plain, outs = [], []
+    for (w, h), img, U in zip(sizes, imgs, Us):
+        mdl = capi.Model.create_2d(w, h, L)
+        mdl.set_unary(U)
+        mdl.add_gaussian(3.0, 3.0, 3.0)
+        mdl.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+        plain.append(mdl)
+        outs.append(torch.empty(w * h * L, device="cuda"))
+
+    def run_plain():
+        for mdl, q in zip(plain, outs):
+            mdl.inference_device(iters, q.data_ptr(), None, stream.cuda_stream)
+
+    # batched
+    t0 = time.perf_counter()
+    b = BatchedDenseCRF2D(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=80, srgb=13, rgbims=imgs, compat=10)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    Qb = torch.empty(total_px * L, device="cuda")
+
+    def run_batched_stream():
+        b.inference_device(iters, Qb.data_ptr(), stream.cuda_stream)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_plain = timed(run_plain)
+    ms_stream = timed(run_batched_stream)
+    b.capture(iters, Qb.data_ptr())
+    ms_graph = timed(lambda: b.replay(stream.cuda_stream))
+    # the batch reproduces the per-image results bit for bit
+    same = all(torch.equal(Qb[b.offsets[i] * L:b.offsets[i + 1] * L], outs[i]) for i in range(len(sizes)))
+    log("batched: %d images, %.1f Mpixel, plain %.2f ms, batched %.2f ms (stream) %.2f ms (graph)"
+        % (len(sizes), total_px / 1e6, ms_plain, ms_stream, ms_graph))
+    return {
+        "workload": "%d images, sides U[256,1024] (seeded), 21 labels, %d iterations, Gaussian+Bilateral Potts"
+                    % (len(sizes), iters),
+        "megapixels": round(total_px / 1e6, 3),
+        "plain_images_per_s": round(len(sizes) / (ms_plain / 1e3), 2),
+        "batched_images_per_s": round(len(sizes) / (ms_graph / 1e3), 2),
+        "batched_stream_images_per_s": round(len(sizes) / (ms_stream / 1e3), 2),
+        "speedup": round(ms_plain / ms_graph, 3),
+        "bitwise_equal_to_plain": bool(same),
+        "batched_lattice_build_s": round(build_s, 3),
+    }
+
+
+def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
+    """512x512x21, 5 iterations, MatrixCompatibility (-I + N(0, 0.1), learnable) and
+    learnable term weights, cross-entropy loss: forward vs forward+backward through
+    the torch.autograd layer (pydensecrf_amd.autograd.CRFLayer)."""
+    import numpy as np
+    from pydensecrf_amd.autograd import CRFLayer
+    cfg = syn.IMAGE_512x512
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, gt = syn.make_image(**cfg)
+    rng = np.random.default_rng(1003)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, c1)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    layer = CRFLayer(m, iters, [c1, c2], kweights=[3.0, 10.0])
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        with torch.no_grad():
+            layer(Ut)
+
+    def fwd_bwd():
+        Q = layer(Ut)
+        loss = torch.nn.functional.nll_loss(torch.log(Q), gtt)
+        loss.backward()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f = timed(fwd)
+    ms_fb = timed(fwd_bwd)
+    log("fwd+bwd 512x512x21: fwd %.3f ms, fwd+bwd %.3f ms" % (ms_f, ms_fb))
+    return {"workload": "DenseCRF2D 512x512, 21 labels, %d iterations, MatrixCompatibility + term weights, "
+                        "cross-entropy, torch.autograd" % iters,
+            "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
+            "fwd_Mpixel_iters_per_s": round(W * H * iters / (ms_f / 1e3) / 1e6, 2),
+            "fwd_bwd_Mpixel_iters_per_s": round(W * H * iters / (ms_fb / 1e3) / 1e6, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +198,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-image batched comparison")
+    ap.add_argument("--no-fwd-bwd", action="store_true", help="skip the 512x512 forward+backward timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy three times (a known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the mean-field path runs on a single MI355X here "
@@ -78,6 +214,13 @@ def main():
     from pydensecrf_amd import capi
     from pydensecrf_amd import synthetic as syn
 
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg = syn.IMAGE_1024x768
     W, H, L = cfg["W"], cfg["H"], cfg["L"]
     iters = args.iters
@@ -102,23 +245,33 @@ def main():
     def step():
         m.inference_device(iters, Q.data_ptr(), None, stream.cuda_stream)
 
+    def timed(fn, steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    # the timed region: K steps of stream-ordered launches, HIP events on every kernel
     m.enable_timing(True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_total = e0.elapsed_time(e1)
-    launches, blur_ms, blur_bytes = m.blur_stats()
+    ms_total = timed(step, args.steps)
+    stats = m.kernel_stats()
     m.enable_timing(False)
     ms_per_step = ms_total / args.steps
     value = N * iters * args.steps / (ms_total / 1e3) / 1e6
+    # the same steps without the kernel events, and as one captured hipGraph
+    ms_plain = timed(step, args.steps) / args.steps
+    m.capture(iters, Q.data_ptr())
+    launch = lambda: m.launch_graph(stream.cuda_stream)  # noqa: E731
+    for _ in range(args.warmup):
+        launch()
+    ms_graph = timed(launch, args.steps) / args.steps
 
     # sanity: Q rows are distributions
     q = Q.view(N, L)
@@ -126,7 +279,15 @@ def main():
     if not row_err < 1e-4:
         raise RuntimeError("Q rows do not sum to one (max error %g)" % row_err)
 
-    blur_gbs = blur_bytes / (blur_ms / 1e3) / 1e9 if blur_ms > 0 else 0.0
+    kernels = {}
+    for name, (n, ms, b) in stats.items():
+        if n:
+            kernels[name] = {"launches": n, "avg_us": round(ms * 1e3 / n, 3), "total_ms": round(ms, 4),
+                             "bytes_per_launch": round(b / n),
+                             "GBs": round(b / (ms / 1e3) / 1e9, 1) if b else None}
+    dominant = max((k for k in kernels if stats[k][2] > 0), key=lambda k: stats[k][1])
+    n_d, ms_d, b_d = stats[dominant]
+    dom_gbs = b_d / (ms_d / 1e3) / 1e9
     iter_bytes = m.iteration_bytes()
     iter_gbs = iter_bytes * iters / (ms_per_step / 1e3) / 1e9
     out = {
@@ -143,23 +304,31 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Voronoi image and unary, pydensecrf_amd.synthetic)",
         "config": {"workload": WORKLOAD, "width": W, "height": H, "labels": L, "iterations": iters,
+                   "launch": "stream-ordered kernels on the torch stream, HIP events around each kernel",
                    "vertices": {"gaussian": Vg, "bilateral": Vb}},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_blur (blur pass)",
-            "achieved": round(blur_gbs, 2),
+            "kernel": dominant,
+            "achieved": round(dom_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(blur_gbs / HBM_PEAK_GBS, 4),
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "launches": launches,
-            "avg_launch_us": round(blur_ms * 1e3 / max(launches, 1), 3),
-            "bytes_per_launch": round(blur_bytes / max(launches, 1)),
+            "avg_launch_us": round(ms_d * 1e3 / n_d, 3),
+            "bytes_per_launch": round(b_d / n_d),
         },
+        "kernels": kernels,
         "iteration": {"ms": round(ms_per_step / iters, 4), "algorithmic_bytes": round(iter_bytes),
                       "achieved_GBs": round(iter_gbs, 2), "frac": round(iter_gbs / HBM_PEAK_GBS, 4)},
+        "without_events": {"ms_per_step": round(ms_plain, 4),
+                           "value": round(N * iters / (ms_plain / 1e3) / 1e6, 3)},
+        "hipgraph": {"ms_per_step": round(ms_graph, 4), "value": round(N * iters / (ms_graph / 1e3) / 1e6, 3)},
         "lattice_build_ms": {"gaussian": round((t1 - t0) * 1e3, 2), "bilateral": round((t2 - t1) * 1e3, 2)},
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(torch, capi, syn, 64, iters, 3, log)
+    if not args.no_fwd_bwd:
+        out["fwd_bwd"] = bench_fwd_bwd(torch, capi, syn, 5, 5, log)
     if not args.no_cpu_baseline:
         log("timing the reference CPU path on the same workload ...")
         cb = cpu_baseline(img, U, W, H, L, iters)

@@ -121,6 +121,47 @@ int dcrf_apply_pairwise(dcrf_model *m, int term, const float *Q, float *out, int
 int dcrf_inference_device(dcrf_model *m, int n_iterations, const float *U_dev, float *Q_dev,
                           void *stream);
 
+/* Record inference(n) on device buffers as a hipGraph (kernels only: no allocation,
+ * copy or memset inside) for repeated launches.  U_dev NULL = the model's unary.  The
+ * graph reads and writes the given pointers and the model's buffers; adding a term or
+ * changing the splat chunk invalidates it.  No reference counterpart (the reference
+ * re-runs DenseCRF::inference, densecrf.cpp:115-131). */
+int dcrf_capture_inference(dcrf_model *m, int n_iterations, const float *U_dev, float *Q_dev);
+/* Launch the captured graph on `stream` (NULL is the null stream). */
+int dcrf_launch_graph(dcrf_model *m, void *stream);
+
+/* ---- parameters and gradients ------------------------------------------------- */
+/* Replace the compatibility of one term; params in the raw layout of dcrf_add_pairwise
+ * (Potts [1], Diagonal [L], Matrix [L*L] row-major, symmetrised as (m+m^T)/2).  The
+ * reference's setLabelCompatibilityParameters (densecrf.cpp:329-341) for one term. */
+int dcrf_set_compat(dcrf_model *m, int term, const float *params);
+/* Per-term weights w_k multiplying each pairwise message, E = U + sum_k C_k(w_k K~_k Q)
+ * ([num_terms] host floats; NULL restores 1).  No reference counterpart: the
+ * reference's terms have unit weight; w_k = 1 reproduces it exactly. */
+int dcrf_set_kernel_weights(dcrf_model *m, const float *w);
+/* labelCompatibilityParameters (densecrf.cpp:314-328): count and packed values (Potts 1,
+ * Diagonal L, Matrix upper triangle W(i, j>=i) row by row). */
+int dcrf_num_compat_params(const dcrf_model *m);
+int dcrf_get_compat_params(const dcrf_model *m, float *params);
+/* DenseCRF::gradient with the LogLikelihood objective (densecrf.cpp:238-297,
+ * objective.cpp:37-50).  gt [N] int16 (labels outside [0, L) ignored).  Outputs (any
+ * may be NULL): objective = mean log-likelihood; unary_grad [N][L] = d objective / dU
+ * (the unary gradient an identity-parameterised unary would report); compat_grad
+ * packed like dcrf_get_compat_params; kweight_grad [num_terms] = d objective / d w_k.
+ * Q_t and the messages K~_k Q_t are kept from the forward pass (no re-filtering). */
+int dcrf_gradient(dcrf_model *m, int n_iterations, const int16_t *gt, float robust, double *objective,
+                  float *unary_grad, float *compat_grad, float *kweight_grad);
+/* Training forward on device buffers: like dcrf_inference_device, but keeps Q_0..Q_n
+ * and the messages for dcrf_backward_device.  compat_dev: raw compatibility parameters
+ * of all terms concatenated (NULL = the model's); kw_dev: [num_terms] term weights (NULL
+ * = the model's); both must stay valid until the backward.  Q_dev may be NULL. */
+int dcrf_forward_device(dcrf_model *m, int n_iterations, const float *U_dev, const float *compat_dev,
+                        const float *kw_dev, float *Q_dev, void *stream);
+/* Adjoint of the last dcrf_forward_device: dQ_dev = dL/dQ_n [N][L].  Writes dU_dev
+ * [N][L], dcompat_dev (raw layout, may be NULL) and dkw_dev ([num_terms], may be NULL). */
+int dcrf_backward_device(dcrf_model *m, const float *dQ_dev, float *dU_dev, float *dcompat_dev, float *dkw_dev,
+                         void *stream);
+
 /* ---- introspection (parity tests, metrics) ------------------------------------- */
 int dcrf_num_variables(const dcrf_model *m);
 int dcrf_num_labels(const dcrf_model *m);
@@ -141,12 +182,17 @@ int dcrf_get_lattice(const dcrf_model *m, int term, int instance, int *offset, i
 int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int *V);
 /* Set the chunk length of the splat's segmented reduction (0 = autotuned default). */
 int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
-/* Kernel timing: when enabled, the library records HIP events around every blur-pass
- * launch on the model's stream.  dcrf_blur_stats returns the number of timed blur
- * launches, their total milliseconds and the algorithmic bytes they moved
- * (sum over launches of V_pass * (2*L*4 + 8)). */
+/* Kernel timing: when enabled, the library records HIP events around every launch of
+ * the inference kernels on the stream they run on (enabling resets the counters).
+ * dcrf_kernel_stats returns, for one kernel class, the number of timed launches, their
+ * total milliseconds and the algorithmic bytes they moved.  Classes:
+ *   0 splat     N*L*4 + sum_k [N*(d_k+1)*8 + N*4 + V_k*L*4]
+ *   1 combine   0 (recombination of split splat segments)
+ *   2 blur      per step s: sum over terms with d_k >= s of V_k*(2*L*4 + 8)
+ *   3 meanfield 2*N*L*4 + sum_k [N*(d_k+1)*8 + N*4 + V_k*L*4]
+ * One iteration's classes add up to dcrf_iteration_bytes. */
 int dcrf_enable_timing(dcrf_model *m, int on);
-int dcrf_blur_stats(dcrf_model *m, long long *launches, double *ms, double *bytes);
+int dcrf_kernel_stats(dcrf_model *m, int kernel, long long *launches, double *ms, double *bytes);
 /* Algorithmic bytes of one mean-field iteration (Q/U streams, lattice entry arrays,
  * normalisers, vertex rows, neighbour tables). */
 int dcrf_iteration_bytes(const dcrf_model *m, double *bytes);

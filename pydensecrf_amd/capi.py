@@ -49,6 +49,15 @@ SIGNATURES = {
     "dcrf_kl_divergence": (_c, [_P, _F, _D]),
     "dcrf_apply_pairwise": (_c, [_P, _c, _F, _F, _c]),
     "dcrf_inference_device": (_c, [_P, _c, _P, _P, _P]),
+    "dcrf_capture_inference": (_c, [_P, _c, _P, _P]),
+    "dcrf_launch_graph": (_c, [_P, _P]),
+    "dcrf_set_compat": (_c, [_P, _c, _F]),
+    "dcrf_set_kernel_weights": (_c, [_P, _F]),
+    "dcrf_num_compat_params": (_c, [_P]),
+    "dcrf_get_compat_params": (_c, [_P, _F]),
+    "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F]),
+    "dcrf_forward_device": (_c, [_P, _c, _P, _P, _P, _P, _P]),
+    "dcrf_backward_device": (_c, [_P, _P, _P, _P, _P, _P]),
     "dcrf_num_variables": (_c, [_P]),
     "dcrf_num_labels": (_c, [_P]),
     "dcrf_num_terms": (_c, [_P]),
@@ -58,7 +67,7 @@ SIGNATURES = {
     "dcrf_term_instance_vertices": (_c, [_P, _c, _c, _I]),
     "dcrf_set_splat_chunk": (_c, [_P, _c]),
     "dcrf_enable_timing": (_c, [_P, _c]),
-    "dcrf_blur_stats": (_c, [_P, _LL, _D, _D]),
+    "dcrf_kernel_stats": (_c, [_P, _c, _LL, _D, _D]),
     "dcrf_iteration_bytes": (_c, [_P, _D]),
 }
 
@@ -246,6 +255,49 @@ class Model:
     def inference_device(self, n, Q_ptr, U_ptr=None, stream=None):
         check(lib().dcrf_inference_device(self.h, n, U_ptr, Q_ptr, stream))
 
+    def capture(self, n, Q_ptr, U_ptr=None):
+        check(lib().dcrf_capture_inference(self.h, n, U_ptr, Q_ptr))
+
+    def launch_graph(self, stream=None):
+        check(lib().dcrf_launch_graph(self.h, stream))
+
+    # -- parameters and gradients
+    def set_compat(self, term, params):
+        p = f32(np.atleast_1d(params))
+        check(lib().dcrf_set_compat(self.h, term, fptr(np.ascontiguousarray(p.ravel()))))
+
+    def set_kernel_weights(self, w):
+        if w is None:
+            check(lib().dcrf_set_kernel_weights(self.h, None))
+            return
+        a = f32(w)
+        if a.size != self.nterms:
+            raise ValueError("need one weight per pairwise term")
+        check(lib().dcrf_set_kernel_weights(self.h, fptr(a)))
+
+    def compat_params(self):
+        p = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
+        check(lib().dcrf_get_compat_params(self.h, fptr(p)))
+        return p[:lib().dcrf_num_compat_params(self.h)]
+
+    def gradient(self, n, gt, robust=0.0):
+        """DenseCRF::gradient with LogLikelihood: (objective, unary_grad (L, N),
+        compat_grad packed, kernel-weight grad [T])."""
+        g = np.ascontiguousarray(gt, dtype=np.int16)
+        obj = ctypes.c_double()
+        ug = self._buf()
+        cg = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
+        kg = np.empty(max(self.nterms, 1), np.float32)
+        check(lib().dcrf_gradient(self.h, n, g.ctypes.data_as(_S), robust, ctypes.byref(obj), fptr(ug), fptr(cg),
+                                  fptr(kg)))
+        return obj.value, ug.T, cg[:lib().dcrf_num_compat_params(self.h)], kg[:self.nterms]
+
+    def forward_device(self, n, U_ptr, Q_ptr, compat_ptr=None, kw_ptr=None, stream=None):
+        check(lib().dcrf_forward_device(self.h, n, U_ptr, compat_ptr, kw_ptr, Q_ptr, stream))
+
+    def backward_device(self, dQ_ptr, dU_ptr, dcompat_ptr=None, dkw_ptr=None, stream=None):
+        check(lib().dcrf_backward_device(self.h, dQ_ptr, dU_ptr, dcompat_ptr, dkw_ptr, stream))
+
     # -- introspection
     def term_info(self, term):
         d, V = ctypes.c_int(), ctypes.c_int()
@@ -280,10 +332,16 @@ class Model:
     def enable_timing(self, on=True):
         check(lib().dcrf_enable_timing(self.h, int(bool(on))))
 
-    def blur_stats(self):
-        n, ms, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
-        check(lib().dcrf_blur_stats(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
-        return n.value, ms.value, b.value
+    KERNEL_CLASSES = ("splat", "combine", "blur", "meanfield")
+
+    def kernel_stats(self):
+        """{class: (launches, total ms, algorithmic bytes)} of the timed launches."""
+        out = {}
+        for i, name in enumerate(self.KERNEL_CLASSES):
+            n, ms, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+            check(lib().dcrf_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+            out[name] = (n.value, ms.value, b.value)
+        return out
 
     def iteration_bytes(self):
         b = ctypes.c_double()

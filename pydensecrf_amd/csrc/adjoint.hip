@@ -1,0 +1,284 @@
+// Adjoint mean-field for gfx950: the backward of E_{t+1} = U + sum_k C_k(w_k K~_k Q_t),
+// Q_t = softmax(-E_t), following DenseCRF::gradient (pydensecrf/densecrf/src/
+// densecrf.cpp:238-297) with the reference's conventions:
+//   b_n   = sumAndNormalize(Q_n .* dL/dQ_n, Q_n)                  (= dL/dE_n, :257-258)
+//   for t = n-1 .. 0:
+//     G_k  += b_{t+1} (K~_k Q_t)^T         compatibility / kernel-weight statistics
+//                                          (labelcompatibility.cpp:57-61, 76-78, 101-108)
+//     g_t   = sum_k w_k K~_k^T C_k^T b_{t+1}   transposed filter (reverse blur order,
+//                                          pairwise.cpp:179-183, permutohedral.cpp:556)
+//     b_t   = sumAndNormalize(Q_t .* g_t, Q_t)                     (:290)
+//   dL/dU = sum_t b_t                                              (:261-262, 293-294)
+// K~_k Q_t comes from the forward pass's message cache, so the backward runs one
+// filter per term per iteration (the reference re-filters, pairwise.cpp:193).
+// Reductions run in a fixed order (per-block float partials, then a double sum over
+// blocks): results are deterministic.
+#include "dcrf_internal.h"
+#include "dcrf_device.h"
+
+#pragma clang fp contract(off)
+
+namespace dcrf {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 32;    // pixels per tile of the L x L statistic
+constexpr int kPairs = 4;    // (a, c) pairs per thread per launch of the L x L statistic
+inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
+
+// sumAndNormalize of x = Q .* g over the lane group: b = sum(x) q - x.
+template <bool WAVE>
+__device__ __forceinline__ float sum_normalize(float q, float gv, LaneGroup<WAVE>& G) {
+    const float x = q * gv;
+    const float s = G.all_sum(x);
+    const float sq = s * q;
+    return sq - x;
+}
+
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, const float* __restrict__ dQ,
+                                                   float* __restrict__ b, float* __restrict__ dU, int N, int L,
+                                                   int per) {
+    extern __shared__ float sh[];
+    LaneGroup<WAVE> G(L, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const size_t idx = (size_t)p * L + G.l;
+    const float q = active ? Q[idx] : 0.0f;
+    const float gv = active ? dQ[idx] : 0.0f;
+    const float bn = sum_normalize<WAVE>(q, gv, G);
+    if (active) {
+        b[idx] = bn;
+        dU[idx] = bn;
+    }
+}
+
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, float* __restrict__ b,
+                                                  float* __restrict__ dU, int N, int L, int per, SliceTable st) {
+    extern __shared__ float sh[];
+    LaneGroup<WAVE> G(L, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int l = G.l;
+    const size_t idx = (size_t)p * L + l;
+    float gsum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < DCRF_MAX_TERMS; k++) {
+        if (k < st.nterms) {
+            const SliceTerm& T = st.t[k];
+            float m = active ? slice_one(T, p, l, L) : 0.0f;
+            if (T.kw) m = m * T.kw[0];
+            const float c = compat_apply<WAVE>(T, m, G);  // C^T = C (W is symmetric)
+            gsum = gsum + c;
+        }
+    }
+    const float q = active ? Q[idx] : 0.0f;
+    const float bn = sum_normalize<WAVE>(q, gsum, G);
+    if (active) {
+        b[idx] = bn;
+        dU[idx] = dU[idx] + bn;
+    }
+}
+
+// partial[block][l] = sum over the block's pixels of b .* M for label l (groups stride
+// over the pixels; the per-label sums of the block's groups are added in group order).
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b, const float* __restrict__ M, int N,
+                                                    int L, int per, float* __restrict__ partial) {
+    extern __shared__ float sh[];
+    LaneGroup<WAVE> G(L, per, sh);
+    const int items = LaneGroup<WAVE>::items_per_block(per);
+    float acc = 0.0f;
+    if (G.on) {
+        for (int p = G.item(); p < N; p += gridDim.x * items) {
+            const size_t idx = (size_t)p * L + G.l;
+            const float prod = b[idx] * M[idx];
+            acc = acc + prod;
+        }
+    }
+    const int t = threadIdx.x;
+    sh[t] = G.on ? acc : 0.0f;
+    __syncthreads();
+    if (t < L) {
+        float s = 0.0f;
+        for (int gi = 0; gi < items; gi++) {
+            const int slot = WAVE ? (gi / per) * 64 + (gi % per) * L : gi * L;
+            s = s + sh[slot + t];
+        }
+        partial[(size_t)blockIdx.x * L + t] = s;
+    }
+}
+
+// partial[block][j] for pairs j in [base, base + 256*kPairs): G[a][c] = sum_p b[p][a] M[p][c].
+__global__ void k_grad_matrix(const float* __restrict__ b, const float* __restrict__ M, int N, int L, int base,
+                              float* __restrict__ partial) {
+    extern __shared__ float tiles[];
+    float* tb = tiles;
+    float* tm = tiles + kTile * L;
+    const int t = threadIdx.x;
+    const int LL = L * L;
+    int pa[kPairs], pc[kPairs];
+    float acc[kPairs];
+#pragma unroll
+    for (int i = 0; i < kPairs; i++) {
+        const int j = base + t + i * kThreads;
+        pa[i] = j < LL ? j / L : 0;
+        pc[i] = j < LL ? j - (j / L) * L : 0;
+        acc[i] = 0.0f;
+    }
+    for (int p0 = blockIdx.x * kTile; p0 < N; p0 += gridDim.x * kTile) {
+        for (int x = t; x < kTile * L; x += kThreads) {
+            const bool in = p0 + x / L < N;
+            tb[x] = in ? b[(size_t)p0 * L + x] : 0.0f;
+            tm[x] = in ? M[(size_t)p0 * L + x] : 0.0f;
+        }
+        __syncthreads();
+        for (int j = 0; j < kTile; j++) {
+#pragma unroll
+            for (int i = 0; i < kPairs; i++) {
+                const float prod = tb[j * L + pa[i]] * tm[j * L + pc[i]];
+                acc[i] = acc[i] + prod;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < kPairs; i++) {
+        const int j = base + t + i * kThreads;
+        if (j < LL) partial[(size_t)blockIdx.x * (kThreads * kPairs) + (j - base)] = acc[i];
+    }
+}
+
+// acc[off + j] += sum over blocks of partial[block][j] (double, block order).
+__global__ void k_reduce_add(const float* __restrict__ partial, int nblocks, int stride, int width,
+                             double* __restrict__ acc, int off) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= width) return;
+    double s = 0.0;
+    for (int k = 0; k < nblocks; k++) s += (double)partial[(size_t)k * stride + j];
+    acc[off + j] += s;
+}
+
+__global__ void k_fill_double(double* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Gradients of one term from its statistic G (diagonal [L] for Potts/Diagonal, full
+// [L*L] for Matrix): raw compatibility parameters and the term weight.
+__global__ void k_compat_finish(const double* __restrict__ G, int ckind, int L, const float* __restrict__ cw,
+                                const float* __restrict__ kw, float* __restrict__ dcompat,
+                                float* __restrict__ dkw) {
+    const int t = threadIdx.x;
+    const double w = kw ? (double)kw[0] : 1.0;
+    if (ckind == DCRF_COMPAT_POTTS) {
+        if (t == 0) {
+            double s = 0.0;
+            for (int a = 0; a < L; a++) s += G[a];
+            if (dcompat) dcompat[0] = (float)(-w * s);
+            if (dkw) dkw[0] = (float)(-(double)cw[0] * s);
+        }
+    } else if (ckind == DCRF_COMPAT_DIAGONAL) {
+        for (int a = t; a < L; a += blockDim.x)
+            if (dcompat) dcompat[a] = (float)(w * G[a]);
+        if (t == 0 && dkw) {
+            double s = 0.0;
+            for (int a = 0; a < L; a++) s += (double)cw[a] * G[a];
+            dkw[0] = (float)s;
+        }
+    } else {
+        for (int j = t; j < L * L; j += blockDim.x) {
+            const int a = j / L, c = j - a * L;
+            if (dcompat) dcompat[j] = (float)(w * 0.5 * (G[a * L + c] + G[c * L + a]));
+        }
+        if (t == 0 && dkw) {
+            double s = 0.0;
+            for (int j = 0; j < L * L; j++) s += (double)cw[j] * G[j];
+            dkw[0] = (float)s;
+        }
+    }
+}
+
+// Compatibility buffer from raw parameters: Matrix W = (m + m^T) / 2
+// (labelcompatibility.cpp:79); Potts / Diagonal copied.
+__global__ void k_set_compat(const float* __restrict__ raw, int ckind, int L, float* __restrict__ cw) {
+    const int n = ckind == DCRF_COMPAT_POTTS ? 1 : (ckind == DCRF_COMPAT_DIAGONAL ? L : L * L);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (ckind == DCRF_COMPAT_MATRIX) {
+            const int a = j / L, c = j - a * L;
+            const float s = raw[a * L + c] + raw[c * L + a];
+            cw[j] = 0.5f * s;
+        } else {
+            cw[j] = raw[j];
+        }
+    }
+}
+
+}  // namespace
+
+#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                  \
+    do {                                                                             \
+        if ((sh).wave)                                                               \
+            kernel<true><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);          \
+        else                                                                         \
+            kernel<false><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);         \
+        DCRF_HIP_CHECK(hipGetLastError());                                           \
+    } while (0)
+
+void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    DCRF_DISPATCH(sh, k_sn_first, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, dQ, b, dU, N, L, sh.per);
+}
+
+void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, b, dU, N, L, sh.per, st);
+}
+
+int grad_partial_blocks(int N, int L) {
+    const LaunchShape sh = shape_for(L);
+    const int need = (N + sh.items - 1) / sh.items;
+    return need < 1024 ? need : 1024;
+}
+
+void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
+                      hipStream_t s) {
+    const int nb = grad_partial_blocks(N, L);
+    if (!full) {
+        const LaunchShape sh = shape_for(L);
+        DCRF_DISPATCH(sh, k_grad_diag, nb, sizeof(float) * sh.block, s, b, M, N, L, sh.per, partial);
+        k_reduce_add<<<blocks_for(L), kThreads, 0, s>>>(partial, nb, L, L, acc, 0);
+        DCRF_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const int LL = L * L, chunk = kThreads * kPairs;
+    const size_t shm = sizeof(float) * 2 * kTile * L;
+    for (int base = 0; base < LL; base += chunk) {
+        k_grad_matrix<<<nb, kThreads, shm, s>>>(b, M, N, L, base, partial);
+        DCRF_HIP_CHECK(hipGetLastError());
+        const int width = LL - base < chunk ? LL - base : chunk;
+        k_reduce_add<<<blocks_for(width), kThreads, 0, s>>>(partial, nb, chunk, width, acc, base);
+        DCRF_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_fill_double(double* p, size_t n, hipStream_t s) {
+    if (!n) return;
+    k_fill_double<<<blocks_for(n), kThreads, 0, s>>>(p, n);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
+                          float* dkw, hipStream_t s) {
+    k_compat_finish<<<1, kThreads, 0, s>>>(G, ckind, L, cw, kw, dcompat, dkw);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s) {
+    const int n = ckind == DCRF_COMPAT_POTTS ? 1 : (ckind == DCRF_COMPAT_DIAGONAL ? L : L * L);
+    k_set_compat<<<blocks_for(n), kThreads, 0, s>>>(raw, ckind, L, cw);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dcrf

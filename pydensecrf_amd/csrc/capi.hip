@@ -39,8 +39,12 @@ struct TermLayout {
     DevBuf<float> swT;   // slice weights, transpose (unused when symmetric)
 };
 
+// Kernel classes timed by dcrf_enable_timing.
+enum { KC_SPLAT = 0, KC_COMBINE = 1, KC_BLUR = 2, KC_MEANFIELD = 3, KC_COUNT = 4 };
+
 struct PendingTiming {
     hipEvent_t start, stop;
+    int cls;
     double bytes;
 };
 
@@ -73,10 +77,24 @@ struct dcrf_model {
     bool all_symmetric = true;
     std::vector<std::unique_ptr<TermLayout>> tl;
 
+    // term weights (device, [T]); null unless set
+    DevBuf<float> kw;
+    bool kw_set = false;
+    // training checkpoints: Q_0..Q_n and the raw messages K~_k Q_t
+    int train_n = -1;
+    const float* train_kw = nullptr;
+    DevBuf<float> qcache, mcache, bbuf, partial;
+    DevBuf<double> gacc;
+
+    // captured inference (dcrf_capture_inference)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+
     // timing
     bool timing = false;
-    long long blur_launches = 0;
-    double blur_ms = 0.0, blur_bytes = 0.0;
+    long long kc_launches[KC_COUNT] = {};
+    double kc_ms[KC_COUNT] = {}, kc_bytes[KC_COUNT] = {};
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> free_events;
 
@@ -86,7 +104,16 @@ struct dcrf_model {
             (void)hipEventDestroy(p.stop);
         }
         for (auto e : free_events) (void)hipEventDestroy(e);
+        drop_graph();
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_valid = false;
     }
 
     int N() const { return geo.N_total(); }
@@ -133,14 +160,36 @@ void collect_timing(dcrf_model& m) {
         DCRF_HIP_CHECK(hipEventSynchronize(p.stop));
         float ms = 0.f;
         DCRF_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
-        m.blur_ms += ms;
-        m.blur_bytes += p.bytes;
-        m.blur_launches++;
+        m.kc_ms[p.cls] += ms;
+        m.kc_bytes[p.cls] += p.bytes;
+        m.kc_launches[p.cls]++;
         m.free_events.push_back(p.start);
         m.free_events.push_back(p.stop);
     }
     m.pending.clear();
 }
+
+// Brackets one launch with events when timing is on.
+struct Timed {
+    dcrf_model& m;
+    hipStream_t s;
+    int cls;
+    double bytes;
+    hipEvent_t e0 = nullptr;
+    Timed(dcrf_model& m_, hipStream_t s_, int cls_, double bytes_) : m(m_), s(s_), cls(cls_), bytes(bytes_) {
+        if (m.timing) {
+            e0 = take_event(m);
+            DCRF_HIP_CHECK(hipEventRecord(e0, s));
+        }
+    }
+    void done() {
+        if (!e0) return;
+        hipEvent_t e1 = take_event(m);
+        DCRF_HIP_CHECK(hipEventRecord(e1, s));
+        m.pending.push_back(PendingTiming{e0, e1, cls, bytes});
+        e0 = nullptr;
+    }
+};
 
 void ensure_device(dcrf_model& m) {
     if (m.dev_ready) return;
@@ -223,7 +272,7 @@ void compute_norm(dcrf_model& m, Term& t) {
     launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(lat.d), N, d1, 0, gv.get(), sw.get(), s);
     SliceTable st{};
     st.nterms = 1;
-    st.t[0] = SliceTerm{gv.get(), sw.get(), bufs[d1 & 1], d1, -1, 0.f, nullptr, 1.0f};
+    st.t[0] = SliceTerm{gv.get(), sw.get(), bufs[d1 & 1], d1, -1, nullptr, nullptr, nullptr};
     t.norm.reserve(N);
     launch_slice_term(t.norm.get(), N, 1, st, 0, s);
     if (t.ntype == DCRF_NO_NORMALIZATION) {
@@ -272,11 +321,9 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
                 t->compat[(size_t)a * L + b] = 0.5f * s;
             }
     }
-    if (ckind != DCRF_COMPAT_POTTS) {
-        t->d_compat.reserve(t->compat.size());
-        DCRF_HIP_CHECK(hipMemcpy(t->d_compat.get(), t->compat.data(), sizeof(float) * t->compat.size(),
-                                 hipMemcpyHostToDevice));
-    }
+    t->d_compat.reserve(t->compat.size());
+    DCRF_HIP_CHECK(hipMemcpy(t->d_compat.get(), t->compat.data(), sizeof(float) * t->compat.size(),
+                             hipMemcpyHostToDevice));
     DevBuf<float> feat;
     feat.reserve((size_t)m.geo.npad_total() * d);
     make_features(feat.get());
@@ -288,10 +335,13 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
     compute_norm(m, *t);
     m.terms.push_back(std::move(t));
     m.layout_valid = false;
+    m.kw_set = false;  // term weights are per term: set them again after adding terms
+    m.train_n = -1;
 }
 
 void ensure_layout(dcrf_model& m) {
     if (m.layout_valid) return;
+    m.drop_graph();
     hipStream_t s = m.stream;
     const int T = (int)m.terms.size();
     const int N = m.N(), L = m.L;
@@ -382,42 +432,63 @@ BlurTable blur_table(const dcrf_model& m) {
     return bt;
 }
 
-SliceTable slice_table(const dcrf_model& m, bool transpose) {
+// kw: device [T] term weights or null; mout: device [T][N][L] raw-message output or null.
+SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw = nullptr, float* mout = nullptr) {
     SliceTable st{};
     st.nterms = (int)m.terms.size();
+    const size_t NL = (size_t)m.N() * m.L;
     for (int k = 0; k < st.nterms; k++) {
         const Term& t = *m.terms[k];
         const TermLayout& lay = *m.tl[k];
         const float* sw = (transpose && !m.all_symmetric) ? lay.swT.get() : lay.swF.get();
         st.t[k] = SliceTerm{lay.vtx.get(), sw, m.vals[(t.lat.d + 1) & 1].get(), t.lat.d + 1, t.ckind,
-                            t.compat.empty() ? 0.f : t.compat[0],
-                            t.ckind == DCRF_COMPAT_POTTS ? nullptr : t.d_compat.get(), 1.0f};
+                            t.d_compat.get(), kw ? kw + k : nullptr, mout ? mout + (size_t)k * NL : nullptr};
     }
     return st;
 }
 
+int compat_raw_len(int ckind, int L) {
+    return ckind == DCRF_COMPAT_POTTS ? 1 : (ckind == DCRF_COMPAT_DIAGONAL ? L : L * L);
+}
+int compat_packed_len(int ckind, int L) {
+    return ckind == DCRF_COMPAT_MATRIX ? L * (L + 1) / 2 : compat_raw_len(ckind, L);
+}
+int stat_len(int ckind, int L) { return ckind == DCRF_COMPAT_MATRIX ? L * L : L; }
+
 // Splat, combine and blur every term's lattice: afterwards term k's filtered rows are
 // in vals[(d_k+1) & 1].  `transpose` selects the transposed filter (reverse blur
 // order and the transpose's normaliser placement, pairwise.cpp:63-80).
+// Algorithmic bytes of the splat (input rows once; per term the (pixel, weight) pairs,
+// the normaliser and the vertex rows written) and of the per-pixel update (unary read,
+// Q written; per term the slice pairs, the normaliser and the vertex rows read).  With
+// the blur steps' V (2 L 4 + 8) they add up to dcrf_iteration_bytes.
+double splat_bytes(const dcrf_model& m) {
+    const double N = m.N(), L = m.L;
+    double b = N * L * 4.0;
+    for (const auto& t : m.terms) b += N * (t->lat.d + 1) * 8.0 + N * 4.0 + (double)t->lat.V * L * 4.0;
+    return b;
+}
+double meanfield_bytes(const dcrf_model& m) { return splat_bytes(m) + m.N() * (double)m.L * 4.0; }
+
 void run_filter(dcrf_model& m, const float* in, bool transpose, hipStream_t s) {
     const int L = m.L;
     const float* w = (transpose && !m.all_symmetric) ? m.g_wT.get() : m.g_wF.get();
-    launch_splat(m.g_chunks.get(), m.C_total, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
-    launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), L, s);
+    {
+        Timed t(m, s, KC_SPLAT, splat_bytes(m));
+        launch_splat(m.g_chunks.get(), m.C_total, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
+        t.done();
+    }
+    if (m.CB_total) {
+        Timed t(m, s, KC_COMBINE, 0.0);
+        launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), L, s);
+        t.done();
+    }
     const BlurTable bt = blur_table(m);
     for (int step = 0; step <= m.max_d; step++) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (m.timing) {
-            e0 = take_event(m);
-            e1 = take_event(m);
-            DCRF_HIP_CHECK(hipEventRecord(e0, s));
-        }
+        Timed t(m, s, KC_BLUR, (double)m.prefixV[step] * (2.0 * L * 4 + 8));
         launch_blur(m.vals[step & 1].get(), m.vals[(step + 1) & 1].get(), m.prefixV[step], L, bt, step,
                     transpose ? 1 : 0, s);
-        if (m.timing) {
-            DCRF_HIP_CHECK(hipEventRecord(e1, s));
-            m.pending.push_back(PendingTiming{e0, e1, (double)m.prefixV[step] * (2.0 * L * 4 + 8)});
-        }
+        t.done();
     }
 }
 
@@ -431,10 +502,72 @@ void prepare(dcrf_model& m) {
 
 void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStream_t s) {
     launch_softmax_neg(U, Q, m.N(), m.L, s);
-    const SliceTable st = slice_table(m, false);
+    const SliceTable st = slice_table(m, false, m.kw_set ? m.kw.get() : nullptr);
     for (int it = 0; it < n_iter; it++) {
         run_filter(m, Q, false, s);
+        Timed t(m, s, KC_MEANFIELD, meanfield_bytes(m));
         launch_meanfield(U, Q, m.N(), m.L, st, nullptr, nullptr, s);
+        t.done();
+    }
+}
+
+// Forward pass that keeps Q_0..Q_n and the raw messages for the adjoint.
+void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compat_raw, const float* kw,
+                   hipStream_t s) {
+    const int N = m.N(), L = m.L, T = (int)m.terms.size();
+    const size_t NL = (size_t)N * L;
+    if (compat_raw) {
+        size_t off = 0;
+        for (auto& t : m.terms) {
+            launch_set_compat(compat_raw + off, t->ckind, L, t->d_compat.get(), s);
+            off += compat_raw_len(t->ckind, L);
+        }
+    }
+    m.qcache.reserve((size_t)(n_iter + 1) * NL);
+    m.mcache.reserve(std::max<size_t>((size_t)n_iter * T * NL, 1));
+    launch_softmax_neg(U, m.qcache.get(), N, L, s);
+    for (int it = 0; it < n_iter; it++) {
+        const float* Qt = m.qcache.get() + (size_t)it * NL;
+        float* Qn = m.qcache.get() + (size_t)(it + 1) * NL;
+        run_filter(m, Qt, false, s);
+        launch_meanfield(U, Qn, N, L, slice_table(m, false, kw, m.mcache.get() + (size_t)it * T * NL), nullptr,
+                         nullptr, s);
+    }
+    m.train_n = n_iter;
+    m.train_kw = kw;
+}
+
+// Adjoint of forward_train.  dQ: dL/dQ_n.  Outputs dU (always), per-term statistics in
+// m.gacc, and optionally raw compatibility / term-weight gradients on the device.
+void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* dkw, hipStream_t s) {
+    if (m.train_n < 0) throw Error{DCRF_ERR_STATE, "backward needs a preceding training forward"};
+    const int N = m.N(), L = m.L, T = (int)m.terms.size(), n = m.train_n;
+    const size_t NL = (size_t)N * L;
+    m.bbuf.reserve(NL);
+    float* b = m.bbuf.get();
+    const float* Qc = m.qcache.get();
+    launch_sn_first(Qc + (size_t)n * NL, dQ, b, dU, N, L, s);
+    std::vector<size_t> goff(T + 1, 0);
+    for (int k = 0; k < T; k++) goff[k + 1] = goff[k] + stat_len(m.terms[k]->ckind, L);
+    m.gacc.reserve(std::max<size_t>(goff[T], 1));
+    launch_fill_double(m.gacc.get(), goff[T], s);
+    m.partial.reserve((size_t)grad_partial_blocks(N, L) * std::max(L, 1024));
+    for (int it = n - 1; it >= 0; it--) {
+        for (int k = 0; k < T; k++)
+            launch_grad_stat(b, m.mcache.get() + ((size_t)it * T + k) * NL, N, L,
+                             m.terms[k]->ckind == DCRF_COMPAT_MATRIX, m.partial.get(), m.gacc.get() + goff[k], s);
+        run_filter(m, b, true, s);
+        launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);
+    }
+    if (dcompat || dkw) {
+        size_t coff = 0;
+        for (int k = 0; k < T; k++) {
+            const Term& t = *m.terms[k];
+            launch_compat_finish(m.gacc.get() + goff[k], t.ckind, L, t.d_compat.get(),
+                                 m.train_kw ? m.train_kw + k : nullptr, dcompat ? dcompat + coff : nullptr,
+                                 dkw ? dkw + k : nullptr, s);
+            coff += compat_raw_len(t.ckind, L);
+        }
     }
 }
 
@@ -611,6 +744,39 @@ int dcrf_inference_device(dcrf_model* m, int n_iter, const float* U_dev, float* 
     });
 }
 
+int dcrf_capture_inference(dcrf_model* m, int n_iter, const float* U_dev, float* Q_dev) {
+    return guard([&] {
+        require(m && Q_dev, "null argument");
+        require(n_iter >= 0, "number of iterations must be non-negative");
+        require(!m->timing, "disable kernel timing before capturing a graph", DCRF_ERR_STATE);
+        prepare(*m);
+        m->drop_graph();
+        const float* U = U_dev ? U_dev : (m->has_unary ? m->unary.get() : nullptr);
+        DCRF_HIP_CHECK(hipStreamSynchronize(m->stream));
+        DCRF_HIP_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_inference(*m, n_iter, U, Q_dev, m->stream);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(m->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        DCRF_HIP_CHECK(hipStreamEndCapture(m->stream, &m->graph));
+        DCRF_HIP_CHECK(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+        m->graph_valid = true;
+    });
+}
+
+int dcrf_launch_graph(dcrf_model* m, void* stream) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(m->graph_valid && m->layout_valid, "no captured inference (or the model changed since)",
+                DCRF_ERR_STATE);
+        DCRF_HIP_CHECK(hipGraphLaunch(m->graph_exec, (hipStream_t)stream));
+    });
+}
+
 int dcrf_start_inference(dcrf_model* m, float* Q) {
     return guard([&] {
         require(m && Q, "null argument");
@@ -639,8 +805,8 @@ int dcrf_step_inference(dcrf_model* m, float* Q, float* tmp1, float* tmp2) {
         DCRF_HIP_CHECK(hipMemcpyAsync(m->Q.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
         if (m->terms.empty()) launch_fill(m->tmp2.get(), 0.f, n, s);
         run_filter(*m, m->Q.get(), false, s);
-        launch_meanfield(m->has_unary ? m->unary.get() : nullptr, m->Q.get(), m->N(), m->L, slice_table(*m, false),
-                         m->tmp1.get(), m->terms.empty() ? nullptr : m->tmp2.get(), s);
+        launch_meanfield(m->has_unary ? m->unary.get() : nullptr, m->Q.get(), m->N(), m->L,
+                         slice_table(*m, false, m->kw_set ? m->kw.get() : nullptr), m->tmp1.get(), m->terms.empty() ? nullptr : m->tmp2.get(), s);
         DCRF_HIP_CHECK(hipMemcpyAsync(Q, m->Q.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
         if (tmp1) DCRF_HIP_CHECK(hipMemcpyAsync(tmp1, m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
         if (tmp2) DCRF_HIP_CHECK(hipMemcpyAsync(tmp2, m->tmp2.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
@@ -659,11 +825,11 @@ int dcrf_kl_divergence(dcrf_model* m, const float* Q, double* kl) {
         DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
         run_filter(*m, m->work.get(), false, s);
         const LaunchShape sh = shape_for(m->L);
-        const int nb = (m->N() + sh.per - 1) / sh.per;
+        const int nb = (m->N() + sh.items - 1) / sh.items;
         DevBuf<double> part;
         part.reserve(nb);
-        launch_kl(m->work.get(), m->has_unary ? m->unary.get() : nullptr, m->N(), m->L, slice_table(*m, false),
-                  part.get(), s);
+        launch_kl(m->work.get(), m->has_unary ? m->unary.get() : nullptr, m->N(), m->L,
+                  slice_table(*m, false, m->kw_set ? m->kw.get() : nullptr), part.get(), s);
         std::vector<double> h(nb);
         DCRF_HIP_CHECK(hipMemcpyAsync(h.data(), part.get(), sizeof(double) * nb, hipMemcpyDeviceToHost, s));
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
@@ -685,10 +851,174 @@ int dcrf_apply_pairwise(dcrf_model* m, int term, const float* Q, float* out, int
         hipStream_t s = m->stream;
         DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
         run_filter(*m, m->work.get(), transpose != 0, s);
-        launch_slice_term(m->tmp1.get(), m->N(), m->L, slice_table(*m, transpose != 0), term, s);
+        launch_slice_term(m->tmp1.get(), m->N(), m->L,
+                          slice_table(*m, transpose != 0, m->kw_set ? m->kw.get() : nullptr), term, s);
         DCRF_HIP_CHECK(hipMemcpyAsync(out, m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
         if (m->timing) collect_timing(*m);
+    });
+}
+
+int dcrf_set_compat(dcrf_model* m, int term, const float* params) {
+    return guard([&] {
+        require(m && params, "null argument");
+        require(term >= 0 && term < (int)m->terms.size(), "pairwise term index out of range");
+        Term& t = *m->terms[term];
+        const int L = m->L;
+        if (t.ckind == DCRF_COMPAT_MATRIX) {
+            for (int a = 0; a < L; a++)
+                for (int b = 0; b < L; b++) {
+                    const float sum = params[a * L + b] + params[b * L + a];
+                    t.compat[(size_t)a * L + b] = 0.5f * sum;
+                }
+        } else {
+            t.compat.assign(params, params + compat_raw_len(t.ckind, L));
+        }
+        ensure_device(*m);
+        DCRF_HIP_CHECK(hipMemcpy(t.d_compat.get(), t.compat.data(), sizeof(float) * t.compat.size(),
+                                 hipMemcpyHostToDevice));
+    });
+}
+
+int dcrf_set_kernel_weights(dcrf_model* m, const float* w) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        const int T = (int)m->terms.size();
+        if (!w) {
+            m->kw_set = false;
+            return;
+        }
+        ensure_device(*m);
+        m->kw.reserve(std::max(T, 1));
+        DCRF_HIP_CHECK(hipMemcpy(m->kw.get(), w, sizeof(float) * T, hipMemcpyHostToDevice));
+        m->kw_set = true;
+    });
+}
+
+int dcrf_num_compat_params(const dcrf_model* m) {
+    if (!m) return 0;
+    int n = 0;
+    for (const auto& t : m->terms) n += compat_packed_len(t->ckind, m->L);
+    return n;
+}
+
+int dcrf_get_compat_params(const dcrf_model* m, float* p) {
+    return guard([&] {
+        require(m && p, "null argument");
+        const int L = m->L;
+        for (const auto& tp : m->terms) {
+            std::vector<float> c(tp->compat.size());
+            if (m->dev_ready)
+                DCRF_HIP_CHECK(hipMemcpy(c.data(), tp->d_compat.get(), sizeof(float) * c.size(), hipMemcpyDeviceToHost));
+            else
+                c = tp->compat;
+            if (tp->ckind == DCRF_COMPAT_MATRIX) {
+                for (int i = 0; i < L; i++)
+                    for (int j = i; j < L; j++) *p++ = c[(size_t)i * L + j];
+            } else {
+                for (float v : c) *p++ = v;
+            }
+        }
+    });
+}
+
+int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, double* objective, float* unary_grad,
+                  float* compat_grad, float* kweight_grad) {
+    return guard([&] {
+        require(m && gt, "null argument");
+        require(n_iter >= 0, "number of iterations must be non-negative");
+        prepare(*m);
+        const int N = m->N(), L = m->L, T = (int)m->terms.size();
+        const size_t NL = (size_t)N * L;
+        hipStream_t s = m->stream;
+        const float* U = m->has_unary ? m->unary.get() : nullptr;
+        const float* kw = m->kw_set ? m->kw.get() : nullptr;
+        forward_train(*m, n_iter, U, nullptr, kw, s);
+        // LogLikelihood (objective.cpp:37-50): r = sum log(max(Q(gt)+robust, 1e-20)) / N,
+        // dL/dQ(gt) = 1 / QQ / N
+        std::vector<float> Qn(NL), dQ(NL, 0.f);
+        DCRF_HIP_CHECK(hipMemcpyAsync(Qn.data(), m->qcache.get() + (size_t)n_iter * NL, sizeof(float) * NL,
+                                      hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        double r = 0.0;
+        for (int i = 0; i < N; i++) {
+            const int g = gt[i];
+            if (g < 0 || g >= L) continue;
+            const float q = Qn[(size_t)i * L + g];
+            const float qr = q + robust;
+            const float QQ = qr > 1e-20f ? qr : 1e-20f;
+            r += std::log((double)QQ) / N;
+            dQ[(size_t)i * L + g] = 1.0f / QQ / (float)N;
+        }
+        if (objective) *objective = r;
+        m->work.reserve(NL);
+        m->tmp1.reserve(NL);
+        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), dQ.data(), sizeof(float) * NL, hipMemcpyHostToDevice, s));
+        backward(*m, m->work.get(), m->tmp1.get(), nullptr, nullptr, s);
+        if (unary_grad)
+            DCRF_HIP_CHECK(hipMemcpyAsync(unary_grad, m->tmp1.get(), sizeof(float) * NL, hipMemcpyDeviceToHost, s));
+        size_t glen = 0;
+        for (auto& t : m->terms) glen += stat_len(t->ckind, L);
+        std::vector<double> G(glen);
+        DCRF_HIP_CHECK(hipMemcpyAsync(G.data(), m->gacc.get(), sizeof(double) * glen, hipMemcpyDeviceToHost, s));
+        std::vector<float> w(T, 1.0f);
+        if (kw) DCRF_HIP_CHECK(hipMemcpyAsync(w.data(), kw, sizeof(float) * T, hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        // pack like DenseCRF::gradient (labelcompatibility.cpp:57-61, 76-78, 101-108)
+        size_t go = 0;
+        float* cp = compat_grad;
+        for (int k = 0; k < T; k++) {
+            const Term& t = *m->terms[k];
+            const double* g = G.data() + go;
+            double kg = 0.0;
+            if (t.ckind == DCRF_COMPAT_POTTS) {
+                double sd = 0.0;
+                for (int a = 0; a < L; a++) sd += g[a];
+                if (cp) *cp++ = (float)(-w[k] * sd);
+                kg = -(double)t.compat[0] * sd;
+            } else if (t.ckind == DCRF_COMPAT_DIAGONAL) {
+                for (int a = 0; a < L; a++) {
+                    if (cp) *cp++ = (float)(w[k] * g[a]);
+                    kg += (double)t.compat[a] * g[a];
+                }
+            } else {
+                for (int i = 0; i < L; i++)
+                    for (int j = i; j < L; j++) {
+                        const double v = i == j ? g[i * L + i] : g[i * L + j] + g[j * L + i];
+                        if (cp) *cp++ = (float)(w[k] * v);
+                    }
+                for (int j = 0; j < L * L; j++) kg += (double)t.compat[j] * g[j];
+            }
+            if (kweight_grad) kweight_grad[k] = (float)kg;
+            go += stat_len(t.ckind, L);
+        }
+        if (m->timing) collect_timing(*m);
+    });
+}
+
+int dcrf_forward_device(dcrf_model* m, int n_iter, const float* U_dev, const float* compat_dev, const float* kw_dev,
+                        float* Q_dev, void* stream) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(n_iter >= 0, "number of iterations must be non-negative");
+        prepare(*m);
+        hipStream_t s = (hipStream_t)stream;
+        const float* U = U_dev ? U_dev : (m->has_unary ? m->unary.get() : nullptr);
+        const float* kw = kw_dev ? kw_dev : (m->kw_set ? m->kw.get() : nullptr);
+        forward_train(*m, n_iter, U, compat_dev, kw, s);
+        if (Q_dev) {
+            const size_t NL = (size_t)m->N() * m->L;
+            DCRF_HIP_CHECK(hipMemcpyAsync(Q_dev, m->qcache.get() + (size_t)n_iter * NL, sizeof(float) * NL,
+                                          hipMemcpyDeviceToDevice, s));
+        }
+    });
+}
+
+int dcrf_backward_device(dcrf_model* m, const float* dQ_dev, float* dU_dev, float* dcompat_dev, float* dkw_dev,
+                         void* stream) {
+    return guard([&] {
+        require(m && dQ_dev && dU_dev, "null argument");
+        backward(*m, dQ_dev, dU_dev, dcompat_dev, dkw_dev, (hipStream_t)stream);
     });
 }
 
@@ -767,19 +1097,22 @@ int dcrf_enable_timing(dcrf_model* m, int on) {
         require(m != nullptr, "null model");
         if (m->timing && !on) collect_timing(*m);
         m->timing = on != 0;
-        m->blur_launches = 0;
-        m->blur_ms = 0.0;
-        m->blur_bytes = 0.0;
+        for (int c = 0; c < KC_COUNT; c++) {
+            m->kc_launches[c] = 0;
+            m->kc_ms[c] = 0.0;
+            m->kc_bytes[c] = 0.0;
+        }
     });
 }
 
-int dcrf_blur_stats(dcrf_model* m, long long* launches, double* ms, double* bytes) {
+int dcrf_kernel_stats(dcrf_model* m, int kernel, long long* launches, double* ms, double* bytes) {
     return guard([&] {
         require(m != nullptr, "null model");
+        require(kernel >= 0 && kernel < KC_COUNT, "unknown kernel class");
         collect_timing(*m);
-        if (launches) *launches = m->blur_launches;
-        if (ms) *ms = m->blur_ms;
-        if (bytes) *bytes = m->blur_bytes;
+        if (launches) *launches = m->kc_launches[kernel];
+        if (ms) *ms = m->kc_ms[kernel];
+        if (bytes) *bytes = m->kc_bytes[kernel];
     });
 }
 

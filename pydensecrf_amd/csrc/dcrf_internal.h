@@ -160,19 +160,22 @@ struct SliceTerm {
     const float* sw;    // [N][d+1] slice weights: bary * alpha * post-normaliser
     const float* vals;  // vertex rows after the last blur pass
     int d1;
-    int ckind;
-    float potts;        // Potts weight
-    const float* cw;    // Diagonal [L] or Matrix [L*L] (device)
-    float scale;        // extra weight multiplying the term's message
+    int ckind;          // DCRF_COMPAT_*, or -1 for none
+    const float* cw;    // device: Potts [1], Diagonal [L], Matrix [L*L] (symmetric)
+    const float* kw;    // device: the term's weight (one float), null = 1
+    float* mout;        // optional output of the raw message K~Q [N][L] (training cache)
 };
 struct SliceTable {
     int nterms;
     SliceTerm t[DCRF_MAX_TERMS];
 };
 
+// Lane-group geometry of the per-item kernels (see dcrf_device.h).
 struct LaunchShape {
     int block;  // threads per block
-    int per;    // items per block (block / L)
+    int per;    // groups per wave (wave mode) or per block (block mode)
+    bool wave;  // L <= 64: groups live inside one wave
+    int items;  // items per block
 };
 LaunchShape shape_for(int L);
 
@@ -197,6 +200,21 @@ void launch_norm_finish(float* norm, int N, int ntype, hipStream_t s);
 void launch_fill(float* p, float v, size_t n, hipStream_t s);
 // out[c][r] = in[r][c] (in is rows x cols row-major).
 void launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s);
+// ---------------------------------------------------------------- adjoint kernels
+// b = sumAndNormalize(Q .* dQ, Q); dU = b.
+void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s);
+// b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) in place; dU += b.
+void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s);
+// acc += sum_p b_p M_p^T (full L*L when `full`, else its diagonal [L]); partial holds
+// grad_partial_blocks(N, L) * max(L, 1024) floats.
+int grad_partial_blocks(int N, int L);
+void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
+                      hipStream_t s);
+void launch_fill_double(double* p, size_t n, hipStream_t s);
+void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
+                          float* dkw, hipStream_t s);
+void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s);
+
 // Global layout remaps.
 void launch_remap_chunks(const Chunk* src, Chunk* dst, int n, int vbase, int ebase, int prow_base,
                          hipStream_t s);

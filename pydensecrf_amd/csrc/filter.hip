@@ -18,6 +18,7 @@
 //   meanfield per pixel: slice every term (sum_j w_j*alpha*norm_p * row_j), apply the
 //            compatibility, subtract from -U, softmax over the lane group.
 #include "dcrf_internal.h"
+#include "dcrf_device.h"
 
 #include <cmath>
 
@@ -26,13 +27,15 @@
 namespace dcrf {
 
 LaunchShape shape_for(int L) {
+    if (L <= 64) return LaunchShape{kWaveBlock, 64 / L, true, (kWaveBlock / 64) * (64 / L)};
     const int block = L <= 256 ? 256 : (L <= 512 ? 512 : 1024);
-    return LaunchShape{block, block / L};
+    return LaunchShape{block, block / L, false, block / L};
 }
 
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBatch = 8;  // contributors / partial rows loaded together
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
 __global__ void k_fill(float* p, float v, size_t n) {
@@ -40,44 +43,71 @@ __global__ void k_fill(float* p, float v, size_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_splat(const Chunk* __restrict__ chunks, int n, const int* __restrict__ pt,
-                        const float* __restrict__ w, const float* __restrict__ in,
-                        float* __restrict__ vals, int L, int per) {
-    const int t = threadIdx.x;
-    const int g = t / L, l = t - g * L;
-    const int c = blockIdx.x * per + g;
-    if (g >= per || c >= n) return;
+// Splat: each group sums one chunk (<= C contributors of a vertex, pixel ascending)
+// sequentially; (pixel, weight) pairs and input rows are fetched kBatch at a time.
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks, int n,
+                                                const int* __restrict__ pt, const float* __restrict__ w,
+                                                const float* __restrict__ in, float* __restrict__ vals, int L,
+                                                int per) {
+    LaneGroup<WAVE> G(L, per, nullptr);
+    const int c = G.item();
+    if (!G.on || c >= n) return;
+    const int l = G.l;
     const Chunk ch = chunks[c];
     float acc = 0.0f;
-#pragma unroll 4
-    for (int i = ch.beg; i < ch.end; i++) {
-        const int p = pt[i];
-        const float wi = w[i];
-        const float x = in[(size_t)p * L + l];
-        const float prod = wi * x;
+    int i = ch.beg;
+    for (; i + kBatch <= ch.end; i += kBatch) {
+        int pp[kBatch];
+        float ww[kBatch], xx[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            pp[u] = pt[i + u];
+            ww[u] = w[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) xx[u] = in[(size_t)pp[u] * L + l];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const float prod = ww[u] * xx[u];
+            acc = acc + prod;
+        }
+    }
+    for (; i < ch.end; i++) {
+        const float prod = w[i] * in[(size_t)pt[i] * L + l];
         acc = acc + prod;
     }
     vals[(size_t)ch.out * L + l] = acc;
 }
 
-__global__ void k_combine(const Combine* __restrict__ cb, int n, float* __restrict__ vals, int L,
-                          int per) {
-    const int t = threadIdx.x;
-    const int g = t / L, l = t - g * L;
-    const int c = blockIdx.x * per + g;
-    if (g >= per || c >= n) return;
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb, int n, float* __restrict__ vals,
+                                                  int L, int per) {
+    LaneGroup<WAVE> G(L, per, nullptr);
+    const int c = G.item();
+    if (!G.on || c >= n) return;
+    const int l = G.l;
     const Combine r = cb[c];
     float acc = 0.0f;
-    for (int i = 0; i < r.count; i++) acc = acc + vals[(size_t)(r.first + i) * L + l];
+    int i = 0;
+    for (; i + kBatch <= r.count; i += kBatch) {
+        float xx[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) xx[u] = vals[(size_t)(r.first + i + u) * L + l];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) acc = acc + xx[u];
+    }
+    for (; i < r.count; i++) acc = acc + vals[(size_t)(r.first + i) * L + l];
     vals[(size_t)r.v * L + l] = acc;
 }
 
-__global__ void k_blur(const float* __restrict__ src, float* __restrict__ dst, int Vp, int L, int per,
-                       BlurTable bt, int step, int reverse) {
-    const int t = threadIdx.x;
-    const int g = t / L, l = t - g * L;
-    const int v = blockIdx.x * per + g;
-    if (g >= per || v >= Vp) return;
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, float* __restrict__ dst, int Vp,
+                                               int L, int per, BlurTable bt, int step, int reverse) {
+    LaneGroup<WAVE> G(L, per, nullptr);
+    const int v = G.item();
+    if (!G.on || v >= Vp) return;
+    const int l = G.l;
     // terms are ordered by vbase; pick the last one starting at or before v
     int vb = bt.t[0].vbase, V = bt.t[0].V, d = bt.t[0].d;
     const int2* nbr = bt.t[0].nbr;
@@ -91,60 +121,26 @@ __global__ void k_blur(const float* __restrict__ src, float* __restrict__ dst, i
         }
     }
     const int j = reverse ? d - step : step;
+    const float o = src[(size_t)v * L + l];
     const int2 nb = nbr[(size_t)j * V + (v - vb)];
     const float a = src[(size_t)nb.x * L + l];
     const float b = src[(size_t)nb.y * L + l];
-    const float o = src[(size_t)v * L + l];
     const float s2 = a + b;
     const float h = 0.5f * s2;
     dst[(size_t)v * L + l] = o + h;
 }
 
-// Slice of term T for pixel p, label l: sum_j sw_j * vals[vtx_j][l] in j order.
-__device__ inline float slice_one(const SliceTerm& T, int p, int l, int L) {
-    float m = 0.0f;
-    const int* vt = T.vtx + (size_t)p * T.d1;
-    const float* sw = T.sw + (size_t)p * T.d1;
-    for (int j = 0; j < T.d1; j++) {
-        const float prod = sw[j] * T.vals[(size_t)vt[j] * L + l];
-        m = m + prod;
-    }
-    return m;
-}
-
-// Compatibility transform of one message value.  Matrix needs the whole label vector
-// of the pixel: it goes through shared memory (sh holds block floats).  Must be
-// called by every thread of the block (it synchronises for the Matrix form).
-__device__ inline float compat_apply(const SliceTerm& T, float m, float* sh, int t, int g, int l,
-                                     int L) {
-    if (T.ckind == DCRF_COMPAT_POTTS) {
-        const float nw = -T.potts;
-        return nw * m;
-    }
-    if (T.ckind == DCRF_COMPAT_DIAGONAL) return T.cw[l] * m;
-    sh[t] = m;
-    __syncthreads();
-    float c = 0.0f;
-    if (l >= 0) {
-        const float* row = T.cw + (size_t)l * L;
-        const float* mv = sh + g * L;
-        for (int b = 0; b < L; b++) {
-            const float prod = row[b] * mv[b];
-            c = c + prod;
-        }
-    }
-    __syncthreads();
-    return c;
-}
-
-__global__ void k_meanfield(const float* __restrict__ U, float* __restrict__ Q, int N, int L, int per,
-                            SliceTable st, float* __restrict__ tmp1, float* __restrict__ tmp2) {
+// Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
+// (densecrf.cpp:122-129, with the per-term weight w_k = 1 unless set).
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U, float* __restrict__ Q, int N,
+                                                    int L, int per, SliceTable st, float* __restrict__ tmp1,
+                                                    float* __restrict__ tmp2) {
     extern __shared__ float sh[];
-    const int t = threadIdx.x;
-    int g = t / L, l = t - g * L;
-    const int p = blockIdx.x * per + g;
-    const bool active = g < per && p < N;
-    if (g >= per) { g = 0; l = 0; }  // idle lanes read a valid shared slot
+    LaneGroup<WAVE> G(L, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int l = G.l;
     const size_t idx = (size_t)p * L + l;
     float e = 0.0f;
     if (active && U) e = -U[idx];
@@ -154,23 +150,16 @@ __global__ void k_meanfield(const float* __restrict__ U, float* __restrict__ Q, 
         if (k < st.nterms) {
             const SliceTerm& T = st.t[k];
             float m = active ? slice_one(T, p, l, L) : 0.0f;
-            if (T.scale != 1.0f) m = m * T.scale;
-            const float c = compat_apply(T, m, sh, t, g, l, L);
+            if (active && T.mout) T.mout[idx] = m;
+            if (T.kw) m = m * T.kw[0];
+            const float c = compat_apply<WAVE>(T, m, G);
             e = e - c;
             last = c;
         }
     }
-    // softmax over the lane group: max, exp, sum in label order
-    sh[t] = e;
-    __syncthreads();
-    float mx = sh[g * L];
-    for (int b = 1; b < L; b++) mx = fmaxf(mx, sh[g * L + b]);
+    const float mx = G.all_max(e);
     const float x = expf(e - mx);
-    __syncthreads();
-    sh[t] = x;
-    __syncthreads();
-    float s = 0.0f;
-    for (int b = 0; b < L; b++) s = s + sh[g * L + b];
+    const float s = G.all_sum(x);
     if (active) {
         Q[idx] = x / s;
         if (tmp1) tmp1[idx] = e;
@@ -179,34 +168,34 @@ __global__ void k_meanfield(const float* __restrict__ U, float* __restrict__ Q, 
 }
 
 // out = compat_which(slice_which) (ckind < 0: the plain slice).
-__global__ void k_slice_term(float* __restrict__ out, int N, int L, int per, SliceTable st, int which) {
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int per, SliceTable st,
+                                                     int which) {
     extern __shared__ float sh[];
-    const int t = threadIdx.x;
-    int g = t / L, l = t - g * L;
-    const int p = blockIdx.x * per + g;
-    const bool active = g < per && p < N;
-    if (g >= per) { g = 0; l = 0; }
+    LaneGroup<WAVE> G(L, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
     SliceTerm T = st.t[0];
 #pragma unroll
     for (int k = 1; k < DCRF_MAX_TERMS; k++)
         if (k == which) T = st.t[k];
-    float m = active ? slice_one(T, p, l, L) : 0.0f;
-    if (T.scale != 1.0f) m = m * T.scale;
-    if (T.ckind >= 0) m = compat_apply(T, m, sh, t, g, l, L);
-    if (active) out[(size_t)p * L + l] = m;
+    float m = active ? slice_one(T, p, G.l, L) : 0.0f;
+    if (T.kw) m = m * T.kw[0];
+    if (T.ckind >= 0) m = compat_apply<WAVE>(T, m, G);
+    if (active) out[(size_t)p * L + G.l] = m;
 }
 
 // KL divergence terms per (pixel, label), reduced per block in double in a fixed order
 // (densecrf.cpp:214-235: q log max(q,1e-20) + u q + sum_k q * pairwise_k(Q)).
-__global__ void k_kl(const float* __restrict__ Q, const float* __restrict__ U, int N, int L, int per,
-                     SliceTable st, double* __restrict__ partial) {
+template <bool WAVE>
+__global__ void __launch_bounds__(1024) k_kl(const float* __restrict__ Q, const float* __restrict__ U, int N, int L,
+                                             int per, SliceTable st, double* __restrict__ partial) {
     extern __shared__ float sh[];
     __shared__ double red[1024];
-    const int t = threadIdx.x;
-    int g = t / L, l = t - g * L;
-    const int p = blockIdx.x * per + g;
-    const bool active = g < per && p < N;
-    if (g >= per) { g = 0; l = 0; }
+    LaneGroup<WAVE> G(L, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int l = G.l;
     const size_t idx = (size_t)p * L + l;
     const float q = active ? Q[idx] : 0.0f;
     double acc = 0.0;
@@ -220,11 +209,12 @@ __global__ void k_kl(const float* __restrict__ Q, const float* __restrict__ U, i
         if (k < st.nterms) {
             const SliceTerm& T = st.t[k];
             float m = active ? slice_one(T, p, l, L) : 0.0f;
-            if (T.scale != 1.0f) m = m * T.scale;
-            const float c = compat_apply(T, m, sh, t, g, l, L);
+            if (T.kw) m = m * T.kw[0];
+            const float c = compat_apply<WAVE>(T, m, G);
             if (active) acc += (double)(q * c);
         }
     }
+    const int t = threadIdx.x;
     red[t] = acc;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -327,35 +317,41 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
+#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                  \
+    do {                                                                             \
+        if ((sh).wave)                                                               \
+            kernel<true><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);          \
+        else                                                                         \
+            kernel<false><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);         \
+        DCRF_HIP_CHECK(hipGetLastError());                                           \
+    } while (0)
+
 void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w, const float* in,
                   float* vals, int L, hipStream_t s) {
     if (!n_chunks) return;
     const LaunchShape sh = shape_for(L);
-    k_splat<<<blocks_for(n_chunks, sh.per), sh.block, 0, s>>>(chunks, n_chunks, csr_pt, csr_w, in, vals, L, sh.per);
-    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), 0, s, chunks, n_chunks, csr_pt, csr_w, in, vals, L,
+                  sh.per);
 }
 
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s) {
     if (!n) return;
     const LaunchShape sh = shape_for(L);
-    k_combine<<<blocks_for(n, sh.per), sh.block, 0, s>>>(c, n, vals, L, sh.per);
-    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, L, sh.per);
 }
 
 void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step, int reverse,
                  hipStream_t s) {
     if (!Vp) return;
     const LaunchShape sh = shape_for(L);
-    k_blur<<<blocks_for(Vp, sh.per), sh.block, 0, s>>>(src, dst, Vp, L, sh.per, bt, step, reverse);
-    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_DISPATCH(sh, k_blur, blocks_for(Vp, sh.items), 0, s, src, dst, Vp, L, sh.per, bt, step, reverse);
 }
 
 void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1, float* tmp2,
                       hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    k_meanfield<<<blocks_for(N, sh.per), sh.block, sizeof(float) * sh.block, s>>>(U, Q, N, L, sh.per, st, tmp1,
-                                                                                  tmp2);
-    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * sh.block, s, U, Q, N, L, sh.per, st,
+                  tmp1, tmp2);
 }
 
 void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
@@ -366,16 +362,15 @@ void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
 
 void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    k_slice_term<<<blocks_for(N, sh.per), sh.block, sizeof(float) * sh.block, s>>>(out, N, L, sh.per, st, which);
-    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * sh.block, s, out, N, L, sh.per, st,
+                  which);
 }
 
 int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st, double* partials,
               hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    const int nb = (int)blocks_for(N, sh.per);
-    k_kl<<<nb, sh.block, sizeof(float) * sh.block, s>>>(Q, U, N, L, sh.per, st, partials);
-    DCRF_HIP_CHECK(hipGetLastError());
+    const int nb = (int)blocks_for(N, sh.items);
+    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * sh.block, s, Q, U, N, L, sh.per, st, partials);
     return nb;
 }
 
