@@ -180,8 +180,12 @@ int dcrf_get_lattice(const dcrf_model *m, int term, int instance, int *offset, i
                      float *bary, int *nbr, int16_t *keys, float *norm);
 /* Vertex count of one instance of a term. */
 int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int *V);
-/* Set the chunk length of the splat's segmented reduction (0 = autotuned default). */
+/* Set the chunk length of the splat's segmented reduction (0 = the default, 64):
+ * vertices with more contributors are summed in pieces that are recombined in order. */
 int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
+/* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
+ * (default 0.5): lower trades memory for shorter probe sequences. */
+int dcrf_set_hash_load(dcrf_model *m, float load);
 /* Kernel timing: when enabled, the library records HIP events around every launch of
  * the inference kernels on the stream they run on (enabling resets the counters).
  * dcrf_kernel_stats returns, for one kernel class, the number of timed launches, their

@@ -66,6 +66,7 @@ SIGNATURES = {
     "dcrf_get_lattice": (_c, [_P, _c, _c, _I, _I, _F, _I, _S, _F]),
     "dcrf_term_instance_vertices": (_c, [_P, _c, _c, _I]),
     "dcrf_set_splat_chunk": (_c, [_P, _c]),
+    "dcrf_set_hash_load": (_c, [_P, _f]),
     "dcrf_enable_timing": (_c, [_P, _c]),
     "dcrf_kernel_stats": (_c, [_P, _c, _LL, _D, _D]),
     "dcrf_iteration_bytes": (_c, [_P, _D]),
@@ -328,6 +329,9 @@ class Model:
 
     def set_splat_chunk(self, c):
         check(lib().dcrf_set_splat_chunk(self.h, c))
+
+    def set_hash_load(self, load):
+        check(lib().dcrf_set_hash_load(self.h, load))
 
     def enable_timing(self, on=True):
         check(lib().dcrf_enable_timing(self.h, int(bool(on))))

@@ -27,84 +27,126 @@ constexpr int kPairs = 4;    // (a, c) pairs per thread per launch of the L x L 
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
 // sumAndNormalize of x = Q .* g over the lane group: b = sum(x) q - x.
-template <bool WAVE>
-__device__ __forceinline__ float sum_normalize(float q, float gv, LaneGroup<WAVE>& G) {
-    const float x = q * gv;
+template <bool WAVE, int R>
+__device__ __forceinline__ void sum_normalize(const float (&q)[R], const float (&gv)[R], float (&b)[R],
+                                              LaneGroup<WAVE, R>& G) {
+    float x[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] = q[r] * gv[r];
     const float s = G.all_sum(x);
-    const float sq = s * q;
-    return sq - x;
-}
-
-template <bool WAVE>
-__global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, const float* __restrict__ dQ,
-                                                   float* __restrict__ b, float* __restrict__ dU, int N, int L,
-                                                   int per) {
-    extern __shared__ float sh[];
-    LaneGroup<WAVE> G(L, per, sh);
-    const int p = G.item();
-    const bool active = G.on && p < N;
-    const size_t idx = (size_t)p * L + G.l;
-    const float q = active ? Q[idx] : 0.0f;
-    const float gv = active ? dQ[idx] : 0.0f;
-    const float bn = sum_normalize<WAVE>(q, gv, G);
-    if (active) {
-        b[idx] = bn;
-        dU[idx] = bn;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const float sq = s * q[r];
+        b[r] = sq - x[r];
     }
 }
 
-template <bool WAVE>
-__global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, float* __restrict__ b,
-                                                  float* __restrict__ dU, int N, int L, int per, SliceTable st) {
+template <bool WAVE, int R>
+__global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, const float* __restrict__ dQ,
+                                                   float* __restrict__ b, float* __restrict__ dU, int N, int L,
+                                                   int S, int per) {
     extern __shared__ float sh[];
-    LaneGroup<WAVE> G(L, per, sh);
+    LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
-    const int l = G.l;
-    const size_t idx = (size_t)p * L + l;
-    float gsum = 0.0f;
+    float q[R], gv[R], bn[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        q[r] = 0.0f;
+        gv[r] = 0.0f;
+    }
+    if (active) {
+        load_piece<R>(Q + (size_t)p * L, G.k0, L, q);
+        load_piece<R>(dQ + (size_t)p * L, G.k0, L, gv);
+    }
+    sum_normalize<WAVE, R>(q, gv, bn, G);
+    if (active) {
+        store_piece<R>(b + (size_t)p * L, G.k0, L, bn);
+        store_piece<R>(dU + (size_t)p * L, G.k0, L, bn);
+    }
+}
+
+template <bool WAVE, int R>
+__global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, float* __restrict__ b,
+                                                  float* __restrict__ dU, int N, int L, int S, int per,
+                                                  SliceTable st) {
+    extern __shared__ float sh[];
+    LaneGroup<WAVE, R> G(L, S, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int k0 = G.k0;
+    float gsum[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) gsum[r] = 0.0f;
 #pragma unroll
     for (int k = 0; k < DCRF_MAX_TERMS; k++) {
         if (k < st.nterms) {
             const SliceTerm& T = st.t[k];
-            float m = active ? slice_one(T, p, l, L) : 0.0f;
-            if (T.kw) m = m * T.kw[0];
-            const float c = compat_apply<WAVE>(T, m, G);  // C^T = C (W is symmetric)
-            gsum = gsum + c;
+            float m[R], c[R];
+            if (active) {
+                slice_piece<R>(T, p, k0, L, m);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = 0.0f;
+            }
+            if (T.kw) {
+                const float kw = T.kw[0];
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+            }
+            compat_piece<WAVE, R>(T, m, c, G);  // C^T = C (W is symmetric)
+#pragma unroll
+            for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[r];
         }
     }
-    const float q = active ? Q[idx] : 0.0f;
-    const float bn = sum_normalize<WAVE>(q, gsum, G);
+    float q[R], bn[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) q[r] = 0.0f;
+    if (active) load_piece<R>(Q + (size_t)p * L, k0, L, q);
+    sum_normalize<WAVE, R>(q, gsum, bn, G);
     if (active) {
-        b[idx] = bn;
-        dU[idx] = dU[idx] + bn;
+        float du[R];
+        load_piece<R>(dU + (size_t)p * L, k0, L, du);
+#pragma unroll
+        for (int r = 0; r < R; r++) du[r] = du[r] + bn[r];
+        store_piece<R>(b + (size_t)p * L, k0, L, bn);
+        store_piece<R>(dU + (size_t)p * L, k0, L, du);
     }
 }
 
-// partial[block][l] = sum over the block's pixels of b .* M for label l (groups stride
-// over the pixels; the per-label sums of the block's groups are added in group order).
-template <bool WAVE>
+// partial[block][label] = sum over the block's pixels of b .* M (groups stride over the
+// pixels; the block's groups are then added in group order per label).
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b, const float* __restrict__ M, int N,
-                                                    int L, int per, float* __restrict__ partial) {
-    extern __shared__ float sh[];
-    LaneGroup<WAVE> G(L, per, sh);
-    const int items = LaneGroup<WAVE>::items_per_block(per);
-    float acc = 0.0f;
+                                                    int L, int S, int per, float* __restrict__ partial) {
+    extern __shared__ float sh[];  // block * R floats
+    LaneGroup<WAVE, R> G(L, S, per, sh);
+    const int items = LaneGroup<WAVE, R>::items_per_block(per);
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
     if (G.on) {
         for (int p = G.item(); p < N; p += gridDim.x * items) {
-            const size_t idx = (size_t)p * L + G.l;
-            const float prod = b[idx] * M[idx];
-            acc = acc + prod;
+            float x[R], y[R];
+            load_piece<R>(b + (size_t)p * L, G.k0, L, x);
+            load_piece<R>(M + (size_t)p * L, G.k0, L, y);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float prod = x[r] * y[r];
+                acc[r] = acc[r] + prod;
+            }
         }
     }
     const int t = threadIdx.x;
-    sh[t] = G.on ? acc : 0.0f;
+#pragma unroll
+    for (int r = 0; r < R; r++) sh[t * R + r] = G.on ? acc[r] : 0.0f;
     __syncthreads();
     if (t < L) {
+        const int ln = t / R, rr = t - ln * R;  // lane and slot holding label t
         float s = 0.0f;
         for (int gi = 0; gi < items; gi++) {
-            const int slot = WAVE ? (gi / per) * 64 + (gi % per) * L : gi * L;
-            s = s + sh[slot + t];
+            const int lane0 = WAVE ? (gi / per) * 64 + (gi % per) * S : gi * S;
+            s = s + sh[(lane0 + ln) * R + rr];
         }
         partial[(size_t)blockIdx.x * L + t] = s;
     }
@@ -217,23 +259,33 @@ __global__ void k_set_compat(const float* __restrict__ raw, int ckind, int L, fl
 
 }  // namespace
 
-#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                  \
-    do {                                                                             \
-        if ((sh).wave)                                                               \
-            kernel<true><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);          \
-        else                                                                         \
-            kernel<false><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);         \
-        DCRF_HIP_CHECK(hipGetLastError());                                           \
+#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                     \
+    do {                                                                                \
+        const unsigned g_ = (grid);                                                     \
+        const size_t m_ = (shm);                                                        \
+        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                     \
+            case 3: kernel<true, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 2: kernel<false, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 5: kernel<true, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 4: kernel<false, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 7: kernel<true, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 6: kernel<false, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 9: kernel<true, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            default: kernel<false, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
+        }                                                                               \
+        DCRF_HIP_CHECK(hipGetLastError());                                              \
     } while (0)
 
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_sn_first, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, dQ, b, dU, N, L, sh.per);
+    DCRF_DISPATCH(sh, k_sn_first, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, dQ, b, dU, N, L, sh.S,
+                  sh.per);
 }
 
 void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, b, dU, N, L, sh.per, st);
+    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, b, dU, N, L, sh.S, sh.per,
+                  st);
 }
 
 int grad_partial_blocks(int N, int L) {
@@ -247,7 +299,7 @@ void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, f
     const int nb = grad_partial_blocks(N, L);
     if (!full) {
         const LaunchShape sh = shape_for(L);
-        DCRF_DISPATCH(sh, k_grad_diag, nb, sizeof(float) * sh.block, s, b, M, N, L, sh.per, partial);
+        DCRF_DISPATCH(sh, k_grad_diag, nb, sizeof(float) * sh.block * sh.R, s, b, M, N, L, sh.S, sh.per, partial);
         k_reduce_add<<<blocks_for(L), kThreads, 0, s>>>(partial, nb, L, L, acc, 0);
         DCRF_HIP_CHECK(hipGetLastError());
         return;

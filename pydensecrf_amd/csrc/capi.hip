@@ -1092,6 +1092,14 @@ int dcrf_set_splat_chunk(dcrf_model* m, int chunk) {
     });
 }
 
+int dcrf_set_hash_load(dcrf_model* m, float load) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(load > 0.0f && load <= 0.5f, "hash load factor must be in (0, 0.5]");
+        m->hash_load = load;
+    });
+}
+
 int dcrf_enable_timing(dcrf_model* m, int on) {
     return guard([&] {
         require(m != nullptr, "null model");

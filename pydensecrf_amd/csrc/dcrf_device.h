@@ -1,15 +1,19 @@
 // Device helpers shared by the lattice-filter, mean-field and adjoint kernels.
 //
-// Lane groups.  A group of L lanes owns one item (a splat chunk, a vertex, a pixel);
-// lane l holds label l, so a row of L floats is one contiguous segment.
-//   WAVE = true  (L <= 64): G = 64 / L groups per 64-wide wave, the block is 4 waves.
-//                 A group never spans waves; group-wide max / sum / peer reads are
-//                 ds_bpermute shuffles (no block barrier).  Lanes G*L..63 are idle.
-//   WAVE = false (L > 64): block / L groups per block of 256, 512 or 1024 threads;
+// Lane groups.  A group of S = ceil(L / R) lanes owns one item (a splat chunk, a
+// vertex, a pixel); lane l owns the R consecutive labels [l*R, l*R + R) of the item's
+// row (the last lane may own fewer), so a row of L floats is read and written as S
+// contiguous R-float pieces (dwordx R accesses).  R is chosen per L (choose_labels_per_lane)
+// to waste as few label slots as possible; per-item work (addresses, reductions, exp)
+// is shared by R labels.
+//   WAVE = true  (S <= 64): 64 / S groups per 64-wide wave, the block is 4 waves.  A
+//                 group never spans waves; group-wide max / sum / peer reads are
+//                 ds_bpermute shuffles with no block barrier.
+//   WAVE = false (S > 64): block / S groups per block of 256, 512 or 1024 threads;
 //                 group-wide operations go through LDS with __syncthreads().
 // Every group-wide operation must be reached by all threads of the block.
-// Reductions have a fixed order (a pairwise tree in wave mode, label order in block
-// mode), so results do not vary from run to run.
+// Reductions have a fixed order (the lane's labels in order, then a pairwise tree over
+// lanes in wave mode or lane order in block mode): results do not vary between runs.
 #pragma once
 
 #include "dcrf_internal.h"
@@ -32,133 +36,189 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
-template <bool WAVE>
-struct LaneGroup {
-    int L, g, l, base, t, per;
-    bool on;
-    float staged;
-    float* sh;
+// R consecutive floats at 4-byte alignment: one global_load/store_dwordxR.
+template <int R>
+struct __attribute__((aligned(4))) Piece {
+    float v[R];
+};
 
-    // per: groups per wave (WAVE) or per block; sh: block-shared floats (block mode)
-    __device__ __forceinline__ LaneGroup(int L_, int per_, float* sh_) : L(L_), per(per_), sh(sh_) {
+// Labels k0..k0+R-1 of a row (zeros past L).
+template <int R>
+__device__ __forceinline__ void load_piece(const float* __restrict__ row, int k0, int L, float (&x)[R]) {
+    if (k0 + R <= L) {
+        const Piece<R> p = *reinterpret_cast<const Piece<R>*>(row + k0);
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = p.v[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = k0 + r < L ? row[k0 + r] : 0.0f;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void store_piece(float* __restrict__ row, int k0, int L, const float (&x)[R]) {
+    if (k0 + R <= L) {
+        Piece<R> p;
+#pragma unroll
+        for (int r = 0; r < R; r++) p.v[r] = x[r];
+        *reinterpret_cast<Piece<R>*>(row + k0) = p;
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (k0 + r < L) row[k0 + r] = x[r];
+    }
+}
+
+template <bool WAVE, int R>
+struct LaneGroup {
+    int L, S, g, l, k0, base, t, per;
+    bool on;
+    float* sh;  // block mode: block * R floats
+
+    // per: groups per wave (WAVE) or per block
+    __device__ __forceinline__ LaneGroup(int L_, int S_, int per_, float* sh_) : L(L_), S(S_), per(per_), sh(sh_) {
         t = threadIdx.x;
         const int lane = WAVE ? (t & 63) : t;
-        g = lane / L;
-        l = lane - g * L;
+        g = lane / S;
+        l = lane - g * S;
         on = g < per;
         if (!on) {  // idle lanes shadow group 0 so that every shuffle index stays valid
             g = 0;
-            l = l < L ? l : 0;
+            l = l < S ? l : 0;
         }
-        base = WAVE ? (t & ~63) + g * L : g * L;
-        staged = 0.0f;
+        k0 = l * R;
+        base = WAVE ? (t & ~63) + g * S : g * S;
     }
     // Index of this group's item.
     __device__ __forceinline__ int item() const {
         const int b = xcd_block(blockIdx.x, gridDim.x);
         return WAVE ? (b * (kWaveBlock / 64) + (t >> 6)) * per + g : b * per + g;
     }
-    // Items per block.
     __device__ static __forceinline__ int items_per_block(int per) { return WAVE ? (kWaveBlock / 64) * per : per; }
+    __device__ __forceinline__ bool has(int r) const { return k0 + r < L; }
 
-    __device__ __forceinline__ void stage(float x) {
-        if (WAVE) {
-            staged = x;
-        } else {
-            sh[t] = x;
-            __syncthreads();
-        }
-    }
-    // Staged value of lane b of this group.
-    __device__ __forceinline__ float peer(int b) const {
-        if (WAVE) return __shfl(staged, (base + b) & 63);
-        return sh[base + b];
-    }
-    __device__ __forceinline__ void unstage() {
-        if (!WAVE) __syncthreads();
+    // Value of lane s of this group (all lanes must call).
+    __device__ __forceinline__ float lane_value(float x, int s) {
+        if (WAVE) return __shfl(x, (base + s) & 63);
+        sh[t] = x;
+        __syncthreads();
+        const float v = sh[base + s];
+        __syncthreads();
+        return v;
     }
 
-    __device__ __forceinline__ float all_max(float x) {
+    __device__ __forceinline__ float all_max(const float (&x)[R]) {
+        float m = x[0];
+#pragma unroll
+        for (int r = 1; r < R; r++)
+            if (has(r)) m = fmaxf(m, x[r]);
         if (WAVE) {
-            // ring doubling: after the step with offset k a lane holds the max of 2k
-            // consecutive lanes (cyclically); max is idempotent, so overlap is harmless
-            for (int k = 1; k < L; k <<= 1) {
+            // ring doubling over the S lanes; max is idempotent, so overlap is harmless
+            for (int k = 1; k < S; k <<= 1) {
                 int s = l + k;
-                if (s >= L) s -= L;
-                x = fmaxf(x, __shfl(x, (base + s) & 63));
+                if (s >= S) s -= S;
+                m = fmaxf(m, __shfl(m, (base + s) & 63));
             }
-            return x;
+            return m;
         }
-        stage(x);
-        float m = peer(0);
-        for (int b = 1; b < L; b++) m = fmaxf(m, peer(b));
-        unstage();
-        return m;
+        sh[t] = m;
+        __syncthreads();
+        float v = sh[base];
+        for (int s = 1; s < S; s++) v = fmaxf(v, sh[base + s]);
+        __syncthreads();
+        return v;
     }
 
-    __device__ __forceinline__ float all_sum(float x) {
+    __device__ __forceinline__ float all_sum(const float (&x)[R]) {
+        float a = x[0];
+#pragma unroll
+        for (int r = 1; r < R; r++)
+            if (has(r)) a = a + x[r];
         if (WAVE) {
-            // pairwise tree towards lane 0 of the group, then a broadcast
-            for (int k = 1; k < L; k <<= 1) {
-                const float y = __shfl(x, (base + l + k) & 63);
-                if ((l & (2 * k - 1)) == 0 && l + k < L) x = x + y;
+            // pairwise tree over the S lanes towards lane 0 of the group, then a broadcast
+            for (int k = 1; k < S; k <<= 1) {
+                const float y = __shfl(a, (base + l + k) & 63);
+                if ((l & (2 * k - 1)) == 0 && l + k < S) a = a + y;
             }
-            return __shfl(x, base & 63);
+            return __shfl(a, base & 63);
         }
-        stage(x);
-        float s = 0.0f;
-        for (int b = 0; b < L; b++) s = s + peer(b);
-        unstage();
-        return s;
+        sh[t] = a;
+        __syncthreads();
+        float v = 0.0f;
+        for (int s = 0; s < S; s++) v = v + sh[base + s];
+        __syncthreads();
+        return v;
     }
 };
 
-// Slice of term T for pixel p, label l: sum_j sw_j * vals[vtx_j][l] in j order.  The
-// (vertex, weight) pairs are loaded first, then all vertex rows, so the d+1 gathers
-// are in flight together.
-__device__ __forceinline__ float slice_one(const SliceTerm& T, int p, int l, int L) {
+// Slice of term T for pixel p: m[r] = sum_j sw_j * vals[vtx_j][k0 + r] in j order.  The
+// (vertex, weight) pairs are loaded first, then all vertex rows, so the d+1 gathers are
+// in flight together.
+template <int R>
+__device__ __forceinline__ void slice_piece(const SliceTerm& T, int p, int k0, int L, float (&m)[R]) {
     const int d1 = T.d1;
     const int* vt = T.vtx + (size_t)p * d1;
     const float* sw = T.sw + (size_t)p * d1;
     int v[kMaxD1];
-    float w[kMaxD1], x[kMaxD1];
+    float w[kMaxD1];
 #pragma unroll
     for (int j = 0; j < kMaxD1; j++)
         if (j < d1) {
             v[j] = vt[j];
             w[j] = sw[j];
         }
+    float x[kMaxD1][R];
 #pragma unroll
     for (int j = 0; j < kMaxD1; j++)
-        if (j < d1) x[j] = T.vals[(size_t)v[j] * L + l];
-    float m = 0.0f;
+        if (j < d1) load_piece<R>(T.vals + (size_t)v[j] * L, k0, L, x[j]);
+#pragma unroll
+    for (int r = 0; r < R; r++) m[r] = 0.0f;
 #pragma unroll
     for (int j = 0; j < kMaxD1; j++)
         if (j < d1) {
-            const float prod = w[j] * x[j];
-            m = m + prod;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float prod = w[j] * x[j][r];
+                m[r] = m[r] + prod;
+            }
         }
-    return m;
 }
 
-// Compatibility transform of one message value (labelcompatibility.cpp:46-48, 66-69,
-// 82-84).  Matrix reads the pixel's whole message vector from its group.
-template <bool WAVE>
-__device__ __forceinline__ float compat_apply(const SliceTerm& T, float m, LaneGroup<WAVE>& G) {
+// Compatibility transform of the lane's message values (labelcompatibility.cpp:46-48,
+// 66-69, 82-84).  Matrix reads the pixel's whole message vector from its group, in
+// label order.  All lanes must call (Matrix synchronises in block mode).
+template <bool WAVE, int R>
+__device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m)[R], float (&c)[R],
+                                             LaneGroup<WAVE, R>& G) {
     if (T.ckind == DCRF_COMPAT_POTTS) {
         const float nw = -T.cw[0];
-        return nw * m;
+#pragma unroll
+        for (int r = 0; r < R; r++) c[r] = nw * m[r];
+        return;
     }
-    if (T.ckind == DCRF_COMPAT_DIAGONAL) return T.cw[G.l] * m;
-    G.stage(m);
-    const float* row = T.cw + (size_t)G.l * G.L;
-    float c = 0.0f;
-    for (int b = 0; b < G.L; b++) {
-        const float prod = row[b] * G.peer(b);
-        c = c + prod;
+    if (T.ckind == DCRF_COMPAT_DIAGONAL) {
+#pragma unroll
+        for (int r = 0; r < R; r++) c[r] = G.has(r) ? T.cw[G.k0 + r] * m[r] : 0.0f;
+        return;
     }
-    G.unstage();
-    return c;
+    const int L = G.L;
+#pragma unroll
+    for (int r = 0; r < R; r++) c[r] = 0.0f;
+    for (int s = 0; s < G.S; s++) {
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+            const float mb = G.lane_value(m[rr], s);
+            const int b = s * R + rr;
+            if (b < L) {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (G.has(r)) {
+                        const float prod = T.cw[(size_t)(G.k0 + r) * L + b] * mb;
+                        c[r] = c[r] + prod;
+                    }
+            }
+        }
+    }
 }
 
 }  // namespace dcrf

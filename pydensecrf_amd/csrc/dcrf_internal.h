@@ -174,9 +174,12 @@ struct SliceTable {
 struct LaunchShape {
     int block;  // threads per block
     int per;    // groups per wave (wave mode) or per block (block mode)
-    bool wave;  // L <= 64: groups live inside one wave
+    bool wave;  // S <= 64: groups live inside one wave
     int items;  // items per block
+    int R;      // labels per lane (1..4)
+    int S;      // lanes per item, ceil(L / R)
 };
+int choose_labels_per_lane(int L);
 LaunchShape shape_for(int L);
 
 void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w,

@@ -26,10 +26,27 @@
 
 namespace dcrf {
 
+// Labels per lane: one lane per item for L <= 4; otherwise the R in {4, 3, 2} that
+// wastes the fewest label slots (ties go to the larger R).
+int choose_labels_per_lane(int L) {
+    if (L <= 4) return L;
+    int best = 4, waste = ((L + 3) / 4) * 4 - L;
+    for (int R = 3; R >= 2; R--) {
+        const int w = ((L + R - 1) / R) * R - L;
+        if (w < waste) {
+            best = R;
+            waste = w;
+        }
+    }
+    return best;
+}
+
 LaunchShape shape_for(int L) {
-    if (L <= 64) return LaunchShape{kWaveBlock, 64 / L, true, (kWaveBlock / 64) * (64 / L)};
-    const int block = L <= 256 ? 256 : (L <= 512 ? 512 : 1024);
-    return LaunchShape{block, block / L, false, block / L};
+    const int R = choose_labels_per_lane(L);
+    const int S = (L + R - 1) / R;
+    if (S <= 64) return LaunchShape{kWaveBlock, 64 / S, true, (kWaveBlock / 64) * (64 / S), R, S};
+    const int block = S <= 256 ? 256 : (S <= 512 ? 512 : 1024);
+    return LaunchShape{block, block / S, false, block / S, R, S};
 }
 
 namespace {
@@ -44,70 +61,107 @@ __global__ void k_fill(float* p, float v, size_t n) {
 }
 
 // Splat: each group sums one chunk (<= C contributors of a vertex, pixel ascending)
-// sequentially; (pixel, weight) pairs and input rows are fetched kBatch at a time.
-template <bool WAVE>
+// sequentially; (pixel, weight) pairs and input pieces are fetched kBatch at a time.
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks, int n,
                                                 const int* __restrict__ pt, const float* __restrict__ w,
                                                 const float* __restrict__ in, float* __restrict__ vals, int L,
-                                                int per) {
-    LaneGroup<WAVE> G(L, per, nullptr);
+                                                int S, int per) {
+    LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
     if (!G.on || c >= n) return;
-    const int l = G.l;
+    const int k0 = G.k0;
     const Chunk ch = chunks[c];
-    float acc = 0.0f;
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
     int i = ch.beg;
     for (; i + kBatch <= ch.end; i += kBatch) {
         int pp[kBatch];
-        float ww[kBatch], xx[kBatch];
+        float ww[kBatch], xx[kBatch][R];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
             pp[u] = pt[i + u];
             ww[u] = w[i + u];
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) xx[u] = in[(size_t)pp[u] * L + l];
+        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)pp[u] * L, k0, L, xx[u]);
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float prod = ww[u] * xx[u][r];
+                acc[r] = acc[r] + prod;
+            }
+    }
+    if (i < ch.end) {  // the tail as one batch: every load in flight, missing slots skipped
+        int pp[kBatch];
+        float ww[kBatch], xx[kBatch][R];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
-            const float prod = ww[u] * xx[u];
-            acc = acc + prod;
+            const int j = i + u < ch.end ? i + u : ch.end - 1;
+            pp[u] = pt[j];
+            ww[u] = w[j];
         }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)pp[u] * L, k0, L, xx[u]);
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (i + u < ch.end) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float prod = ww[u] * xx[u][r];
+                    acc[r] = acc[r] + prod;
+                }
+            }
     }
-    for (; i < ch.end; i++) {
-        const float prod = w[i] * in[(size_t)pt[i] * L + l];
-        acc = acc + prod;
-    }
-    vals[(size_t)ch.out * L + l] = acc;
+    store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
 }
 
-template <bool WAVE>
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb, int n, float* __restrict__ vals,
-                                                  int L, int per) {
-    LaneGroup<WAVE> G(L, per, nullptr);
+                                                  int L, int S, int per) {
+    LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
     if (!G.on || c >= n) return;
-    const int l = G.l;
-    const Combine r = cb[c];
-    float acc = 0.0f;
+    const int k0 = G.k0;
+    const Combine rc = cb[c];
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
     int i = 0;
-    for (; i + kBatch <= r.count; i += kBatch) {
-        float xx[kBatch];
+    for (; i + kBatch <= rc.count; i += kBatch) {
+        float xx[kBatch][R];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) xx[u] = vals[(size_t)(r.first + i + u) * L + l];
+        for (int u = 0; u < kBatch; u++) load_piece<R>(vals + (size_t)(rc.first + i + u) * L, k0, L, xx[u]);
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) acc = acc + xx[u];
+        for (int u = 0; u < kBatch; u++)
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r] = acc[r] + xx[u][r];
     }
-    for (; i < r.count; i++) acc = acc + vals[(size_t)(r.first + i) * L + l];
-    vals[(size_t)r.v * L + l] = acc;
+    if (i < rc.count) {
+        float xx[kBatch][R];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int j = i + u < rc.count ? i + u : rc.count - 1;
+            load_piece<R>(vals + (size_t)(rc.first + j) * L, k0, L, xx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (i + u < rc.count)
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[r] = acc[r] + xx[u][r];
+    }
+    store_piece<R>(vals + (size_t)rc.v * L, k0, L, acc);
 }
 
-template <bool WAVE>
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, float* __restrict__ dst, int Vp,
-                                               int L, int per, BlurTable bt, int step, int reverse) {
-    LaneGroup<WAVE> G(L, per, nullptr);
+                                               int L, int S, int per, BlurTable bt, int step, int reverse) {
+    LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int v = G.item();
     if (!G.on || v >= Vp) return;
-    const int l = G.l;
+    const int k0 = G.k0;
     // terms are ordered by vbase; pick the last one starting at or before v
     int vb = bt.t[0].vbase, V = bt.t[0].V, d = bt.t[0].d;
     const int2* nbr = bt.t[0].nbr;
@@ -121,97 +175,165 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
         }
     }
     const int j = reverse ? d - step : step;
-    const float o = src[(size_t)v * L + l];
+    float o[R], a[R], b[R], out[R];
+    load_piece<R>(src + (size_t)v * L, k0, L, o);
     const int2 nb = nbr[(size_t)j * V + (v - vb)];
-    const float a = src[(size_t)nb.x * L + l];
-    const float b = src[(size_t)nb.y * L + l];
-    const float s2 = a + b;
-    const float h = 0.5f * s2;
-    dst[(size_t)v * L + l] = o + h;
+    load_piece<R>(src + (size_t)nb.x * L, k0, L, a);
+    load_piece<R>(src + (size_t)nb.y * L, k0, L, b);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const float s2 = a[r] + b[r];
+        const float h = 0.5f * s2;
+        out[r] = o[r] + h;
+    }
+    store_piece<R>(dst + (size_t)v * L, k0, L, out);
 }
 
 // Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
 // (densecrf.cpp:122-129, with the per-term weight w_k = 1 unless set).
-template <bool WAVE>
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U, float* __restrict__ Q, int N,
-                                                    int L, int per, SliceTable st, float* __restrict__ tmp1,
+                                                    int L, int S, int per, SliceTable st, float* __restrict__ tmp1,
                                                     float* __restrict__ tmp2) {
     extern __shared__ float sh[];
-    LaneGroup<WAVE> G(L, per, sh);
+    LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
-    const int l = G.l;
-    const size_t idx = (size_t)p * L + l;
-    float e = 0.0f;
-    if (active && U) e = -U[idx];
-    float last = 0.0f;
+    const int k0 = G.k0;
+    float e[R], last[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        e[r] = 0.0f;
+        last[r] = 0.0f;
+    }
+    if (active && U) {
+        load_piece<R>(U + (size_t)p * L, k0, L, e);
+#pragma unroll
+        for (int r = 0; r < R; r++) e[r] = -e[r];
+    }
 #pragma unroll
     for (int k = 0; k < DCRF_MAX_TERMS; k++) {
         if (k < st.nterms) {
             const SliceTerm& T = st.t[k];
-            float m = active ? slice_one(T, p, l, L) : 0.0f;
-            if (active && T.mout) T.mout[idx] = m;
-            if (T.kw) m = m * T.kw[0];
-            const float c = compat_apply<WAVE>(T, m, G);
-            e = e - c;
-            last = c;
+            float m[R], c[R];
+            if (active) {
+                slice_piece<R>(T, p, k0, L, m);
+                if (T.mout) store_piece<R>(T.mout + (size_t)p * L, k0, L, m);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = 0.0f;
+            }
+            if (T.kw) {
+                const float kw = T.kw[0];
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+            }
+            compat_piece<WAVE, R>(T, m, c, G);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                e[r] = e[r] - c[r];
+                last[r] = c[r];
+            }
         }
     }
     const float mx = G.all_max(e);
-    const float x = expf(e - mx);
+    float x[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] = expf(e[r] - mx);
     const float s = G.all_sum(x);
     if (active) {
-        Q[idx] = x / s;
-        if (tmp1) tmp1[idx] = e;
-        if (tmp2) tmp2[idx] = last;
+        float q[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) q[r] = x[r] / s;
+        store_piece<R>(Q + (size_t)p * L, k0, L, q);
+        if (tmp1) store_piece<R>(tmp1 + (size_t)p * L, k0, L, e);
+        if (tmp2) store_piece<R>(tmp2 + (size_t)p * L, k0, L, last);
     }
 }
 
 // out = compat_which(slice_which) (ckind < 0: the plain slice).
-template <bool WAVE>
-__global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int per, SliceTable st,
-                                                     int which) {
+template <bool WAVE, int R>
+__global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int S, int per,
+                                                     SliceTable st, int which) {
     extern __shared__ float sh[];
-    LaneGroup<WAVE> G(L, per, sh);
+    LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
     SliceTerm T = st.t[0];
 #pragma unroll
     for (int k = 1; k < DCRF_MAX_TERMS; k++)
         if (k == which) T = st.t[k];
-    float m = active ? slice_one(T, p, G.l, L) : 0.0f;
-    if (T.kw) m = m * T.kw[0];
-    if (T.ckind >= 0) m = compat_apply<WAVE>(T, m, G);
-    if (active) out[(size_t)p * L + G.l] = m;
+    float m[R], c[R];
+    if (active) {
+        slice_piece<R>(T, p, G.k0, L, m);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = 0.0f;
+    }
+    if (T.kw) {
+        const float kw = T.kw[0];
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+    }
+    if (T.ckind >= 0) {
+        compat_piece<WAVE, R>(T, m, c, G);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) c[r] = m[r];
+    }
+    if (active) store_piece<R>(out + (size_t)p * L, G.k0, L, c);
 }
 
-// KL divergence terms per (pixel, label), reduced per block in double in a fixed order
+// KL divergence terms per pixel, reduced per block in double in a fixed order
 // (densecrf.cpp:214-235: q log max(q,1e-20) + u q + sum_k q * pairwise_k(Q)).
-template <bool WAVE>
+template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_kl(const float* __restrict__ Q, const float* __restrict__ U, int N, int L,
-                                             int per, SliceTable st, double* __restrict__ partial) {
+                                             int S, int per, SliceTable st, double* __restrict__ partial) {
     extern __shared__ float sh[];
     __shared__ double red[1024];
-    LaneGroup<WAVE> G(L, per, sh);
+    LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
-    const int l = G.l;
-    const size_t idx = (size_t)p * L + l;
-    const float q = active ? Q[idx] : 0.0f;
+    const int k0 = G.k0;
+    float q[R], u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        q[r] = 0.0f;
+        u[r] = 0.0f;
+    }
     double acc = 0.0;
     if (active) {
-        const float qc = q > 1e-20f ? q : 1e-20f;
-        acc += (double)q * log((double)qc);
-        if (U) acc += (double)(U[idx] * q);
+        load_piece<R>(Q + (size_t)p * L, k0, L, q);
+        if (U) load_piece<R>(U + (size_t)p * L, k0, L, u);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (G.has(r)) {
+                const float qc = q[r] > 1e-20f ? q[r] : 1e-20f;
+                acc += (double)q[r] * log((double)qc);
+                if (U) acc += (double)(u[r] * q[r]);
+            }
     }
 #pragma unroll
     for (int k = 0; k < DCRF_MAX_TERMS; k++) {
         if (k < st.nterms) {
             const SliceTerm& T = st.t[k];
-            float m = active ? slice_one(T, p, l, L) : 0.0f;
-            if (T.kw) m = m * T.kw[0];
-            const float c = compat_apply<WAVE>(T, m, G);
-            if (active) acc += (double)(q * c);
+            float m[R], c[R];
+            if (active) {
+                slice_piece<R>(T, p, k0, L, m);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = 0.0f;
+            }
+            if (T.kw) {
+                const float kw = T.kw[0];
+#pragma unroll
+                for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+            }
+            compat_piece<WAVE, R>(T, m, c, G);
+            if (active)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (G.has(r)) acc += (double)(q[r] * c[r]);
         }
     }
     const int t = threadIdx.x;
@@ -317,41 +439,50 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
-#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                  \
-    do {                                                                             \
-        if ((sh).wave)                                                               \
-            kernel<true><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);          \
-        else                                                                         \
-            kernel<false><<<(grid), (sh).block, (shm), (s)>>>(__VA_ARGS__);         \
-        DCRF_HIP_CHECK(hipGetLastError());                                           \
+#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                     \
+    do {                                                                                \
+        const unsigned g_ = (grid);                                                     \
+        const size_t m_ = (shm);                                                        \
+        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                     \
+            case 3: kernel<true, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 2: kernel<false, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 5: kernel<true, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 4: kernel<false, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 7: kernel<true, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 6: kernel<false, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 9: kernel<true, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            default: kernel<false, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
+        }                                                                               \
+        DCRF_HIP_CHECK(hipGetLastError());                                              \
     } while (0)
+
 
 void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w, const float* in,
                   float* vals, int L, hipStream_t s) {
     if (!n_chunks) return;
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), 0, s, chunks, n_chunks, csr_pt, csr_w, in, vals, L,
-                  sh.per);
+                  sh.S, sh.per);
 }
 
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s) {
     if (!n) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, L, sh.per);
+    DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, L, sh.S, sh.per);
 }
 
 void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step, int reverse,
                  hipStream_t s) {
     if (!Vp) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_blur, blocks_for(Vp, sh.items), 0, s, src, dst, Vp, L, sh.per, bt, step, reverse);
+    DCRF_DISPATCH(sh, k_blur, blocks_for(Vp, sh.items), 0, s, src, dst, Vp, L, sh.S, sh.per, bt, step, reverse);
 }
 
 void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1, float* tmp2,
                       hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * sh.block, s, U, Q, N, L, sh.per, st,
-                  tmp1, tmp2);
+    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * sh.block, s, U, Q, N, L, sh.S, sh.per,
+                  st, tmp1, tmp2);
 }
 
 void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
@@ -362,15 +493,15 @@ void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
 
 void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * sh.block, s, out, N, L, sh.per, st,
-                  which);
+    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * sh.block, s, out, N, L, sh.S, sh.per,
+                  st, which);
 }
 
 int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st, double* partials,
               hipStream_t s) {
     const LaunchShape sh = shape_for(L);
     const int nb = (int)blocks_for(N, sh.items);
-    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * sh.block, s, Q, U, N, L, sh.per, st, partials);
+    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * sh.block, s, Q, U, N, L, sh.S, sh.per, st, partials);
     return nb;
 }
 

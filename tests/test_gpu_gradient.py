@@ -191,3 +191,23 @@ def test_autograd_repeatable_and_guarded():
     layer(Ut)  # a second forward invalidates the first graph's checkpoints
     with pytest.raises(RuntimeError):
         Q1.sum().backward()
+
+
+@pytest.mark.parametrize("L", [4, 7, 130])
+def test_gradient_label_shapes(L):
+    W, H, n = 19, 13, 3
+    img, U, gt = image_case(W, H, L, seed=L)
+    rng = np.random.default_rng(L)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = rng.uniform(0.5, 2, L).astype(np.float32)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, c1)
+        m.add_bilateral(10.0, 10.0, 13.0, 13.0, 13.0, img, c2)
+    obj, ug, cg, _ = g.gradient(n, gt)
+    r, ugo, cgo = o.gradient(n, gt)
+    assert abs(obj - r) <= 1e-5 * abs(r)
+    assert _rel_close(ug, ugo, 1e-4)
+    assert _rel_close(cg, cgo, 1e-4)

@@ -153,3 +153,36 @@ def test_splat_chunk_invariance(chunk):
     g, o = build_both(W, H, L, U, _std_terms(img))
     g.set_splat_chunk(chunk)
     assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL
+
+
+@pytest.mark.parametrize("L", [4, 7, 19, 64, 65, 130, 300])
+@pytest.mark.parametrize("ck", ["potts", "matrix"])
+def test_label_count_shapes(L, ck):
+    """Labels per lane and group layouts: L = 4 (one lane per pixel), 7 / 19 (a partial
+    last lane), 64, 65, 130 and 300 (groups wider than a wave, LDS path)."""
+    W, H = 23, 17
+    img, U, _ = image_case(W, H, L, seed=L + 100)
+    if ck == "potts":
+        c1, c2 = 3.0, 10.0
+    else:
+        rng = np.random.default_rng(L)
+        c1 = (-2 * np.eye(L) + rng.normal(0, 0.05, (L, L))).astype(np.float32)
+        c2 = (-4 * np.eye(L) + rng.normal(0, 0.05, (L, L))).astype(np.float32)
+    g, o = build_both(W, H, L, U, [("gauss", 3.0, 3.0, c1, SYM), ("bilat", 20.0, 20.0, 13., 13., 13., img, c2, SYM)])
+    assert np.abs(g.inference(4) - o.inference(4)).max() <= Q_TOL
+    Q = o.start()
+    assert abs(g.kl(Q) - o.kl(Q)) <= 1e-5 * max(1.0, abs(o.kl(Q)))
+    np.testing.assert_allclose(g.apply(1, Q, True), o.apply(1, Q, True), rtol=1e-5, atol=1e-5)
+
+
+def test_autotuned_chunk_keeps_results(tmp_path, monkeypatch):
+    from pydensecrf_amd import tune
+    monkeypatch.setenv("PYDENSECRF_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    W, H, L = 96, 72, 21
+    img, U, _ = image_case(W, H, L, seed=77)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    best, times = tune.tune_splat_chunk(g, iters=2, reps=2)
+    assert best in tune.CHUNKS and set(times) == set(tune.CHUNKS)
+    assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL
+    again, t2 = tune.tune_splat_chunk(g)
+    assert again == best and t2 is None  # cached
