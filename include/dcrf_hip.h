@@ -183,6 +183,12 @@ int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int
 /* Set the chunk length of the splat's segmented reduction (0 = the default, 64):
  * vertices with more contributors are summed in pieces that are recombined in order. */
 int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
+/* Side T of the splat tiles (default 32; call before adding terms).  Pixels are
+ * grouped into tiles (T x T squares of a 2-D image, runs of T*T pixels otherwise); a
+ * vertex with more contributors than the splat chunk is cut at tile boundaries, and
+ * the splat processes its segments tile by tile, so the input rows of a tile are
+ * fetched from memory once.  No counterpart in the reference (a scheduling knob). */
+int dcrf_set_splat_tile(dcrf_model *m, int side);
 /* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
  * (default 0.5): lower trades memory for shorter probe sequences. */
 int dcrf_set_hash_load(dcrf_model *m, float load);
@@ -200,6 +206,10 @@ int dcrf_kernel_stats(dcrf_model *m, int kernel, long long *launches, double *ms
 /* Algorithmic bytes of one mean-field iteration (Q/U streams, lattice entry arrays,
  * normalisers, vertex rows, neighbour tables). */
 int dcrf_iteration_bytes(const dcrf_model *m, double *bytes);
+/* Splat layout of the model (built on first use): whether the LDS-tiled splat runs
+ * (else the global form), the tile side, tile count, splat segments and partial rows.
+ * No counterpart in the reference (introspection for tests and benchmarks). */
+int dcrf_splat_layout(dcrf_model *m, int *tiled, int *tile_side, int *n_tiles, int *segments, int *partials);
 
 #ifdef __cplusplus
 }

@@ -67,9 +67,11 @@ SIGNATURES = {
     "dcrf_term_instance_vertices": (_c, [_P, _c, _c, _I]),
     "dcrf_set_splat_chunk": (_c, [_P, _c]),
     "dcrf_set_hash_load": (_c, [_P, _f]),
+    "dcrf_set_splat_tile": (_c, [_P, _c]),
     "dcrf_enable_timing": (_c, [_P, _c]),
     "dcrf_kernel_stats": (_c, [_P, _c, _LL, _D, _D]),
     "dcrf_iteration_bytes": (_c, [_P, _D]),
+    "dcrf_splat_layout": (_c, [_P, _I, _I, _I, _I, _I]),
 }
 
 _lib = None
@@ -333,6 +335,9 @@ class Model:
     def set_hash_load(self, load):
         check(lib().dcrf_set_hash_load(self.h, load))
 
+    def set_splat_tile(self, side):
+        check(lib().dcrf_set_splat_tile(self.h, side))
+
     def enable_timing(self, on=True):
         check(lib().dcrf_enable_timing(self.h, int(bool(on))))
 
@@ -346,6 +351,12 @@ class Model:
             check(lib().dcrf_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
             out[name] = (n.value, ms.value, b.value)
         return out
+
+    def splat_layout(self):
+        """{tiled, tile_side, n_tiles, segments, partials} of the splat layout."""
+        v = [ctypes.c_int() for _ in range(5)]
+        check(lib().dcrf_splat_layout(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("tiled", "tile_side", "n_tiles", "segments", "partials"), [x.value for x in v]))
 
     def iteration_bytes(self):
         b = ctypes.c_double()

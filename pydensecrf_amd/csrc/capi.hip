@@ -63,6 +63,9 @@ struct dcrf_model {
     DevBuf<float> Q, work, tmp1, tmp2, stage;
     float hash_load = 0.5f;
     int splat_chunk = 0;  // 0: default
+    int tile_side = 0;  // 0: default_tile_side(L)
+    TileMap tiles;
+    bool tiles_ready = false;
 
     // global filter layout
     bool layout_valid = false;
@@ -72,6 +75,9 @@ struct dcrf_model {
     DevBuf<float> vals[2];
     DevBuf<Chunk> g_chunks;
     DevBuf<Combine> g_comb;
+    DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
+    bool tiled = false;                // splat through LDS tiles (else the global form)
+    TiledSplat tsplat{};
     DevBuf<int> g_pt;
     DevBuf<float> g_wF, g_wT;
     bool all_symmetric = true;
@@ -233,12 +239,31 @@ inline bool post_norm(int ntype, bool transpose) {
 // Alpha of the slice (permutohedral.cpp:571): 1 / (1 + 2^-d) in float.
 inline float slice_alpha(int d) { return 1.0f / (1.0f + std::pow(2.0f, (float)-d)); }
 
+TileGeom tile_geom(const dcrf_model& m) {
+    return TileGeom{m.tiles.base_dev.get(), m.geod.pix_base.get(), m.geo.is2d ? m.geod.W.get() : nullptr,
+                    m.geo.n_inst, m.geo.is2d ? 1 : 0, m.tiles.side};
+}
+
+// Most entries and segments of one tile, from first-of-tile tables.
+int max_run(const std::vector<int>& first) {
+    int mx = 0;
+    for (size_t t = 0; t + 1 < first.size(); t++) mx = std::max(mx, first[t + 1] - first[t]);
+    return mx;
+}
+
+template <typename T>
+void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t s) {
+    dst.reserve(src.size());
+    if (!src.empty())
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
+}
+
 // norm = K*1 for one term (pairwise.cpp:44), then the normalisation of :46-61.
 void compute_norm(dcrf_model& m, Term& t) {
     TermLattice& lat = t.lat;
     const int N = m.N(), d1 = lat.d + 1, V = lat.V, E = N * d1;
     hipStream_t s = m.stream;
-    DevBuf<int> pt;
+    DevBuf<int> pt, tseg, te;
     DevBuf<float> w, sw, vals0, vals1, ones;
     DevBuf<Chunk> ch;
     DevBuf<Combine> cb;
@@ -246,22 +271,35 @@ void compute_norm(dcrf_model& m, Term& t) {
     pt.reserve(E);
     w.reserve(E);
     sw.reserve(E);
-    launch_csr_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, pt.get(), w.get(), s);
+    // one term: entry positions are the sorted positions; rows V+1.. hold the partials
+    std::vector<Chunk> hch = lat.chunks;
+    for (auto& c : hch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
+    std::vector<Combine> hcb = lat.combines;
+    for (auto& c : hcb) c.first += V + 1;
+    upload(ch, hch, s);
+    upload(cb, hcb, s);
+    upload(tseg, lat.chunk_tile_ptr, s);
+    upload(te, lat.tile_ptr, s);
+    const TileGeom tg = tile_geom(m);
+    const TiledSplat ts{ch.get(), tseg.get(), te.get(), lat.n_tiles, tg, max_run(lat.tile_ptr),
+                        max_run(lat.chunk_tile_ptr)};
+    const bool tiled = tiled_splat_lds(1, tg.side, ts.e_max, ts.s_max) > 0;
+    launch_entry_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, tiled ? &tg : nullptr,
+                        pt.get(), w.get(), s);
     const int rows = V + 1 + lat.n_partials;
     vals0.reserve(rows);
     vals1.reserve(V + 1);
     launch_fill(vals0.get(), 0.f, rows, s);
     launch_fill(vals1.get(), 0.f, V + 1, s);
-    ch.reserve(lat.n_chunks);
-    cb.reserve(lat.n_combines);
-    launch_remap_chunks(lat.chunks.get(), ch.get(), lat.n_chunks, 0, 0, V + 1, s);
-    launch_remap_combines(lat.combines.get(), cb.get(), lat.n_combines, 0, V + 1, s);
     nbr.reserve((size_t)d1 * V);
     launch_remap_nbr(lat.nbr.get(), nbr.get(), d1 * V, 0, V, s);
     ones.reserve(N);
     launch_fill(ones.get(), 1.f, N, s);
-    launch_splat(ch.get(), lat.n_chunks, pt.get(), w.get(), ones.get(), vals0.get(), 1, s);
-    launch_combine(cb.get(), lat.n_combines, vals0.get(), 1, s);
+    if (tiled)
+        launch_splat_tiled(ts, pt.get(), w.get(), ones.get(), vals0.get(), 1, s);
+    else
+        launch_splat(ch.get(), (int)hch.size(), pt.get(), w.get(), ones.get(), vals0.get(), 1, s);
+    launch_combine(cb.get(), (int)hcb.size(), vals0.get(), 1, s);
     BlurTable bt{};
     bt.nterms = 1;
     bt.t[0] = BlurTerm{0, V, lat.d, nbr.get()};
@@ -327,11 +365,15 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
     DevBuf<float> feat;
     feat.reserve((size_t)m.geo.npad_total() * d);
     make_features(feat.get());
-    build_lattice(m.geo, m.geod, feat.get(), d, t->lat, m.hash_load, m.stream);
+    if (!m.tiles_ready) {
+        make_tile_map(m.geo, m.tile_side ? m.tile_side : default_tile_side(m.L), m.tiles);
+        m.tiles_ready = true;
+    }
+    build_lattice(m.geo, m.geod, m.tiles, feat.get(), d, t->lat, m.hash_load, m.stream);
     if (t->lat.overflow)
         throw Error{DCRF_ERR_OVERFLOW,
                     "a lattice coordinate exceeds the int16 range of the lattice keys; scale the features down"};
-    build_chunks(t->lat, m.chunk(), m.stream);
+    build_chunks(t->lat, m.chunk());
     compute_norm(m, *t);
     m.terms.push_back(std::move(t));
     m.layout_valid = false;
@@ -346,55 +388,104 @@ void ensure_layout(dcrf_model& m) {
     const int T = (int)m.terms.size();
     const int N = m.N(), L = m.L;
     for (auto& t : m.terms)
-        if (t->lat.chunk_len != m.chunk()) build_chunks(t->lat, m.chunk(), s);
+        if (t->lat.chunk_len != m.chunk()) build_chunks(t->lat, m.chunk());
     m.order.resize(T);
     std::iota(m.order.begin(), m.order.end(), 0);
     std::stable_sort(m.order.begin(), m.order.end(),
                      [&](int a, int b) { return m.terms[a]->lat.d > m.terms[b]->lat.d; });
     m.vbase.assign(T, 0);
-    int V = 0, P = 0, E = 0, C = 0, CB = 0;
+    std::vector<int> pbase(T, 0);
+    int V = 0, P = 0, E = 0;
     m.max_d = -1;
     m.all_symmetric = true;
     for (int k : m.order) {
         const TermLattice& lat = m.terms[k]->lat;
         m.vbase[k] = V;
         V += lat.V;
-        P += lat.n_partials;
         E += N * (lat.d + 1);
-        C += lat.n_chunks;
-        CB += lat.n_combines;
         m.max_d = std::max(m.max_d, lat.d);
         if (m.terms[k]->ntype != DCRF_NORMALIZE_SYMMETRIC && m.terms[k]->ntype != DCRF_NO_NORMALIZATION)
             m.all_symmetric = false;
     }
+    for (int k : m.order) {
+        pbase[k] = V + 1 + P;
+        P += m.terms[k]->lat.n_partials;
+    }
     m.V_total = V;
     m.rows0 = V + 1 + P;
     m.E_total = E;
-    m.C_total = C;
-    m.CB_total = CB;
+    // Segments and entries, tile by tile, the terms in layout order inside a tile: tile
+    // t's entries of term k (sorted positions tile_ptr[t] ..) move to ebase[k][t] + j.
+    const int NT = m.tiles.n_tiles();
+    std::vector<Chunk> gch;
+    std::vector<Combine> gcb;
+    std::vector<int> gts(NT + 1), gte(NT + 1);
+    std::vector<std::vector<int>> ebase(T, std::vector<int>(NT));
+    int e = 0, e_max = 0, s_max = 0;
+    for (int t = 0; t < NT; t++) {
+        gts[t] = (int)gch.size();
+        gte[t] = e;
+        for (int k : m.order) {
+            const TermLattice& lat = m.terms[k]->lat;
+            const int eb = e - lat.tile_ptr[t];
+            ebase[k][t] = eb;
+            for (int c = lat.chunk_tile_ptr[t]; c < lat.chunk_tile_ptr[t + 1]; c++) {
+                Chunk x = lat.chunks[c];
+                x.v += m.vbase[k];
+                x.beg += eb;
+                x.end += eb;
+                x.out = x.out >= 0 ? x.out + m.vbase[k] : pbase[k] + (-x.out - 1);
+                gch.push_back(x);
+            }
+            e += lat.tile_ptr[t + 1] - lat.tile_ptr[t];
+        }
+        e_max = std::max(e_max, e - gte[t]);
+        s_max = std::max(s_max, (int)gch.size() - gts[t]);
+    }
+    gts[NT] = (int)gch.size();
+    gte[NT] = e;
+    if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
+    for (int k : m.order)
+        for (Combine c : m.terms[k]->lat.combines) {
+            c.v += m.vbase[k];
+            c.first += pbase[k];
+            gcb.push_back(c);
+        }
+    m.C_total = (int)gch.size();
+    m.CB_total = (int)gcb.size();
+    upload(m.g_chunks, gch, s);
+    upload(m.g_comb, gcb, s);
+    upload(m.g_tile_seg, gts, s);
+    upload(m.g_tile_e, gte, s);
+    const TileGeom tg = tile_geom(m);
+    m.tsplat = TiledSplat{m.g_chunks.get(), m.g_tile_seg.get(), m.g_tile_e.get(), NT, tg, e_max, s_max};
+    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max) > 0;
+
     m.vals[0].reserve((size_t)m.rows0 * L);
     m.vals[1].reserve((size_t)(V + 1) * L);
     launch_fill(m.vals[0].get(), 0.f, (size_t)m.rows0 * L, s);
     launch_fill(m.vals[1].get(), 0.f, (size_t)(V + 1) * L, s);
-    m.g_chunks.reserve(C);
-    m.g_comb.reserve(CB);
     m.g_pt.reserve(E);
     m.g_wF.reserve(E);
     if (!m.all_symmetric) m.g_wT.reserve(E);
     m.tl.clear();
     m.tl.resize(T);
-    int ebase = 0, cbase = 0, cbbase = 0, pbase = V + 1;
-    for (int k : m.order) {
+    std::vector<DevBuf<int>> staging(2 * T);  // live until the synchronisation below
+    for (int i = 0; i < T; i++) {
+        const int k = m.order[i];
         Term& t = *m.terms[k];
         const TermLattice& lat = t.lat;
         const int d1 = lat.d + 1, Ek = N * d1;
-        launch_csr_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, false),
-                          m.g_pt.get() + ebase, m.g_wF.get() + ebase, s);
+        DevBuf<int>& tptr = staging[2 * i];
+        DevBuf<int>& eb = staging[2 * i + 1];
+        upload(tptr, lat.tile_ptr, s);
+        upload(eb, ebase[k], s);
+        const TileGeom* local = m.tiled ? &tg : nullptr;
+        launch_entry_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, false),
+                            tptr.get(), eb.get(), NT, local, m.g_pt.get(), m.g_wF.get(), s);
         if (!m.all_symmetric)
-            launch_csr_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, true),
-                              m.g_pt.get() + ebase, m.g_wT.get() + ebase, s);
-        launch_remap_chunks(lat.chunks.get(), m.g_chunks.get() + cbase, lat.n_chunks, m.vbase[k], ebase, pbase, s);
-        launch_remap_combines(lat.combines.get(), m.g_comb.get() + cbbase, lat.n_combines, m.vbase[k], pbase, s);
+            launch_entry_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, true),
+                                tptr.get(), eb.get(), NT, local, m.g_pt.get(), m.g_wT.get(), s);
         auto lay = std::make_unique<TermLayout>();
         lay->nbr.reserve((size_t)d1 * lat.V);
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
@@ -409,10 +500,6 @@ void ensure_layout(dcrf_model& m) {
                                  m.vbase[k], lay->vtx.get(), lay->swT.get(), s);
         }
         m.tl[k] = std::move(lay);
-        ebase += Ek;
-        cbase += lat.n_chunks;
-        cbbase += lat.n_combines;
-        pbase += lat.n_partials;
     }
     m.prefixV.assign(std::max(m.max_d + 1, 0), 0);
     for (int st = 0; st <= m.max_d; st++)
@@ -475,7 +562,10 @@ void run_filter(dcrf_model& m, const float* in, bool transpose, hipStream_t s) {
     const float* w = (transpose && !m.all_symmetric) ? m.g_wT.get() : m.g_wF.get();
     {
         Timed t(m, s, KC_SPLAT, splat_bytes(m));
-        launch_splat(m.g_chunks.get(), m.C_total, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
+        if (m.tiled)
+            launch_splat_tiled(m.tsplat, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
+        else
+            launch_splat(m.g_chunks.get(), m.C_total, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
         t.done();
     }
     if (m.CB_total) {
@@ -1092,6 +1182,16 @@ int dcrf_set_splat_chunk(dcrf_model* m, int chunk) {
     });
 }
 
+int dcrf_set_splat_tile(dcrf_model* m, int side) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(side >= 0 && side <= 4096, "tile side must be in [0, 4096]");
+        require(m->terms.empty(), "set the splat tile before adding pairwise terms", DCRF_ERR_STATE);
+        m->tile_side = side;
+        m->tiles_ready = false;
+    });
+}
+
 int dcrf_set_hash_load(dcrf_model* m, float load) {
     return guard([&] {
         require(m != nullptr, "null model");
@@ -1121,6 +1221,19 @@ int dcrf_kernel_stats(dcrf_model* m, int kernel, long long* launches, double* ms
         if (launches) *launches = m->kc_launches[kernel];
         if (ms) *ms = m->kc_ms[kernel];
         if (bytes) *bytes = m->kc_bytes[kernel];
+    });
+}
+
+int dcrf_splat_layout(dcrf_model* m, int* tiled, int* tile_side, int* n_tiles, int* segments, int* partials) {
+    return guard([&] {
+        require(m && tiled && tile_side && n_tiles && segments && partials, "null argument");
+        ensure_device(*m);
+        ensure_layout(*m);
+        *tiled = m->tiled ? 1 : 0;
+        *tile_side = m->tiles.side;
+        *n_tiles = m->tiles.n_tiles();
+        *segments = m->C_total;
+        *partials = m->rows0 - m->V_total - 1;
     });
 }
 

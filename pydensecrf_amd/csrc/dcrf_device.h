@@ -36,6 +36,58 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Last i in [0, n) with base[i] <= q (base ascending, base[0] <= q).
+__device__ __forceinline__ int last_le(const int* __restrict__ base, int n, int q) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= q)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Pixels of tile t: rows of tw pixels starting at p0, p0 + stride, ... (th rows).
+// 2-D instances are cut into side x side squares (row-major over the image), other
+// instances into runs of side*side pixels.
+__device__ __forceinline__ void tile_rect(const TileGeom& g, int t, int& p0, int& tw, int& th, int& stride) {
+    const int i = g.n_inst > 1 ? last_le(g.tile_base, g.n_inst, t) : 0;
+    const int lt = t - g.tile_base[i];
+    const int Ni = g.pix_base[i + 1] - g.pix_base[i];
+    if (g.is2d) {
+        const int W = g.W[i], H = Ni / W;
+        const int ntx = (W + g.side - 1) / g.side;
+        const int ty = lt / ntx, tx = lt - ty * ntx;
+        const int x0 = tx * g.side, y0 = ty * g.side;
+        tw = min(g.side, W - x0);
+        th = min(g.side, H - y0);
+        p0 = g.pix_base[i] + y0 * W + x0;
+        stride = W;
+    } else {
+        const int run = g.side * g.side;
+        p0 = g.pix_base[i] + lt * run;
+        tw = min(run, Ni - lt * run);
+        th = 1;
+        stride = 0;
+    }
+}
+
+// Index of pixel p inside its tile (row-major in the tile).
+__device__ __forceinline__ int tile_local(const TileGeom& g, int p) {
+    const int i = g.n_inst > 1 ? last_le(g.pix_base, g.n_inst, p) : 0;
+    const int local = p - g.pix_base[i];
+    if (g.is2d) {
+        const int W = g.W[i];
+        const int y = local / W, x = local - y * W;
+        const int x0 = (x / g.side) * g.side;
+        const int tw = min(g.side, W - x0);
+        return (y % g.side) * tw + (x - x0);
+    }
+    return local % (g.side * g.side);
+}
+
 // R consecutive floats at 4-byte alignment: one global_load/store_dwordxR.
 template <int R>
 struct __attribute__((aligned(4))) Piece {

@@ -71,8 +71,8 @@ struct DevBuf {
 // ---------------------------------------------------------------- lattice of one term
 struct Chunk {
     int v;    // vertex (term-local, later global)
-    int beg;  // first CSR entry
-    int end;  // one past the last CSR entry
+    int beg;  // first entry
+    int end;  // one past the last entry
     int out;  // output row: >= 0 vertex row; < 0 partial row -(r+1) (term-local)
 };
 struct Combine {
@@ -92,12 +92,16 @@ struct TermLattice {
     DevBuf<int> rank;    // [N][d+1]
     DevBuf<int> nbr;     // [(d+1)][V][2]
     DevBuf<int16_t> keys;  // [V][d]
-    DevBuf<int> csr_ptr;   // [V+1]
-    DevBuf<int> csr_e;     // [N*(d+1)] entry index p*(d+1)+r, sorted by (vertex, entry)
-    DevBuf<Chunk> chunks;  // splat segments
-    int n_chunks = 0;
-    DevBuf<Combine> combines;  // multi-chunk vertices
-    int n_combines = 0;
+    DevBuf<int> csr_e;     // [N*(d+1)] entry index p*(d+1)+r, sorted by (tile of p, vertex, p)
+    int n_tiles = 0;
+    std::vector<int> run_beg;   // runs of equal (tile, vertex) in csr_e order (size n_runs + 1)
+    std::vector<int> run_v, run_t;
+    std::vector<int> tile_ptr;  // first csr_e position of each tile (size n_tiles + 1)
+    // splat segments, term-local, listed tile by tile: out >= 0 vertex row, < 0 partial
+    // row -(r+1); chunk_tile_ptr[t] = first segment of tile t
+    std::vector<Chunk> chunks;
+    std::vector<int> chunk_tile_ptr;
+    std::vector<Combine> combines;  // vertices with more than one segment
     int n_partials = 0;
     int chunk_len = 0;
 };
@@ -127,13 +131,36 @@ struct GeometryDev {
     DevBuf<int> pix_base, pad_base, W;
 };
 
+// Splat tiles: pixels are grouped into tiles (T x T squares of a 2-D instance, runs of
+// T*T consecutive pixels otherwise, never crossing instances); the contributors of a
+// heavy vertex are cut at tile boundaries and all segments are processed tile by tile,
+// so the input rows of one tile are fetched once and re-read from cache.
+struct TileMap {
+    int side = 0;               // T
+    std::vector<int> base;      // first tile of each instance (size n_inst + 1)
+    DevBuf<int> base_dev;
+    int n_tiles() const { return base.empty() ? 0 : base.back(); }
+};
+void make_tile_map(const Geometry& g, int side, TileMap& tm);
+// Default tile side for L labels: a tile's input rows take at most 24 KiB of LDS.
+int default_tile_side(int L);
+
+// Tile geometry as the kernels see it.
+struct TileGeom {
+    const int* tile_base;  // [n_inst + 1]
+    const int* pix_base;   // [n_inst + 1]
+    const int* W;          // 2-D: width of each instance
+    int n_inst, is2d, side;
+};
+
 // ---------------------------------------------------------------- lattice build
 // Builds the lattice of one term from padded point-major features feat [npad][d]
 // (pad points zero).  Fills lat.{vtx,bary,rank,nbr,keys,csr_ptr,csr_e,V,inst_vbase}.
-void build_lattice(const Geometry& g, const GeometryDev& gd, const float* feat, int d,
+void build_lattice(const Geometry& g, const GeometryDev& gd, const TileMap& tm, const float* feat, int d,
                    TermLattice& lat, float hash_load, hipStream_t s);
-// Splat segments of at most `chunk` entries per vertex.
-void build_chunks(TermLattice& lat, int chunk, hipStream_t s);
+// Splat segments (host): the runs of equal (tile, vertex) cut into pieces of at most
+// `chunk` entries, listed tile by tile.
+void build_chunks(TermLattice& lat, int chunk);
 // Feature generators into padded point-major layout [npad][d].
 void features_from_dxN(const Geometry& g, const GeometryDev& gd, const float* feat_dxN, int d,
                        float* out, hipStream_t s);
@@ -182,8 +209,27 @@ struct LaunchShape {
 int choose_labels_per_lane(int L);
 LaunchShape shape_for(int L);
 
-void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w,
+// Splat: vals[out] = sum over a segment's entries i of w[i] * in[pixel of i], in entry
+// order.  Global form: pt[i] is the pixel (row of `in`) of entry i.
+void launch_splat(const Chunk* chunks, int n_chunks, const int* pt, const float* w,
                   const float* in, float* vals, int L, hipStream_t s);
+// Tiled form: one block per tile stages the tile's input rows, entries and segments in
+// LDS; pt[i] is then the index of the entry's pixel inside its tile (row-major in the
+// tile).  The entries of tile t are [tile_e[t], tile_e[t+1]), its segments
+// [tile_seg[t], tile_seg[t+1]).
+struct TiledSplat {
+    const Chunk* chunks;
+    const int* tile_seg;
+    const int* tile_e;
+    int n_tiles;
+    TileGeom geom;
+    int e_max, s_max;  // most entries / segments of one tile
+};
+// LDS bytes of the tiled splat (0 if it does not fit a block or L needs the block-mode
+// lane groups: use the global form).
+size_t tiled_splat_lds(int L, int side, int e_max, int s_max);
+void launch_splat_tiled(const TiledSplat& ts, const int* pt, const float* w, const float* in, float* vals,
+                        int L, hipStream_t s);
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s);
 void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);
@@ -219,13 +265,14 @@ void launch_compat_finish(const double* G, int ckind, int L, const float* cw, co
 void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s);
 
 // Global layout remaps.
-void launch_remap_chunks(const Chunk* src, Chunk* dst, int n, int vbase, int ebase, int prow_base,
-                         hipStream_t s);
-void launch_remap_combines(const Combine* src, Combine* dst, int n, int vbase, int prow_base,
-                           hipStream_t s);
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);
-void launch_csr_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm,
-                       int use_norm, int* pt_out, float* w_out, hipStream_t s);
+// Entry arrays of one term: sorted entry j (csr_e[j] = p*(d+1)+r) goes to position
+// base[t] + j, t the tile of j (tile_ptr: first j of each tile; base null: position j);
+// pt = the pixel p, or its index inside its tile when `local` is given; w = the
+// barycentric weight, times norm[p] when use_norm.
+void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
+                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int* pt_out,
+                         float* w_out, hipStream_t s);
 void launch_slice_weights(const int* vtx, const float* bary, const float* norm, int use_norm,
                           float alpha, int N, int d1, int vbase, int* gvtx, float* sw,
                           hipStream_t s);

@@ -53,6 +53,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBatch = 8;  // contributors / partial rows loaded together
+constexpr size_t kTiledLdsMax = 64 * 1024;  // LDS of one tiled-splat block
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
 __global__ void k_fill(float* p, float v, size_t n) {
@@ -116,6 +117,113 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
             }
     }
     store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
+}
+
+// Tiled splat: block = one tile.  Stage 1 copies the tile's input rows, its entries
+// (pixel index in the tile, weight) and its segments (beg, end, out) into LDS, every
+// thread keeping kStage loads in flight; stage 2 gives the segments to the lane groups
+// of the block, which sum them from LDS in entry order (the same order, hence the same
+// bits, as k_splat).
+constexpr int kStage = 32;
+
+template <int R>
+__global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const int* __restrict__ pt,
+                                                            const float* __restrict__ w,
+                                                            const float* __restrict__ in, float* __restrict__ vals,
+                                                            int L, int S, int per, int q_max) {
+    extern __shared__ float sh[];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    int p0, tw, th, stride;
+    tile_rect(ts.geom, t, p0, tw, th, stride);
+    const int e0 = ts.tile_e[t], ne = ts.tile_e[t + 1] - e0;
+    const int c0 = ts.tile_seg[t], ns = ts.tile_seg[t + 1] - c0;
+    const int rowf = tw * L;
+    const int n1 = th * rowf, n2 = n1 + ne, n3 = n2 + ne, n4 = n3 + 3 * ns;
+    const float inv_rowf = 1.0f / (float)rowf;
+    const long long rskip = (long long)stride * L - rowf;  // gap between two tile rows in `in`
+    const unsigned* __restrict__ qsrc = reinterpret_cast<const unsigned*>(in) + (size_t)p0 * L;
+    const unsigned* __restrict__ psrc = reinterpret_cast<const unsigned*>(pt) + e0;
+    const unsigned* __restrict__ wsrc = reinterpret_cast<const unsigned*>(w) + e0;
+    const unsigned* __restrict__ csrc = reinterpret_cast<const unsigned*>(ts.chunks + c0);
+    unsigned* shu = reinterpret_cast<unsigned*>(sh);
+    // LDS: [rows: q_max floats][pixel index: e_max][weight: e_max][segments: 3 s_max]
+    for (int b = threadIdx.x; b < n4; b += kStage * kWaveBlock) {
+        unsigned x[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int i = b + u * kWaveBlock;
+            if (i < n1) {
+                int r = (int)((float)i * inv_rowf);  // row of the tile (i < 2^24: corrected by one)
+                if (r * rowf > i)
+                    r--;
+                else if ((r + 1) * rowf <= i)
+                    r++;
+                x[u] = qsrc[i + r * rskip];
+            } else if (i < n2) {
+                x[u] = psrc[i - n1];
+            } else if (i < n3) {
+                x[u] = wsrc[i - n2];
+            } else if (i < n4) {
+                const int j = i - n3;
+                const int c = j / 3;
+                x[u] = csrc[4 * c + 1 + (j - 3 * c)];  // Chunk {v, beg, end, out}
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int i = b + u * kWaveBlock;
+            if (i < n4) {
+                const int off = i < n1 ? 0 : (i < n2 ? q_max - n1 : (i < n3 ? q_max + ts.e_max - n2
+                                                                            : q_max + 2 * ts.e_max - n3));
+                shu[i + off] = x[u];
+            }
+        }
+    }
+    __syncthreads();
+    const float* tq = sh;
+    const int* tp = reinterpret_cast<const int*>(sh + q_max);
+    const float* tw8 = sh + q_max + ts.e_max;
+    const int* tsg = reinterpret_cast<const int*>(sh + q_max + 2 * ts.e_max);
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    if (!G.on) return;  // no barrier below
+    const int groups = (kWaveBlock / 64) * per;
+    const int k0 = G.k0;
+    for (int c = (threadIdx.x >> 6) * per + G.g; c < ns; c += groups) {
+        const int b = tsg[3 * c] - e0, e = tsg[3 * c + 1] - e0, out = tsg[3 * c + 2];
+        float acc[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] = 0.0f;
+        int i = b;
+        for (; i + 4 <= e; i += 4) {
+            int pp[4];
+            float ww[4], xx[4][R];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                pp[u] = tp[i + u];
+                ww[u] = tw8[i + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) load_piece<R>(tq + pp[u] * L, k0, L, xx[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float prod = ww[u] * xx[u][r];
+                    acc[r] = acc[r] + prod;
+                }
+        }
+        for (; i < e; i++) {
+            float xx[R];
+            const float ww = tw8[i];
+            load_piece<R>(tq + tp[i] * L, k0, L, xx);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float prod = ww * xx[r];
+                acc[r] = acc[r] + prod;
+            }
+        }
+        store_piece<R>(vals + (size_t)out * L, k0, L, acc);
+    }
 }
 
 template <bool WAVE, int R>
@@ -355,28 +463,6 @@ __global__ void k_norm_finish(float* norm, int N, int ntype) {
     else if (ntype == DCRF_NORMALIZE_BEFORE || ntype == DCRF_NORMALIZE_AFTER) norm[i] = (float)(1.0 / x);
 }
 
-__global__ void k_remap_chunks(const Chunk* __restrict__ src, Chunk* __restrict__ dst, int n, int vbase,
-                               int ebase, int prow_base) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Chunk c = src[i];
-    c.v += vbase;
-    c.beg += ebase;
-    c.end += ebase;
-    c.out = c.out >= 0 ? c.out + vbase : prow_base + (-c.out - 1);
-    dst[i] = c;
-}
-
-__global__ void k_remap_combines(const Combine* __restrict__ src, Combine* __restrict__ dst, int n,
-                                 int vbase, int prow_base) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Combine c = src[i];
-    c.v += vbase;
-    c.first += prow_base;
-    dst[i] = c;
-}
-
 __global__ void k_remap_nbr(const int* __restrict__ src, int2* __restrict__ dst, int n, int vbase,
                             int zero_row) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -385,16 +471,18 @@ __global__ void k_remap_nbr(const int* __restrict__ src, int2* __restrict__ dst,
     dst[i] = make_int2(a >= 0 ? a + vbase : zero_row, b >= 0 ? b + vbase : zero_row);
 }
 
-__global__ void k_csr_gather(const int* __restrict__ csr_e, int E, int d1, const float* __restrict__ bary,
-                             const float* __restrict__ norm, int use_norm, int* __restrict__ pt,
-                             float* __restrict__ w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E) return;
-    const int e = csr_e[i];
+__global__ void k_entry_gather(const int* __restrict__ csr_e, int E, int d1, const float* __restrict__ bary,
+                               const float* __restrict__ norm, int use_norm, const int* __restrict__ tile_ptr,
+                               const int* __restrict__ base, int n_tiles, TileGeom g, int local,
+                               int* __restrict__ pt, float* __restrict__ w) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= E) return;
+    const int e = csr_e[j];
     const int p = e / d1;
-    pt[i] = p;
+    const int pos = base ? base[last_le(tile_ptr, n_tiles, j)] + j : j;
+    pt[pos] = local ? tile_local(g, p) : p;
     const float b = bary[e];
-    w[i] = use_norm ? b * norm[p] : b;
+    w[pos] = use_norm ? b * norm[p] : b;
 }
 
 __global__ void k_slice_weights(const int* __restrict__ vtx, const float* __restrict__ bary,
@@ -465,6 +553,30 @@ void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const fl
                   sh.S, sh.per);
 }
 
+size_t tiled_splat_lds(int L, int side, int e_max, int s_max) {
+    const LaunchShape sh = shape_for(L);
+    if (!sh.wave) return 0;
+    const size_t words = (size_t)side * side * L + 2 * (size_t)e_max + 3 * (size_t)s_max;
+    const size_t bytes = words * sizeof(float);
+    return bytes <= kTiledLdsMax ? bytes : 0;
+}
+
+void launch_splat_tiled(const TiledSplat& ts, const int* pt, const float* w, const float* in, float* vals, int L,
+                        hipStream_t s) {
+    if (!ts.n_tiles) return;
+    const LaunchShape sh = shape_for(L);
+    const size_t lds = tiled_splat_lds(L, ts.geom.side, ts.e_max, ts.s_max);
+    if (!lds) throw Error{DCRF_ERR_VALUE, "tiled splat does not fit the LDS of a block"};
+    const int q_max = ts.geom.side * ts.geom.side * L;
+    switch (sh.R) {
+        case 1: k_splat_tiled<1><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
+        case 2: k_splat_tiled<2><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
+        case 3: k_splat_tiled<3><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
+        default: k_splat_tiled<4><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
+    }
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s) {
     if (!n) return;
     const LaunchShape sh = shape_for(L);
@@ -510,28 +622,19 @@ void launch_norm_finish(float* norm, int N, int ntype, hipStream_t s) {
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
-void launch_remap_chunks(const Chunk* src, Chunk* dst, int n, int vbase, int ebase, int prow_base, hipStream_t s) {
-    if (!n) return;
-    k_remap_chunks<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, ebase, prow_base);
-    DCRF_HIP_CHECK(hipGetLastError());
-}
-
-void launch_remap_combines(const Combine* src, Combine* dst, int n, int vbase, int prow_base, hipStream_t s) {
-    if (!n) return;
-    k_remap_combines<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, prow_base);
-    DCRF_HIP_CHECK(hipGetLastError());
-}
-
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s) {
     if (!n) return;
     k_remap_nbr<<<blocks_for(n), kThreads, 0, s>>>(src, dst, n, vbase, zero_row);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
-void launch_csr_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
-                       int* pt_out, float* w_out, hipStream_t s) {
+void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
+                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int* pt_out,
+                         float* w_out, hipStream_t s) {
     if (!E) return;
-    k_csr_gather<<<blocks_for(E), kThreads, 0, s>>>(csr_e, E, d1, bary, norm, use_norm, pt_out, w_out);
+    const TileGeom g = local ? *local : TileGeom{};
+    k_entry_gather<<<blocks_for(E), kThreads, 0, s>>>(csr_e, E, d1, bary, norm, use_norm, tile_ptr, base, n_tiles, g,
+                                                      local ? 1 : 0, pt_out, w_out);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

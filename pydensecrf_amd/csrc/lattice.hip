@@ -343,16 +343,40 @@ __global__ void k_iota(int* p, int n) {
     if (i < n) p[i] = i;
 }
 
-__global__ void k_lower_bound(const int* __restrict__ sorted, int n, int V, int* __restrict__ ptr) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v > V) return;
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (sorted[mid] < v) lo = mid + 1;
-        else hi = mid;
+// Sort key of a real entry: (tile of its pixel << vbits) | vertex.
+__global__ void k_entry_keys(const int* __restrict__ vtx, int Er, int d1, const int* pix_base, const int* Wd,
+                             const int* tile_base, int n_inst, int is2d, int side, int vbits,
+                             unsigned long long* __restrict__ key) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Er) return;
+    const int p = e / d1;
+    const int i = n_inst > 1 ? find_instance(pix_base, n_inst, p) : 0;
+    const int local = p - pix_base[i];
+    int tile;
+    if (is2d) {
+        const int W = Wd[i];
+        const int ntx = (W + side - 1) / side;
+        tile = tile_base[i] + (local / W / side) * ntx + (local % W) / side;
+    } else {
+        tile = tile_base[i] + local / (side * side);
     }
-    ptr[v] = lo;
+    key[e] = ((unsigned long long)(unsigned)tile << vbits) | (unsigned)vtx[e];
+}
+
+// Starts of the runs of equal keys in a sorted array.
+__global__ void k_run_flags(const unsigned long long* __restrict__ key, int n, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_run_compact(const unsigned long long* __restrict__ key, const int* __restrict__ flag,
+                              const int* __restrict__ rid, int n, int* __restrict__ beg,
+                              unsigned long long* __restrict__ rkey) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    beg[rid[i]] = i;
+    rkey[rid[i]] = key[i];
 }
 
 __global__ void k_gather_int(const int* __restrict__ src, const int* __restrict__ idx, int n,
@@ -369,8 +393,8 @@ uint32_t next_pow2(uint64_t x) {
 }
 
 template <int D>
-void build_lattice_d(const Geometry& g, const GeometryDev& gd, const float* feat, TermLattice& lat,
-                     float hash_load, hipStream_t s) {
+void build_lattice_d(const Geometry& g, const GeometryDev& gd, const TileMap& tm, const float* feat,
+                     TermLattice& lat, float hash_load, hipStream_t s) {
     constexpr int D1 = D + 1;
     const int npad = g.npad_total();
     const int N = g.N_total();
@@ -460,37 +484,106 @@ void build_lattice_d(const Geometry& g, const GeometryDev& gd, const float* feat
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
     }
 
-    // CSR by vertex: stable sort of real entries by vertex id
+    // Entries sorted by (tile of the pixel, vertex), stably (pixel order within a key),
+    // and the runs of equal (tile, vertex): the splat segments are cut from the runs.
     lat.csr_e.reserve((size_t)Er);
-    lat.csr_ptr.reserve((size_t)V + 1);
+    lat.n_tiles = tm.n_tiles();
     {
-        DevBuf<int> iota, sorted_v;
+        int vbits = 1;
+        while ((1ll << vbits) <= (long long)V) vbits++;
+        int tbits = 1;
+        while ((1ll << tbits) <= (long long)lat.n_tiles) tbits++;
+        if (vbits + tbits > 63) throw Error{DCRF_ERR_VALUE, "too many lattice vertices and tiles"};
+        DevBuf<int> iota, flag2, rid;
+        DevBuf<unsigned long long> key, sorted_key;
         iota.reserve(Er);
-        sorted_v.reserve(Er);
+        key.reserve(Er);
+        sorted_key.reserve(Er);
         k_iota<<<blocks_for(Er), kThreads, 0, s>>>(iota.get(), Er);
         DCRF_HIP_CHECK(hipGetLastError());
-        int bits = 1;
-        while ((1ll << bits) <= (long long)V) bits++;
+        k_entry_keys<<<blocks_for(Er), kThreads, 0, s>>>(lat.vtx.get(), Er, D1, gd.pix_base.get(), gd.W.get(),
+                                                         tm.base_dev.get(), g.n_inst, g.is2d ? 1 : 0, tm.side,
+                                                         vbits, key.get());
+        DCRF_HIP_CHECK(hipGetLastError());
         size_t tmp_bytes = 0;
-        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, lat.vtx.get(), sorted_v.get(), iota.get(),
-                                                 lat.csr_e.get(), (size_t)Er, 0, bits, s));
+        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, key.get(), sorted_key.get(), iota.get(),
+                                                 lat.csr_e.get(), (size_t)Er, 0, vbits + tbits, s));
         DevBuf<char> tmp;
         tmp.reserve(tmp_bytes);
-        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, lat.vtx.get(), sorted_v.get(), iota.get(),
-                                                 lat.csr_e.get(), (size_t)Er, 0, bits, s));
-        k_lower_bound<<<blocks_for((size_t)V + 1), kThreads, 0, s>>>(sorted_v.get(), Er, V, lat.csr_ptr.get());
+        DCRF_HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, key.get(), sorted_key.get(), iota.get(),
+                                                 lat.csr_e.get(), (size_t)Er, 0, vbits + tbits, s));
+        flag2.reserve(Er);
+        rid.reserve(Er);
+        k_run_flags<<<blocks_for(Er), kThreads, 0, s>>>(sorted_key.get(), Er, flag2.get());
         DCRF_HIP_CHECK(hipGetLastError());
+        size_t scan_bytes = 0;
+        DCRF_HIP_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes, flag2.get(), rid.get(), 0, (size_t)Er,
+                                               rocprim::plus<int>(), s));
+        DevBuf<char> stmp;
+        stmp.reserve(scan_bytes);
+        DCRF_HIP_CHECK(rocprim::exclusive_scan(stmp.get(), scan_bytes, flag2.get(), rid.get(), 0, (size_t)Er,
+                                               rocprim::plus<int>(), s));
+        int tail[2];
+        DCRF_HIP_CHECK(hipMemcpyAsync(&tail[0], rid.get() + (Er - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipMemcpyAsync(&tail[1], flag2.get() + (Er - 1), sizeof(int), hipMemcpyDeviceToHost, s));
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        const int n_runs = tail[0] + tail[1];
+        DevBuf<int> rbeg;
+        DevBuf<unsigned long long> rkey;
+        rbeg.reserve(n_runs);
+        rkey.reserve(n_runs);
+        k_run_compact<<<blocks_for(Er), kThreads, 0, s>>>(sorted_key.get(), flag2.get(), rid.get(), Er, rbeg.get(),
+                                                          rkey.get());
+        DCRF_HIP_CHECK(hipGetLastError());
+        lat.run_beg.resize(n_runs + 1);
+        std::vector<unsigned long long> hk(n_runs);
+        DCRF_HIP_CHECK(hipMemcpyAsync(lat.run_beg.data(), rbeg.get(), sizeof(int) * n_runs, hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipMemcpyAsync(hk.data(), rkey.get(), sizeof(unsigned long long) * n_runs,
+                                      hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        lat.run_beg[n_runs] = Er;
+        lat.run_v.resize(n_runs);
+        lat.run_t.resize(n_runs);
+        const unsigned long long vmask = (1ull << vbits) - 1;
+        for (int r = 0; r < n_runs; r++) {
+            lat.run_v[r] = (int)(hk[r] & vmask);
+            lat.run_t[r] = (int)(hk[r] >> vbits);
+        }
+        // first sorted entry of every tile
+        lat.tile_ptr.assign(lat.n_tiles + 1, Er);
+        for (int r = n_runs - 1; r >= 0; r--) lat.tile_ptr[lat.run_t[r]] = lat.run_beg[r];
+        for (int t = lat.n_tiles - 1; t >= 0; t--) lat.tile_ptr[t] = std::min(lat.tile_ptr[t], lat.tile_ptr[t + 1]);
     }
 }
 
 }  // namespace
 
-void build_lattice(const Geometry& g, const GeometryDev& gd, const float* feat, int d,
+int default_tile_side(int L) {
+    int side = (int)std::sqrt(24.0 * 1024.0 / (4.0 * std::max(L, 1)));
+    side = std::max(4, std::min(64, side & ~3));
+    return side;
+}
+
+void make_tile_map(const Geometry& g, int side, TileMap& tm) {
+    tm.side = side;
+    tm.base.assign(1, 0);
+    for (int i = 0; i < g.n_inst; i++) {
+        long long nt;
+        if (g.is2d)
+            nt = (long long)((g.W[i] + side - 1) / side) * ((g.H[i] + side - 1) / side);
+        else
+            nt = ((long long)g.N[i] + (long long)side * side - 1) / ((long long)side * side);
+        tm.base.push_back((int)(tm.base.back() + nt));
+    }
+    tm.base_dev.reserve(tm.base.size());
+    DCRF_HIP_CHECK(hipMemcpy(tm.base_dev.get(), tm.base.data(), sizeof(int) * tm.base.size(), hipMemcpyHostToDevice));
+}
+
+void build_lattice(const Geometry& g, const GeometryDev& gd, const TileMap& tm, const float* feat, int d,
                    TermLattice& lat, float hash_load, hipStream_t s) {
     switch (d) {
 #define DCRF_CASE(D) \
-    case D: build_lattice_d<D>(g, gd, feat, lat, hash_load, s); break;
+    case D: build_lattice_d<D>(g, gd, tm, feat, lat, hash_load, s); break;
         DCRF_CASE(1) DCRF_CASE(2) DCRF_CASE(3) DCRF_CASE(4) DCRF_CASE(5) DCRF_CASE(6) DCRF_CASE(7)
         DCRF_CASE(8) DCRF_CASE(9) DCRF_CASE(10) DCRF_CASE(11) DCRF_CASE(12) DCRF_CASE(13)
         DCRF_CASE(14) DCRF_CASE(15)
@@ -501,39 +594,53 @@ void build_lattice(const Geometry& g, const GeometryDev& gd, const float* feat, 
     }
 }
 
-void build_chunks(TermLattice& lat, int chunk, hipStream_t s) {
-    const int V = lat.V;
-    std::vector<int> ptr(V + 1);
-    DCRF_HIP_CHECK(hipMemcpyAsync(ptr.data(), lat.csr_ptr.get(), sizeof(int) * (V + 1), hipMemcpyDeviceToHost, s));
-    DCRF_HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<Chunk> ch;
-    std::vector<Combine> cb;
-    ch.reserve(V + V / 4);
-    int prow = 0;
-    for (int v = 0; v < V; v++) {
-        const int b = ptr[v], e = ptr[v + 1];
-        const int deg = e - b;
-        const int n = deg <= chunk ? 1 : (deg + chunk - 1) / chunk;
-        if (n == 1) {
-            ch.push_back(Chunk{v, b, e, v});
-        } else {
-            for (int c = 0; c < n; c++)
-                ch.push_back(Chunk{v, b + c * chunk, std::min(e, b + (c + 1) * chunk), -(prow + c) - 1});
-            cb.push_back(Combine{v, prow, n, 0});
-            prow += n;
+void build_chunks(TermLattice& lat, int chunk) {
+    const int V = lat.V, n_runs = (int)lat.run_v.size(), T = lat.n_tiles;
+    std::vector<Chunk>& ch = lat.chunks;
+    ch.clear();
+    ch.reserve(n_runs + n_runs / 4 + 16);
+    std::vector<int> seg_tile;
+    seg_tile.reserve(ch.capacity());
+    // vertices without real contributors (created by padding lanes only) still need
+    // their row zeroed every iteration: an empty segment each, listed with tile 0
+    std::vector<char> seen(V, 0);
+    for (int r = 0; r < n_runs; r++) seen[lat.run_v[r]] = 1;
+    for (int v = 0; v < V; v++)
+        if (!seen[v]) {
+            ch.push_back(Chunk{v, 0, 0, v});
+            seg_tile.push_back(0);
+        }
+    std::vector<int> cnt(V, 0);
+    const size_t first_real = ch.size();
+    for (int r = 0; r < n_runs; r++) {
+        const int b = lat.run_beg[r], e = lat.run_beg[r + 1], v = lat.run_v[r];
+        for (int i = b; i < e; i += chunk) {
+            ch.push_back(Chunk{v, i, std::min(e, i + chunk), 0});
+            seg_tile.push_back(lat.run_t[r]);
+            cnt[v]++;
         }
     }
-    lat.n_chunks = (int)ch.size();
-    lat.n_combines = (int)cb.size();
+    // vertices with one segment write their row directly; the others get partial rows,
+    // contiguous per vertex in segment (= tile) order, recombined by the combine pass
+    std::vector<int> pfirst(V, -1);
+    lat.combines.clear();
+    int prow = 0;
+    for (int v = 0; v < V; v++)
+        if (cnt[v] > 1) {
+            pfirst[v] = prow;
+            lat.combines.push_back(Combine{v, prow, cnt[v], 0});
+            prow += cnt[v];
+        }
+    std::vector<int> used(V, 0);
+    for (size_t c = first_real; c < ch.size(); c++) {
+        const int v = ch[c].v;
+        ch[c].out = pfirst[v] < 0 ? v : -(pfirst[v] + used[v]++) - 1;
+    }
+    lat.chunk_tile_ptr.assign(T + 1, (int)ch.size());
+    for (int c = (int)ch.size() - 1; c >= 0; c--) lat.chunk_tile_ptr[seg_tile[c]] = c;
+    for (int t = T - 1; t >= 0; t--) lat.chunk_tile_ptr[t] = std::min(lat.chunk_tile_ptr[t], lat.chunk_tile_ptr[t + 1]);
     lat.n_partials = prow;
     lat.chunk_len = chunk;
-    lat.chunks.reserve(ch.size());
-    DCRF_HIP_CHECK(hipMemcpyAsync(lat.chunks.get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, s));
-    lat.combines.reserve(cb.size());
-    if (!cb.empty())
-        DCRF_HIP_CHECK(hipMemcpyAsync(lat.combines.get(), cb.data(), sizeof(Combine) * cb.size(),
-                                      hipMemcpyHostToDevice, s));
-    DCRF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void features_from_dxN(const Geometry& g, const GeometryDev& gd, const float* feat_dxN, int d, float* out,

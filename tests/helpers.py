@@ -12,10 +12,12 @@ def image_case(W, H, L, seed=0, sites=5):
     return img, U, gt
 
 
-def build_both(W, H, L, U, terms, generic=False):
+def build_both(W, H, L, U, terms, generic=False, tile=None):
     """terms: list of ('gauss', sx, sy, compat, ntype) / ('bilat', sx, sy, sr, sg, sb, img, compat, ntype) /
-    ('feat', features (d,N), compat, ntype)."""
+    ('feat', features (d,N), compat, ntype).  tile: splat tile side of the GPU model."""
     g = capi.Model.create(W * H, L) if generic else capi.Model.create_2d(W, H, L)
+    if tile is not None:
+        g.set_splat_tile(tile)
     o = po.OracleCRF(N=W * H, L=L) if generic else po.OracleCRF(W=W, H=H, L=L)
     if U is not None:
         g.set_unary(U)

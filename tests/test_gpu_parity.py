@@ -175,6 +175,30 @@ def test_label_count_shapes(L, ck):
     np.testing.assert_allclose(g.apply(1, Q, True), o.apply(1, Q, True), rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("tile", [0, 1, 5, 16, 32, 4096])
+@pytest.mark.parametrize("generic", [False, True])
+def test_splat_tile_invariance(tile, generic):
+    """Splat segments cut at tile boundaries give the same messages up to summation
+    order: side 0 (the default, 16 at 21 labels), 1 (a segment per contributor), 5
+    and 16 run the LDS-tiled splat; 32 and 4096 (one tile per image) exceed its LDS and
+    run the global form."""
+    W, H, L = 120, 90, 21
+    img, U, _ = image_case(W, H, L, seed=19)
+    if generic:
+        yy, xx = np.mgrid[0:H, 0:W]
+        f = np.concatenate([np.stack([xx, yy]).reshape(2, -1) / 20.0,
+                            img.reshape(-1, 3).T / 13.0]).astype(np.float32)
+        terms = [("feat", f, 10.0, SYM)]
+    else:
+        terms = _std_terms(img)
+    g, o = build_both(W, H, L, U, terms, generic=generic, tile=tile)
+    g.set_splat_chunk(16)
+    lay = g.splat_layout()
+    assert lay["tiled"] == (tile in (0, 1, 5, 16)), lay
+    assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL
+    np.testing.assert_allclose(g.apply(0, o.start(), True), o.apply(0, o.start(), True), rtol=1e-5, atol=1e-5)
+
+
 def test_autotuned_chunk_keeps_results(tmp_path, monkeypatch):
     from pydensecrf_amd import tune
     monkeypatch.setenv("PYDENSECRF_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
