@@ -42,7 +42,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_meanfield_1024x768x21.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "meanfield_1024x768x21.txt")
 
 
 def measured_traffic(kernel_class):

@@ -183,11 +183,13 @@ int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int
 /* Set the chunk length of the splat's segmented reduction (0 = the default, 64):
  * vertices with more contributors are summed in pieces that are recombined in order. */
 int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
-/* Side T of the splat tiles (default 32; call before adding terms).  Pixels are
- * grouped into tiles (T x T squares of a 2-D image, runs of T*T pixels otherwise); a
- * vertex with more contributors than the splat chunk is cut at tile boundaries, and
- * the splat processes its segments tile by tile, so the input rows of a tile are
- * fetched from memory once.  No counterpart in the reference (a scheduling knob). */
+/* Side T of the splat tiles (0 = the default, 128; call before adding terms).  Pixels
+ * are grouped into tiles (T x T squares of a 2-D image, runs of T*T pixels otherwise);
+ * each vertex's contributors are cut at tile boundaries and the splat processes the
+ * segments tile by tile, so a tile's input rows are fetched from memory about once.
+ * When a tile's rows and entries fit a workgroup's LDS (T*T*L*4 bytes plus 8 per
+ * entry, at most 64 KiB) the splat stages them there (one workgroup per tile).  No
+ * counterpart in the reference (a scheduling knob). */
 int dcrf_set_splat_tile(dcrf_model *m, int side);
 /* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
  * (default 0.5): lower trades memory for shorter probe sequences. */

@@ -78,8 +78,9 @@ struct dcrf_model {
     DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
     bool tiled = false;                // splat through LDS tiles (else the global form)
     TiledSplat tsplat{};
-    DevBuf<int> g_pt;
-    DevBuf<float> g_wF, g_wT;
+    DevBuf<int2> g_entF, g_entT;  // splat entries (pixel, weight bits), forward / transposed
+    bool fused = false;           // inference through k_meanfield_fused
+    DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
     bool all_symmetric = true;
     std::vector<std::unique_ptr<TermLayout>> tl;
 
@@ -263,13 +264,13 @@ void compute_norm(dcrf_model& m, Term& t) {
     TermLattice& lat = t.lat;
     const int N = m.N(), d1 = lat.d + 1, V = lat.V, E = N * d1;
     hipStream_t s = m.stream;
-    DevBuf<int> pt, tseg, te;
-    DevBuf<float> w, sw, vals0, vals1, ones;
+    DevBuf<int> tseg, te;
+    DevBuf<int2> ent;
+    DevBuf<float> sw, vals0, vals1, ones;
     DevBuf<Chunk> ch;
     DevBuf<Combine> cb;
     DevBuf<int2> nbr;
-    pt.reserve(E);
-    w.reserve(E);
+    ent.reserve(E);
     sw.reserve(E);
     // one term: entry positions are the sorted positions; rows V+1.. hold the partials
     std::vector<Chunk> hch = lat.chunks;
@@ -285,7 +286,7 @@ void compute_norm(dcrf_model& m, Term& t) {
                         max_run(lat.chunk_tile_ptr)};
     const bool tiled = tiled_splat_lds(1, tg.side, ts.e_max, ts.s_max) > 0;
     launch_entry_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, tiled ? &tg : nullptr,
-                        pt.get(), w.get(), s);
+                        ent.get(), s);
     const int rows = V + 1 + lat.n_partials;
     vals0.reserve(rows);
     vals1.reserve(V + 1);
@@ -296,9 +297,9 @@ void compute_norm(dcrf_model& m, Term& t) {
     ones.reserve(N);
     launch_fill(ones.get(), 1.f, N, s);
     if (tiled)
-        launch_splat_tiled(ts, pt.get(), w.get(), ones.get(), vals0.get(), 1, s);
+        launch_splat_tiled(ts, ent.get(), ones.get(), vals0.get(), 1, s);
     else
-        launch_splat(ch.get(), (int)hch.size(), pt.get(), w.get(), ones.get(), vals0.get(), 1, s);
+        launch_splat(ch.get(), (int)hch.size(), ent.get(), ones.get(), vals0.get(), 1, s);
     launch_combine(cb.get(), (int)hcb.size(), vals0.get(), 1, s);
     BlurTable bt{};
     bt.nterms = 1;
@@ -465,9 +466,8 @@ void ensure_layout(dcrf_model& m) {
     m.vals[1].reserve((size_t)(V + 1) * L);
     launch_fill(m.vals[0].get(), 0.f, (size_t)m.rows0 * L, s);
     launch_fill(m.vals[1].get(), 0.f, (size_t)(V + 1) * L, s);
-    m.g_pt.reserve(E);
-    m.g_wF.reserve(E);
-    if (!m.all_symmetric) m.g_wT.reserve(E);
+    m.g_entF.reserve(E);
+    if (!m.all_symmetric) m.g_entT.reserve(E);
     m.tl.clear();
     m.tl.resize(T);
     std::vector<DevBuf<int>> staging(2 * T);  // live until the synchronisation below
@@ -482,10 +482,10 @@ void ensure_layout(dcrf_model& m) {
         upload(eb, ebase[k], s);
         const TileGeom* local = m.tiled ? &tg : nullptr;
         launch_entry_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, false),
-                            tptr.get(), eb.get(), NT, local, m.g_pt.get(), m.g_wF.get(), s);
+                            tptr.get(), eb.get(), NT, local, m.g_entF.get(), s);
         if (!m.all_symmetric)
             launch_entry_gather(lat.csr_e.get(), Ek, d1, lat.bary.get(), t.norm.get(), pre_norm(t.ntype, true),
-                                tptr.get(), eb.get(), NT, local, m.g_pt.get(), m.g_wT.get(), s);
+                                tptr.get(), eb.get(), NT, local, m.g_entT.get(), s);
         auto lay = std::make_unique<TermLayout>();
         lay->nbr.reserve((size_t)d1 * lat.V);
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
@@ -500,6 +500,19 @@ void ensure_layout(dcrf_model& m) {
                                  m.vbase[k], lay->vtx.get(), lay->swT.get(), s);
         }
         m.tl[k] = std::move(lay);
+    }
+    // interleaved slice pairs for the fused mean-field kernel
+    m.fused = fused_meanfield_ok(L, T, T > 0 ? m.terms[0]->lat.d + 1 : 0, T > 1 ? m.terms[1]->lat.d + 1 : 0);
+    if (m.fused) {
+        int ds = 0;
+        for (int k = 0; k < T; k++) ds += m.terms[k]->lat.d + 1;
+        m.g_slc.reserve((size_t)N * ds);
+        int off = 0;
+        for (int k = 0; k < T; k++) {
+            const int d1 = m.terms[k]->lat.d + 1;
+            launch_interleave_slice(m.tl[k]->vtx.get(), m.tl[k]->swF.get(), N, d1, ds, off, m.g_slc.get(), s);
+            off += d1;
+        }
     }
     m.prefixV.assign(std::max(m.max_d + 1, 0), 0);
     for (int st = 0; st <= m.max_d; st++)
@@ -559,13 +572,13 @@ double meanfield_bytes(const dcrf_model& m) { return splat_bytes(m) + m.N() * (d
 
 void run_filter(dcrf_model& m, const float* in, bool transpose, hipStream_t s) {
     const int L = m.L;
-    const float* w = (transpose && !m.all_symmetric) ? m.g_wT.get() : m.g_wF.get();
+    const int2* ent = (transpose && !m.all_symmetric) ? m.g_entT.get() : m.g_entF.get();
     {
         Timed t(m, s, KC_SPLAT, splat_bytes(m));
         if (m.tiled)
-            launch_splat_tiled(m.tsplat, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
+            launch_splat_tiled(m.tsplat, ent, in, m.vals[0].get(), L, s);
         else
-            launch_splat(m.g_chunks.get(), m.C_total, m.g_pt.get(), w, in, m.vals[0].get(), L, s);
+            launch_splat(m.g_chunks.get(), m.C_total, ent, in, m.vals[0].get(), L, s);
         t.done();
     }
     if (m.CB_total) {
@@ -596,7 +609,10 @@ void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStrea
     for (int it = 0; it < n_iter; it++) {
         run_filter(m, Q, false, s);
         Timed t(m, s, KC_MEANFIELD, meanfield_bytes(m));
-        launch_meanfield(U, Q, m.N(), m.L, st, nullptr, nullptr, s);
+        if (m.fused)
+            launch_meanfield_fused(U, Q, m.N(), m.L, st, m.g_slc.get(), s);
+        else
+            launch_meanfield(U, Q, m.N(), m.L, st, nullptr, nullptr, s);
         t.done();
     }
 }

@@ -142,7 +142,9 @@ struct TileMap {
     int n_tiles() const { return base.empty() ? 0 : base.back(); }
 };
 void make_tile_map(const Geometry& g, int side, TileMap& tm);
-// Default tile side for L labels: a tile's input rows take at most 24 KiB of LDS.
+// Default tile side (128: measured fastest at 1024x768 with 21 labels, where the
+// bilateral lattice's vertices span hundreds of pixels and smaller tiles multiply the
+// partial rows).
 int default_tile_side(int L);
 
 // Tile geometry as the kernels see it.
@@ -209,12 +211,12 @@ struct LaunchShape {
 int choose_labels_per_lane(int L);
 LaunchShape shape_for(int L);
 
-// Splat: vals[out] = sum over a segment's entries i of w[i] * in[pixel of i], in entry
-// order.  Global form: pt[i] is the pixel (row of `in`) of entry i.
-void launch_splat(const Chunk* chunks, int n_chunks, const int* pt, const float* w,
-                  const float* in, float* vals, int L, hipStream_t s);
+// Splat: vals[out] = sum over a segment's entries i of w_i * in[p_i], in entry order;
+// ent[i] = (p_i, bits of w_i).  Global form: p_i is the pixel (row of `in`).
+void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, float* vals, int L,
+                  hipStream_t s);
 // Tiled form: one block per tile stages the tile's input rows, entries and segments in
-// LDS; pt[i] is then the index of the entry's pixel inside its tile (row-major in the
+// LDS; p_i is then the index of the entry's pixel inside its tile (row-major in the
 // tile).  The entries of tile t are [tile_e[t], tile_e[t+1]), its segments
 // [tile_seg[t], tile_seg[t+1]).
 struct TiledSplat {
@@ -228,8 +230,7 @@ struct TiledSplat {
 // LDS bytes of the tiled splat (0 if it does not fit a block or L needs the block-mode
 // lane groups: use the global form).
 size_t tiled_splat_lds(int L, int side, int e_max, int s_max);
-void launch_splat_tiled(const TiledSplat& ts, const int* pt, const float* w, const float* in, float* vals,
-                        int L, hipStream_t s);
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, hipStream_t s);
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s);
 void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);
@@ -237,6 +238,14 @@ void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& b
 // tmp1/tmp2 optional outputs (energy, last term's message).
 void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1,
                       float* tmp2, hipStream_t s);
+// The same update for models of one term with d+1 in {3, 6} or two terms with
+// (d+1) = (3, 6) / (6, 3) (fused_meanfield_ok): slc[p] holds the (vertex, weight bits)
+// pairs of every term of pixel p, term 0 first (launch_interleave_slice builds it).
+bool fused_meanfield_ok(int L, int nterms, int d1_0, int d1_1);
+void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st, const int2* slc,
+                            hipStream_t s);
+void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,
+                             hipStream_t s);
 // out = compat_k(slice_k) for term index `which` of st.
 void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s);
 // Q0 = softmax(-U).
@@ -268,11 +277,11 @@ void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);
 // Entry arrays of one term: sorted entry j (csr_e[j] = p*(d+1)+r) goes to position
 // base[t] + j, t the tile of j (tile_ptr: first j of each tile; base null: position j);
-// pt = the pixel p, or its index inside its tile when `local` is given; w = the
-// barycentric weight, times norm[p] when use_norm.
+// ent = (the pixel p, or its index inside its tile when `local` is given; bits of the
+// barycentric weight, times norm[p] when use_norm).
 void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
-                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int* pt_out,
-                         float* w_out, hipStream_t s);
+                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int2* ent_out,
+                         hipStream_t s);
 void launch_slice_weights(const int* vtx, const float* bary, const float* norm, int use_norm,
                           float alpha, int N, int d1, int vbase, int* gvtx, float* sw,
                           hipStream_t s);

@@ -61,13 +61,13 @@ __global__ void k_fill(float* p, float v, size_t n) {
     if (i < n) p[i] = v;
 }
 
-// Splat: each group sums one chunk (<= C contributors of a vertex, pixel ascending)
-// sequentially; (pixel, weight) pairs and input pieces are fetched kBatch at a time.
+// Splat: each group sums one chunk (<= C contributors of a vertex, in entry order)
+// sequentially.  Entries are (pixel, weight bits) pairs; the pairs of the next kBatch
+// entries are loaded while the input pieces of the current ones are in flight.
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks, int n,
-                                                const int* __restrict__ pt, const float* __restrict__ w,
-                                                const float* __restrict__ in, float* __restrict__ vals, int L,
-                                                int S, int per) {
+                                                const int2* __restrict__ ent, const float* __restrict__ in,
+                                                float* __restrict__ vals, int L, int S, int per) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
     if (!G.on || c >= n) return;
@@ -76,45 +76,33 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
-    int i = ch.beg;
-    for (; i + kBatch <= ch.end; i += kBatch) {
-        int pp[kBatch];
-        float ww[kBatch], xx[kBatch][R];
+    int2 cur[kBatch];
+    if (ch.beg < ch.end) {
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            pp[u] = pt[i + u];
-            ww[u] = w[i + u];
-        }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)pp[u] * L, k0, L, xx[u]);
-#pragma unroll
-        for (int u = 0; u < kBatch; u++)
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const float prod = ww[u] * xx[u][r];
-                acc[r] = acc[r] + prod;
-            }
+        for (int u = 0; u < kBatch; u++) cur[u] = ent[min(ch.beg + u, ch.end - 1)];
     }
-    if (i < ch.end) {  // the tail as one batch: every load in flight, missing slots skipped
-        int pp[kBatch];
-        float ww[kBatch], xx[kBatch][R];
+    for (int i = ch.beg; i < ch.end; i += kBatch) {
+        float xx[kBatch][R];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int j = i + u < ch.end ? i + u : ch.end - 1;
-            pp[u] = pt[j];
-            ww[u] = w[j];
+        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)cur[u].x * L, k0, L, xx[u]);
+        const int i2 = i + kBatch;
+        int2 nxt[kBatch];
+        if (i2 < ch.end) {
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) nxt[u] = ent[min(i2 + u, ch.end - 1)];
         }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)pp[u] * L, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
             if (i + u < ch.end) {
+                const float wu = __int_as_float(cur[u].y);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const float prod = ww[u] * xx[u][r];
+                    const float prod = wu * xx[u][r];
                     acc[r] = acc[r] + prod;
                 }
             }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) cur[u] = nxt[u];
     }
     store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
 }
@@ -127,8 +115,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
 constexpr int kStage = 32;
 
 template <int R>
-__global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const int* __restrict__ pt,
-                                                            const float* __restrict__ w,
+__global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const int2* __restrict__ ent,
                                                             const float* __restrict__ in, float* __restrict__ vals,
                                                             int L, int S, int per, int q_max) {
     extern __shared__ float sh[];
@@ -138,15 +125,14 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
     const int e0 = ts.tile_e[t], ne = ts.tile_e[t + 1] - e0;
     const int c0 = ts.tile_seg[t], ns = ts.tile_seg[t + 1] - c0;
     const int rowf = tw * L;
-    const int n1 = th * rowf, n2 = n1 + ne, n3 = n2 + ne, n4 = n3 + 3 * ns;
+    const int n1 = th * rowf, n3 = n1 + 2 * ne, n4 = n3 + 3 * ns;
     const float inv_rowf = 1.0f / (float)rowf;
     const long long rskip = (long long)stride * L - rowf;  // gap between two tile rows in `in`
     const unsigned* __restrict__ qsrc = reinterpret_cast<const unsigned*>(in) + (size_t)p0 * L;
-    const unsigned* __restrict__ psrc = reinterpret_cast<const unsigned*>(pt) + e0;
-    const unsigned* __restrict__ wsrc = reinterpret_cast<const unsigned*>(w) + e0;
+    const unsigned* __restrict__ esrc = reinterpret_cast<const unsigned*>(ent + e0);
     const unsigned* __restrict__ csrc = reinterpret_cast<const unsigned*>(ts.chunks + c0);
     unsigned* shu = reinterpret_cast<unsigned*>(sh);
-    // LDS: [rows: q_max floats][pixel index: e_max][weight: e_max][segments: 3 s_max]
+    // LDS: [rows: q_max floats][entries: e_max (index, weight) pairs][segments: 3 s_max]
     for (int b = threadIdx.x; b < n4; b += kStage * kWaveBlock) {
         unsigned x[kStage];
 #pragma unroll
@@ -159,10 +145,8 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
                 else if ((r + 1) * rowf <= i)
                     r++;
                 x[u] = qsrc[i + r * rskip];
-            } else if (i < n2) {
-                x[u] = psrc[i - n1];
             } else if (i < n3) {
-                x[u] = wsrc[i - n2];
+                x[u] = esrc[i - n1];
             } else if (i < n4) {
                 const int j = i - n3;
                 const int c = j / 3;
@@ -173,16 +157,14 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
         for (int u = 0; u < kStage; u++) {
             const int i = b + u * kWaveBlock;
             if (i < n4) {
-                const int off = i < n1 ? 0 : (i < n2 ? q_max - n1 : (i < n3 ? q_max + ts.e_max - n2
-                                                                            : q_max + 2 * ts.e_max - n3));
+                const int off = i < n1 ? 0 : (i < n3 ? q_max - n1 : q_max + 2 * ts.e_max - n3);
                 shu[i + off] = x[u];
             }
         }
     }
     __syncthreads();
     const float* tq = sh;
-    const int* tp = reinterpret_cast<const int*>(sh + q_max);
-    const float* tw8 = sh + q_max + ts.e_max;
+    const int2* te = reinterpret_cast<const int2*>(sh + q_max);
     const int* tsg = reinterpret_cast<const int*>(sh + q_max + 2 * ts.e_max);
     LaneGroup<true, R> G(L, S, per, nullptr);
     if (!G.on) return;  // no barrier below
@@ -199,8 +181,9 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
             float ww[4], xx[4][R];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                pp[u] = tp[i + u];
-                ww[u] = tw8[i + u];
+                const int2 x = te[i + u];
+                pp[u] = x.x;
+                ww[u] = __int_as_float(x.y);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) load_piece<R>(tq + pp[u] * L, k0, L, xx[u]);
@@ -214,8 +197,9 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
         }
         for (; i < e; i++) {
             float xx[R];
-            const float ww = tw8[i];
-            load_piece<R>(tq + tp[i] * L, k0, L, xx);
+            const int2 x = te[i];
+            const float ww = __int_as_float(x.y);
+            load_piece<R>(tq + x.x * L, k0, L, xx);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const float prod = ww * xx[r];
@@ -359,6 +343,85 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
     }
 }
 
+// Mean-field update for models of one or two terms with d+1 = D0 (and D1) lattice
+// entries per pixel: slc holds every pixel's (vertex, weight bits) pairs of both terms
+// side by side, so all D0 + D1 vertex rows are gathered after one dependent load.  Same
+// arithmetic, in the same order, as k_meanfield.
+template <int R, int D0, int D1>
+__global__ void __launch_bounds__(kWaveBlock) k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
+                                                                int N, int L, int S, int per, SliceTable st,
+                                                                const int2* __restrict__ slc) {
+    constexpr int DS = D0 + D1;
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int k0 = G.k0;
+    int2 e[DS];
+    float u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = 0.0f;
+    if (active) {
+        const int2* sl = slc + (size_t)p * DS;
+#pragma unroll
+        for (int j = 0; j < DS; j++) e[j] = sl[j];
+        if (U) load_piece<R>(U + (size_t)p * L, k0, L, u);
+    } else {
+#pragma unroll
+        for (int j = 0; j < DS; j++) e[j] = make_int2(0, 0);  // vertex 0: a valid row, weight 0
+    }
+    float x[DS][R];
+#pragma unroll
+    for (int j = 0; j < DS; j++) load_piece<R>((j < D0 ? st.t[0].vals : st.t[1].vals) + (size_t)e[j].x * L, k0, L, x[j]);
+    float en[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) en[r] = -u[r];
+#pragma unroll
+    for (int k = 0; k < (D1 ? 2 : 1); k++) {
+        const SliceTerm& T = st.t[k];
+        const int jb = k ? D0 : 0, je = k ? DS : D0;
+        float m[R], c[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = 0.0f;
+#pragma unroll
+        for (int j = jb; j < je; j++) {
+            const float w = __int_as_float(e[j].y);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float prod = w * x[j][r];
+                m[r] = m[r] + prod;
+            }
+        }
+        if (T.kw) {
+            const float kw = T.kw[0];
+#pragma unroll
+            for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+        }
+        compat_piece<true, R>(T, m, c, G);
+#pragma unroll
+        for (int r = 0; r < R; r++) en[r] = en[r] - c[r];
+    }
+    const float mx = G.all_max(en);
+    float y[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) y[r] = expf(en[r] - mx);
+    const float sum = G.all_sum(y);
+    if (active) {
+        float q[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) q[r] = y[r] / sum;
+        store_piece<R>(Q + (size_t)p * L, k0, L, q);
+    }
+}
+
+// slc[p][off + j] = (vtx[p][j], bits of sw[p][j]) for j < d1; rows of ds pairs.
+__global__ void k_interleave_slice(const int* __restrict__ vtx, const float* __restrict__ sw, int N, int d1, int ds,
+                                   int off, int2* __restrict__ slc) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * d1) return;
+    const size_t p = i / d1, j = i - p * d1;
+    slc[p * ds + off + j] = make_int2(vtx[i], __float_as_int(sw[i]));
+}
+
 // out = compat_which(slice_which) (ckind < 0: the plain slice).
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int S, int per,
@@ -474,15 +537,15 @@ __global__ void k_remap_nbr(const int* __restrict__ src, int2* __restrict__ dst,
 __global__ void k_entry_gather(const int* __restrict__ csr_e, int E, int d1, const float* __restrict__ bary,
                                const float* __restrict__ norm, int use_norm, const int* __restrict__ tile_ptr,
                                const int* __restrict__ base, int n_tiles, TileGeom g, int local,
-                               int* __restrict__ pt, float* __restrict__ w) {
+                               int2* __restrict__ ent) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= E) return;
     const int e = csr_e[j];
     const int p = e / d1;
     const int pos = base ? base[last_le(tile_ptr, n_tiles, j)] + j : j;
-    pt[pos] = local ? tile_local(g, p) : p;
     const float b = bary[e];
-    w[pos] = use_norm ? b * norm[p] : b;
+    const float w = use_norm ? b * norm[p] : b;
+    ent[pos] = make_int2(local ? tile_local(g, p) : p, __float_as_int(w));
 }
 
 __global__ void k_slice_weights(const int* __restrict__ vtx, const float* __restrict__ bary,
@@ -545,34 +608,33 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
     } while (0)
 
 
-void launch_splat(const Chunk* chunks, int n_chunks, const int* csr_pt, const float* csr_w, const float* in,
-                  float* vals, int L, hipStream_t s) {
+void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, float* vals, int L,
+                  hipStream_t s) {
     if (!n_chunks) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), 0, s, chunks, n_chunks, csr_pt, csr_w, in, vals, L,
-                  sh.S, sh.per);
+    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), 0, s, chunks, n_chunks, ent, in, vals, L, sh.S,
+                  sh.per);
 }
 
 size_t tiled_splat_lds(int L, int side, int e_max, int s_max) {
     const LaunchShape sh = shape_for(L);
     if (!sh.wave) return 0;
-    const size_t words = (size_t)side * side * L + 2 * (size_t)e_max + 3 * (size_t)s_max;
+    const size_t words = (((size_t)side * side * L + 1) & ~(size_t)1) + 2 * (size_t)e_max + 3 * (size_t)s_max;
     const size_t bytes = words * sizeof(float);
     return bytes <= kTiledLdsMax ? bytes : 0;
 }
 
-void launch_splat_tiled(const TiledSplat& ts, const int* pt, const float* w, const float* in, float* vals, int L,
-                        hipStream_t s) {
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, hipStream_t s) {
     if (!ts.n_tiles) return;
     const LaunchShape sh = shape_for(L);
     const size_t lds = tiled_splat_lds(L, ts.geom.side, ts.e_max, ts.s_max);
     if (!lds) throw Error{DCRF_ERR_VALUE, "tiled splat does not fit the LDS of a block"};
-    const int q_max = ts.geom.side * ts.geom.side * L;
+    const int q_max = (ts.geom.side * ts.geom.side * L + 1) & ~1;  // the entry pairs start 8-byte aligned
     switch (sh.R) {
-        case 1: k_splat_tiled<1><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
-        case 2: k_splat_tiled<2><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
-        case 3: k_splat_tiled<3><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
-        default: k_splat_tiled<4><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, pt, w, in, vals, L, sh.S, sh.per, q_max); break;
+        case 1: k_splat_tiled<1><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 2: k_splat_tiled<2><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 3: k_splat_tiled<3><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        default: k_splat_tiled<4><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
     }
     DCRF_HIP_CHECK(hipGetLastError());
 }
@@ -588,6 +650,50 @@ void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& b
     if (!Vp) return;
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_blur, blocks_for(Vp, sh.items), 0, s, src, dst, Vp, L, sh.S, sh.per, bt, step, reverse);
+}
+
+bool fused_meanfield_ok(int L, int nterms, int d1_0, int d1_1) {
+    if (!shape_for(L).wave) return false;
+    if (nterms == 1) return d1_0 == 3 || d1_0 == 6;
+    if (nterms == 2) return (d1_0 == 3 && d1_1 == 6) || (d1_0 == 6 && d1_1 == 3);
+    return false;
+}
+
+void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,
+                             hipStream_t s) {
+    if (!N) return;
+    k_interleave_slice<<<blocks_for((size_t)N * d1), kThreads, 0, s>>>(vtx, sw, N, d1, ds, off, slc);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+#define DCRF_FUSED(R_, D0_, D1_)                                                                          \
+    k_meanfield_fused<R_, D0_, D1_><<<blocks_for(N, sh.items), kWaveBlock, 0, s>>>(U, Q, N, L, sh.S, sh.per, st, \
+                                                                                slc)
+
+template <int R>
+void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int N, int L, const SliceTable& st,
+                    const int2* slc, hipStream_t s) {
+    const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
+    if (a == 3 && b == 0) DCRF_FUSED(R, 3, 0);
+    else if (a == 6 && b == 0) DCRF_FUSED(R, 6, 0);
+    else if (a == 3 && b == 6) DCRF_FUSED(R, 3, 6);
+    else if (a == 6 && b == 3) DCRF_FUSED(R, 6, 3);
+    else throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+#undef DCRF_FUSED
+
+void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st, const int2* slc,
+                            hipStream_t s) {
+    const LaunchShape sh = shape_for(L);
+    if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
+        throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
+    switch (sh.R) {
+        case 1: launch_fused_r<1>(sh, U, Q, N, L, st, slc, s); break;
+        case 2: launch_fused_r<2>(sh, U, Q, N, L, st, slc, s); break;
+        case 3: launch_fused_r<3>(sh, U, Q, N, L, st, slc, s); break;
+        default: launch_fused_r<4>(sh, U, Q, N, L, st, slc, s); break;
+    }
 }
 
 void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1, float* tmp2,
@@ -629,12 +735,12 @@ void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row,
 }
 
 void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
-                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int* pt_out,
-                         float* w_out, hipStream_t s) {
+                         const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int2* ent_out,
+                         hipStream_t s) {
     if (!E) return;
     const TileGeom g = local ? *local : TileGeom{};
     k_entry_gather<<<blocks_for(E), kThreads, 0, s>>>(csr_e, E, d1, bary, norm, use_norm, tile_ptr, base, n_tiles, g,
-                                                      local ? 1 : 0, pt_out, w_out);
+                                                      local ? 1 : 0, ent_out);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

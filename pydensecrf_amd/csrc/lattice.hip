@@ -559,9 +559,8 @@ void build_lattice_d(const Geometry& g, const GeometryDev& gd, const TileMap& tm
 }  // namespace
 
 int default_tile_side(int L) {
-    int side = (int)std::sqrt(24.0 * 1024.0 / (4.0 * std::max(L, 1)));
-    side = std::max(4, std::min(64, side & ~3));
-    return side;
+    (void)L;
+    return 128;
 }
 
 void make_tile_map(const Geometry& g, int side, TileMap& tm) {

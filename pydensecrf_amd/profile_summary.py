@@ -9,7 +9,8 @@ TRACE_DIR holds the *_kernel_stats.csv of `rocprofv3 --kernel-trace --stats
 in the run, a copy of K KiB (bench.py --copy-calibration copies 262144 KiB three
 times): scale = K / counter reading.  With --copy-kib the summary ends with lines
     traffic <class> <HBM bytes per launch>
-for the mean-field kernel classes (the kernel of the class with the most dispatches).
+for the mean-field kernel classes (of the kernels whose name starts with the class's
+kernel prefix, the one with the most dispatches).
 """
 import csv
 import glob
@@ -81,7 +82,7 @@ def main(argv):
             print("copy calibration: %.0f KiB copy reads FETCH_SIZE %.1f, WRITE_SIZE %.1f -> scales %.3f, %.3f"
                   % (copy_kib, copy_max["FETCH_SIZE"], copy_max["WRITE_SIZE"], fs, ws))
             for cls, prefix in CLASSES.items():
-                names = [k for k in merged if _short(k).startswith(prefix + "<") or _short(k) == prefix]
+                names = [k for k in merged if _short(k).startswith(prefix)]
                 if not names:
                     continue
                 k = max(names, key=lambda n: calls[n])

@@ -128,6 +128,10 @@ def run_fd(path, jobs):
     parts = {"unary": rel[:nu], "compat": rel[nu:nu + len(pairs)], "kweights": rel[nu + len(pairs):]}
     for k, r in parts.items():
         print("%-8s n=%4d  max rel err %.3g  median %.3g" % (k, r.size, r.max(), np.median(r)))
+    labels = ["u%d" % i for i in g["sel"]] + ["c%d:%d,%d" % p for p in pairs] + ["kw0", "kw1"]
+    for i in np.argsort(-rel)[:8]:
+        print("  worst %-12s analytic % .6e  fd % .6e  rel %.3g  |err|/max|fd| %.3g"
+              % (labels[i], an[i], fd[i], rel[i], abs(an[i] - fd[i]) / np.abs(fd).max()))
     print("overall max rel err %.3g (bar 1e-3)" % rel.max())
     return rel.max()
 
