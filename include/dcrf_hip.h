@@ -180,7 +180,7 @@ int dcrf_get_lattice(const dcrf_model *m, int term, int instance, int *offset, i
                      float *bary, int *nbr, int16_t *keys, float *norm);
 /* Vertex count of one instance of a term. */
 int dcrf_term_instance_vertices(const dcrf_model *m, int term, int instance, int *V);
-/* Set the chunk length of the splat's segmented reduction (0 = the default, 64):
+/* Set the chunk length of the splat's segmented reduction (0 = the default, 32):
  * vertices with more contributors are summed in pieces that are recombined in order. */
 int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
 /* Side T of the splat tiles (0 = the default, 128; call before adding terms).  Pixels

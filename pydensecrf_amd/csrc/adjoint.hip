@@ -22,8 +22,6 @@ namespace dcrf {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 32;    // pixels per tile of the L x L statistic
-constexpr int kPairs = 4;    // (a, c) pairs per thread per launch of the L x L statistic
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
 // sumAndNormalize of x = Q .* g over the lane group: b = sum(x) q - x.
@@ -71,6 +69,8 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
                                                   float* __restrict__ dU, int N, int L, int S, int per,
                                                   SliceTable st) {
     extern __shared__ float sh[];
+    float* shc = sh + blockDim.x;
+    stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
 #pragma unroll
                 for (int r = 0; r < R; r++) m[r] = m[r] * kw;
             }
-            compat_piece<WAVE, R>(T, m, c, G);  // C^T = C (W is symmetric)
+            compat_piece<WAVE, R>(T, m, c, G, shc);  // C^T = C (W is symmetric)
 #pragma unroll
             for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[r];
         }
@@ -152,54 +152,70 @@ __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b,
     }
 }
 
-// partial[block][j] for pairs j in [base, base + 256*kPairs): G[a][c] = sum_p b[p][a] M[p][c].
-__global__ void k_grad_matrix(const float* __restrict__ b, const float* __restrict__ M, int N, int L, int base,
-                              float* __restrict__ partial) {
-    extern __shared__ float tiles[];
-    float* tb = tiles;
-    float* tm = tiles + kTile * L;
-    const int t = threadIdx.x;
-    const int LL = L * L;
-    int pa[kPairs], pc[kPairs];
-    float acc[kPairs];
+// The L x L statistic G[a][c] = sum_p b[p][a] M[p][c] is a tall-skinny GEMM (K = N
+// pixels): one wave per (32 x 32 output tile, pixel range) accumulates it on the
+// matrix cores with v_mfma_f32_32x32x2_f32, two pixels per instruction, kMfmaBatch
+// pixel pairs of operands in flight; lane l supplies A[l % 32][l / 32] = b[p + l/32][a]
+// and B[l / 32][l % 32] = M[p + l/32][c] (labels past L read as 0).  The range's tile
+// goes to partial[range][a * L + c]; k_reduce_cols adds the ranges in double.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kMfmaBatch = 8;
+
+__global__ void __launch_bounds__(256) k_grad_mfma(const float* __restrict__ b, const float* __restrict__ M, int N,
+                                                   int L, int tiles1, int ppw, int n_waves,
+                                                   float* __restrict__ partial) {
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= n_waves) return;  // whole waves only; no barrier in this kernel
+    const int lane = threadIdx.x & 63;
+    const int tiles2 = tiles1 * tiles1;
+    const int range = w / tiles2, tile = w - range * tiles2;
+    const int ta = tile / tiles1, tc = tile - ta * tiles1;
+    const int i = lane & 31, k = lane >> 5;
+    const int a = ta * 32 + i, c = tc * 32 + i;
+    const int ac = min(a, L - 1), cc = min(c, L - 1);  // clamped: loads stay in bounds
+    const float ma = a < L ? 1.0f : 0.0f, mc = c < L ? 1.0f : 0.0f;
+    const int p0 = range * ppw, p1 = min(N, p0 + ppw);
+    floatx16 acc;
 #pragma unroll
-    for (int i = 0; i < kPairs; i++) {
-        const int j = base + t + i * kThreads;
-        pa[i] = j < LL ? j / L : 0;
-        pc[i] = j < LL ? j - (j / L) * L : 0;
-        acc[i] = 0.0f;
-    }
-    for (int p0 = blockIdx.x * kTile; p0 < N; p0 += gridDim.x * kTile) {
-        for (int x = t; x < kTile * L; x += kThreads) {
-            const bool in = p0 + x / L < N;
-            tb[x] = in ? b[(size_t)p0 * L + x] : 0.0f;
-            tm[x] = in ? M[(size_t)p0 * L + x] : 0.0f;
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    for (int p = p0; p < p1; p += 2 * kMfmaBatch) {
+        float av[kMfmaBatch], bv[kMfmaBatch];
+#pragma unroll
+        for (int u = 0; u < kMfmaBatch; u++) {
+            const int pp = p + 2 * u + k;
+            const int pc = min(pp, p1 - 1);
+            av[u] = b[(size_t)pc * L + ac];
+            bv[u] = M[(size_t)pc * L + cc];
         }
-        __syncthreads();
-        for (int j = 0; j < kTile; j++) {
 #pragma unroll
-            for (int i = 0; i < kPairs; i++) {
-                const float prod = tb[j * L + pa[i]] * tm[j * L + pc[i]];
-                acc[i] = acc[i] + prod;
-            }
+        for (int u = 0; u < kMfmaBatch; u++) {
+            const float live = p + 2 * u + k < p1 ? 1.0f : 0.0f;  // pixels past the range add 0
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u] * (ma * live), bv[u] * mc, acc, 0, 0, 0);
         }
-        __syncthreads();
     }
+    float* out = partial + (size_t)range * L * L;
 #pragma unroll
-    for (int i = 0; i < kPairs; i++) {
-        const int j = base + t + i * kThreads;
-        if (j < LL) partial[(size_t)blockIdx.x * (kThreads * kPairs) + (j - base)] = acc[i];
+    for (int r = 0; r < 16; r++) {
+        const int row = ta * 32 + 8 * (r >> 2) + 4 * k + (r & 3), col = tc * 32 + i;
+        if (row < L && col < L) out[(size_t)row * L + col] = acc[r];
     }
 }
 
-// acc[off + j] += sum over blocks of partial[block][j] (double, block order).
-__global__ void k_reduce_add(const float* __restrict__ partial, int nblocks, int stride, int width,
-                             double* __restrict__ acc, int off) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= width) return;
+// acc[off + j] += sum over k < nblocks of partial[k * stride + j], in double: one block
+// per column j, a fixed split over the threads, then a fixed tree (deterministic).
+__global__ void k_reduce_cols(const float* __restrict__ partial, int nblocks, int stride, double* __restrict__ acc,
+                              int off) {
+    __shared__ double red[256];
+    const int j = blockIdx.x, t = threadIdx.x;
     double s = 0.0;
-    for (int k = 0; k < nblocks; k++) s += (double)partial[(size_t)k * stride + j];
-    acc[off + j] += s;
+    for (int k = t; k < nblocks; k += blockDim.x) s += (double)partial[(size_t)k * stride + j];
+    red[t] = s;
+    __syncthreads();
+    for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) acc[off + j] += red[0];
 }
 
 __global__ void k_fill_double(double* p, size_t n) {
@@ -282,10 +298,12 @@ void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N
                   sh.per);
 }
 
-void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s) {
+void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st_in, hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, b, dU, N, L, sh.S, sh.per,
-                  st);
+    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, Q, b, dU, N, L, sh.S,
+                  sh.per, st);
 }
 
 int grad_partial_blocks(int N, int L) {
@@ -294,25 +312,33 @@ int grad_partial_blocks(int N, int L) {
     return need < 1024 ? need : 1024;
 }
 
+size_t grad_partial_floats() { return (size_t)1 << 20; }
+
 void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
                       hipStream_t s) {
-    const int nb = grad_partial_blocks(N, L);
     if (!full) {
+        const int nb = grad_partial_blocks(N, L);
         const LaunchShape sh = shape_for(L);
         DCRF_DISPATCH(sh, k_grad_diag, nb, sizeof(float) * sh.block * sh.R, s, b, M, N, L, sh.S, sh.per, partial);
-        k_reduce_add<<<blocks_for(L), kThreads, 0, s>>>(partial, nb, L, L, acc, 0);
+        k_reduce_cols<<<L, 256, 0, s>>>(partial, nb, L, acc, 0);
         DCRF_HIP_CHECK(hipGetLastError());
         return;
     }
-    const int LL = L * L, chunk = kThreads * kPairs;
-    const size_t shm = sizeof(float) * 2 * kTile * L;
-    for (int base = 0; base < LL; base += chunk) {
-        k_grad_matrix<<<nb, kThreads, shm, s>>>(b, M, N, L, base, partial);
-        DCRF_HIP_CHECK(hipGetLastError());
-        const int width = LL - base < chunk ? LL - base : chunk;
-        k_reduce_add<<<blocks_for(width), kThreads, 0, s>>>(partial, nb, chunk, width, acc, base);
-        DCRF_HIP_CHECK(hipGetLastError());
-    }
+    // pixel ranges: as many as the partial buffer holds (L*L floats each), at least 16
+    // pixels per range; an even range length keeps the MFMA pixel pairs aligned
+    const long long LL = (long long)L * L;
+    long long ranges = std::max<long long>(1, std::min<long long>((long long)grad_partial_floats() / LL, (N + 15) / 16));
+    int ppw = (int)((N + ranges - 1) / ranges);
+    ppw = (ppw + 1) & ~1;
+    ranges = (N + ppw - 1) / ppw;
+    const int tiles1 = (L + 31) / 32;
+    const long long waves = ranges * tiles1 * tiles1;
+    if (waves > 0x7fffffffll / 64) throw Error{DCRF_ERR_VALUE, "gradient statistic too large"};
+    k_grad_mfma<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(b, M, N, L, tiles1, ppw, (int)waves, partial);
+    DCRF_HIP_CHECK(hipGetLastError());
+    const int threads = ranges >= 256 ? 256 : (ranges >= 64 ? 64 : 32);
+    k_reduce_cols<<<(unsigned)LL, threads, 0, s>>>(partial, (int)ranges, (int)LL, acc, 0);
+    DCRF_HIP_CHECK(hipGetLastError());
 }
 
 void launch_fill_double(double* p, size_t n, hipStream_t s) {

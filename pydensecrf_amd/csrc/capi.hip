@@ -30,7 +30,7 @@ void set_error(int code, const std::string& msg) {
 
 namespace {
 
-constexpr int kDefaultChunk = 64;
+constexpr int kDefaultChunk = 32;
 
 struct TermLayout {
     DevBuf<int2> nbr;    // [(d+1)][V] global rows
@@ -77,6 +77,7 @@ struct dcrf_model {
     DevBuf<Combine> g_comb;
     DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
     bool tiled = false;                // splat through LDS tiles (else the global form)
+    size_t lds_cap = 0;                // LDS bytes one workgroup may allocate
     TiledSplat tsplat{};
     DevBuf<int2> g_entF, g_entT;  // splat entries (pixel, weight bits), forward / transposed
     bool fused = false;           // inference through k_meanfield_fused
@@ -204,6 +205,10 @@ void ensure_device(dcrf_model& m) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
         throw Error{DCRF_ERR_DEVICE, "no HIP device is available"};
     DCRF_HIP_CHECK(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+    int dev = 0, lds = 0;
+    DCRF_HIP_CHECK(hipGetDevice(&dev));
+    DCRF_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    m.lds_cap = (size_t)std::max(lds, 0);
     const Geometry& g = m.geo;
     m.geod.pix_base.reserve(g.pix_base.size());
     m.geod.pad_base.reserve(g.pad_base.size());
@@ -259,6 +264,13 @@ void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t s) {
         DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
 }
 
+// The splat's wave mode loads a wave's entries as one range: consecutive segments must
+// adjoin (segment c+1 begins where segment c ends), which the tile-major layout gives.
+void require_adjoining(const std::vector<Chunk>& ch) {
+    for (size_t c = 1; c < ch.size(); c++)
+        if (ch[c].beg != ch[c - 1].end) throw Error{DCRF_ERR_STATE, "splat layout: segments do not adjoin"};
+}
+
 // norm = K*1 for one term (pairwise.cpp:44), then the normalisation of :46-61.
 void compute_norm(dcrf_model& m, Term& t) {
     TermLattice& lat = t.lat;
@@ -275,6 +287,7 @@ void compute_norm(dcrf_model& m, Term& t) {
     // one term: entry positions are the sorted positions; rows V+1.. hold the partials
     std::vector<Chunk> hch = lat.chunks;
     for (auto& c : hch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
+    require_adjoining(hch);
     std::vector<Combine> hcb = lat.combines;
     for (auto& c : hcb) c.first += V + 1;
     upload(ch, hch, s);
@@ -284,7 +297,7 @@ void compute_norm(dcrf_model& m, Term& t) {
     const TileGeom tg = tile_geom(m);
     const TiledSplat ts{ch.get(), tseg.get(), te.get(), lat.n_tiles, tg, max_run(lat.tile_ptr),
                         max_run(lat.chunk_tile_ptr)};
-    const bool tiled = tiled_splat_lds(1, tg.side, ts.e_max, ts.s_max) > 0;
+    const bool tiled = tiled_splat_lds(1, tg.side, ts.e_max, ts.s_max, m.lds_cap) > 0;
     launch_entry_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, tiled ? &tg : nullptr,
                         ent.get(), s);
     const int rows = V + 1 + lat.n_partials;
@@ -297,7 +310,7 @@ void compute_norm(dcrf_model& m, Term& t) {
     ones.reserve(N);
     launch_fill(ones.get(), 1.f, N, s);
     if (tiled)
-        launch_splat_tiled(ts, ent.get(), ones.get(), vals0.get(), 1, s);
+        launch_splat_tiled(ts, ent.get(), ones.get(), vals0.get(), 1, m.lds_cap, s);
     else
         launch_splat(ch.get(), (int)hch.size(), ent.get(), ones.get(), vals0.get(), 1, s);
     launch_combine(cb.get(), (int)hcb.size(), vals0.get(), 1, s);
@@ -446,6 +459,7 @@ void ensure_layout(dcrf_model& m) {
     gts[NT] = (int)gch.size();
     gte[NT] = e;
     if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
+    require_adjoining(gch);
     for (int k : m.order)
         for (Combine c : m.terms[k]->lat.combines) {
             c.v += m.vbase[k];
@@ -460,7 +474,7 @@ void ensure_layout(dcrf_model& m) {
     upload(m.g_tile_e, gte, s);
     const TileGeom tg = tile_geom(m);
     m.tsplat = TiledSplat{m.g_chunks.get(), m.g_tile_seg.get(), m.g_tile_e.get(), NT, tg, e_max, s_max};
-    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max) > 0;
+    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
 
     m.vals[0].reserve((size_t)m.rows0 * L);
     m.vals[1].reserve((size_t)(V + 1) * L);
@@ -576,7 +590,7 @@ void run_filter(dcrf_model& m, const float* in, bool transpose, hipStream_t s) {
     {
         Timed t(m, s, KC_SPLAT, splat_bytes(m));
         if (m.tiled)
-            launch_splat_tiled(m.tsplat, ent, in, m.vals[0].get(), L, s);
+            launch_splat_tiled(m.tsplat, ent, in, m.vals[0].get(), L, m.lds_cap, s);
         else
             launch_splat(m.g_chunks.get(), m.C_total, ent, in, m.vals[0].get(), L, s);
         t.done();
@@ -657,7 +671,7 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     for (int k = 0; k < T; k++) goff[k + 1] = goff[k] + stat_len(m.terms[k]->ckind, L);
     m.gacc.reserve(std::max<size_t>(goff[T], 1));
     launch_fill_double(m.gacc.get(), goff[T], s);
-    m.partial.reserve((size_t)grad_partial_blocks(N, L) * std::max(L, 1024));
+    m.partial.reserve(std::max(grad_partial_floats(), (size_t)grad_partial_blocks(N, L) * L));
     for (int it = n - 1; it >= 0; it--) {
         for (int k = 0; k < T; k++)
             launch_grad_stat(b, m.mcache.get() + ((size_t)it * T + k) * NL, N, L,

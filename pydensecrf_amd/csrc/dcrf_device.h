@@ -1,11 +1,11 @@
 // Device helpers shared by the lattice-filter, mean-field and adjoint kernels.
 //
-// Lane groups.  A group of S = ceil(L / R) lanes owns one item (a splat chunk, a
-// vertex, a pixel); lane l owns the R consecutive labels [l*R, l*R + R) of the item's
-// row (the last lane may own fewer), so a row of L floats is read and written as S
-// contiguous R-float pieces (dwordx R accesses).  R is chosen per L (choose_labels_per_lane)
-// to waste as few label slots as possible; per-item work (addresses, reductions, exp)
-// is shared by R labels.
+// Lane groups.  A group of S = L / R lanes owns one item (a splat chunk, a vertex, a
+// pixel); lane l owns the R consecutive labels [l*R, l*R + R) of the item's row, so a
+// row of L floats is read and written as S contiguous R-float pieces (dwordx R
+// accesses).  R is the largest of 4, 3, 2 dividing L, else 1 (choose_labels_per_lane):
+// no lane holds a partial piece, so no access is conditional; per-item work (addresses,
+// reductions, exp) is shared by R labels.
 //   WAVE = true  (S <= 64): 64 / S groups per 64-wide wave, the block is 4 waves.  A
 //                 group never spans waves; group-wide max / sum / peer reads are
 //                 ds_bpermute shuffles with no block barrier.
@@ -94,31 +94,23 @@ struct __attribute__((aligned(4))) Piece {
     float v[R];
 };
 
-// Labels k0..k0+R-1 of a row (zeros past L).
+// Labels k0..k0+R-1 of a row: one R-float access (R divides L, so every lane's piece
+// is whole and no access is conditional: loads issued in a row stay independent).
 template <int R>
 __device__ __forceinline__ void load_piece(const float* __restrict__ row, int k0, int L, float (&x)[R]) {
-    if (k0 + R <= L) {
-        const Piece<R> p = *reinterpret_cast<const Piece<R>*>(row + k0);
+    (void)L;
+    const Piece<R> p = *reinterpret_cast<const Piece<R>*>(row + k0);
 #pragma unroll
-        for (int r = 0; r < R; r++) x[r] = p.v[r];
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) x[r] = k0 + r < L ? row[k0 + r] : 0.0f;
-    }
+    for (int r = 0; r < R; r++) x[r] = p.v[r];
 }
 
 template <int R>
 __device__ __forceinline__ void store_piece(float* __restrict__ row, int k0, int L, const float (&x)[R]) {
-    if (k0 + R <= L) {
-        Piece<R> p;
+    (void)L;
+    Piece<R> p;
 #pragma unroll
-        for (int r = 0; r < R; r++) p.v[r] = x[r];
-        *reinterpret_cast<Piece<R>*>(row + k0) = p;
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if (k0 + r < L) row[k0 + r] = x[r];
-    }
+    for (int r = 0; r < R; r++) p.v[r] = x[r];
+    *reinterpret_cast<Piece<R>*>(row + k0) = p;
 }
 
 template <bool WAVE, int R>
@@ -147,7 +139,7 @@ struct LaneGroup {
         return WAVE ? (b * (kWaveBlock / 64) + (t >> 6)) * per + g : b * per + g;
     }
     __device__ static __forceinline__ int items_per_block(int per) { return WAVE ? (kWaveBlock / 64) * per : per; }
-    __device__ __forceinline__ bool has(int r) const { return k0 + r < L; }
+    __device__ __forceinline__ bool has(int r) const { return true; }  // R divides L: every slot is a label
 
     // Value of lane s of this group (all lanes must call).
     __device__ __forceinline__ float lane_value(float x, int s) {
@@ -236,12 +228,29 @@ __device__ __forceinline__ void slice_piece(const SliceTerm& T, int p, int k0, i
         }
 }
 
+// Copy the Matrix compatibilities that have an LDS slot (cw_lds >= 0) into the compat
+// region shc.  Every thread of the block must call it (one barrier when anything is
+// staged; the condition is uniform).
+__device__ __forceinline__ void stage_compat(const SliceTable& st, int L, float* shc) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < DCRF_MAX_TERMS; k++)
+        if (k < st.nterms && st.t[k].cw_lds >= 0) {
+            any = true;
+            const float* src = st.t[k].cw;
+            float* dst = shc + st.t[k].cw_lds;
+            for (int i = threadIdx.x; i < L * L; i += blockDim.x) dst[i] = src[i];
+        }
+    if (any) __syncthreads();
+}
+
 // Compatibility transform of the lane's message values (labelcompatibility.cpp:46-48,
 // 66-69, 82-84).  Matrix reads the pixel's whole message vector from its group, in
-// label order.  All lanes must call (Matrix synchronises in block mode).
+// label order, and its weights from the LDS copy when there is one.  All lanes must
+// call (Matrix synchronises in block mode).
 template <bool WAVE, int R>
 __device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m)[R], float (&c)[R],
-                                             LaneGroup<WAVE, R>& G) {
+                                             LaneGroup<WAVE, R>& G, const float* shc) {
     if (T.ckind == DCRF_COMPAT_POTTS) {
         const float nw = -T.cw[0];
 #pragma unroll
@@ -254,6 +263,7 @@ __device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m
         return;
     }
     const int L = G.L;
+    const float* cw = T.cw_lds >= 0 ? shc + T.cw_lds : T.cw;
 #pragma unroll
     for (int r = 0; r < R; r++) c[r] = 0.0f;
     for (int s = 0; s < G.S; s++) {
@@ -265,7 +275,7 @@ __device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if (G.has(r)) {
-                        const float prod = T.cw[(size_t)(G.k0 + r) * L + b] * mb;
+                        const float prod = cw[(size_t)(G.k0 + r) * L + b] * mb;
                         c[r] = c[r] + prod;
                     }
             }

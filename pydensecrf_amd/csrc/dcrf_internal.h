@@ -193,11 +193,17 @@ struct SliceTerm {
     const float* cw;    // device: Potts [1], Diagonal [L], Matrix [L*L] (symmetric)
     const float* kw;    // device: the term's weight (one float), null = 1
     float* mout;        // optional output of the raw message K~Q [N][L] (training cache)
+    int cw_lds = -1;    // Matrix: offset (floats) of the LDS copy of cw in the kernel's compat
+                        // region, or -1 to read cw from global memory (set by the launchers)
 };
 struct SliceTable {
     int nterms;
     SliceTerm t[DCRF_MAX_TERMS];
 };
+// Give each Matrix term an LDS copy of its L x L compatibility (cw_lds) while they fit
+// kCompatLdsFloats in total; returns the floats of the compat LDS region.
+constexpr int kCompatLdsFloats = 8192;
+int assign_compat_lds(SliceTable& st, int L);
 
 // Lane-group geometry of the per-item kernels (see dcrf_device.h).
 struct LaunchShape {
@@ -227,10 +233,11 @@ struct TiledSplat {
     TileGeom geom;
     int e_max, s_max;  // most entries / segments of one tile
 };
-// LDS bytes of the tiled splat (0 if it does not fit a block or L needs the block-mode
-// lane groups: use the global form).
-size_t tiled_splat_lds(int L, int side, int e_max, int s_max);
-void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, hipStream_t s);
+// LDS bytes of the tiled splat (0 if it exceeds lds_cap, the LDS one workgroup may
+// allocate, or L needs the block-mode lane groups: use the global form).
+size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap);
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, size_t lds_cap,
+                        hipStream_t s);
 void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s);
 void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);
@@ -264,8 +271,9 @@ void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N
 // b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) in place; dU += b.
 void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s);
 // acc += sum_p b_p M_p^T (full L*L when `full`, else its diagonal [L]); partial holds
-// grad_partial_blocks(N, L) * max(L, 1024) floats.
+// max(grad_partial_floats(), grad_partial_blocks(N, L) * L) floats.
 int grad_partial_blocks(int N, int L);
+size_t grad_partial_floats();
 void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
                       hipStream_t s);
 void launch_fill_double(double* p, size_t n, hipStream_t s);

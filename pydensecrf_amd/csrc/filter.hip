@@ -4,11 +4,11 @@
 // DenseKernel::filter (pairwise.cpp:63-80), LabelCompatibility::apply
 // (labelcompatibility.cpp:46-87) and expAndNormalize (densecrf.cpp:98-106).
 //
-// Thread mapping: a lane group of L consecutive threads handles one item (a splat
-// chunk, a lattice vertex or a pixel), lane l owning label l, so every vertex/pixel
-// row of L floats is read and written as one contiguous segment.  A block holds
-// block/L groups.  All reductions run in a fixed order: results are deterministic
-// from run to run.
+// Thread mapping: a lane group of S = L / R consecutive threads handles one item (a
+// splat chunk, a lattice vertex or a pixel), lane l owning the R labels [l*R, l*R+R),
+// so every vertex/pixel row of L floats is read and written as S contiguous R-float
+// pieces (dcrf_device.h).  All reductions run in a fixed order: results are
+// deterministic from run to run.
 //   splat    each chunk (<= C contributors of one vertex, pixel ascending) is summed
 //            sequentially; vertices with several chunks write partial rows that the
 //            combine kernel adds in chunk order.  No atomics.
@@ -26,19 +26,12 @@
 
 namespace dcrf {
 
-// Labels per lane: one lane per item for L <= 4; otherwise the R in {4, 3, 2} that
-// wastes the fewest label slots (ties go to the larger R).
+// Labels per lane: the largest R in {4, 3, 2} that divides L, else 1 (so every lane's
+// piece of a row is whole: no partial lanes, no conditional accesses).
 int choose_labels_per_lane(int L) {
-    if (L <= 4) return L;
-    int best = 4, waste = ((L + 3) / 4) * 4 - L;
-    for (int R = 3; R >= 2; R--) {
-        const int w = ((L + R - 1) / R) * R - L;
-        if (w < waste) {
-            best = R;
-            waste = w;
-        }
-    }
-    return best;
+    for (int R = 4; R >= 2; R--)
+        if (L % R == 0) return R;
+    return 1;
 }
 
 LaunchShape shape_for(int L) {
@@ -53,7 +46,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBatch = 8;  // contributors / partial rows loaded together
-constexpr size_t kTiledLdsMax = 64 * 1024;  // LDS of one tiled-splat block
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
 __global__ void k_fill(float* p, float v, size_t n) {
@@ -62,47 +54,94 @@ __global__ void k_fill(float* p, float v, size_t n) {
 }
 
 // Splat: each group sums one chunk (<= C contributors of a vertex, in entry order)
-// sequentially.  Entries are (pixel, weight bits) pairs; the pairs of the next kBatch
-// entries are loaded while the input pieces of the current ones are in flight.
+// sequentially.  Entries are (pixel, weight bits) pairs.
+//   WAVE mode: the chunks of a wave are consecutive and their entry ranges adjoin (the
+//   layout guarantees chunk c+1 begins where chunk c ends), so the wave loads its whole
+//   entry range cooperatively (64 consecutive pairs per load instruction) into an LDS
+//   window of kWin pairs, and each group reads its pairs from there: one cache line
+//   serves 16 pairs instead of one load per lane and pair.  Input rows are gathered
+//   kBatch at a time.
+//   block mode (a group wider than a wave): the pairs are loaded by every lane.
+constexpr int kWin = 512;  // entry pairs per wave window
+
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks, int n,
                                                 const int2* __restrict__ ent, const float* __restrict__ in,
                                                 float* __restrict__ vals, int L, int S, int per) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
-    if (!G.on || c >= n) return;
     const int k0 = G.k0;
-    const Chunk ch = chunks[c];
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
-    int2 cur[kBatch];
-    if (ch.beg < ch.end) {
+    if (WAVE) {
+        extern __shared__ int2 win_all[];
+        int2* win = win_all + (threadIdx.x >> 6) * (kWin + kBatch);
+        const int lane = threadIdx.x & 63;
+        const int cw0 = c - G.g;  // the wave's first chunk
+        if (cw0 >= n) return;     // the whole wave is idle (no block barrier below)
+        const bool mine = G.on && c < n;
+        const Chunk ch = chunks[mine ? c : cw0];
+        const int r0 = chunks[cw0].beg, r1 = chunks[min(cw0 + per - 1, n - 1)].end;
+        for (int w0 = r0; w0 < r1; w0 += kWin) {  // wave-uniform
+            const int wn = min(kWin, r1 - w0);
+            // all loads first (clamped, unconditional), then the stores: the loads stay
+            // in flight together
+            int2 t[kWin / 64];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) cur[u] = ent[min(ch.beg + u, ch.end - 1)];
+            for (int u = 0; u < kWin / 64; u++) t[u] = ent[w0 + min(lane + 64 * u, wn - 1)];
+            __builtin_amdgcn_wave_barrier();  // the previous window's reads come first
+#pragma unroll
+            for (int u = 0; u < kWin / 64; u++)
+                if (lane + 64 * u < wn) win[lane + 64 * u] = t[u];
+            if (lane < kBatch) win[wn + lane] = make_int2(0, 0);  // padding: pixel 0, weight 0
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (mine) {
+                const int a = max(ch.beg, w0) - w0, b = min(ch.end, w0 + wn) - w0;
+                for (int i = a; i < b; i += kBatch) {
+                    int2 e[kBatch];
+                    float xx[kBatch][R];
+#pragma unroll
+                    for (int u = 0; u < kBatch; u++) e[u] = win[i + u];  // i + u < wn + kBatch
+#pragma unroll
+                    for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * L, k0, L, xx[u]);
+#pragma unroll
+                    for (int u = 0; u < kBatch; u++)
+                        if (i + u < b) {
+                            const float wu = __int_as_float(e[u].y);
+#pragma unroll
+                            for (int r = 0; r < R; r++) {
+                                const float prod = wu * xx[u][r];
+                                acc[r] = acc[r] + prod;
+                            }
+                        }
+                }
+            }
+        }
+        if (mine) store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
+        return;
     }
+    if (!G.on || c >= n) return;
+    const Chunk ch = chunks[c];
     for (int i = ch.beg; i < ch.end; i += kBatch) {
+        int2 e[kBatch];
         float xx[kBatch][R];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)cur[u].x * L, k0, L, xx[u]);
-        const int i2 = i + kBatch;
-        int2 nxt[kBatch];
-        if (i2 < ch.end) {
+        for (int u = 0; u < kBatch; u++) e[u] = ent[min(i + u, ch.end - 1)];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) nxt[u] = ent[min(i2 + u, ch.end - 1)];
-        }
+        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * L, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
             if (i + u < ch.end) {
-                const float wu = __int_as_float(cur[u].y);
+                const float wu = __int_as_float(e[u].y);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const float prod = wu * xx[u][r];
                     acc[r] = acc[r] + prod;
                 }
             }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) cur[u] = nxt[u];
     }
     store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
 }
@@ -115,7 +154,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
 constexpr int kStage = 32;
 
 template <int R>
-__global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const int2* __restrict__ ent,
+__global__ void __launch_bounds__(1024) k_splat_tiled(TiledSplat ts, const int2* __restrict__ ent,
                                                             const float* __restrict__ in, float* __restrict__ vals,
                                                             int L, int S, int per, int q_max) {
     extern __shared__ float sh[];
@@ -133,29 +172,27 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
     const unsigned* __restrict__ csrc = reinterpret_cast<const unsigned*>(ts.chunks + c0);
     unsigned* shu = reinterpret_cast<unsigned*>(sh);
     // LDS: [rows: q_max floats][entries: e_max (index, weight) pairs][segments: 3 s_max]
-    for (int b = threadIdx.x; b < n4; b += kStage * kWaveBlock) {
+    const int nt = blockDim.x;
+    // every load is unconditional (index clamped to the last element, source chosen by
+    // select), so the kStage loads of a thread are in flight together; stores follow
+    for (int b = threadIdx.x; b < n4; b += kStage * nt) {
         unsigned x[kStage];
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
-            const int i = b + u * kWaveBlock;
-            if (i < n1) {
-                int r = (int)((float)i * inv_rowf);  // row of the tile (i < 2^24: corrected by one)
-                if (r * rowf > i)
-                    r--;
-                else if ((r + 1) * rowf <= i)
-                    r++;
-                x[u] = qsrc[i + r * rskip];
-            } else if (i < n3) {
-                x[u] = esrc[i - n1];
-            } else if (i < n4) {
-                const int j = i - n3;
-                const int c = j / 3;
-                x[u] = csrc[4 * c + 1 + (j - 3 * c)];  // Chunk {v, beg, end, out}
-            }
+            const int i = min(b + u * nt, n4 - 1);
+            int r = (int)((float)i * inv_rowf);  // row of the tile (exact within one for i < 2^24)
+            r -= r * rowf > i ? 1 : 0;
+            r += (r + 1) * rowf <= i ? 1 : 0;
+            const int j = i - n3;
+            const int c = j / 3;
+            const unsigned* q = qsrc + (i + (long long)r * rskip);
+            const unsigned* e = esrc + (i - n1);
+            const unsigned* h = csrc + (4 * c + 1 + (j - 3 * c));  // Chunk {v, beg, end, out}
+            x[u] = *(i < n1 ? q : (i < n3 ? e : h));
         }
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
-            const int i = b + u * kWaveBlock;
+            const int i = b + u * nt;
             if (i < n4) {
                 const int off = i < n1 ? 0 : (i < n3 ? q_max - n1 : q_max + 2 * ts.e_max - n3);
                 shu[i + off] = x[u];
@@ -168,7 +205,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_splat_tiled(TiledSplat ts, const
     const int* tsg = reinterpret_cast<const int*>(sh + q_max + 2 * ts.e_max);
     LaneGroup<true, R> G(L, S, per, nullptr);
     if (!G.on) return;  // no barrier below
-    const int groups = (kWaveBlock / 64) * per;
+    const int groups = (nt / 64) * per;
     const int k0 = G.k0;
     for (int c = (threadIdx.x >> 6) * per + G.g; c < ns; c += groups) {
         const int b = tsg[3 * c] - e0, e = tsg[3 * c + 1] - e0, out = tsg[3 * c + 2];
@@ -221,23 +258,11 @@ __global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
-    int i = 0;
-    for (; i + kBatch <= rc.count; i += kBatch) {
+    for (int i = 0; i < rc.count; i += kBatch) {  // count >= 2; loads clamped to the last row
         float xx[kBatch][R];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) load_piece<R>(vals + (size_t)(rc.first + i + u) * L, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
-#pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = acc[r] + xx[u][r];
-    }
-    if (i < rc.count) {
-        float xx[kBatch][R];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int j = i + u < rc.count ? i + u : rc.count - 1;
-            load_piece<R>(vals + (size_t)(rc.first + j) * L, k0, L, xx[u]);
-        }
+            load_piece<R>(vals + (size_t)(rc.first + min(i + u, rc.count - 1)) * L, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
             if (i + u < rc.count)
@@ -288,6 +313,8 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
                                                     int L, int S, int per, SliceTable st, float* __restrict__ tmp1,
                                                     float* __restrict__ tmp2) {
     extern __shared__ float sh[];
+    float* shc = sh + blockDim.x;  // compat region after the group scratch
+    stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -320,7 +347,7 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
 #pragma unroll
                 for (int r = 0; r < R; r++) m[r] = m[r] * kw;
             }
-            compat_piece<WAVE, R>(T, m, c, G);
+            compat_piece<WAVE, R>(T, m, c, G, shc);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 e[r] = e[r] - c[r];
@@ -356,18 +383,39 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_fused(const float* __r
     const int p = G.item();
     const bool active = G.on && p < N;
     const int k0 = G.k0;
+    const int pc = active ? p : 0;  // idle lanes read pixel 0 (in bounds; results unused)
+    // the wave's pixels are consecutive: load their slice pairs as one contiguous run
+    // (64 pairs per load instruction) into LDS, then each group reads its own
+    extern __shared__ int2 pairs_all[];
+    int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
+    float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
+    stage_compat(st, L, shc);
+    {
+        const int pw0 = p - G.g;  // the wave's first pixel
+        const int np = max(0, min(per, N - pw0)) * DS;
+        const int lane = threadIdx.x & 63;
+        // two loads per lane per round (clamped, unconditional), then the stores
+        const size_t base = (size_t)pw0 * DS;
+        const int last = max(np - 1, 0);
+        for (int i0 = 0; i0 < np; i0 += 128) {
+            const int2 t0 = slc[base + min(i0 + lane, last)];
+            const int2 t1 = slc[base + min(i0 + 64 + lane, last)];
+            if (i0 + lane < np) wp[i0 + lane] = t0;
+            if (i0 + 64 + lane < np) wp[i0 + 64 + lane] = t1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     int2 e[DS];
     float u[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) u[r] = 0.0f;
-    if (active) {
-        const int2* sl = slc + (size_t)p * DS;
-#pragma unroll
-        for (int j = 0; j < DS; j++) e[j] = sl[j];
-        if (U) load_piece<R>(U + (size_t)p * L, k0, L, u);
+    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
+    if (U) {
+        load_piece<R>(U + (size_t)pc * L, k0, L, u);
     } else {
 #pragma unroll
-        for (int j = 0; j < DS; j++) e[j] = make_int2(0, 0);  // vertex 0: a valid row, weight 0
+        for (int r = 0; r < R; r++) u[r] = 0.0f;
     }
     float x[DS][R];
 #pragma unroll
@@ -396,7 +444,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_fused(const float* __r
 #pragma unroll
             for (int r = 0; r < R; r++) m[r] = m[r] * kw;
         }
-        compat_piece<true, R>(T, m, c, G);
+        compat_piece<true, R>(T, m, c, G, shc);
 #pragma unroll
         for (int r = 0; r < R; r++) en[r] = en[r] - c[r];
     }
@@ -427,6 +475,8 @@ template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int S, int per,
                                                      SliceTable st, int which) {
     extern __shared__ float sh[];
+    float* shc = sh + blockDim.x;
+    stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -447,7 +497,7 @@ __global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, in
         for (int r = 0; r < R; r++) m[r] = m[r] * kw;
     }
     if (T.ckind >= 0) {
-        compat_piece<WAVE, R>(T, m, c, G);
+        compat_piece<WAVE, R>(T, m, c, G, shc);
     } else {
 #pragma unroll
         for (int r = 0; r < R; r++) c[r] = m[r];
@@ -462,6 +512,8 @@ __global__ void __launch_bounds__(1024) k_kl(const float* __restrict__ Q, const 
                                              int S, int per, SliceTable st, double* __restrict__ partial) {
     extern __shared__ float sh[];
     __shared__ double red[1024];
+    float* shc = sh + blockDim.x;
+    stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -500,7 +552,7 @@ __global__ void __launch_bounds__(1024) k_kl(const float* __restrict__ Q, const 
 #pragma unroll
                 for (int r = 0; r < R; r++) m[r] = m[r] * kw;
             }
-            compat_piece<WAVE, R>(T, m, c, G);
+            compat_piece<WAVE, R>(T, m, c, G, shc);
             if (active)
 #pragma unroll
                 for (int r = 0; r < R; r++)
@@ -612,29 +664,50 @@ void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const floa
                   hipStream_t s) {
     if (!n_chunks) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), 0, s, chunks, n_chunks, ent, in, vals, L, sh.S,
+    const size_t shm = sh.wave ? (size_t)(kWaveBlock / 64) * (kWin + kBatch) * sizeof(int2) : 0;
+    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), shm, s, chunks, n_chunks, ent, in, vals, L, sh.S,
                   sh.per);
 }
 
-size_t tiled_splat_lds(int L, int side, int e_max, int s_max) {
+size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap) {
     const LaunchShape sh = shape_for(L);
     if (!sh.wave) return 0;
     const size_t words = (((size_t)side * side * L + 1) & ~(size_t)1) + 2 * (size_t)e_max + 3 * (size_t)s_max;
     const size_t bytes = words * sizeof(float);
-    return bytes <= kTiledLdsMax ? bytes : 0;
+    return bytes <= lds_cap ? bytes : 0;
 }
 
-void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, hipStream_t s) {
+// Workgroup size of the tiled splat: 1024 threads (16 waves of lane groups) once a tile
+// takes more than a quarter of the workgroup LDS limit, else 256.
+int tiled_splat_block(size_t lds, size_t lds_cap) { return lds * 4 > lds_cap ? 1024 : kWaveBlock; }
+
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, size_t lds_cap,
+                        hipStream_t s) {
     if (!ts.n_tiles) return;
     const LaunchShape sh = shape_for(L);
-    const size_t lds = tiled_splat_lds(L, ts.geom.side, ts.e_max, ts.s_max);
+    const size_t lds = tiled_splat_lds(L, ts.geom.side, ts.e_max, ts.s_max, lds_cap);
     if (!lds) throw Error{DCRF_ERR_VALUE, "tiled splat does not fit the LDS of a block"};
     const int q_max = (ts.geom.side * ts.geom.side * L + 1) & ~1;  // the entry pairs start 8-byte aligned
+    const int block = tiled_splat_block(lds, lds_cap);
+    // dynamic LDS beyond the default launch limit must be granted per kernel
+    static size_t granted[5] = {};
+    auto grant = [&](const void* fn, int r) {
+        if (lds > granted[r]) {
+            DCRF_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            granted[r] = lds;
+        }
+    };
     switch (sh.R) {
-        case 1: k_splat_tiled<1><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        case 2: k_splat_tiled<2><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        case 3: k_splat_tiled<3><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        default: k_splat_tiled<4><<<ts.n_tiles, kWaveBlock, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 1: grant(reinterpret_cast<const void*>(&k_splat_tiled<1>), 1); break;
+        case 2: grant(reinterpret_cast<const void*>(&k_splat_tiled<2>), 2); break;
+        case 3: grant(reinterpret_cast<const void*>(&k_splat_tiled<3>), 3); break;
+        default: grant(reinterpret_cast<const void*>(&k_splat_tiled<4>), 4); break;
+    }
+    switch (sh.R) {
+        case 1: k_splat_tiled<1><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 2: k_splat_tiled<2><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 3: k_splat_tiled<3><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        default: k_splat_tiled<4><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
     }
     DCRF_HIP_CHECK(hipGetLastError());
 }
@@ -667,12 +740,15 @@ void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int
 }
 
 #define DCRF_FUSED(R_, D0_, D1_)                                                                          \
-    k_meanfield_fused<R_, D0_, D1_><<<blocks_for(N, sh.items), kWaveBlock, 0, s>>>(U, Q, N, L, sh.S, sh.per, st, \
+    k_meanfield_fused<R_, D0_, D1_><<<blocks_for(N, sh.items), kWaveBlock,                                     \
+                                      (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) +            \
+                                          (size_t)cwf * sizeof(float), s>>>(                                          \
+        U, Q, N, L, sh.S, sh.per, st, \
                                                                                 slc)
 
 template <int R>
 void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int N, int L, const SliceTable& st,
-                    const int2* slc, hipStream_t s) {
+                    const int2* slc, int cwf, hipStream_t s) {
     const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
     if (a == 3 && b == 0) DCRF_FUSED(R, 3, 0);
     else if (a == 6 && b == 0) DCRF_FUSED(R, 6, 0);
@@ -683,24 +759,39 @@ void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int N, int 
 }
 #undef DCRF_FUSED
 
-void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st, const int2* slc,
+void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st_in, const int2* slc,
                             hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
     if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
         throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
     switch (sh.R) {
-        case 1: launch_fused_r<1>(sh, U, Q, N, L, st, slc, s); break;
-        case 2: launch_fused_r<2>(sh, U, Q, N, L, st, slc, s); break;
-        case 3: launch_fused_r<3>(sh, U, Q, N, L, st, slc, s); break;
-        default: launch_fused_r<4>(sh, U, Q, N, L, st, slc, s); break;
+        case 1: launch_fused_r<1>(sh, U, Q, N, L, st, slc, cwf, s); break;
+        case 2: launch_fused_r<2>(sh, U, Q, N, L, st, slc, cwf, s); break;
+        case 3: launch_fused_r<3>(sh, U, Q, N, L, st, slc, cwf, s); break;
+        default: launch_fused_r<4>(sh, U, Q, N, L, st, slc, cwf, s); break;
     }
 }
 
-void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1, float* tmp2,
+int assign_compat_lds(SliceTable& st, int L) {
+    int total = 0;
+    for (int k = 0; k < st.nterms; k++) st.t[k].cw_lds = -1;
+    for (int k = 0; k < st.nterms; k++)
+        if (st.t[k].ckind == DCRF_COMPAT_MATRIX && total + L * L <= kCompatLdsFloats) {
+            st.t[k].cw_lds = total;
+            total += L * L;
+        }
+    return total;
+}
+
+void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st_in, float* tmp1, float* tmp2,
                       hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * sh.block, s, U, Q, N, L, sh.S, sh.per,
-                  st, tmp1, tmp2);
+    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, U, Q, N, L, sh.S,
+                  sh.per, st, tmp1, tmp2);
 }
 
 void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
@@ -709,17 +800,21 @@ void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
     launch_meanfield(U, Q, N, L, st, nullptr, nullptr, s);
 }
 
-void launch_slice_term(float* out, int N, int L, const SliceTable& st, int which, hipStream_t s) {
+void launch_slice_term(float* out, int N, int L, const SliceTable& st_in, int which, hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * sh.block, s, out, N, L, sh.S, sh.per,
-                  st, which);
+    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, out, N, L, sh.S,
+                  sh.per, st, which);
 }
 
-int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st, double* partials,
+int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st_in, double* partials,
               hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
     const int nb = (int)blocks_for(N, sh.items);
-    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * sh.block, s, Q, U, N, L, sh.S, sh.per, st, partials);
+    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * (sh.block + cwf), s, Q, U, N, L, sh.S, sh.per, st, partials);
     return nb;
 }
 
