@@ -191,6 +191,14 @@ int dcrf_set_splat_chunk(dcrf_model *m, int chunk);
  * entry, at most 64 KiB) the splat stages them there (one workgroup per tile).  No
  * counterpart in the reference (a scheduling knob). */
 int dcrf_set_splat_tile(dcrf_model *m, int side);
+/* Inference mean-field kernel: 0 = automatic (the fused kernel over interleaved slice
+ * pairs where the term layout has one, else the generic kernel), 1 = generic, 2 =
+ * fused, 3 = the fused kernel over pixel runs whose distinct vertex rows are staged in
+ * LDS (where a run's rows and pairs fit 64 KiB; measured slower than 2 at 1024x768 with
+ * 21 labels).  run_rows (0 = 16) is the height of a run rectangle, whose width is two
+ * waves of pixels.  Every choice computes the same arithmetic in the same order.  No
+ * counterpart in the reference (a kernel choice for testing and benchmarking). */
+int dcrf_set_meanfield_kernel(dcrf_model *m, int mode, int run_rows);
 /* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
  * (default 0.5): lower trades memory for shorter probe sequences. */
 int dcrf_set_hash_load(dcrf_model *m, float load);
@@ -208,10 +216,13 @@ int dcrf_kernel_stats(dcrf_model *m, int kernel, long long *launches, double *ms
 /* Algorithmic bytes of one mean-field iteration (Q/U streams, lattice entry arrays,
  * normalisers, vertex rows, neighbour tables). */
 int dcrf_iteration_bytes(const dcrf_model *m, double *bytes);
-/* Splat layout of the model (built on first use): whether the LDS-tiled splat runs
- * (else the global form), the tile side, tile count, splat segments and partial rows.
- * No counterpart in the reference (introspection for tests and benchmarks). */
-int dcrf_splat_layout(dcrf_model *m, int *tiled, int *tile_side, int *n_tiles, int *segments, int *partials);
+/* Filter layout of the model (built on first use): whether the LDS-tiled splat runs
+ * (else the global form), the tile side, tile count, splat segments, partial rows, and
+ * the inference mean-field kernel (0 generic, 1 fused over interleaved slice pairs,
+ * 2 fused over LDS-staged pixel runs).  No counterpart in the reference
+ * (introspection for tests and benchmarks). */
+int dcrf_splat_layout(dcrf_model *m, int *tiled, int *tile_side, int *n_tiles, int *segments, int *partials,
+                      int *meanfield);
 
 #ifdef __cplusplus
 }

@@ -68,10 +68,11 @@ SIGNATURES = {
     "dcrf_set_splat_chunk": (_c, [_P, _c]),
     "dcrf_set_hash_load": (_c, [_P, _f]),
     "dcrf_set_splat_tile": (_c, [_P, _c]),
+    "dcrf_set_meanfield_kernel": (_c, [_P, _c, _c]),
     "dcrf_enable_timing": (_c, [_P, _c]),
     "dcrf_kernel_stats": (_c, [_P, _c, _LL, _D, _D]),
     "dcrf_iteration_bytes": (_c, [_P, _D]),
-    "dcrf_splat_layout": (_c, [_P, _I, _I, _I, _I, _I]),
+    "dcrf_splat_layout": (_c, [_P, _I, _I, _I, _I, _I, _I]),
 }
 
 _lib = None
@@ -338,6 +339,11 @@ class Model:
     def set_splat_tile(self, side):
         check(lib().dcrf_set_splat_tile(self.h, side))
 
+    def set_meanfield_kernel(self, mode, run_rows=0):
+        """0 automatic (fused where available), 1 generic, 2 fused, 3 fused over
+        LDS-staged pixel runs; run_rows: height of the pixel-run rectangles (0 = 16)."""
+        check(lib().dcrf_set_meanfield_kernel(self.h, mode, run_rows))
+
     def enable_timing(self, on=True):
         check(lib().dcrf_enable_timing(self.h, int(bool(on))))
 
@@ -353,10 +359,11 @@ class Model:
         return out
 
     def splat_layout(self):
-        """{tiled, tile_side, n_tiles, segments, partials} of the splat layout."""
-        v = [ctypes.c_int() for _ in range(5)]
+        """{tiled, tile_side, n_tiles, segments, partials, meanfield} of the filter layout
+        (meanfield: 0 generic, 1 fused, 2 fused over pixel runs)."""
+        v = [ctypes.c_int() for _ in range(6)]
         check(lib().dcrf_splat_layout(self.h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("tiled", "tile_side", "n_tiles", "segments", "partials"), [x.value for x in v]))
+        return dict(zip(("tiled", "tile_side", "n_tiles", "segments", "partials", "meanfield"), [x.value for x in v]))
 
     def iteration_bytes(self):
         b = ctypes.c_double()

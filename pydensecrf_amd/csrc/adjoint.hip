@@ -64,13 +64,13 @@ __global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, 
     }
 }
 
-template <bool WAVE, int R>
+template <bool WAVE, int R, bool CWL>
 __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, float* __restrict__ b,
                                                   float* __restrict__ dU, int N, int L, int S, int per,
                                                   SliceTable st) {
     extern __shared__ float sh[];
     float* shc = sh + blockDim.x;
-    stage_compat(st, L, shc);
+    if (CWL) stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -292,6 +292,32 @@ __global__ void k_set_compat(const float* __restrict__ raw, int ckind, int L, fl
         DCRF_HIP_CHECK(hipGetLastError());                                              \
     } while (0)
 
+// As DCRF_DISPATCH for kernels with a third template argument CWL (compiled-in LDS
+// staging of Matrix compatibilities): CWL = (cwl != 0).
+#define DCRF_DISPATCH_T(sh, kernel, C, grid, shm, s, ...)                                   \
+    do {                                                                                   \
+        const unsigned g_ = (grid);                                                        \
+        const size_t m_ = (shm);                                                           \
+        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                        \
+            case 3: kernel<true, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 2: kernel<false, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 5: kernel<true, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 4: kernel<false, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 7: kernel<true, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 6: kernel<false, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 9: kernel<true, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            default: kernel<false, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
+        }                                                                                  \
+        DCRF_HIP_CHECK(hipGetLastError());                                                 \
+    } while (0)
+#define DCRF_DISPATCH_CWL(sh, cwl, kernel, grid, shm, s, ...)                     \
+    do {                                                                         \
+        if (cwl)                                                                 \
+            DCRF_DISPATCH_T(sh, kernel, true, grid, shm, s, __VA_ARGS__);        \
+        else                                                                     \
+            DCRF_DISPATCH_T(sh, kernel, false, grid, shm, s, __VA_ARGS__);       \
+    } while (0)
+
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_sn_first, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, dQ, b, dU, N, L, sh.S,
@@ -302,8 +328,8 @@ void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const Sli
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_adjoint, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, Q, b, dU, N, L, sh.S,
-                  sh.per, st);
+    DCRF_DISPATCH_CWL(sh, cwf, k_adjoint, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, Q, b, dU,
+                      N, L, sh.S, sh.per, st);
 }
 
 int grad_partial_blocks(int N, int L) {

@@ -82,6 +82,11 @@ struct dcrf_model {
     DevBuf<int2> g_entF, g_entT;  // splat entries (pixel, weight bits), forward / transposed
     bool fused = false;           // inference through k_meanfield_fused
     DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
+    bool runs = false;            // ... or through k_meanfield_runs (rows staged per pixel run)
+    int mf_mode = 0;              // dcrf_set_meanfield_kernel: 0 auto (fused), 1 generic, 2 fused, 3 runs
+    int run_rows = 16;            // pixel rows of a run rectangle
+    RunSlice rslice;
+    int rows_lds = 0;
     bool all_symmetric = true;
     std::vector<std::unique_ptr<TermLayout>> tl;
 
@@ -395,6 +400,8 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
     m.train_n = -1;
 }
 
+SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw = nullptr, float* mout = nullptr);
+
 void ensure_layout(dcrf_model& m) {
     if (m.layout_valid) return;
     m.drop_graph();
@@ -516,7 +523,9 @@ void ensure_layout(dcrf_model& m) {
         m.tl[k] = std::move(lay);
     }
     // interleaved slice pairs for the fused mean-field kernel
-    m.fused = fused_meanfield_ok(L, T, T > 0 ? m.terms[0]->lat.d + 1 : 0, T > 1 ? m.terms[1]->lat.d + 1 : 0);
+    m.runs = false;
+    m.fused = m.mf_mode != 1 &&
+              fused_meanfield_ok(L, T, T > 0 ? m.terms[0]->lat.d + 1 : 0, T > 1 ? m.terms[1]->lat.d + 1 : 0);
     if (m.fused) {
         int ds = 0;
         for (int k = 0; k < T; k++) ds += m.terms[k]->lat.d + 1;
@@ -526,6 +535,29 @@ void ensure_layout(dcrf_model& m) {
             const int d1 = m.terms[k]->lat.d + 1;
             launch_interleave_slice(m.tl[k]->vtx.get(), m.tl[k]->swF.get(), N, d1, ds, off, m.g_slc.get(), s);
             off += d1;
+        }
+        // pixel runs (on request): entry j reads vals[(d+1) & 1] of its term
+        m.runs = false;
+        if (m.mf_mode == 3 && m.V_total < (1 << 30)) {
+            unsigned par = 0;
+            int j = 0;
+            for (int k = 0; k < T; k++) {
+                const int d1 = m.terms[k]->lat.d + 1;
+                for (int q = 0; q < d1; q++, j++)
+                    if (d1 & 1) par |= 1u << j;
+            }
+            // rectangles of 2 waves' pixels per row x 16 rows (18 x 16 at 21 labels)
+            // rectangles two waves of pixels wide, at most run_rows high and at most
+            // 24 KiB of slice pairs
+            const LaunchShape shp = shape_for(L);
+            const int TW = 2 * shp.per;
+            const int TH = std::max(1, std::min(m.run_rows, (24 * 1024) / (TW * ds * 8)));
+            build_run_slice(m.geo, m.geod, m.g_slc.get(), ds, par, TW, TH, m.rslice, s);
+            m.rows_lds = (m.rslice.u_max * L + 1) & ~1;
+            SliceTable probe = slice_table(m, false);
+            const int cwf = assign_compat_lds(probe, L);
+            const size_t lds = meanfield_runs_lds(L, ds, m.rslice.P, m.rows_lds, cwf);
+            m.runs = m.mf_mode == 3 && lds <= 64 * 1024 && lds <= m.lds_cap;
         }
     }
     m.prefixV.assign(std::max(m.max_d + 1, 0), 0);
@@ -547,7 +579,7 @@ BlurTable blur_table(const dcrf_model& m) {
 }
 
 // kw: device [T] term weights or null; mout: device [T][N][L] raw-message output or null.
-SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw = nullptr, float* mout = nullptr) {
+SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw, float* mout) {
     SliceTable st{};
     st.nterms = (int)m.terms.size();
     const size_t NL = (size_t)m.N() * m.L;
@@ -623,7 +655,13 @@ void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStrea
     for (int it = 0; it < n_iter; it++) {
         run_filter(m, Q, false, s);
         Timed t(m, s, KC_MEANFIELD, meanfield_bytes(m));
-        if (m.fused)
+        if (m.runs)
+            launch_meanfield_runs(U, Q, m.N(), m.L, st,
+                                  RunSliceDev{m.rslice.P, m.rslice.n_runs, m.rows_lds, m.rslice.base.get(),
+                                              m.rslice.rows.get(), m.rslice.pix_base.get(), m.rslice.rpix.get(),
+                                              m.rslice.loc.get(), m.vals[0].get(), m.vals[1].get()},
+                                  s);
+        else if (m.fused)
             launch_meanfield_fused(U, Q, m.N(), m.L, st, m.g_slc.get(), s);
         else
             launch_meanfield(U, Q, m.N(), m.L, st, nullptr, nullptr, s);
@@ -1222,6 +1260,20 @@ int dcrf_set_splat_tile(dcrf_model* m, int side) {
     });
 }
 
+int dcrf_set_meanfield_kernel(dcrf_model* m, int mode, int run_rows) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(mode >= 0 && mode <= 3, "mean-field kernel mode must be 0 (auto), 1, 2 or 3");
+        require(run_rows >= 0 && run_rows <= 64, "run rows must be in [0, 64]");
+        const int rr = run_rows ? run_rows : 16;
+        if (mode != m->mf_mode || rr != m->run_rows) {
+            m->mf_mode = mode;
+            m->run_rows = rr;
+            m->layout_valid = false;
+        }
+    });
+}
+
 int dcrf_set_hash_load(dcrf_model* m, float load) {
     return guard([&] {
         require(m != nullptr, "null model");
@@ -1254,9 +1306,10 @@ int dcrf_kernel_stats(dcrf_model* m, int kernel, long long* launches, double* ms
     });
 }
 
-int dcrf_splat_layout(dcrf_model* m, int* tiled, int* tile_side, int* n_tiles, int* segments, int* partials) {
+int dcrf_splat_layout(dcrf_model* m, int* tiled, int* tile_side, int* n_tiles, int* segments, int* partials,
+                      int* meanfield) {
     return guard([&] {
-        require(m && tiled && tile_side && n_tiles && segments && partials, "null argument");
+        require(m && tiled && tile_side && n_tiles && segments && partials && meanfield, "null argument");
         ensure_device(*m);
         ensure_layout(*m);
         *tiled = m->tiled ? 1 : 0;
@@ -1264,6 +1317,7 @@ int dcrf_splat_layout(dcrf_model* m, int* tiled, int* tile_side, int* n_tiles, i
         *n_tiles = m->tiles.n_tiles();
         *segments = m->C_total;
         *partials = m->rows0 - m->V_total - 1;
+        *meanfield = m->runs ? 2 : (m->fused ? 1 : 0);
     });
 }
 

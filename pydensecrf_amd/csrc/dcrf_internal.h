@@ -249,6 +249,32 @@ void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& 
 // (d+1) = (3, 6) / (6, 3) (fused_meanfield_ok): slc[p] holds the (vertex, weight bits)
 // pairs of every term of pixel p, term 0 first (launch_interleave_slice builds it).
 bool fused_meanfield_ok(int L, int nterms, int d1_0, int d1_1);
+// Run form of the fused update: pixels grouped into runs (TW x TH rectangles of 2-D
+// instances, runs of TW*TH consecutive pixels otherwise) and visited in run-major
+// order; per run the distinct vertex rows its slice pairs read (tagged (row << 1) |
+// buffer: vals v0 or v1), and the pairs with the row replaced by its index in the run's
+// list.  The kernel stages a run's rows in LDS.
+struct RunSliceDev {
+    int P, n_runs, rows_lds;  // rows_lds: LDS floats for the largest run's rows (even)
+    const int* base;          // [n_runs + 1] first distinct row of each run
+    const int* rows;          // tagged rows
+    const int* pix_base;      // [n_runs + 1] first position of each run in run-major order
+    const int* rpix;          // [N] pixel at each run-major position
+    const int2* loc;          // [N][DS] by run-major position: (index in the run's list, weight bits)
+    const float* v0;
+    const float* v1;
+};
+struct RunSlice {
+    int P = 0, n_runs = 0, u_max = 0;
+    DevBuf<int> base, rows, pix_base, rpix;
+    DevBuf<int2> loc;
+};
+// par_mask bit j: entry j of a pixel's slice pairs reads the second value buffer.
+void build_run_slice(const Geometry& g, const GeometryDev& gd, const int2* slc, int ds, unsigned par_mask, int TW,
+                     int TH, RunSlice& rs, hipStream_t s);
+size_t meanfield_runs_lds(int L, int ds, int P, int rows_lds, int cwf);
+void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTable& st, const RunSliceDev& rs,
+                           hipStream_t s);
 void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st, const int2* slc,
                             hipStream_t s);
 void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,

@@ -21,6 +21,7 @@
 #include "dcrf_device.h"
 
 #include <cmath>
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -308,13 +309,13 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
 
 // Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
 // (densecrf.cpp:122-129, with the per-term weight w_k = 1 unless set).
-template <bool WAVE, int R>
+template <bool WAVE, int R, bool CWL>
 __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U, float* __restrict__ Q, int N,
                                                     int L, int S, int per, SliceTable st, float* __restrict__ tmp1,
                                                     float* __restrict__ tmp2) {
     extern __shared__ float sh[];
     float* shc = sh + blockDim.x;  // compat region after the group scratch
-    stage_compat(st, L, shc);
+    if (CWL) stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -370,56 +371,14 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
     }
 }
 
-// Mean-field update for models of one or two terms with d+1 = D0 (and D1) lattice
-// entries per pixel: slc holds every pixel's (vertex, weight bits) pairs of both terms
-// side by side, so all D0 + D1 vertex rows are gathered after one dependent load.  Same
-// arithmetic, in the same order, as k_meanfield.
+// The per-pixel tail of the fused mean-field kernels: per term the weighted sum of the
+// gathered rows (entry order), the term weight and the compatibility, subtracted from
+// -U; then the softmax over the lane group, stored when `active`.
 template <int R, int D0, int D1>
-__global__ void __launch_bounds__(kWaveBlock) k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
-                                                                int N, int L, int S, int per, SliceTable st,
-                                                                const int2* __restrict__ slc) {
+__device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceTable& st, const int2 (&e)[D0 + D1],
+                                             const float (&x)[D0 + D1][R], const float (&u)[R], bool active,
+                                             float* __restrict__ qrow, const float* shc) {
     constexpr int DS = D0 + D1;
-    LaneGroup<true, R> G(L, S, per, nullptr);
-    const int p = G.item();
-    const bool active = G.on && p < N;
-    const int k0 = G.k0;
-    const int pc = active ? p : 0;  // idle lanes read pixel 0 (in bounds; results unused)
-    // the wave's pixels are consecutive: load their slice pairs as one contiguous run
-    // (64 pairs per load instruction) into LDS, then each group reads its own
-    extern __shared__ int2 pairs_all[];
-    int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
-    float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
-    stage_compat(st, L, shc);
-    {
-        const int pw0 = p - G.g;  // the wave's first pixel
-        const int np = max(0, min(per, N - pw0)) * DS;
-        const int lane = threadIdx.x & 63;
-        // two loads per lane per round (clamped, unconditional), then the stores
-        const size_t base = (size_t)pw0 * DS;
-        const int last = max(np - 1, 0);
-        for (int i0 = 0; i0 < np; i0 += 128) {
-            const int2 t0 = slc[base + min(i0 + lane, last)];
-            const int2 t1 = slc[base + min(i0 + 64 + lane, last)];
-            if (i0 + lane < np) wp[i0 + lane] = t0;
-            if (i0 + 64 + lane < np) wp[i0 + 64 + lane] = t1;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    int2 e[DS];
-    float u[R];
-#pragma unroll
-    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
-    if (U) {
-        load_piece<R>(U + (size_t)pc * L, k0, L, u);
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) u[r] = 0.0f;
-    }
-    float x[DS][R];
-#pragma unroll
-    for (int j = 0; j < DS; j++) load_piece<R>((j < D0 ? st.t[0].vals : st.t[1].vals) + (size_t)e[j].x * L, k0, L, x[j]);
     float en[R];
 #pragma unroll
     for (int r = 0; r < R; r++) en[r] = -u[r];
@@ -457,7 +416,146 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_fused(const float* __r
         float q[R];
 #pragma unroll
         for (int r = 0; r < R; r++) q[r] = y[r] / sum;
-        store_piece<R>(Q + (size_t)p * L, k0, L, q);
+        store_piece<R>(qrow, G.k0, G.L, q);
+    }
+}
+
+// Mean-field update for models of one or two terms with d+1 = D0 (and D1) lattice
+// entries per pixel: slc holds every pixel's (vertex, weight bits) pairs of both terms
+// side by side, so all D0 + D1 vertex rows are gathered after one dependent load.  Same
+// arithmetic, in the same order, as k_meanfield.
+template <int R, int D0, int D1, bool CWL>
+__global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))  // <= 64 VGPRs
+k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
+                                                                int N, int L, int S, int per, SliceTable st,
+                                                                const int2* __restrict__ slc) {
+    constexpr int DS = D0 + D1;
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int k0 = G.k0;
+    const int pc = active ? p : 0;  // idle lanes read pixel 0 (in bounds; results unused)
+    // the wave's pixels are consecutive: load their slice pairs as one contiguous run
+    // (64 pairs per load instruction) into LDS, then each group reads its own
+    extern __shared__ int2 pairs_all[];
+    int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
+    // CWL: some Matrix term has an LDS copy of its weights (staging costs a barrier, so
+    // it is compiled in only then)
+    float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
+    if (CWL) stage_compat(st, L, shc);
+    {
+        const int pw0 = p - G.g;  // the wave's first pixel
+        const int np = max(0, min(per, N - pw0)) * DS;
+        const int lane = threadIdx.x & 63;
+        // two loads per lane per round (clamped, unconditional), then the stores
+        const size_t base = (size_t)pw0 * DS;
+        const int last = max(np - 1, 0);
+        for (int i0 = 0; i0 < np; i0 += 128) {
+            const int2 t0 = slc[base + min(i0 + lane, last)];
+            const int2 t1 = slc[base + min(i0 + 64 + lane, last)];
+            if (i0 + lane < np) wp[i0 + lane] = t0;
+            if (i0 + 64 + lane < np) wp[i0 + 64 + lane] = t1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    int2 e[DS];
+    float u[R];
+#pragma unroll
+    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
+    if (U) {
+        load_piece<R>(U + (size_t)pc * L, k0, L, u);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) u[r] = 0.0f;
+    }
+    float x[DS][R];
+#pragma unroll
+    for (int j = 0; j < DS; j++) load_piece<R>((j < D0 ? st.t[0].vals : st.t[1].vals) + (size_t)e[j].x * L, k0, L, x[j]);
+    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc);
+}
+
+// The same update over pixel runs (2-D rectangles whose rows hold whole waves of
+// pixels), one workgroup per run: the run's distinct vertex rows (a few hundred, against
+// P*(D0+D1) gathers) are staged in LDS once, then every pixel's slice reads them from
+// there (rs.loc holds the pairs with the vertex replaced by its index in the run's
+// list, in run-major pixel order).
+template <int R, int D0, int D1>
+__global__ void __launch_bounds__(kWaveBlock) k_meanfield_runs(const float* __restrict__ U, float* __restrict__ Q,
+                                                               int N, int L, int S, int per, SliceTable st,
+                                                               RunSliceDev rs) {
+    constexpr int DS = D0 + D1;
+    extern __shared__ float lds[];
+    float* tq = lds;  // rs.rows_lds floats: the run's rows
+    int2* tp = reinterpret_cast<int2*>(lds + rs.rows_lds);  // rs.P * DS pairs: the run's slice pairs
+    float* shc = reinterpret_cast<float*>(tp + (size_t)rs.P * DS);
+    stage_compat(st, L, shc);
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int k0 = G.k0;
+    const int run = xcd_block(blockIdx.x, gridDim.x);
+    const int k0r = rs.pix_base[run], np = rs.pix_base[run + 1] - k0r;  // run-major positions
+    const int u0 = rs.base[run], nu = rs.base[run + 1] - u0;
+    const int items = (blockDim.x >> 6) * per;
+    const int gi = (threadIdx.x >> 6) * per + G.g;
+    // stage the rows (group gi takes rows gi, gi + items, ...) and the pairs (all threads,
+    // consecutive words): loads first, then stores
+    for (int q0 = 0; q0 < nu; q0 += 4 * items) {
+        float xs[4][R];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int q = min(q0 + b * items + gi, nu - 1);
+            const int tag = rs.rows[u0 + q];
+            load_piece<R>(((tag & 1) ? rs.v1 : rs.v0) + (size_t)(tag >> 1) * L, k0, L, xs[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int q = q0 + b * items + gi;
+            if (G.on && q < nu) store_piece<R>(tq + (size_t)q * L, k0, L, xs[b]);
+        }
+    }
+    {
+        const int npairs = np * DS;
+        const int2* src = rs.loc + (size_t)k0r * DS;
+        for (int i0 = 0; i0 < npairs; i0 += 4 * (int)blockDim.x) {
+            int2 t[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) t[b] = src[min(i0 + b * (int)blockDim.x + (int)threadIdx.x, npairs - 1)];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int i = i0 + b * (int)blockDim.x + (int)threadIdx.x;
+                if (i < npairs) tp[i] = t[b];
+            }
+        }
+    }
+    __syncthreads();
+    // pixel passes: each pass's unary rows are loaded one pass ahead
+    auto pixel = [&](int b0) { return rs.rpix[b0 + gi < np && G.on ? k0r + b0 + gi : k0r]; };
+    float un[R];
+    int pn = pixel(0);
+    if (U) {
+        load_piece<R>(U + (size_t)pn * L, k0, L, un);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) un[r] = 0.0f;
+    }
+    for (int b0 = 0; b0 < np; b0 += items) {
+        const bool active = G.on && b0 + gi < np;
+        const int p = pn;
+        float u[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) u[r] = un[r];
+        if (b0 + items < np) {
+            pn = pixel(b0 + items);
+            if (U) load_piece<R>(U + (size_t)pn * L, k0, L, un);
+        }
+        int2 e[DS];
+#pragma unroll
+        for (int j = 0; j < DS; j++) e[j] = active ? tp[(b0 + gi) * DS + j] : make_int2(0, 0);
+        float x[DS][R];
+#pragma unroll
+        for (int j = 0; j < DS; j++) load_piece<R>(tq + (size_t)e[j].x * L, k0, L, x[j]);
+        finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc);
     }
 }
 
@@ -471,12 +569,12 @@ __global__ void k_interleave_slice(const int* __restrict__ vtx, const float* __r
 }
 
 // out = compat_which(slice_which) (ckind < 0: the plain slice).
-template <bool WAVE, int R>
+template <bool WAVE, int R, bool CWL>
 __global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, int N, int L, int S, int per,
                                                      SliceTable st, int which) {
     extern __shared__ float sh[];
     float* shc = sh + blockDim.x;
-    stage_compat(st, L, shc);
+    if (CWL) stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -507,13 +605,13 @@ __global__ void __launch_bounds__(1024) k_slice_term(float* __restrict__ out, in
 
 // KL divergence terms per pixel, reduced per block in double in a fixed order
 // (densecrf.cpp:214-235: q log max(q,1e-20) + u q + sum_k q * pairwise_k(Q)).
-template <bool WAVE, int R>
+template <bool WAVE, int R, bool CWL>
 __global__ void __launch_bounds__(1024) k_kl(const float* __restrict__ Q, const float* __restrict__ U, int N, int L,
                                              int S, int per, SliceTable st, double* __restrict__ partial) {
     extern __shared__ float sh[];
     __shared__ double red[1024];
     float* shc = sh + blockDim.x;
-    stage_compat(st, L, shc);
+    if (CWL) stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
@@ -659,6 +757,32 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
         DCRF_HIP_CHECK(hipGetLastError());                                              \
     } while (0)
 
+// As DCRF_DISPATCH for kernels with a third template argument CWL (compiled-in LDS
+// staging of Matrix compatibilities): CWL = (cwl != 0).
+#define DCRF_DISPATCH_T(sh, kernel, C, grid, shm, s, ...)                                   \
+    do {                                                                                   \
+        const unsigned g_ = (grid);                                                        \
+        const size_t m_ = (shm);                                                           \
+        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                        \
+            case 3: kernel<true, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 2: kernel<false, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 5: kernel<true, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 4: kernel<false, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 7: kernel<true, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            case 6: kernel<false, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
+            case 9: kernel<true, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
+            default: kernel<false, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
+        }                                                                                  \
+        DCRF_HIP_CHECK(hipGetLastError());                                                 \
+    } while (0)
+#define DCRF_DISPATCH_CWL(sh, cwl, kernel, grid, shm, s, ...)                     \
+    do {                                                                         \
+        if (cwl)                                                                 \
+            DCRF_DISPATCH_T(sh, kernel, true, grid, shm, s, __VA_ARGS__);        \
+        else                                                                     \
+            DCRF_DISPATCH_T(sh, kernel, false, grid, shm, s, __VA_ARGS__);       \
+    } while (0)
+
 
 void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, float* vals, int L,
                   hipStream_t s) {
@@ -740,7 +864,8 @@ void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int
 }
 
 #define DCRF_FUSED(R_, D0_, D1_)                                                                          \
-    k_meanfield_fused<R_, D0_, D1_><<<blocks_for(N, sh.items), kWaveBlock,                                     \
+    (cwf ? k_meanfield_fused<R_, D0_, D1_, true> : k_meanfield_fused<R_, D0_, D1_, false>)<<<                   \
+        blocks_for(N, sh.items), kWaveBlock,                                                                    \
                                       (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) +            \
                                           (size_t)cwf * sizeof(float), s>>>(                                          \
         U, Q, N, L, sh.S, sh.per, st, \
@@ -758,6 +883,59 @@ void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int N, int 
     DCRF_HIP_CHECK(hipGetLastError());
 }
 #undef DCRF_FUSED
+
+#define DCRF_RUNS(R_, D0_, D1_)                                                                           \
+    k_meanfield_runs<R_, D0_, D1_><<<rs.n_runs, kWaveBlock, lds, s>>>(U, Q, N, L, sh.S, sh.per, st, rs)
+
+template <int R>
+void launch_runs_r(const LaunchShape& sh, const float* U, float* Q, int N, int L, const SliceTable& st,
+                   const RunSliceDev& rs, size_t lds, hipStream_t s) {
+    const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
+    if (a == 3 && b == 0) DCRF_RUNS(R, 3, 0);
+    else if (a == 6 && b == 0) DCRF_RUNS(R, 6, 0);
+    else if (a == 3 && b == 6) DCRF_RUNS(R, 3, 6);
+    else if (a == 6 && b == 3) DCRF_RUNS(R, 6, 3);
+    else throw Error{DCRF_ERR_VALUE, "no run mean-field kernel for these terms"};
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+#undef DCRF_RUNS
+
+size_t meanfield_runs_lds(int L, int ds, int P, int rows_lds, int cwf) {
+    (void)L;
+    return (size_t)rows_lds * sizeof(float) + (size_t)P * ds * sizeof(int2) + (size_t)cwf * sizeof(float);
+}
+
+void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTable& st_in, const RunSliceDev& rs,
+                           hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
+    const LaunchShape sh = shape_for(L);
+    if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
+        throw Error{DCRF_ERR_VALUE, "no run mean-field kernel for these terms"};
+    const int ds = st.t[0].d1 + (st.nterms > 1 ? st.t[1].d1 : 0);
+    const size_t lds = meanfield_runs_lds(L, ds, rs.P, rs.rows_lds, cwf);
+    static size_t granted[5] = {};
+    auto grant = [&](const void* fn, int r) {
+        if (lds > granted[r]) {
+            DCRF_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            granted[r] = lds;
+        }
+    };
+    const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
+    auto fn_for = [&](auto rtag) -> const void* {
+        constexpr int R_ = decltype(rtag)::value;
+        if (a == 3 && b == 0) return reinterpret_cast<const void*>(&k_meanfield_runs<R_, 3, 0>);
+        if (a == 6 && b == 0) return reinterpret_cast<const void*>(&k_meanfield_runs<R_, 6, 0>);
+        if (a == 3 && b == 6) return reinterpret_cast<const void*>(&k_meanfield_runs<R_, 3, 6>);
+        return reinterpret_cast<const void*>(&k_meanfield_runs<R_, 6, 3>);
+    };
+    switch (sh.R) {
+        case 1: grant(fn_for(std::integral_constant<int, 1>{}), 1); launch_runs_r<1>(sh, U, Q, N, L, st, rs, lds, s); break;
+        case 2: grant(fn_for(std::integral_constant<int, 2>{}), 2); launch_runs_r<2>(sh, U, Q, N, L, st, rs, lds, s); break;
+        case 3: grant(fn_for(std::integral_constant<int, 3>{}), 3); launch_runs_r<3>(sh, U, Q, N, L, st, rs, lds, s); break;
+        default: grant(fn_for(std::integral_constant<int, 4>{}), 4); launch_runs_r<4>(sh, U, Q, N, L, st, rs, lds, s); break;
+    }
+}
 
 void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st_in, const int2* slc,
                             hipStream_t s) {
@@ -790,8 +968,8 @@ void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& 
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_meanfield, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, U, Q, N, L, sh.S,
-                  sh.per, st, tmp1, tmp2);
+    DCRF_DISPATCH_CWL(sh, cwf, k_meanfield, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, U, Q, N,
+                      L, sh.S, sh.per, st, tmp1, tmp2);
 }
 
 void launch_softmax_neg(const float* U, float* Q, int N, int L, hipStream_t s) {
@@ -804,8 +982,8 @@ void launch_slice_term(float* out, int N, int L, const SliceTable& st_in, int wh
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_slice_term, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, out, N, L, sh.S,
-                  sh.per, st, which);
+    DCRF_DISPATCH_CWL(sh, cwf, k_slice_term, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, out, N,
+                      L, sh.S, sh.per, st, which);
 }
 
 int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st_in, double* partials,
@@ -814,7 +992,8 @@ int launch_kl(const float* Q, const float* U, int N, int L, const SliceTable& st
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
     const int nb = (int)blocks_for(N, sh.items);
-    DCRF_DISPATCH(sh, k_kl, nb, sizeof(float) * (sh.block + cwf), s, Q, U, N, L, sh.S, sh.per, st, partials);
+    DCRF_DISPATCH_CWL(sh, cwf, k_kl, nb, sizeof(float) * (sh.block + cwf), s, Q, U, N, L, sh.S, sh.per, st,
+                      partials);
     return nb;
 }
 

@@ -556,7 +556,167 @@ void build_lattice_d(const Geometry& g, const GeometryDev& gd, const TileMap& tm
     }
 }
 
+// Run of pixel p: a TW x TH rectangle of its 2-D instance (row-major over the image), or
+// a run of TW*TH consecutive pixels of other instances.
+__device__ inline int run_of(int p, const int* pix_base, const int* Wd, const int* run_base, int n_inst, int is2d,
+                             int TW, int TH) {
+    const int i = n_inst > 1 ? find_instance(pix_base, n_inst, p) : 0;
+    const int local = p - pix_base[i];
+    if (is2d) {
+        const int W = Wd[i];
+        const int y = local / W, x = local - y * W;
+        const int ntx = (W + TW - 1) / TW;
+        return run_base[i] + (y / TH) * ntx + x / TW;
+    }
+    return run_base[i] + local / (TW * TH);
+}
+
+// Pixel keys (run << 32) | p: sorted, they give the run-major pixel order.
+__global__ void k_rs_pixkeys(int N, const int* pix_base, const int* Wd, const int* run_base, int n_inst, int is2d,
+                             int TW, int TH, unsigned long long* __restrict__ key) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    key[p] = ((unsigned long long)(unsigned)run_of(p, pix_base, Wd, run_base, n_inst, is2d, TW, TH) << 32) | (unsigned)p;
+}
+
+// rpix[k] = the k-th pixel in run-major order, pos[p] = k, pbase[run] = first k of the run.
+__global__ void k_rs_order(const unsigned long long* __restrict__ skey, int N, int* __restrict__ rpix,
+                           int* __restrict__ pos, int* __restrict__ pbase) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int p = (int)(unsigned)(skey[k] & 0xffffffffu);
+    const int run = (int)(skey[k] >> 32);
+    rpix[k] = p;
+    pos[p] = k;
+    if (k == 0 || (int)(skey[k - 1] >> 32) != run) pbase[run] = k;
+}
+
+// Run slice: key of slice pair e = p*ds + j: (run of p << 32) | ((vertex << 1) | buffer).
+__global__ void k_rs_keys(const int2* __restrict__ slc, int n, int ds, unsigned par_mask, const int* pix_base,
+                          const int* Wd, const int* run_base, int n_inst, int is2d, int TW, int TH,
+                          unsigned long long* __restrict__ key) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int p = e / ds, j = e - p * ds;
+    const unsigned tag = ((unsigned)slc[e].x << 1) | ((par_mask >> j) & 1u);
+    key[e] = ((unsigned long long)(unsigned)run_of(p, pix_base, Wd, run_base, n_inst, is2d, TW, TH) << 32) | tag;
+}
+
+__global__ void k_rs_compact(const unsigned long long* __restrict__ key, const int* __restrict__ flag,
+                             const int* __restrict__ uid, int n, int* __restrict__ rows, int* __restrict__ base) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    rows[uid[i]] = (int)(unsigned)(key[i] & 0xffffffffu);
+    if (i == 0 || (key[i] >> 32) != (key[i - 1] >> 32)) base[(int)(key[i] >> 32)] = uid[i];
+}
+
+// uid is the exclusive scan of the key-start flags: the distinct-key index of sorted
+// entry i is uid[i] + flag[i] - 1.
+__global__ void k_rs_loc(const unsigned long long* __restrict__ key, const int* __restrict__ perm,
+                         const int* __restrict__ uid, const int* __restrict__ flag, const int* __restrict__ base,
+                         const int2* __restrict__ slc, const int* __restrict__ pos, int ds, int n,
+                         int2* __restrict__ loc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int e = perm[i];
+    const int p = e / ds, j = e - p * ds;
+    loc[(size_t)pos[p] * ds + j] = make_int2(uid[i] + flag[i] - 1 - base[(int)(key[i] >> 32)], slc[e].y);
+}
+
 }  // namespace
+
+void build_run_slice(const Geometry& g, const GeometryDev& gd, const int2* slc, int ds, unsigned par_mask, int TW,
+                     int TH, RunSlice& rs, hipStream_t s) {
+    const int N = g.N_total();
+    const long long n_ll = (long long)N * ds;
+    if (n_ll > 0x7ffffff0ll) throw Error{DCRF_ERR_VALUE, "too many slice pairs"};
+    const int n = (int)n_ll;
+    rs.P = TW * TH;
+    std::vector<int> rb(1, 0);
+    for (int i = 0; i < g.n_inst; i++) {
+        long long nr;
+        if (g.is2d)
+            nr = (long long)((g.W[i] + TW - 1) / TW) * ((g.H[i] + TH - 1) / TH);
+        else
+            nr = ((long long)g.N[i] + rs.P - 1) / rs.P;
+        rb.push_back((int)(rb.back() + nr));
+    }
+    rs.n_runs = rb.back();
+    int rbits = 1;
+    while ((1ll << rbits) <= (long long)rs.n_runs) rbits++;
+    DevBuf<int> run_base, pos;
+    run_base.reserve(rb.size());
+    DCRF_HIP_CHECK(hipMemcpyAsync(run_base.get(), rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, s));
+    const int* Wd = g.is2d ? gd.W.get() : nullptr;
+    const int is2d = g.is2d ? 1 : 0;
+    // run-major pixel order
+    rs.rpix.reserve(N);
+    rs.pix_base.reserve(rs.n_runs + 1);
+    pos.reserve(N);
+    {
+        DevBuf<unsigned long long> pk, spk;
+        pk.reserve(N);
+        spk.reserve(N);
+        k_rs_pixkeys<<<blocks_for(N), kThreads, 0, s>>>(N, gd.pix_base.get(), Wd, run_base.get(), g.n_inst, is2d, TW,
+                                                        TH, pk.get());
+        DCRF_HIP_CHECK(hipGetLastError());
+        size_t tb = 0;
+        DCRF_HIP_CHECK(rocprim::radix_sort_keys(nullptr, tb, pk.get(), spk.get(), (size_t)N, 0, 32 + rbits, s));
+        DevBuf<char> tmp;
+        tmp.reserve(tb);
+        DCRF_HIP_CHECK(rocprim::radix_sort_keys(tmp.get(), tb, pk.get(), spk.get(), (size_t)N, 0, 32 + rbits, s));
+        k_rs_order<<<blocks_for(N), kThreads, 0, s>>>(spk.get(), N, rs.rpix.get(), pos.get(), rs.pix_base.get());
+        DCRF_HIP_CHECK(hipGetLastError());
+        DCRF_HIP_CHECK(hipMemcpyAsync(rs.pix_base.get() + rs.n_runs, &N, sizeof(int), hipMemcpyHostToDevice, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    DevBuf<unsigned long long> key, skey;
+    DevBuf<int> iota, perm, flag, uid;
+    key.reserve(n);
+    skey.reserve(n);
+    iota.reserve(n);
+    perm.reserve(n);
+    flag.reserve(n);
+    uid.reserve(n);
+    k_rs_keys<<<blocks_for(n), kThreads, 0, s>>>(slc, n, ds, par_mask, gd.pix_base.get(), Wd, run_base.get(),
+                                                  g.n_inst, is2d, TW, TH, key.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    k_iota<<<blocks_for(n), kThreads, 0, s>>>(iota.get(), n);
+    DCRF_HIP_CHECK(hipGetLastError());
+    size_t tb = 0;
+    DCRF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, key.get(), skey.get(), iota.get(), perm.get(), (size_t)n, 0,
+                                             32 + rbits, s));
+    DevBuf<char> tmp;
+    tmp.reserve(tb);
+    DCRF_HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tb, key.get(), skey.get(), iota.get(), perm.get(), (size_t)n, 0,
+                                             32 + rbits, s));
+    k_run_flags<<<blocks_for(n), kThreads, 0, s>>>(skey.get(), n, flag.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    size_t sb = 0;
+    DCRF_HIP_CHECK(rocprim::exclusive_scan(nullptr, sb, flag.get(), uid.get(), 0, (size_t)n, rocprim::plus<int>(), s));
+    DevBuf<char> stmp;
+    stmp.reserve(sb);
+    DCRF_HIP_CHECK(rocprim::exclusive_scan(stmp.get(), sb, flag.get(), uid.get(), 0, (size_t)n, rocprim::plus<int>(), s));
+    int tail[2];
+    DCRF_HIP_CHECK(hipMemcpyAsync(&tail[0], uid.get() + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipMemcpyAsync(&tail[1], flag.get() + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    const int total = tail[0] + tail[1];
+    rs.rows.reserve(total);
+    rs.base.reserve(rs.n_runs + 1);
+    k_rs_compact<<<blocks_for(n), kThreads, 0, s>>>(skey.get(), flag.get(), uid.get(), n, rs.rows.get(), rs.base.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    DCRF_HIP_CHECK(hipMemcpyAsync(rs.base.get() + rs.n_runs, &total, sizeof(int), hipMemcpyHostToDevice, s));
+    rs.loc.reserve(n);
+    k_rs_loc<<<blocks_for(n), kThreads, 0, s>>>(skey.get(), perm.get(), uid.get(), flag.get(), rs.base.get(), slc,
+                                                  pos.get(), ds, n, rs.loc.get());
+    DCRF_HIP_CHECK(hipGetLastError());
+    std::vector<int> hb(rs.n_runs + 1);
+    DCRF_HIP_CHECK(hipMemcpyAsync(hb.data(), rs.base.get(), sizeof(int) * hb.size(), hipMemcpyDeviceToHost, s));
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    rs.u_max = 0;
+    for (int r = 0; r < rs.n_runs; r++) rs.u_max = std::max(rs.u_max, hb[r + 1] - hb[r]);
+}
 
 int default_tile_side(int L) {
     (void)L;

@@ -132,18 +132,64 @@ def test_start_step_kl_apply():
 
 @pytest.mark.parametrize("terms", ["g", "b", "gb", "bg"])
 @pytest.mark.parametrize("L", [4, 21])
-def test_fused_meanfield_bitwise(terms, L):
+@pytest.mark.parametrize("W,H", [(45, 31), (64, 48), (37, 23)])
+def test_fused_meanfield_bitwise(terms, L, W, H):
     """inference() runs the fused mean-field kernel for these term layouts, step() the
-    generic one: the same arithmetic in the same order, so Q agrees bit for bit."""
-    W, H = 45, 31
+    generic one: the same arithmetic in the same order, so Q agrees bit for bit; so do
+    the pixel-run form (whole and partial runs, two run heights) and the generic kernel
+    when selected."""
     img, U, _ = image_case(W, H, L, seed=len(terms) * 7 + L)
     spec = {"g": ("gauss", 3.0, 3.0, 3.0, SYM), "b": ("bilat", 20.0, 20.0, 13.0, 13.0, 13.0, img, 10.0, SYM)}
     g, _ = build_both(W, H, L, U, [spec[c] for c in terms])
+    assert g.splat_layout()["meanfield"] == 1  # the fused form by default
     Qi = g.inference(3)
     Q = g.start()
     for _ in range(3):
         Q, _, _ = g.step(Q)
     assert np.array_equal(Qi.view(np.int32), Q.view(np.int32))
+    for mode, rows, path in ((3, 0, 2), (3, 4, 2), (1, 0, 0)):  # pixel runs, generic
+        g.set_meanfield_kernel(mode, rows)
+        assert g.splat_layout()["meanfield"] == path
+        assert np.array_equal(g.inference(3).view(np.int32), Qi.view(np.int32)), (mode, rows)
+
+
+def test_fused_meanfield_generic_runs_bitwise():
+    """A generic (not 2-D) model whose one term has d = 2: the pixel-run kernel runs
+    over runs of consecutive pixels (a partial last run included)."""
+    rng = np.random.default_rng(29)
+    Wg, Hg, L = 50, 41, 21
+    yy, xx = np.mgrid[0:Hg, 0:Wg]
+    f = (np.stack([xx, yy]).reshape(2, -1) / 3.0).astype(np.float32)
+    U = rng.standard_normal((L, Wg * Hg)).astype(np.float32)
+    g, o = build_both(Wg * Hg, 1, L, U, [("feat", f, 3.0, SYM)], generic=True)
+    g.set_meanfield_kernel(3)
+    assert g.splat_layout()["meanfield"] == 2
+    Qi = g.inference(3)
+    Q = g.start()
+    for _ in range(3):
+        Q, _, _ = g.step(Q)
+    assert np.array_equal(Qi.view(np.int32), Q.view(np.int32))
+    assert np.abs(Qi - o.inference(3)).max() <= Q_TOL
+
+
+def test_fused_meanfield_without_runs_bitwise():
+    """Random 5-D features share almost no lattice vertices between neighbouring
+    pixels, so even when pixel runs are requested a run's rows exceed the LDS budget:
+    inference takes the plain fused kernel (interleaved slice pairs, rows gathered from
+    memory), bit-identical to the generic kernel of step()."""
+    rng = np.random.default_rng(23)
+    N, L = 3000, 21
+    f = (rng.standard_normal((5, N)) * 6).astype(np.float32)
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    g, o = build_both(N, 1, L, U, [("feat", f, 5.0, SYM)], generic=True)
+    g.set_meanfield_kernel(3)
+    assert g.splat_layout()["meanfield"] == 1  # the runs' rows exceed the LDS budget
+    Qi = g.inference(3)
+    Q = g.start()
+    for _ in range(3):
+        Q, _, _ = g.step(Q)
+    assert np.array_equal(Qi.view(np.int32), Q.view(np.int32))
+    assert np.abs(Qi - o.inference(3)).max() <= Q_TOL
 
 
 def test_no_unary_and_no_terms():
