@@ -18,6 +18,8 @@ Also reported on the same line:
   batched    64 images of uneven size in one batched model vs 64 plain per-image calls
   fwd_bwd    512x512x21, 5 iterations, learnable Matrix compatibility: forward and
              forward+backward through the torch.autograd layer
+  volumes    8 volumes 96x96xZ, 4 labels, 10 iterations, one batched model: forward and
+             forward+backward
   cpu_baseline  the reference CPU path (oracle/_ref, the reference's own C++ built from
              its sources; the oracle restatement if that build is absent) on the host,
              on the same 1024x768x21 workload
@@ -217,6 +219,71 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
             "fwd_bwd_Mpixel_iters_per_s": round(W * H * iters / (ms_fb / 1e3) / 1e6, 2)}
 
 
+def bench_volumes(torch, capi, syn, iters, reps, log):
+    """8 volumes of 96x96xZ, Z ~ U{32..64} (seeded), L = 4, Gaussian (z,y,x)/3 (d = 3) +
+    bilateral (z,y,x)/40, I/10 (d = 4), Diagonal compatibilities [1, 2, 2, 3], one batched
+    model: forward and forward+backward through the torch.autograd layer (cross-entropy
+    of the generator's true labels)."""
+    import numpy as np
+    from pydensecrf_amd.autograd import CRFLayer
+    from pydensecrf_amd.batched import BatchedDenseCRF
+    depths = syn.batch_volume_depths(8)
+    L = 4
+    feats_g, feats_b, Us, gts, sizes = [], [], [], [], []
+    for i, Z in enumerate(depths):
+        vol, U, gt = syn.make_volume(Z, 96, 96, L=L, seed=2000 + i)
+        zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(96), np.arange(96), indexing="ij")
+        pos = np.stack([zz, yy, xx]).reshape(3, -1).astype(np.float32)
+        feats_g.append(pos / np.float32(3))
+        feats_b.append(np.concatenate([pos / np.float32(40), vol.reshape(1, -1) / np.float32(0.1)]))
+        Us.append(U)
+        gts.append(gt)
+        sizes.append(Z * 96 * 96)
+    w = np.array([1, 2, 2, 3], np.float32)
+    t0 = time.perf_counter()
+    b = BatchedDenseCRF(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseEnergy(feats_g, compat=w)
+    b.addPairwiseEnergy(feats_b, compat=w)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    layer = CRFLayer(b.model, iters, [w, w])
+    Ut = torch.tensor(np.ascontiguousarray(np.concatenate(Us, axis=1).T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(np.concatenate(gts).astype(np.int64), device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        with torch.no_grad():
+            layer(Ut)
+
+    def fwd_bwd():
+        Q = layer(Ut)
+        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+        loss.backward()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f = timed(fwd)
+    ms_fb = timed(fwd_bwd)
+    nvox = sum(sizes)
+    log("volumes: %d voxels, fwd %.3f ms, fwd+bwd %.3f ms" % (nvox, ms_f, ms_fb))
+    return {"workload": "8 volumes 96x96xZ, Z in [32,64] (seeded), 4 labels, %d iterations, Gaussian d=3 + "
+                        "bilateral d=4, Diagonal compat, one batched model, torch.autograd" % iters,
+            "voxels": nvox, "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
+            "fwd_Mvoxel_iters_per_s": round(nvox * iters / (ms_f / 1e3) / 1e6, 2),
+            "fwd_bwd_Mvoxel_iters_per_s": round(nvox * iters / (ms_fb / 1e3) / 1e6, 2),
+            "lattice_build_s": round(build_s, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-image batched comparison")
     ap.add_argument("--no-fwd-bwd", action="store_true", help="skip the 512x512 forward+backward timing")
+    ap.add_argument("--no-volumes", action="store_true", help="skip the 8-volume batched forward+backward timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy three times (a known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -386,6 +454,8 @@ def main():
         out["batched"] = bench_batched(torch, capi, syn, 64, iters, 3, log)
     if not args.no_fwd_bwd:
         out["fwd_bwd"] = bench_fwd_bwd(torch, capi, syn, 5, 5, log)
+    if not args.no_volumes:
+        out["volumes"] = bench_volumes(torch, capi, syn, iters, 3, log)
     if not args.no_cpu_baseline:
         log("timing the reference CPU path on the same workload ...")
         cb = cpu_baseline(img, U, W, H, L, iters)

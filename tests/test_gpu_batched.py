@@ -93,3 +93,62 @@ def test_graph_replay_bitexact():
         got = Q.view(b.N, L).cpu().numpy().T
         assert np.array_equal(got.view(np.int32), ref.view(np.int32)), "replay %d" % r
         _ = b.inference(1)  # eager work between replays
+
+
+def _volume_case(dims, L, seed):
+    rng = np.random.default_rng(seed)
+    feats_g, feats_b, Us, gts, sizes = [], [], [], [], []
+    for (Z, Y, X) in dims:
+        zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+        pos = np.stack([zz, yy, xx]).reshape(3, -1).astype(np.float32)
+        inten = rng.uniform(0, 1, (1, Z * Y * X)).astype(np.float32)
+        feats_g.append(pos / np.float32(3))
+        feats_b.append(np.concatenate([pos / np.float32(4), inten / np.float32(0.1)]))
+        Us.append(rng.standard_normal((L, Z * Y * X)).astype(np.float32))
+        gts.append(rng.integers(0, L, Z * Y * X))
+        sizes.append(Z * Y * X)
+    return feats_g, feats_b, Us, gts, sizes
+
+
+def test_batched_volume_gradients_match_single():
+    """Forward+backward through the autograd layer on a batch of volumes (the C4 shape:
+    d = 3 Gaussian and d = 4 bilateral features, Diagonal compatibilities, L = 4) equals
+    the per-volume passes: dU bit for bit (every per-voxel kernel of the batch is the
+    per-volume kernel), compatibility and term-weight gradients (sums over voxels,
+    grouped differently) to 1e-5 relative."""
+    import torch
+    from pydensecrf_amd.autograd import CRFLayer
+    L, n = 4, 5
+    dims = [(6, 9, 10), (8, 7, 6), (5, 10, 12)]
+    feats_g, feats_b, Us, gts, sizes = _volume_case(dims, L, 31)
+    w1 = np.array([1, 2, 2, 3], np.float32)
+    w2 = np.array([2, 1, 3, 1], np.float32)
+
+    def run(model, U, gt):
+        layer = CRFLayer(model, n, [w1, w2], kweights=[1.5, 0.5])
+        Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+        Q = layer(Ut)
+        gtt = torch.tensor(gt, device="cuda")
+        loss = -torch.log(Q.gather(1, gtt[:, None])).sum()
+        loss.backward()
+        return (Q.detach().cpu().numpy(), Ut.grad.cpu().numpy(), layer.compat.grad.cpu().numpy(),
+                layer.kweights.grad.cpu().numpy())
+
+    b = BatchedDenseCRF(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseEnergy(feats_g, compat=w1)
+    b.addPairwiseEnergy(feats_b, compat=w2)
+    Qb, dUb, dcb, dkb = run(b.model, np.concatenate(Us, axis=1), np.concatenate(gts))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    dc_sum, dk_sum = 0.0, 0.0
+    for i in range(len(dims)):
+        m = capi.Model.create(sizes[i], L)
+        m.add_pairwise(feats_g[i], w1)
+        m.add_pairwise(feats_b[i], w2)
+        Qs, dUs, dcs, dks = run(m, Us[i], gts[i])
+        assert np.array_equal(Qb[off[i]:off[i + 1]].view(np.int32), Qs.view(np.int32)), i
+        assert np.array_equal(dUb[off[i]:off[i + 1]].view(np.int32), dUs.view(np.int32)), i
+        dc_sum = dc_sum + dcs.astype(np.float64)
+        dk_sum = dk_sum + dks.astype(np.float64)
+    np.testing.assert_allclose(dcb, dc_sum, rtol=1e-5, atol=1e-6 * np.abs(dc_sum).max())
+    np.testing.assert_allclose(dkb, dk_sum, rtol=1e-5, atol=1e-6 * np.abs(dk_sum).max())
