@@ -214,7 +214,7 @@ __device__ __forceinline__ void slice_piece(const SliceTerm& T, int p, int k0, i
     float x[kMaxD1][R];
 #pragma unroll
     for (int j = 0; j < kMaxD1; j++)
-        if (j < d1) load_piece<R>(T.vals + (size_t)v[j] * L, k0, L, x[j]);
+        if (j < d1) load_piece<R>(T.vals + (size_t)v[j] * T.ldv, k0, L, x[j]);
 #pragma unroll
     for (int r = 0; r < R; r++) m[r] = 0.0f;
 #pragma unroll

@@ -193,6 +193,7 @@ struct SliceTerm {
     const float* cw;    // device: Potts [1], Diagonal [L], Matrix [L*L] (symmetric)
     const float* kw;    // device: the term's weight (one float), null = 1
     float* mout;        // optional output of the raw message K~Q [N][L] (training cache)
+    int ldv = 0;        // row stride of vals (floats)
     int cw_lds = -1;    // Matrix: offset (floats) of the LDS copy of cw in the kernel's compat
                         // region, or -1 to read cw from global memory (set by the launchers)
 };
@@ -219,8 +220,8 @@ LaunchShape shape_for(int L);
 
 // Splat: vals[out] = sum over a segment's entries i of w_i * in[p_i], in entry order;
 // ent[i] = (p_i, bits of w_i).  Global form: p_i is the pixel (row of `in`).
-void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, float* vals, int L,
-                  hipStream_t s);
+void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, int ldi, float* vals,
+                  int ldv, int L, hipStream_t s);
 // Tiled form: one block per tile stages the tile's input rows, entries and segments in
 // LDS; p_i is then the index of the entry's pixel inside its tile (row-major in the
 // tile).  The entries of tile t are [tile_e[t], tile_e[t+1]), its segments
@@ -236,10 +237,11 @@ struct TiledSplat {
 // LDS bytes of the tiled splat (0 if it exceeds lds_cap, the LDS one workgroup may
 // allocate, or L needs the block-mode lane groups: use the global form).
 size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap);
-void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, size_t lds_cap,
-                        hipStream_t s);
-void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s);
-void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step,
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int ldv, int L,
+                        size_t lds_cap, hipStream_t s);
+void launch_combine(const Combine* c, int n, float* vals, int ldv, int L, hipStream_t s);
+// One blur step over the vertices [v0, Vp) (a range of the terms with d >= step).
+void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);
 // Mean-field update: E = -U - sum_k compat_k(slice_k); Q = softmax(E).  U may be null.
 // tmp1/tmp2 optional outputs (energy, last term's message).
@@ -275,8 +277,8 @@ void build_run_slice(const Geometry& g, const GeometryDev& gd, const int2* slc, 
 size_t meanfield_runs_lds(int L, int ds, int P, int rows_lds, int cwf);
 void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTable& st, const RunSliceDev& rs,
                            hipStream_t s);
-void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st, const int2* slc,
-                            hipStream_t s);
+void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
+                            const int2* slc, hipStream_t s);
 void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,
                              hipStream_t s);
 // out = compat_k(slice_k) for term index `which` of st.

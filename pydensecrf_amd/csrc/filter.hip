@@ -67,8 +67,8 @@ constexpr int kWin = 512;  // entry pairs per wave window
 
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks, int n,
-                                                const int2* __restrict__ ent, const float* __restrict__ in,
-                                                float* __restrict__ vals, int L, int S, int per) {
+                                                const int2* __restrict__ ent, const float* __restrict__ in, int ldi,
+                                                float* __restrict__ vals, int ldv, int L, int S, int per) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
     const int k0 = G.k0;
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
 #pragma unroll
                     for (int u = 0; u < kBatch; u++) e[u] = win[i + u];  // i + u < wn + kBatch
 #pragma unroll
-                    for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * L, k0, L, xx[u]);
+                    for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * ldi, k0, L, xx[u]);
 #pragma unroll
                     for (int u = 0; u < kBatch; u++)
                         if (i + u < b) {
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
                 }
             }
         }
-        if (mine) store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
+        if (mine) store_piece<R>(vals + (size_t)ch.out * ldv, k0, L, acc);
         return;
     }
     if (!G.on || c >= n) return;
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
 #pragma unroll
         for (int u = 0; u < kBatch; u++) e[u] = ent[min(i + u, ch.end - 1)];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * L, k0, L, xx[u]);
+        for (int u = 0; u < kBatch; u++) load_piece<R>(in + (size_t)e[u].x * ldi, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
             if (i + u < ch.end) {
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
                 }
             }
     }
-    store_piece<R>(vals + (size_t)ch.out * L, k0, L, acc);
+    store_piece<R>(vals + (size_t)ch.out * ldv, k0, L, acc);
 }
 
 // Tiled splat: block = one tile.  Stage 1 copies the tile's input rows, its entries
@@ -157,7 +157,7 @@ constexpr int kStage = 32;
 template <int R>
 __global__ void __launch_bounds__(1024) k_splat_tiled(TiledSplat ts, const int2* __restrict__ ent,
                                                             const float* __restrict__ in, float* __restrict__ vals,
-                                                            int L, int S, int per, int q_max) {
+                                                            int ldv, int L, int S, int per, int q_max) {
     extern __shared__ float sh[];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     int p0, tw, th, stride;
@@ -244,13 +244,13 @@ __global__ void __launch_bounds__(1024) k_splat_tiled(TiledSplat ts, const int2*
                 acc[r] = acc[r] + prod;
             }
         }
-        store_piece<R>(vals + (size_t)out * L, k0, L, acc);
+        store_piece<R>(vals + (size_t)out * ldv, k0, L, acc);
     }
 }
 
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb, int n, float* __restrict__ vals,
-                                                  int L, int S, int per) {
+                                                  int ldv, int L, int S, int per) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int c = G.item();
     if (!G.on || c >= n) return;
@@ -263,21 +263,22 @@ __global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb
         float xx[kBatch][R];
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
-            load_piece<R>(vals + (size_t)(rc.first + min(i + u, rc.count - 1)) * L, k0, L, xx[u]);
+            load_piece<R>(vals + (size_t)(rc.first + min(i + u, rc.count - 1)) * ldv, k0, L, xx[u]);
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
             if (i + u < rc.count)
 #pragma unroll
                 for (int r = 0; r < R; r++) acc[r] = acc[r] + xx[u][r];
     }
-    store_piece<R>(vals + (size_t)rc.v * L, k0, L, acc);
+    store_piece<R>(vals + (size_t)rc.v * ldv, k0, L, acc);
 }
 
 template <bool WAVE, int R>
-__global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, float* __restrict__ dst, int Vp,
-                                               int L, int S, int per, BlurTable bt, int step, int reverse) {
+__global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, float* __restrict__ dst, int v0,
+                                               int Vp, int ldv, int L, int S, int per, BlurTable bt, int step,
+                                               int reverse) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
-    const int v = G.item();
+    const int v = v0 + G.item();  // vertices [v0, Vp)
     if (!G.on || v >= Vp) return;
     const int k0 = G.k0;
     // terms are ordered by vbase; pick the last one starting at or before v
@@ -294,17 +295,17 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
     }
     const int j = reverse ? d - step : step;
     float o[R], a[R], b[R], out[R];
-    load_piece<R>(src + (size_t)v * L, k0, L, o);
     const int2 nb = nbr[(size_t)j * V + (v - vb)];
-    load_piece<R>(src + (size_t)nb.x * L, k0, L, a);
-    load_piece<R>(src + (size_t)nb.y * L, k0, L, b);
+    load_piece<R>(src + (size_t)v * ldv, k0, L, o);
+    load_piece<R>(src + (size_t)nb.x * ldv, k0, L, a);
+    load_piece<R>(src + (size_t)nb.y * ldv, k0, L, b);
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const float s2 = a[r] + b[r];
         const float h = 0.5f * s2;
         out[r] = o[r] + h;
     }
-    store_piece<R>(dst + (size_t)v * L, k0, L, out);
+    store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
 }
 
 // Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
@@ -427,8 +428,8 @@ __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceT
 template <int R, int D0, int D1, bool CWL>
 __global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))  // <= 64 VGPRs
 k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
-                                                                int N, int L, int S, int per, SliceTable st,
-                                                                const int2* __restrict__ slc) {
+                                                                int ldq, int N, int L, int S, int per,
+                                                                SliceTable st, const int2* __restrict__ slc) {
     constexpr int DS = D0 + D1;
     LaneGroup<true, R> G(L, S, per, nullptr);
     const int p = G.item();
@@ -472,8 +473,11 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
     }
     float x[DS][R];
 #pragma unroll
-    for (int j = 0; j < DS; j++) load_piece<R>((j < D0 ? st.t[0].vals : st.t[1].vals) + (size_t)e[j].x * L, k0, L, x[j]);
-    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc);
+    for (int j = 0; j < DS; j++) {
+        const SliceTerm& T = j < D0 ? st.t[0] : st.t[1];
+        load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
+    }
+    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc);
 }
 
 // The same update over pixel runs (2-D rectangles whose rows hold whole waves of
@@ -506,7 +510,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_runs(const float* __re
         for (int b = 0; b < 4; b++) {
             const int q = min(q0 + b * items + gi, nu - 1);
             const int tag = rs.rows[u0 + q];
-            load_piece<R>(((tag & 1) ? rs.v1 : rs.v0) + (size_t)(tag >> 1) * L, k0, L, xs[b]);
+            load_piece<R>(((tag & 1) ? rs.v1 : rs.v0) + (size_t)(tag >> 1) * st.t[0].ldv, k0, L, xs[b]);
         }
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -784,13 +788,13 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
     } while (0)
 
 
-void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, float* vals, int L,
-                  hipStream_t s) {
+void launch_splat(const Chunk* chunks, int n_chunks, const int2* ent, const float* in, int ldi, float* vals,
+                  int ldv, int L, hipStream_t s) {
     if (!n_chunks) return;
     const LaunchShape sh = shape_for(L);
     const size_t shm = sh.wave ? (size_t)(kWaveBlock / 64) * (kWin + kBatch) * sizeof(int2) : 0;
-    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), shm, s, chunks, n_chunks, ent, in, vals, L, sh.S,
-                  sh.per);
+    DCRF_DISPATCH(sh, k_splat, blocks_for(n_chunks, sh.items), shm, s, chunks, n_chunks, ent, in, ldi, vals, ldv, L,
+                  sh.S, sh.per);
 }
 
 size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap) {
@@ -805,8 +809,8 @@ size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap) {
 // takes more than a quarter of the workgroup LDS limit, else 256.
 int tiled_splat_block(size_t lds, size_t lds_cap) { return lds * 4 > lds_cap ? 1024 : kWaveBlock; }
 
-void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int L, size_t lds_cap,
-                        hipStream_t s) {
+void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int ldv, int L,
+                        size_t lds_cap, hipStream_t s) {
     if (!ts.n_tiles) return;
     const LaunchShape sh = shape_for(L);
     const size_t lds = tiled_splat_lds(L, ts.geom.side, ts.e_max, ts.s_max, lds_cap);
@@ -828,25 +832,26 @@ void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, 
         default: grant(reinterpret_cast<const void*>(&k_splat_tiled<4>), 4); break;
     }
     switch (sh.R) {
-        case 1: k_splat_tiled<1><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        case 2: k_splat_tiled<2><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        case 3: k_splat_tiled<3><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
-        default: k_splat_tiled<4><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, L, sh.S, sh.per, q_max); break;
+        case 1: k_splat_tiled<1><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, ldv, L, sh.S, sh.per, q_max); break;
+        case 2: k_splat_tiled<2><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, ldv, L, sh.S, sh.per, q_max); break;
+        case 3: k_splat_tiled<3><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, ldv, L, sh.S, sh.per, q_max); break;
+        default: k_splat_tiled<4><<<ts.n_tiles, block, lds, s>>>(ts, ent, in, vals, ldv, L, sh.S, sh.per, q_max); break;
     }
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
-void launch_combine(const Combine* c, int n, float* vals, int L, hipStream_t s) {
+void launch_combine(const Combine* c, int n, float* vals, int ldv, int L, hipStream_t s) {
     if (!n) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, L, sh.S, sh.per);
+    DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, ldv, L, sh.S, sh.per);
 }
 
-void launch_blur(const float* src, float* dst, int Vp, int L, const BlurTable& bt, int step, int reverse,
-                 hipStream_t s) {
-    if (!Vp) return;
+void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, const BlurTable& bt, int step,
+                 int reverse, hipStream_t s) {
+    if (Vp <= v0) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_blur, blocks_for(Vp, sh.items), 0, s, src, dst, Vp, L, sh.S, sh.per, bt, step, reverse);
+    DCRF_DISPATCH(sh, k_blur, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, bt, step,
+                  reverse);
 }
 
 bool fused_meanfield_ok(int L, int nterms, int d1_0, int d1_1) {
@@ -868,11 +873,11 @@ void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int
         blocks_for(N, sh.items), kWaveBlock,                                                                    \
                                       (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) +            \
                                           (size_t)cwf * sizeof(float), s>>>(                                          \
-        U, Q, N, L, sh.S, sh.per, st, \
+        U, Q, ldq, N, L, sh.S, sh.per, st, \
                                                                                 slc)
 
 template <int R>
-void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int N, int L, const SliceTable& st,
+void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
                     const int2* slc, int cwf, hipStream_t s) {
     const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
     if (a == 3 && b == 0) DCRF_FUSED(R, 3, 0);
@@ -937,18 +942,18 @@ void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTa
     }
 }
 
-void launch_meanfield_fused(const float* U, float* Q, int N, int L, const SliceTable& st_in, const int2* slc,
-                            hipStream_t s) {
+void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st_in,
+                            const int2* slc, hipStream_t s) {
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
     if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
         throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
     switch (sh.R) {
-        case 1: launch_fused_r<1>(sh, U, Q, N, L, st, slc, cwf, s); break;
-        case 2: launch_fused_r<2>(sh, U, Q, N, L, st, slc, cwf, s); break;
-        case 3: launch_fused_r<3>(sh, U, Q, N, L, st, slc, cwf, s); break;
-        default: launch_fused_r<4>(sh, U, Q, N, L, st, slc, cwf, s); break;
+        case 1: launch_fused_r<1>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
+        case 2: launch_fused_r<2>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
+        case 3: launch_fused_r<3>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
+        default: launch_fused_r<4>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
     }
 }
 
