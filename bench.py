@@ -17,7 +17,8 @@ Also reported on the same line:
   hipgraph   the same steps replayed as one captured hipGraph
   batched    64 images of uneven size in one batched model vs 64 plain per-image calls
   fwd_bwd    512x512x21, 5 iterations, learnable Matrix compatibility: forward and
-             forward+backward through the torch.autograd layer
+             forward+backward through the torch.autograd layer, also with learnable
+             kernel parameters (their gradient and the lattice rebuild)
   volumes    8 volumes 96x96xZ, 4 labels, 10 iterations, one batched model: forward and
              forward+backward
   cpu_baseline  the reference CPU path (oracle/_ref, the reference's own C++ built from
@@ -212,11 +213,25 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
     ms_f = timed(fwd)
     ms_fb = timed(fwd_bwd)
     log("fwd+bwd 512x512x21: fwd %.3f ms, fwd+bwd %.3f ms" % (ms_f, ms_fb))
+    # the same with the kernel parameters (DIAG feature scales, 2 + 5) learnable as well:
+    # the backward adds the kernel-parameter gradient; a changed parameter vector
+    # rebuilds both lattices before the next forward (timed on its own)
+    layer = CRFLayer(m, iters, [c1, c2], kweights=[3.0, 10.0], learn_kernel=True)
+    ms_fbk = timed(fwd_bwd)
+    kp = m.kernel_params()
+    t0 = time.perf_counter()
+    for r in range(3):
+        m.set_kernel_params(kp * np.float32(1.0 + 0.01 * (r + 1)))
+    ms_rebuild = (time.perf_counter() - t0) * 1e3 / 3
+    m.set_kernel_params(kp)
+    log("fwd+bwd with kernel gradient %.3f ms; lattice rebuild %.2f ms" % (ms_fbk, ms_rebuild))
     return {"workload": "DenseCRF2D 512x512, 21 labels, %d iterations, MatrixCompatibility + term weights, "
                         "cross-entropy, torch.autograd" % iters,
             "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
             "fwd_Mpixel_iters_per_s": round(W * H * iters / (ms_f / 1e3) / 1e6, 2),
-            "fwd_bwd_Mpixel_iters_per_s": round(W * H * iters / (ms_fb / 1e3) / 1e6, 2)}
+            "fwd_bwd_Mpixel_iters_per_s": round(W * H * iters / (ms_fb / 1e3) / 1e6, 2),
+            "fwd_bwd_kernel_ms": round(ms_fbk, 4), "kernel_ratio": round(ms_fbk / ms_f, 3),
+            "kernel_param_rebuild_ms": round(ms_rebuild, 3)}
 
 
 def bench_volumes(torch, capi, syn, iters, reps, log):

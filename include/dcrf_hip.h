@@ -143,14 +143,24 @@ int dcrf_set_kernel_weights(dcrf_model *m, const float *w);
  * Diagonal L, Matrix upper triangle W(i, j>=i) row by row). */
 int dcrf_num_compat_params(const dcrf_model *m);
 int dcrf_get_compat_params(const dcrf_model *m, float *params);
+/* kernelParameters / setKernelParameters (densecrf.cpp:334-360, pairwise.cpp:129-151):
+ * per term none (CONST_KERNEL), d scales (DIAG_KERNEL, ones at first) or a d x d matrix
+ * stored column-major (FULL_KERNEL, the identity at first), concatenated over terms.
+ * Setting them rebuilds each DIAG/FULL term's lattice and normaliser from diag(p) f or
+ * P f, f the features the term was added with. */
+int dcrf_num_kernel_params(const dcrf_model *m);
+int dcrf_get_kernel_params(const dcrf_model *m, float *params);
+int dcrf_set_kernel_params(dcrf_model *m, const float *params);
 /* DenseCRF::gradient with the LogLikelihood objective (densecrf.cpp:238-297,
  * objective.cpp:37-50).  gt [N] int16 (labels outside [0, L) ignored).  Outputs (any
  * may be NULL): objective = mean log-likelihood; unary_grad [N][L] = d objective / dU
  * (the unary gradient an identity-parameterised unary would report); compat_grad
- * packed like dcrf_get_compat_params; kweight_grad [num_terms] = d objective / d w_k.
+ * packed like dcrf_get_compat_params; kweight_grad [num_terms] = d objective / d w_k
+ * (no counterpart in the reference: its terms have no weight); kernel_grad packed like
+ * dcrf_get_kernel_params (the reference's kernel_grad, pairwise.cpp:152-163, 202-207).
  * Q_t and the messages K~_k Q_t are kept from the forward pass (no re-filtering). */
 int dcrf_gradient(dcrf_model *m, int n_iterations, const int16_t *gt, float robust, double *objective,
-                  float *unary_grad, float *compat_grad, float *kweight_grad);
+                  float *unary_grad, float *compat_grad, float *kweight_grad, float *kernel_grad);
 /* Training forward on device buffers: like dcrf_inference_device, but keeps Q_0..Q_n
  * and the messages for dcrf_backward_device.  compat_dev: raw compatibility parameters
  * of all terms concatenated (NULL = the model's); kw_dev: [num_terms] term weights (NULL
@@ -158,9 +168,12 @@ int dcrf_gradient(dcrf_model *m, int n_iterations, const int16_t *gt, float robu
 int dcrf_forward_device(dcrf_model *m, int n_iterations, const float *U_dev, const float *compat_dev,
                         const float *kw_dev, float *Q_dev, void *stream);
 /* Adjoint of the last dcrf_forward_device: dQ_dev = dL/dQ_n [N][L].  Writes dU_dev
- * [N][L], dcompat_dev (raw layout, may be NULL) and dkw_dev ([num_terms], may be NULL). */
+ * [N][L], dcompat_dev (raw layout, may be NULL), dkw_dev ([num_terms], may be NULL) and
+ * dkernel_dev (packed like dcrf_get_kernel_params, may be NULL: the reference's
+ * kernel_grad recursion, densecrf.cpp:280-284, which costs two extra filters and four
+ * lattice-gradient passes per term and iteration). */
 int dcrf_backward_device(dcrf_model *m, const float *dQ_dev, float *dU_dev, float *dcompat_dev, float *dkw_dev,
-                         void *stream);
+                         float *dkernel_dev, void *stream);
 
 /* ---- introspection (parity tests, metrics) ------------------------------------- */
 int dcrf_num_variables(const dcrf_model *m);

@@ -357,12 +357,96 @@ void orc_compute_d(const orc_lattice *lat, const double *in, double *out, int L,
     compute_d(lat, in, out, L, reverse, 0);
 }
 
+/* Permutohedral::gradient (permutohedral.cpp:612-695).  df [N][d] is zeroed, then for
+ * dir 0 the lattice splats a and blurs axes 0..d, for dir 1 it splats b and blurs d..0
+ * (its own scalar splat and blur: the old value plus a double 0.5*(n1+n2), :645-659);
+ * the slicing gradient rotates each point's vertex rows by rank into differences
+ * (alpha = 1/(1+2^-d)/(d+1), :628) and pairs their elevation with b (dir 0) or a
+ * (dir 1), summed over the vs values (:662-691). */
+void orc_lattice_gradient(const orc_lattice *lat, float *df, const float *a, const float *b, int vs)
+{
+    const int N = lat->N, d = lat->d, V = lat->V;
+    float *values = (float *)malloc(sizeof(float) * (size_t)(V + 2) * vs);
+    float *nv = (float *)malloc(sizeof(float) * (size_t)(V + 2) * vs);
+    float *r_a = (float *)malloc(sizeof(float) * (size_t)(d + 1) * vs);
+    float *sm = (float *)malloc(sizeof(float) * (size_t)vs);
+    memset(df, 0, sizeof(float) * (size_t)N * d);
+    float sf[32];
+    const float inv_std_dev = (float)(sqrt(2.0 / 3.0) * (d + 1));
+    for (int i = 0; i < d; i++) sf[i] = (float)(1.0 / sqrt((double)((i + 2) * (i + 1))) * inv_std_dev);
+    const float alpha = 1.0f / (1 + powf(2, (float)-d)) / (float)(d + 1);
+    for (int dir = 0; dir < 2; dir++) {
+        memset(values, 0, sizeof(float) * (size_t)(V + 2) * vs);
+        memset(nv, 0, sizeof(float) * (size_t)(V + 2) * vs);
+        const float *src = dir ? b : a, *other = dir ? a : b;
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j <= d; j++) {
+                const int o = lat->offset[i * (d + 1) + j] + 1;
+                const float w = lat->bary[i * (d + 1) + j];
+                for (int k = 0; k < vs; k++) {
+                    const float p = w * src[(size_t)i * vs + k];
+                    values[(size_t)o * vs + k] = values[(size_t)o * vs + k] + p;
+                }
+            }
+        for (int j = dir ? d : 0; j <= d && j >= 0; dir ? j-- : j++) {
+            for (int i = 0; i < V; i++) {
+                const float *ov = values + (size_t)(i + 1) * vs;
+                float *w = nv + (size_t)(i + 1) * vs;
+                const int n1 = lat->nbr[2 * ((size_t)j * V + i)] + 1;
+                const int n2 = lat->nbr[2 * ((size_t)j * V + i) + 1] + 1;
+                const float *v1 = values + (size_t)n1 * vs, *v2 = values + (size_t)n2 * vs;
+                for (int k = 0; k < vs; k++) {
+                    const float s2 = v1[k] + v2[k];
+                    w[k] = (float)((double)ov[k] + 0.5 * (double)s2);
+                }
+            }
+            float *t = values;
+            values = nv;
+            nv = t;
+        }
+        for (int i = 0; i < N; i++) {
+            memset(r_a, 0, sizeof(float) * (size_t)(d + 1) * vs);
+            for (int j = 0; j <= d; j++) {
+                const int r0 = d - lat->rank[i * (d + 1) + j];
+                const int r1 = r0 + 1 > d ? 0 : r0 + 1;
+                const int o0 = lat->offset[i * (d + 1) + r0] + 1;
+                const int o1 = lat->offset[i * (d + 1) + r1] + 1;
+                for (int k = 0; k < vs; k++) {
+                    const float p0 = alpha * values[(size_t)o0 * vs + k];
+                    r_a[j * vs + k] = r_a[j * vs + k] + p0;
+                    const float p1 = alpha * values[(size_t)o1 * vs + k];
+                    r_a[j * vs + k] = r_a[j * vs + k] - p1;
+                }
+            }
+            memcpy(sm, r_a, sizeof(float) * (size_t)vs);
+            for (int j = 1; j <= d; j++) {
+                float grad = 0;
+                for (int k = 0; k < vs; k++) {
+                    const float jr = (float)j * r_a[j * vs + k];
+                    const float diff = sm[k] - jr;
+                    const float v = sf[j - 1] * diff;
+                    const float p = other[(size_t)i * vs + k] * v;
+                    grad = grad + p;
+                    sm[k] = sm[k] + r_a[j * vs + k];
+                }
+                df[(size_t)i * d + j - 1] = df[(size_t)i * d + j - 1] + grad;
+            }
+        }
+    }
+    free(values);
+    free(nv);
+    free(r_a);
+    free(sm);
+}
+
 /* ------------------------------------------------------------------ */
 /* DenseCRF                                                             */
 /* ------------------------------------------------------------------ */
 
 typedef struct {
     orc_lattice lat;
+    float *feat;     /* [N][d] the term's features (DenseKernel::f_) */
+    float *kparams;  /* DIAG [d] scales, FULL [d*d] column-major matrix (parameters_) */
     int ktype, ntype, ckind, ncompat;
     float *compat;   /* Potts [1], Diagonal [L], Matrix [L*L] symmetrised W */
     float *norm;     /* [N] */
@@ -398,6 +482,8 @@ void orc_crf_free(orc_crf *c)
     if (!c) return;
     for (int k = 0; k < c->nterms; k++) {
         orc_lattice_free(&c->terms[k].lat);
+        free(c->terms[k].feat);
+        free(c->terms[k].kparams);
         free(c->terms[k].compat);
         free(c->terms[k].norm);
         free(c->terms[k].norm_d);
@@ -442,6 +528,11 @@ int orc_crf_add_pairwise(orc_crf *c, const float *feat, int d, int ckind, const 
     orc_term *t = &c->terms[c->nterms];
     memset(t, 0, sizeof(*t));
     if (orc_lattice_build(&t->lat, feat, c->N, d) != 0) return -1;
+    t->feat = (float *)malloc(sizeof(float) * (size_t)c->N * d);
+    memcpy(t->feat, feat, sizeof(float) * (size_t)c->N * d);
+    /* DenseKernel constructor (pairwise.cpp:116-122): ones, resp. the identity */
+    t->kparams = (float *)calloc((size_t)d * d, sizeof(float));
+    for (int j = 0; j < d; j++) t->kparams[ktype == ORC_FULL_KERNEL ? j * d + j : j] = 1.0f;
     t->ktype = ktype;
     t->ntype = ntype;
     t->ckind = ckind;
@@ -680,9 +771,186 @@ static void compat_gradient_f(const orc_term *t, int N, int L, const float *b, c
     }
 }
 
-/* DenseCRF::gradient (densecrf.cpp:238-297) with LogLikelihood (objective.cpp:37-50). */
+/* DenseCRF::kernelParameters (densecrf.cpp:334-347, pairwise.cpp:129-140). */
+void orc_crf_get_kernel_params(const orc_crf *c, float *p)
+{
+    for (int k = 0; k < c->nterms; k++) {
+        const orc_term *t = &c->terms[k];
+        const int d = t->lat.d;
+        const int n = t->ktype == ORC_CONST_KERNEL ? 0 : (t->ktype == ORC_DIAG_KERNEL ? d : d * d);
+        memcpy(p, t->kparams, sizeof(float) * (size_t)n);
+        p += n;
+    }
+}
+
+/* DenseCRF::setKernelParameters (densecrf.cpp:348-360) -> DenseKernel::setParameters
+ * (pairwise.cpp:141-151): the lattice and normaliser are rebuilt from diag(p) f (DIAG)
+ * or P f (FULL, P column-major); f itself is kept. */
+int orc_crf_set_kernel_params(orc_crf *c, const float *p)
+{
+    for (int k = 0; k < c->nterms; k++) {
+        orc_term *t = &c->terms[k];
+        const int d = t->lat.d, N = c->N;
+        if (t->ktype == ORC_CONST_KERNEL) continue;
+        const int n = t->ktype == ORC_DIAG_KERNEL ? d : d * d;
+        memcpy(t->kparams, p, sizeof(float) * (size_t)n);
+        p += n;
+        float *g = (float *)malloc(sizeof(float) * (size_t)N * d);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < d; j++) {
+                const float *f = t->feat + (size_t)i * d;
+                if (t->ktype == ORC_DIAG_KERNEL) {
+                    g[(size_t)i * d + j] = t->kparams[j] * f[j];
+                } else {
+                    float s = 0.0f;
+                    for (int q = 0; q < d; q++) {
+                        const float m = t->kparams[q * d + j] * f[q];
+                        s = s + m;
+                    }
+                    g[(size_t)i * d + j] = s;
+                }
+            }
+        orc_lattice_free(&t->lat);
+        const int r = orc_lattice_build(&t->lat, g, N, d);
+        free(g);
+        if (r != 0) return -1;
+        make_norm_f(&t->lat, t->ntype, t->norm);
+        make_norm_d(&t->lat, t->ntype, t->norm_d);
+    }
+    return 0;
+}
+
+int orc_crf_num_kernel_params(const orc_crf *c)
+{
+    int n = 0;
+    for (int k = 0; k < c->nterms; k++) {
+        const orc_term *t = &c->terms[k];
+        const int d = t->lat.d;
+        n += t->ktype == ORC_CONST_KERNEL ? 0 : (t->ktype == ORC_DIAG_KERNEL ? d : d * d);
+    }
+    return n;
+}
+
+/* DenseKernel::featureGradient (pairwise.cpp:88-114): fg [N][d] for the term's
+ * normalisation, from lattice gradients (above) and plain lattice filters. */
+static void feature_gradient_f(const orc_crf *c, const orc_term *t, const float *a, const float *b, float *fg)
+{
+    const int N = c->N, L = c->L, d = t->lat.d;
+    const size_t n = (size_t)N * L;
+    if (t->ntype == ORC_NO_NORMALIZATION) {
+        orc_lattice_gradient(&t->lat, fg, a, b, L);
+        return;
+    }
+    const float *nm = t->norm;
+    float *x = (float *)malloc(sizeof(float) * n), *ones = (float *)malloc(sizeof(float) * n);
+    float *u = (float *)malloc(sizeof(float) * n), *v = (float *)malloc(sizeof(float) * n);
+    float *r = (float *)malloc(sizeof(float) * (size_t)N * d);
+    for (size_t i = 0; i < n; i++) ones[i] = 1.0f;
+    if (t->ntype == ORC_NORMALIZE_SYMMETRIC) {
+        /* fa = K^T(a n), fb = K(b n); r = grad(0.5 (a fb + fa b) n^3, 1); g = grad(a n, b n) */
+        float *an = (float *)malloc(sizeof(float) * n), *bn = (float *)malloc(sizeof(float) * n);
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < L; k++) {
+                an[(size_t)i * L + k] = a[(size_t)i * L + k] * nm[i];
+                bn[(size_t)i * L + k] = b[(size_t)i * L + k] * nm[i];
+            }
+        orc_compute_f(&t->lat, an, u, L, 1); /* fa */
+        orc_compute_f(&t->lat, bn, v, L, 0); /* fb */
+        for (int i = 0; i < N; i++) {
+            const float n2 = nm[i] * nm[i];
+            const float n3 = n2 * nm[i];
+            for (int k = 0; k < L; k++) {
+                const size_t q = (size_t)i * L + k;
+                const float p1 = a[q] * v[q], p2 = u[q] * b[q];
+                const float sum = p1 + p2;
+                const float h = 0.5f * sum;
+                x[q] = h * n3;
+            }
+        }
+        orc_lattice_gradient(&t->lat, r, x, ones, L);
+        orc_lattice_gradient(&t->lat, fg, an, bn, L);
+        free(an);
+        free(bn);
+    } else if (t->ntype == ORC_NORMALIZE_AFTER) {
+        /* fb = K(b); r = grad(a fb n^2, 1); g = grad(a n, b) */
+        orc_compute_f(&t->lat, b, v, L, 0);
+        for (int i = 0; i < N; i++) {
+            const float n2 = nm[i] * nm[i];
+            for (int k = 0; k < L; k++) {
+                const size_t q = (size_t)i * L + k;
+                const float p = a[q] * v[q];
+                x[q] = p * n2;
+                u[q] = a[q] * nm[i];
+            }
+        }
+        orc_lattice_gradient(&t->lat, r, x, ones, L);
+        orc_lattice_gradient(&t->lat, fg, u, b, L);
+    } else {
+        /* NORMALIZE_BEFORE: fa = K^T(a); r = grad(fa b n^2, 1); g = grad(a, b n) */
+        orc_compute_f(&t->lat, a, u, L, 1);
+        for (int i = 0; i < N; i++) {
+            const float n2 = nm[i] * nm[i];
+            for (int k = 0; k < L; k++) {
+                const size_t q = (size_t)i * L + k;
+                const float p = u[q] * b[q];
+                x[q] = p * n2;
+                v[q] = b[q] * nm[i];
+            }
+        }
+        orc_lattice_gradient(&t->lat, r, x, ones, L);
+        orc_lattice_gradient(&t->lat, fg, a, v, L);
+    }
+    for (size_t i = 0; i < (size_t)N * d; i++) fg[i] = -r[i] + fg[i];
+    free(x);
+    free(ones);
+    free(u);
+    free(v);
+    free(r);
+}
+
+/* PairwisePotential::kernelGradient (pairwise.cpp:202-207) -> DenseKernel::gradient
+ * (:152-163): lbl = compatibility applied to Q; fg = featureGradient(b, lbl); DIAG
+ * returns the row sums of f .* fg (a sequential float sum per dimension), FULL the
+ * d x d product fg f^T packed column-major (accumulated in double: the reference's
+ * GEMM order depends on the host's cache blocking). */
+static void kernel_gradient_f(const orc_crf *c, const orc_term *t, const float *b, const float *Q, float *g)
+{
+    const int N = c->N, L = c->L, d = t->lat.d;
+    if (t->ktype == ORC_CONST_KERNEL) return;
+    float *lbl = (float *)malloc(sizeof(float) * (size_t)N * L);
+    float *fg = (float *)malloc(sizeof(float) * (size_t)N * d);
+    compat_apply_f(t->ckind, t->compat, N, L, Q, lbl, 0);
+    feature_gradient_f(c, t, b, lbl, fg);
+    if (t->ktype == ORC_DIAG_KERNEL) {
+        for (int j = 0; j < d; j++) {
+            float s = t->feat[j] * fg[j];
+            for (int i = 1; i < N; i++) {
+                const float p = t->feat[(size_t)i * d + j] * fg[(size_t)i * d + j];
+                s = s + p;
+            }
+            g[j] = s;
+        }
+    } else {
+        for (int col = 0; col < d; col++)
+            for (int row = 0; row < d; row++) {
+                double s = 0;
+                for (int i = 0; i < N; i++) s += (double)fg[(size_t)i * d + row] * (double)t->feat[(size_t)i * d + col];
+                g[col * d + row] = (float)s;
+            }
+    }
+    free(lbl);
+    free(fg);
+}
+
 double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
                         float *compat_grad)
+{
+    return orc_crf_gradient_k(c, n_iter, gt, robust, unary_grad, compat_grad, NULL);
+}
+
+/* DenseCRF::gradient (densecrf.cpp:238-297) with LogLikelihood (objective.cpp:37-50). */
+double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
+                          float *compat_grad, float *kernel_grad)
 {
     const int N = c->N, L = c->L;
     const size_t n = (size_t)N * L;
@@ -716,11 +984,14 @@ double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, f
     sum_and_normalize_f(b, Qn, b, N, L);
     if (unary_grad) memcpy(unary_grad, b, sizeof(float) * n);
     const int ncp = orc_crf_num_compat_params(c);
+    const int nkp = orc_crf_num_kernel_params(c);
     if (compat_grad) memset(compat_grad, 0, sizeof(float) * ncp);
+    if (kernel_grad) memset(kernel_grad, 0, sizeof(float) * nkp);
     float *pg = (float *)malloc(sizeof(float) * (ncp > 0 ? ncp : 1));
+    float *kg = (float *)malloc(sizeof(float) * (nkp > 0 ? nkp : 1));
     for (int it = n_iter - 1; it >= 0; it--) {
         memset(tmp1, 0, sizeof(float) * n);
-        int ip = 0;
+        int ip = 0, ik = 0;
         for (int k = 0; k < c->nterms; k++) {
             const orc_term *t = &c->terms[k];
             const int np = t->ckind == ORC_MATRIX ? L * (L + 1) / 2 : t->ncompat;
@@ -731,6 +1002,12 @@ double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, f
                 for (int j = 0; j < np; j++) compat_grad[ip + j] = compat_grad[ip + j] + pg[j];
             }
             ip += np;
+            const int nk = t->ktype == ORC_CONST_KERNEL ? 0 : (t->ktype == ORC_DIAG_KERNEL ? t->lat.d : t->lat.d * t->lat.d);
+            if (kernel_grad && nk) {
+                kernel_gradient_f(c, t, b, Q[it], kg);
+                for (int j = 0; j < nk; j++) kernel_grad[ik + j] = kernel_grad[ik + j] + kg[j];
+            }
+            ik += nk;
             term_apply_f(c, t, b, tmp2, 1);
             for (size_t i = 0; i < n; i++) tmp1[i] = tmp1[i] + tmp2[i];
         }
@@ -746,6 +1023,7 @@ double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, f
     free(b);
     free(kq);
     free(pg);
+    free(kg);
     return r;
 }
 

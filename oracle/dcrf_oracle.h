@@ -75,6 +75,18 @@ void orc_crf_set_compat_params(orc_crf *c, const float *p);
  * log-likelihood).  unary_grad [N][L] (sum of b over iterations), compat_grad packed. */
 double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
                         float *compat_grad);
+/* The same with the kernel-parameter gradient (DenseKernel::gradient, pairwise.cpp:
+ * 152-163): per term none (CONST_KERNEL), d values (DIAG_KERNEL) or d*d column-major
+ * (FULL_KERNEL), at the current kernel parameters. */
+double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
+                          float *compat_grad, float *kernel_grad);
+int orc_crf_num_kernel_params(const orc_crf *c);
+/* Kernel parameters of all terms, packed like the reference (ones / identity at first);
+ * setting them rebuilds each DIAG/FULL term's lattice from the scaled features. */
+void orc_crf_get_kernel_params(const orc_crf *c, float *p);
+int orc_crf_set_kernel_params(orc_crf *c, const float *p);
+/* Permutohedral::gradient: df [N][d] from a, b [N][vs]. */
+void orc_lattice_gradient(const orc_lattice *lat, float *df, const float *a, const float *b, int vs);
 
 /* FP64 re-run on the same lattices (normalisers recomputed in double). */
 void orc_crf_inference_fp64(orc_crf *c, int n_iter, double *Q);

@@ -112,6 +112,12 @@ def _oracle():
         lib.orc_crf_set_compat_params.argtypes = [_P, _F]
         lib.orc_crf_gradient.argtypes = [_P, ctypes.c_int, _S, ctypes.c_float, _F, _F]
         lib.orc_crf_gradient.restype = ctypes.c_double
+        lib.orc_crf_gradient_k.argtypes = [_P, ctypes.c_int, _S, ctypes.c_float, _F, _F, _F]
+        lib.orc_crf_gradient_k.restype = ctypes.c_double
+        lib.orc_crf_num_kernel_params.argtypes = [_P]
+        lib.orc_crf_get_kernel_params.argtypes = [_P, _F]
+        lib.orc_crf_set_kernel_params.argtypes = [_P, _F]
+        lib.orc_lattice_gradient.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_int]
         lib.orc_crf_inference_fp64.argtypes = [_P, ctypes.c_int, _D]
         lib.orc_crf_ce_fp64.argtypes = [_P, ctypes.c_int, _S, _D, _D, _D]
         lib.orc_crf_ce_fp64.restype = ctypes.c_double
@@ -166,6 +172,14 @@ class OracleLattice:
             out = np.empty_like(pm)
             self._lib.orc_compute_f(ctypes.byref(self._lat), _fp(pm), _fp(out), L, int(reverse))
         return out.T
+
+    def gradient(self, a_LxN, b_LxN):
+        """Gradient of a^T K b with respect to the features, (d, N)."""
+        a = np.ascontiguousarray(_c32(a_LxN).T)
+        b = np.ascontiguousarray(_c32(b_LxN).T)
+        df = np.zeros((self.N, self.d), np.float32)
+        self._lib.orc_lattice_gradient(ctypes.byref(self._lat), _fp(df), _fp(a), _fp(b), a.shape[1])
+        return df.T
 
     def __del__(self):
         try:
@@ -287,12 +301,29 @@ class OracleCRF(_CRFBase):
         p = _c32(p)
         self._lib.orc_crf_set_compat_params(self._h, _fp(p))
 
-    def gradient(self, n, gt, robust=0.0):
+    def kernel_params(self):
+        p = np.empty(self._lib.orc_crf_num_kernel_params(self._h), dtype=np.float32)
+        self._lib.orc_crf_get_kernel_params(self._h, _fp(p))
+        return p
+
+    def set_kernel_params(self, p):
+        p = _c32(p)
+        if p.size != self._lib.orc_crf_num_kernel_params(self._h):
+            raise ValueError("kernel parameter count mismatch")
+        if self._lib.orc_crf_set_kernel_params(self._h, _fp(p)) != 0:
+            raise ValueError("oracle lattice rebuild failed")
+
+    def gradient(self, n, gt, robust=0.0, kernel=False):
+        """(objective, dL/dU (L, N), compat gradient[, kernel-parameter gradient])."""
         gt = np.ascontiguousarray(gt, dtype=np.int16)
         ug = self._out()
         cg = np.empty(self._lib.orc_crf_num_compat_params(self._h), dtype=np.float32)
-        r = self._lib.orc_crf_gradient(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg))
-        return r, ug.T, cg
+        if not kernel:
+            r = self._lib.orc_crf_gradient(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg))
+            return r, ug.T, cg
+        kg = np.empty(self._lib.orc_crf_num_kernel_params(self._h), dtype=np.float32)
+        r = self._lib.orc_crf_gradient_k(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg), _fp(kg))
+        return r, ug.T, cg, kg
 
     def ce_fp64(self, n, gt, U_LxN=None, compat_raw=None, kweights=None):
         gt = np.ascontiguousarray(gt, dtype=np.int16)
@@ -465,9 +496,23 @@ class RefCRF(_CRFBase):
         p = _c32(p)
         self._lib.ref_crf_set_compat_params(self._h, _fp(p), p.size)
 
-    def gradient(self, n, gt, robust=0.0):
+    def kernel_params(self):
+        p = np.empty(self._lib.ref_crf_num_kernel_params(self._h), dtype=np.float32)
+        self._lib.ref_crf_get_kernel_params(self._h, _fp(p))
+        return p
+
+    def set_kernel_params(self, p):
+        p = _c32(p)
+        self._lib.ref_crf_set_kernel_params(self._h, _fp(p), p.size)
+
+    def gradient(self, n, gt, robust=0.0, kernel=False):
+        """(objective, dL/dU (L, N), compat gradient[, kernel-parameter gradient])."""
         gt = np.ascontiguousarray(gt, dtype=np.int16)
         ug = self._out()
         cg = np.empty(self._lib.ref_crf_num_compat_params(self._h), dtype=np.float32)
-        r = self._lib.ref_crf_gradient(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg), None)
-        return r, ug.T, cg
+        if not kernel:
+            r = self._lib.ref_crf_gradient(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg), None)
+            return r, ug.T, cg
+        kg = np.empty(self._lib.ref_crf_num_kernel_params(self._h), dtype=np.float32)
+        r = self._lib.ref_crf_gradient(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg), _fp(kg))
+        return r, ug.T, cg, kg

@@ -55,9 +55,12 @@ SIGNATURES = {
     "dcrf_set_kernel_weights": (_c, [_P, _F]),
     "dcrf_num_compat_params": (_c, [_P]),
     "dcrf_get_compat_params": (_c, [_P, _F]),
-    "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F]),
+    "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
+    "dcrf_num_kernel_params": (_c, [_P]),
+    "dcrf_get_kernel_params": (_c, [_P, _F]),
+    "dcrf_set_kernel_params": (_c, [_P, _F]),
     "dcrf_forward_device": (_c, [_P, _c, _P, _P, _P, _P, _P]),
-    "dcrf_backward_device": (_c, [_P, _P, _P, _P, _P, _P]),
+    "dcrf_backward_device": (_c, [_P, _P, _P, _P, _P, _P, _P]),
     "dcrf_num_variables": (_c, [_P]),
     "dcrf_num_labels": (_c, [_P]),
     "dcrf_num_terms": (_c, [_P]),
@@ -284,23 +287,38 @@ class Model:
         check(lib().dcrf_get_compat_params(self.h, fptr(p)))
         return p[:lib().dcrf_num_compat_params(self.h)]
 
-    def gradient(self, n, gt, robust=0.0):
+    def kernel_params(self):
+        n = lib().dcrf_num_kernel_params(self.h)
+        p = np.empty(max(n, 1), np.float32)
+        check(lib().dcrf_get_kernel_params(self.h, fptr(p)))
+        return p[:n]
+
+    def set_kernel_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        if a.size != lib().dcrf_num_kernel_params(self.h):
+            raise ValueError("expected %d kernel parameters, got %d" % (lib().dcrf_num_kernel_params(self.h), a.size))
+        check(lib().dcrf_set_kernel_params(self.h, fptr(a)))
+
+    def gradient(self, n, gt, robust=0.0, kernel=False):
         """DenseCRF::gradient with LogLikelihood: (objective, unary_grad (L, N),
-        compat_grad packed, kernel-weight grad [T])."""
+        compat_grad packed, kernel-weight grad [T][, kernel-parameter grad])."""
         g = np.ascontiguousarray(gt, dtype=np.int16)
         obj = ctypes.c_double()
         ug = self._buf()
         cg = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         kg = np.empty(max(self.nterms, 1), np.float32)
+        nk = lib().dcrf_num_kernel_params(self.h)
+        kp = np.empty(max(nk, 1), np.float32) if kernel else None
         check(lib().dcrf_gradient(self.h, n, g.ctypes.data_as(_S), robust, ctypes.byref(obj), fptr(ug), fptr(cg),
-                                  fptr(kg)))
-        return obj.value, ug.T, cg[:lib().dcrf_num_compat_params(self.h)], kg[:self.nterms]
+                                  fptr(kg), fptr(kp) if kernel else None))
+        out = (obj.value, ug.T, cg[:lib().dcrf_num_compat_params(self.h)], kg[:self.nterms])
+        return out + (kp[:nk],) if kernel else out
 
     def forward_device(self, n, U_ptr, Q_ptr, compat_ptr=None, kw_ptr=None, stream=None):
         check(lib().dcrf_forward_device(self.h, n, U_ptr, compat_ptr, kw_ptr, Q_ptr, stream))
 
-    def backward_device(self, dQ_ptr, dU_ptr, dcompat_ptr=None, dkw_ptr=None, stream=None):
-        check(lib().dcrf_backward_device(self.h, dQ_ptr, dU_ptr, dcompat_ptr, dkw_ptr, stream))
+    def backward_device(self, dQ_ptr, dU_ptr, dcompat_ptr=None, dkw_ptr=None, stream=None, dkernel_ptr=None):
+        check(lib().dcrf_backward_device(self.h, dQ_ptr, dU_ptr, dcompat_ptr, dkw_ptr, dkernel_ptr, stream))
 
     # -- introspection
     def term_info(self, term):

@@ -100,6 +100,10 @@ struct dcrf_model {
     const float* train_kw = nullptr;
     DevBuf<float> qcache, mcache, bbuf, partial;
     DevBuf<double> gacc;
+    // kernel-parameter gradient: 5 [N][L] work slabs, 2 [N][d] feature gradients,
+    // per-block partials and the accumulated gradient of all terms
+    DevBuf<float> kgbuf, kgdf;
+    DevBuf<double> kpartial, kacc;
 
     // captured inference (dcrf_capture_inference)
     hipGraph_t graph = nullptr;
@@ -354,6 +358,148 @@ void compute_norm(dcrf_model& m, Term& t) {
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------- kernel-parameter gradient
+// The term's own lattice filter without normaliser (the lattice_.compute calls of
+// DenseKernel::featureGradient, pairwise.cpp:88-114), built on first use: segments and
+// entries in the term's sorted order, term-local rows (zero row V, partials from V+1),
+// plus the all-ones input splatted and blurred in reverse order (one value per vertex).
+RawPlan& raw_plan(dcrf_model& m, Term& t) {
+    if (t.raw) return *t.raw;
+    const TermLattice& lat = t.lat;
+    const int N = m.N(), d1 = lat.d + 1, V = lat.V, E = N * d1;
+    hipStream_t s = m.stream;
+    auto P = std::make_unique<RawPlan>();
+    std::vector<Chunk> hch = lat.chunks;
+    for (auto& c : hch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
+    require_adjoining(hch);
+    std::vector<Combine> hcb = lat.combines;
+    for (auto& c : hcb) c.first += V + 1;
+    upload(P->ch, hch, s);
+    upload(P->cb, hcb, s);
+    P->nch = (int)hch.size();
+    P->ncb = (int)hcb.size();
+    P->ent.reserve(E);
+    launch_entry_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
+                        P->ent.get(), s);
+    P->nbr.reserve((size_t)d1 * V);
+    launch_remap_nbr(lat.nbr.get(), P->nbr.get(), d1 * V, 0, V, s);
+    DevBuf<int> gv;
+    gv.reserve(E);
+    P->sw.reserve(E);
+    launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(lat.d), N, d1, 0, gv.get(),
+                         P->sw.get(), s);
+    DevBuf<float> ones, v0, v1;
+    ones.reserve(N);
+    launch_fill(ones.get(), 1.f, N, s);
+    const int rows = V + 1 + lat.n_partials;
+    v0.reserve(rows);
+    v1.reserve(V + 1);
+    launch_fill(v0.get(), 0.f, rows, s);
+    launch_fill(v1.get(), 0.f, V + 1, s);
+    launch_splat(P->ch.get(), P->nch, P->ent.get(), ones.get(), 1, v0.get(), 1, 1, s);
+    launch_combine(P->cb.get(), P->ncb, v0.get(), 1, 1, s);
+    BlurTable bt{};
+    bt.nterms = 1;
+    bt.t[0] = BlurTerm{0, V, lat.d, P->nbr.get()};
+    float* bufs[2] = {v0.get(), v1.get()};
+    for (int step = 0; step <= lat.d; step++)
+        launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, 1, 1, bt, step, 1, s);
+    if (d1 & 1)
+        P->ones_rev = std::move(v1);
+    else
+        P->ones_rev = std::move(v0);
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));
+    t.raw = std::move(P);
+    return *t.raw;
+}
+
+// Splat `in` ([N][L]) on the term's own lattice and blur it (reverse: axes d..0); the
+// rows are left in the model's vertex buffers, which the next global filter overwrites.
+const float* raw_lattice(dcrf_model& m, Term& t, const float* in, bool reverse, hipStream_t s) {
+    const RawPlan& P = *t.raw;
+    const int V = t.lat.V, L = m.L, ldv = m.ldv;
+    float* bufs[2] = {m.vals[0].get(), m.vals[1].get()};
+    launch_fill(bufs[0] + (size_t)V * ldv, 0.f, ldv, s);
+    launch_fill(bufs[1] + (size_t)V * ldv, 0.f, ldv, s);
+    launch_splat(P.ch.get(), P.nch, P.ent.get(), in, L, bufs[0], ldv, L, s);
+    launch_combine(P.cb.get(), P.ncb, bufs[0], ldv, L, s);
+    BlurTable bt{};
+    bt.nterms = 1;
+    bt.t[0] = BlurTerm{0, V, t.lat.d, P.nbr.get()};
+    for (int step = 0; step <= t.lat.d; step++)
+        launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, ldv, L, bt, step, reverse ? 1 : 0, s);
+    return bufs[(t.lat.d + 1) & 1];
+}
+
+// out = K in (or K^T in): Permutohedral::compute without normaliser.
+void raw_filter(dcrf_model& m, Term& t, const float* in, float* out, bool reverse, hipStream_t s) {
+    const float* v = raw_lattice(m, t, in, reverse, s);
+    SliceTable st{};
+    st.nterms = 1;
+    st.t[0] = SliceTerm{t.lat.vtx.get(), t.raw->sw.get(), v, t.lat.d + 1, -1, nullptr, nullptr, nullptr, m.ldv};
+    launch_slice_term(out, m.N(), m.L, st, 0, s);
+}
+
+// acc += kernelGradient(b, Q_t) of one term (pairwise.cpp:202-207, 152-163, 88-114).
+void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, const float* kw, double* acc,
+                      hipStream_t s) {
+    const int N = m.N(), L = m.L, d = t.lat.d;
+    const size_t NL = (size_t)N * L;
+    raw_plan(m, t);
+    float* lbl = m.kgbuf.get();
+    float* A = lbl + NL;
+    float* B = A + NL;
+    float* F1 = B + NL;
+    float* F2 = F1 + NL;
+    float* dfr = m.kgdf.get();
+    float* dfg = dfr + (size_t)N * d;
+    const float* nm = t.norm.get();
+    launch_compat_rows(Qt, N, L, t.ckind, t.d_compat.get(), kw, lbl, s);
+    // G(a, c) below; x the first argument of the normaliser term G(x, 1)
+    const float* a = b;
+    const float* c = lbl;
+    const float* x = nullptr;
+    if (t.ntype == DCRF_NORMALIZE_SYMMETRIC) {
+        launch_kg_scale(b, lbl, nm, N, L, A, B, s);
+        a = A;
+        c = B;
+        raw_filter(m, t, A, F1, true, s);   // fa = K^T (b n)
+        raw_filter(m, t, B, F2, false, s);  // fb = K (lbl n)
+        launch_kg_x(b, lbl, F1, F2, nm, N, L, t.ntype, F1, s);
+        x = F1;
+    } else if (t.ntype == DCRF_NORMALIZE_AFTER) {
+        launch_kg_scale(b, lbl, nm, N, L, A, nullptr, s);
+        a = A;
+        raw_filter(m, t, lbl, F2, false, s);  // fb = K lbl
+        launch_kg_x(b, lbl, nullptr, F2, nm, N, L, t.ntype, F1, s);
+        x = F1;
+    } else if (t.ntype == DCRF_NORMALIZE_BEFORE) {
+        launch_kg_scale(b, lbl, nm, N, L, nullptr, B, s);
+        c = B;
+        raw_filter(m, t, b, F1, true, s);  // fa = K^T b
+        launch_kg_x(b, lbl, F1, nullptr, nm, N, L, t.ntype, F1, s);
+        x = F1;
+    }
+    const int* vtx = t.lat.vtx.get();
+    const int* rank = t.lat.rank.get();
+    if (x) {
+        // r = G(x, 1): dir 0 pairs the forward-blurred x with ones, dir 1 the
+        // reverse-blurred ones with x
+        launch_slice_grad(raw_lattice(m, t, x, false, s), m.ldv, 1, vtx, rank, nullptr, N, L, d, 0, dfr, s);
+        launch_slice_grad(t.raw->ones_rev.get(), 1, 0, vtx, rank, x, N, L, d, 1, dfr, s);
+    }
+    launch_slice_grad(raw_lattice(m, t, a, false, s), m.ldv, 1, vtx, rank, c, N, L, d, 0, dfg, s);
+    launch_slice_grad(raw_lattice(m, t, c, true, s), m.ldv, 1, vtx, rank, a, N, L, d, 1, dfg, s);
+    launch_kparam_reduce(x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
+                         m.geo.n_inst, N, d, t.ktype == DCRF_FULL_KERNEL, m.kpartial.get(), acc, s);
+}
+
+int num_kernel_params(const dcrf_model& m) {
+    int n = 0;
+    for (const auto& t : m.terms) n += t->num_kparams();
+    return n;
+}
+
 void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, int ntype,
               const std::function<void(float*)>& make_features) {
     require((int)m.terms.size() < DCRF_MAX_TERMS, "at most " + std::to_string(DCRF_MAX_TERMS) + " pairwise terms");
@@ -397,6 +543,10 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
                     "a lattice coordinate exceeds the int16 range of the lattice keys; scale the features down"};
     build_chunks(t->lat, m.chunk());
     compute_norm(m, *t);
+    // DenseKernel keeps f_ and starts from unit scales / the identity (pairwise.cpp:116-122)
+    t->feat0 = std::move(feat);
+    t->kparams.assign(ktype == DCRF_FULL_KERNEL ? (size_t)d * d : (size_t)d, 0.0f);
+    for (int j = 0; j < d; j++) t->kparams[ktype == DCRF_FULL_KERNEL ? j * d + j : j] = 1.0f;
     m.terms.push_back(std::move(t));
     m.layout_valid = false;
     m.kw_set = false;  // term weights are per term: set them again after adding terms
@@ -711,7 +861,9 @@ void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compa
 
 // Adjoint of forward_train.  dQ: dL/dQ_n.  Outputs dU (always), per-term statistics in
 // m.gacc, and optionally raw compatibility / term-weight gradients on the device.
-void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* dkw, hipStream_t s) {
+// kernel: also accumulate the kernel-parameter gradient of every term into m.kacc.
+void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* dkw, hipStream_t s,
+              bool kernel = false) {
     if (m.train_n < 0) throw Error{DCRF_ERR_STATE, "backward needs a preceding training forward"};
     const int N = m.N(), L = m.L, T = (int)m.terms.size(), n = m.train_n;
     const size_t NL = (size_t)N * L;
@@ -724,10 +876,33 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     m.gacc.reserve(std::max<size_t>(goff[T], 1));
     launch_fill_double(m.gacc.get(), goff[T], s);
     m.partial.reserve(std::max(grad_partial_floats(), (size_t)grad_partial_blocks(N, L) * L));
+    std::vector<int> koff(T + 1, 0);
+    for (int k = 0; k < T; k++) koff[k + 1] = koff[k] + m.terms[k]->num_kparams();
+    kernel = kernel && koff[T] > 0;
+    if (kernel) {
+        int dmax = 1;
+        for (auto& t : m.terms) dmax = std::max(dmax, t->lat.d);
+        m.kgbuf.reserve(5 * NL);
+        m.kgdf.reserve((size_t)2 * N * dmax);
+        m.kpartial.reserve(kparam_partial_doubles(dmax));
+        m.kacc.reserve(koff[T]);
+        launch_fill_double(m.kacc.get(), koff[T], s);
+    }
     for (int it = n - 1; it >= 0; it--) {
         for (int k = 0; k < T; k++)
             launch_grad_stat(b, m.mcache.get() + ((size_t)it * T + k) * NL, N, L,
                              m.terms[k]->ckind == DCRF_COMPAT_MATRIX, m.partial.get(), m.gacc.get() + goff[k], s);
+        if (kernel) {
+            // DenseCRF::gradient takes the kernel gradient with b before its update
+            // (densecrf.cpp:280-284)
+            for (int k = 0; k < T; k++)
+                if (m.terms[k]->num_kparams())
+                    kernel_grad_term(m, *m.terms[k], b, Qc + (size_t)it * NL, m.train_kw ? m.train_kw + k : nullptr,
+                                     m.kacc.get() + koff[k], s);
+            // the term-local filters may have written the global zero rows
+            launch_fill(m.vals[0].get() + (size_t)m.V_total * m.ldv, 0.f, m.ldv, s);
+            launch_fill(m.vals[1].get() + (size_t)m.V_total * m.ldv, 0.f, m.ldv, s);
+        }
         run_filter(m, b, L, true, s);
         launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);
     }
@@ -1094,8 +1269,72 @@ int dcrf_get_compat_params(const dcrf_model* m, float* p) {
     });
 }
 
+int dcrf_num_kernel_params(const dcrf_model* m) { return m ? num_kernel_params(*m) : 0; }
+
+int dcrf_get_kernel_params(const dcrf_model* m, float* params) {
+    return guard([&] {
+        require(m && params, "null argument");
+        for (const auto& t : m->terms) {
+            std::copy(t->kparams.begin(), t->kparams.begin() + t->num_kparams(), params);
+            params += t->num_kparams();
+        }
+    });
+}
+
+int dcrf_set_kernel_params(dcrf_model* m, const float* params) {
+    return guard([&] {
+        require(m && params, "null argument");
+        const int np = num_kernel_params(*m);
+        for (int i = 0; i < np; i++) require(std::isfinite(params[i]), "kernel parameters must be finite");
+        ensure_device(*m);
+        hipStream_t s = m->stream;
+        // new lattices and normalisers of every scaled term first, then commit them all
+        struct Rebuilt {
+            Term* t;
+            Term tmp;
+            std::vector<float> kp;
+        };
+        std::vector<std::unique_ptr<Rebuilt>> done;
+        const float* p = params;
+        const int npad = m->geo.npad_total();
+        for (auto& tp : m->terms) {
+            Term& t = *tp;
+            const int n = t.num_kparams();
+            if (!n) continue;
+            auto r = std::make_unique<Rebuilt>();
+            r->t = &t;
+            r->kp.assign(p, p + n);
+            p += n;
+            const int d = t.lat.d;
+            DevBuf<float> P, feat;
+            upload(P, r->kp, s);
+            feat.reserve((size_t)npad * d);
+            launch_scale_features(t.feat0.get(), npad, d, t.ktype == DCRF_FULL_KERNEL, P.get(), feat.get(), s);
+            r->tmp.ntype = t.ntype;
+            build_lattice(m->geo, m->geod, m->tiles, feat.get(), d, r->tmp.lat, m->hash_load, s);
+            if (r->tmp.lat.overflow)
+                throw Error{DCRF_ERR_OVERFLOW, "a scaled lattice coordinate exceeds the int16 range of the lattice "
+                                               "keys; scale the kernel parameters down"};
+            build_chunks(r->tmp.lat, m->chunk());
+            compute_norm(*m, r->tmp);
+            done.push_back(std::move(r));
+        }
+        for (auto& r : done) {
+            r->t->lat = std::move(r->tmp.lat);
+            r->t->norm = std::move(r->tmp.norm);
+            r->t->kparams = std::move(r->kp);
+            r->t->raw.reset();
+        }
+        if (!done.empty()) {
+            m->layout_valid = false;
+            m->train_n = -1;
+            m->drop_graph();
+        }
+    });
+}
+
 int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, double* objective, float* unary_grad,
-                  float* compat_grad, float* kweight_grad) {
+                  float* compat_grad, float* kweight_grad, float* kernel_grad) {
     return guard([&] {
         require(m && gt, "null argument");
         require(n_iter >= 0, "number of iterations must be non-negative");
@@ -1126,7 +1365,11 @@ int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, do
         m->work.reserve(NL);
         m->tmp1.reserve(NL);
         DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), dQ.data(), sizeof(float) * NL, hipMemcpyHostToDevice, s));
-        backward(*m, m->work.get(), m->tmp1.get(), nullptr, nullptr, s);
+        const int nkp = num_kernel_params(*m);
+        backward(*m, m->work.get(), m->tmp1.get(), nullptr, nullptr, s, kernel_grad && nkp > 0);
+        std::vector<double> KG(nkp, 0.0);
+        if (kernel_grad && nkp && n_iter > 0)
+            DCRF_HIP_CHECK(hipMemcpyAsync(KG.data(), m->kacc.get(), sizeof(double) * nkp, hipMemcpyDeviceToHost, s));
         if (unary_grad)
             DCRF_HIP_CHECK(hipMemcpyAsync(unary_grad, m->tmp1.get(), sizeof(float) * NL, hipMemcpyDeviceToHost, s));
         size_t glen = 0;
@@ -1164,6 +1407,8 @@ int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, do
             if (kweight_grad) kweight_grad[k] = (float)kg;
             go += stat_len(t.ckind, L);
         }
+        if (kernel_grad)
+            for (int i = 0; i < nkp; i++) kernel_grad[i] = (float)KG[i];
         if (m->timing) collect_timing(*m);
     });
 }
@@ -1187,10 +1432,19 @@ int dcrf_forward_device(dcrf_model* m, int n_iter, const float* U_dev, const flo
 }
 
 int dcrf_backward_device(dcrf_model* m, const float* dQ_dev, float* dU_dev, float* dcompat_dev, float* dkw_dev,
-                         void* stream) {
+                         float* dkernel_dev, void* stream) {
     return guard([&] {
         require(m && dQ_dev && dU_dev, "null argument");
-        backward(*m, dQ_dev, dU_dev, dcompat_dev, dkw_dev, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        const int nkp = num_kernel_params(*m);
+        const bool kernel = dkernel_dev && nkp > 0;
+        backward(*m, dQ_dev, dU_dev, dcompat_dev, dkw_dev, s, kernel);
+        if (kernel) {
+            if (m->train_n > 0)
+                launch_kparam_store(m->kacc.get(), nkp, dkernel_dev, s);
+            else
+                launch_fill(dkernel_dev, 0.f, nkp, s);
+        }
     });
 }
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -106,12 +107,30 @@ struct TermLattice {
     int chunk_len = 0;
 };
 
+// One term's lattice filter on its own (term-local vertex rows, zero row V, partial
+// rows from V+1), without normaliser: the filters of the kernel-parameter gradient.
+struct RawPlan {
+    DevBuf<Chunk> ch;
+    DevBuf<Combine> cb;
+    int nch = 0, ncb = 0;
+    DevBuf<int2> ent;        // (pixel, barycentric weight bits) in segment order
+    DevBuf<int2> nbr;        // [(d+1)][V] neighbour rows, zero row V for an absent one
+    DevBuf<float> sw;        // [N][d+1] slice weights bary / (1 + 2^-d)
+    DevBuf<float> ones_rev;  // [V+1] splat of ones blurred in reverse order (row V zero)
+};
+
 struct Term {
     int ktype = DCRF_DIAG_KERNEL, ntype = DCRF_NORMALIZE_SYMMETRIC, ckind = DCRF_COMPAT_POTTS;
     std::vector<float> compat;  // Potts [1], Diagonal [L], Matrix [L*L] symmetrised
     DevBuf<float> d_compat;
     TermLattice lat;
     DevBuf<float> norm;  // [N]
+    DevBuf<float> feat0;         // [npad][d] features as given (DenseKernel::f_)
+    std::vector<float> kparams;  // DIAG [d] scales, FULL [d*d] column-major (parameters_)
+    std::unique_ptr<RawPlan> raw;
+    int num_kparams() const {
+        return ktype == DCRF_CONST_KERNEL ? 0 : (ktype == DCRF_DIAG_KERNEL ? lat.d : lat.d * lat.d);
+    }
 };
 
 // Instance geometry of a model (a plain model is one instance).
@@ -308,6 +327,30 @@ void launch_fill_double(double* p, size_t n, hipStream_t s);
 void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
                           float* dkw, hipStream_t s);
 void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s);
+
+// ---------------------------------------------------------------- kernel-parameter gradient
+// lbl = compat(w Q) [N][L] (w = kw[0], or 1 when kw is null).
+void launch_compat_rows(const float* Q, int N, int L, int ckind, const float* cw, const float* kw, float* out,
+                        hipStream_t s);
+// bn = b n, ln = lbl n per pixel (either output may be null).
+void launch_kg_scale(const float* b, const float* lbl, const float* norm, int N, int L, float* bn, float* ln,
+                     hipStream_t s);
+// The normaliser-derivative input x of featureGradient for ntype (may overwrite fa).
+void launch_kg_x(const float* b, const float* lbl, const float* fa, const float* fb, const float* norm, int N, int L,
+                 int ntype, float* x, hipStream_t s);
+// One direction of the lattice gradient: df (+)= slicing gradient of the blurred rows
+// vals (row stride ldv, label stride ks) paired with c [N][L] (null: ones).
+void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int N,
+                       int L, int d, int accumulate, float* df, hipStream_t s);
+// acc (d or d*d doubles) += parameter gradient of fg = (r ? -r + g : g) against the
+// padded feature table f; partial holds kparam_partial_doubles(d).
+size_t kparam_partial_doubles(int d);
+void launch_kparam_reduce(const float* r, const float* g, const float* f, const int* pix_base, const int* pad_base,
+                          int n_inst, int N, int d, bool full, double* partial, double* acc, hipStream_t s);
+// out[i] = (float)acc[i].
+void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
+// Scaled features diag(P) f or P f (P column-major) over the padded table.
+void launch_scale_features(const float* f, int npad, int d, bool full, const float* P, float* out, hipStream_t s);
 
 // Global layout remaps.
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);

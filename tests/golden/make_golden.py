@@ -5,7 +5,7 @@ compiled from its sources by oracle/Makefile (`make -C oracle ref`) into
 oracle/_ref/libdcrf_ref.so; this script drives that build through
 oracle/pyoracle.py (RefLattice / RefCRF) and stores inputs and outputs as .npz data.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [fixture ...]
 
 Fixtures
   lattice_small.npz  Permutohedral::init on small feature sets (d = 1..6, N % 4 != 0
@@ -14,6 +14,10 @@ Fixtures
                      normalisations, L = 1, 2, 3, 5, 21): Q after n iterations, one
                      stepInference (Q, tmp1, tmp2), KL, DenseCRF::gradient (LogLikelihood,
                      identity-gradient unary) for unary and compatibility parameters.
+  kgrad_small.npz    Permutohedral::gradient on small feature sets, and
+                     DenseCRF::gradient's kernel_grad (DIAG / FULL / CONST kernels, all
+                     normalisations, default and set kernel parameters) with the
+                     objective and unary / compatibility gradients of the same call.
   simple_10x10.npz   the reference test suite's own fixture (test_dcrf.py:8-26):
                      unary, image and Q of test_call_dcrf2d / test_call_dcrf.
   image_320x213.npz  the 320x213x21 5-iteration workload: V per term, SHA-256 of the
@@ -102,6 +106,67 @@ def crf_small():
     np.savez_compressed(os.path.join(HERE, "crf_small.npz"), **out)
 
 
+KG_CASES = [
+    # (W, H, L, compat kind, ktype gaussian, ktype bilateral, ntype, n_iter, kernel-parameter mode)
+    (11, 9, 3, "potts", 1, 1, 3, 3, "default"),
+    (11, 9, 3, "diag", 1, 1, 1, 3, "scaled"),
+    (13, 8, 4, "matrix", 2, 2, 2, 2, "scaled"),
+    (9, 9, 2, "potts", 2, 2, 0, 3, "default"),
+    (16, 12, 5, "matrix", 1, 1, 3, 4, "scaled"),
+    (10, 10, 3, "potts", 0, 1, 1, 2, "scaled"),
+    (14, 10, 21, "potts", 1, 2, 3, 2, "scaled"),
+    (12, 7, 1, "potts", 1, 1, 2, 3, "default"),
+]
+
+
+def kgrad_small():
+    out = {}
+    rng = np.random.default_rng(4321)
+    lat_cases = [(2, 61, 1), (3, 80, 2), (5, 97, 4), (4, 40, 3)]
+    for i, (d, N, L) in enumerate(lat_cases):
+        f = (rng.standard_normal((d, N)) * 3).astype(np.float32)
+        a = rng.standard_normal((L, N)).astype(np.float32)
+        b = rng.standard_normal((L, N)).astype(np.float32)
+        out["l%d_feat" % i], out["l%d_a" % i], out["l%d_b" % i] = f, a, b
+        out["l%d_df" % i] = po.RefLattice(f).gradient(a, b)
+    out["nlat"] = np.int32(len(lat_cases))
+    for i, (W, H, L, ck, kt1, kt2, nt, n, mode) in enumerate(KG_CASES):
+        r2 = np.random.default_rng(700 + i)
+        img, U, gt = syn.make_image(W, H, L=L, sites=4, seed=700 + i)
+        if ck == "potts":
+            c1, c2 = np.float32(3.0), np.float32(10.0)
+        elif ck == "diag":
+            c1 = r2.uniform(0.5, 3, L).astype(np.float32)
+            c2 = r2.uniform(1, 8, L).astype(np.float32)
+        else:
+            c1 = (-2 * np.eye(L) + r2.normal(0, 0.3, (L, L))).astype(np.float32)
+            c2 = (-5 * np.eye(L) + r2.normal(0, 0.5, (L, L))).astype(np.float32)
+        crf = po.RefCRF(W=W, H=H, L=L, identity_unary=True)
+        crf.set_unary(U)
+        crf.add_gaussian(3.0, 2.0, float(c1) if ck == "potts" else c1, ktype=kt1, ntype=nt)
+        crf.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, img, float(c2) if ck == "potts" else c2, ktype=kt2,
+                          ntype=nt)
+        kp = crf.kernel_params()
+        if mode == "scaled":
+            segs, j = [], 0
+            for kt, d in ((kt1, 2), (kt2, 5)):
+                if kt == 1:
+                    segs.append(r2.uniform(0.7, 1.4, d))
+                elif kt == 2:
+                    segs.append((np.eye(d) + r2.normal(0, 0.1, (d, d))).T.ravel())
+            kp = np.concatenate(segs).astype(np.float32)
+            crf.set_kernel_params(kp)
+        r, ug, cg, kg = crf.gradient(n, gt, kernel=True)
+        p = "c%d_" % i
+        out.update({p + "img": img, p + "U": U, p + "gt": gt, p + "c1": np.atleast_1d(c1), p + "c2": np.atleast_1d(c2),
+                    p + "kparams": kp, p + "obj": np.float64(r), p + "unary_grad": ug, p + "compat_grad": cg,
+                    p + "kernel_grad": kg, p + "Q": crf.inference(n),
+                    p + "meta": np.array([W, H, L, ["potts", "diag", "matrix"].index(ck), kt1, kt2, nt, n],
+                                         np.int32)})
+    out["ncases"] = np.int32(len(KG_CASES))
+    np.savez_compressed(os.path.join(HERE, "kgrad_small.npz"), **out)
+
+
 def simple_10x10():
     # the reference suite's fixture, pydensecrf/tests/test_dcrf.py:8-26
     unary1 = np.zeros((10, 10), dtype=np.float32)
@@ -147,11 +212,13 @@ def image_workload(name, cfg, fname):
 if __name__ == "__main__":
     if not po.ref_available():
         sys.exit("build the reference first: make -C oracle ref")
-    lattice_small()
-    crf_small()
-    simple_10x10()
-    image_workload("320x213", syn.IMAGE_320x213, "image_320x213.npz")
-    image_workload("1024x768", syn.IMAGE_1024x768, "image_1024x768.npz")
+    writers = {"lattice_small": lattice_small, "crf_small": crf_small, "kgrad_small": kgrad_small,
+               "simple_10x10": simple_10x10,
+               "image_320x213": lambda: image_workload("320x213", syn.IMAGE_320x213, "image_320x213.npz"),
+               "image_1024x768": lambda: image_workload("1024x768", syn.IMAGE_1024x768, "image_1024x768.npz")}
+    # python tests/golden/make_golden.py [fixture ...]: all of them by default
+    for name in sys.argv[1:] or list(writers):
+        writers[name]()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

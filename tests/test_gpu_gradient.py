@@ -211,3 +211,154 @@ def test_gradient_label_shapes(L):
     assert abs(obj - r) <= 1e-5 * abs(r)
     assert _rel_close(ug, ugo, 1e-4)
     assert _rel_close(cg, cgo, 1e-4)
+
+
+# ---------------------------------------------------------------- kernel parameters
+# DenseCRF::gradient's kernel_grad (pairwise.cpp:152-163, 202-207, permutohedral.cpp:
+# 612-695) and kernelParameters / setKernelParameters (densecrf.cpp:334-360).  The GPU
+# splat sums a vertex's entries in another order than the reference, and the kernel
+# gradient is a difference of two lattice gradients summed over all points: the bar is
+# 1e-4 of the largest entry (objective 1e-5 relative).
+KG_TOL = 1e-4
+# 3072 points x 21 labels: with FULL kernels and NORMALIZE_AFTER the bilateral term's
+# 25 entries are ~1e-4 of the Gaussian's and sit at the float noise floor (the
+# reference's own algorithm moves them by tens of percent under a permutation of the
+# pixel order, which changes only its summation order)
+KG_TOL_L21 = 5e-4
+
+
+def _kg_model(z, i):
+    W, H, L, ck, kt1, kt2, nt, n = [int(v) for v in z["c%d_meta" % i]]
+    p = "c%d_" % i
+    c1, c2 = z[p + "c1"], z[p + "c2"]
+    if ck == 0:
+        c1, c2 = float(c1[0]), float(c2[0])
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(z[p + "U"])
+    g.add_gaussian(3.0, 2.0, c1, ktype=kt1, ntype=nt)
+    g.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], c2, ktype=kt2, ntype=nt)
+    return g, n, p
+
+
+@pytest.mark.parametrize("i", range(8))
+def test_kernel_gradient_reference_fixtures(i):
+    z = np.load(os.path.join(GOLDEN, "kgrad_small.npz"), allow_pickle=False)
+    g, n, p = _kg_model(z, i)
+    kp = z[p + "kparams"]
+    assert g.kernel_params().size == kp.size
+    if kp.size:
+        g.set_kernel_params(kp)
+    np.testing.assert_array_equal(g.kernel_params(), kp)
+    np.testing.assert_allclose(g.inference(n), z[p + "Q"], atol=2e-5)
+    obj, ug, cg, _, kg = g.gradient(n, z[p + "gt"], kernel=True)
+    ref = z[p + "kernel_grad"]
+    assert abs(obj - float(z[p + "obj"])) <= 1e-5 * max(1.0, abs(obj))
+    assert _rel_close(ug, z[p + "unary_grad"], 1e-4)
+    assert _rel_close(cg, z[p + "compat_grad"], 1e-4)
+    assert kg.shape == ref.shape
+    if ref.size and not np.any(ref):
+        # one label: Q = 1 and b = 0, every gradient vanishes exactly
+        assert not np.any(kg)
+    elif ref.size:
+        err = np.abs(kg - ref).max() / np.abs(ref).max()
+        print("kernel gradient case %d: max error / max entry %.3g" % (i, err))
+        assert err <= KG_TOL
+
+
+@pytest.mark.parametrize("kt,nt", [(1, 3), (2, 2), (1, 1), (2, 0)])
+def test_kernel_gradient_vs_oracle_l21(kt, nt):
+    W, H, L, n = 64, 48, 21, 3
+    img, U, gt = image_case(W, H, L, seed=70 + kt * 4 + nt)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    # unnormalised messages grow with K1 (tens of points): weaker compatibilities keep
+    # Q away from saturation
+    c1, c2 = (0.05, 0.1) if nt == po.NO_NORMALIZATION else (3.0, 10.0)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, c1, ktype=kt, ntype=nt)
+        m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, c2, ktype=kt, ntype=nt)
+    p0 = o.kernel_params()
+    p = (p0 * np.float32(1.1)).astype(np.float32)
+    for m in (g, o):
+        m.set_kernel_params(p)
+    obj, ug, cg, _, kg = g.gradient(n, gt, kernel=True)
+    r, ugo, cgo, kgo = o.gradient(n, gt, kernel=True)
+    assert abs(obj - r) <= 1e-5 * abs(r)
+    assert _rel_close(ug, ugo, 1e-4)
+    err = np.abs(kg - kgo).max() / np.abs(kgo).max()
+    print("kernel gradient kt=%d nt=%d: max error / max entry %.3g" % (kt, nt, err))
+    print("  gpu   ", np.array2string(kg, precision=5))
+    print("  oracle", np.array2string(kgo, precision=5))
+    assert err <= KG_TOL_L21
+
+
+def test_kernel_params_rebuild_and_reset():
+    """setKernelParameters rebuilds the lattices: inference follows the oracle at the new
+    parameters, and setting the initial parameters again restores the first result."""
+    W, H, L, n = 48, 40, 5, 4
+    img, U, _ = image_case(W, H, L, seed=9)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0, ktype=po.CONST_KERNEL)
+        m.add_bilateral(30.0, 30.0, 13.0, 13.0, 13.0, img, 10.0, ktype=po.FULL_KERNEL)
+    p0 = g.kernel_params()
+    np.testing.assert_array_equal(p0, np.eye(5, dtype=np.float32).ravel())
+    Q0 = g.inference(n)
+    rng = np.random.default_rng(3)
+    p = (np.eye(5) + rng.normal(0, 0.1, (5, 5))).astype(np.float32).ravel()
+    for m in (g, o):
+        m.set_kernel_params(p)
+    Q1 = g.inference(n)
+    np.testing.assert_allclose(Q1, o.inference(n), atol=2e-5)
+    assert np.abs(Q1 - Q0).max() > 1e-3
+    with pytest.raises(ValueError):
+        g.set_kernel_params(p[:-1])
+    g.set_kernel_params(p0)
+    np.testing.assert_array_equal(g.inference(n), Q0)
+
+
+def test_autograd_kernel_params():
+    """CRFLayer(learn_kernel=True): the kernel-parameter gradient of the cross-entropy is
+    minus dcrf_gradient's kernel_grad (of the mean log-likelihood), and a forward after
+    the parameters moved runs on lattices rebuilt at the new values."""
+    import torch
+    from pydensecrf_amd.autograd import CRFLayer
+
+    W, H, L, n = 32, 24, 4, 3
+    N = W * H
+    img, U, gt = image_case(W, H, L, seed=21)
+    gt = np.where(gt < 0, 0, gt).astype(np.int16)
+    rng = np.random.default_rng(5)
+    kp = np.concatenate([[1.2, 0.9], (np.eye(5) + rng.normal(0, 0.05, (5, 5))).T.ravel()]).astype(np.float32)
+
+    def model():
+        g = capi.Model.create_2d(W, H, L)
+        g.add_gaussian(3.0, 3.0, 3.0, ktype=capi.DIAG_KERNEL)
+        g.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.FULL_KERNEL)
+        return g
+
+    layer = CRFLayer(model(), n, [3.0, 10.0], kparams=kp, learn_kernel=True)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+    Q = layer(Ut)
+    loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+    loss.backward()
+    dkp = layer.kparams.grad.cpu().numpy()
+
+    ref = model()
+    ref.set_unary(U)
+    ref.set_kernel_params(kp)
+    obj, _, _, _, kg = ref.gradient(n, gt, kernel=True)
+    assert abs(float(loss) + obj) <= 1e-5 * abs(obj)
+    assert np.abs(dkp + kg).max() <= 1e-4 * np.abs(kg).max()
+
+    with torch.no_grad():
+        layer.kparams -= 0.05 * layer.kparams.grad / layer.kparams.grad.abs().max()
+        Q2 = layer(Ut).cpu().numpy()
+    fresh = model()
+    fresh.set_unary(U)
+    fresh.set_kernel_params(layer.kparams.detach().cpu().numpy())
+    np.testing.assert_allclose(Q2, fresh.inference(n).T, atol=1e-6)

@@ -266,6 +266,87 @@ def test_gradient_central_differences(ck):
     np.testing.assert_allclose(-cg, fd, rtol=1e-3, atol=1e-5)
 
 
+@pytest.mark.parametrize("d,N", [(2, 30), (3, 24), (5, 16)])
+def test_lattice_gradient_central_differences(d, N):
+    """Permutohedral::gradient(a, c) is the gradient of a^T K^T c (K^T: the reverse blur
+    order) with respect to the features: central differences (h = 1e-3) of the FP64
+    filter on lattices rebuilt at f +- h e_ij, generic feature positions."""
+    rng = np.random.default_rng(40 + d)
+    f = (rng.standard_normal((d, N)) * 2).astype(np.float32)
+    a = rng.standard_normal((2, N))
+    c = rng.standard_normal((2, N))
+    df = po.OracleLattice(f).gradient(a.astype(np.float32), c.astype(np.float32))
+    h = 1e-3
+    fd = np.zeros((d, N))
+    for j in range(d):
+        for i in range(N):
+            fp, fm = f.copy(), f.copy()
+            fp[j, i] += h
+            fm[j, i] -= h
+            vp = (a * po.OracleLattice(fp).compute(c, True, fp64=True)).sum()
+            vm = (a * po.OracleLattice(fm).compute(c, True, fp64=True)).sum()
+            fd[j, i] = (vp - vm) / (2 * h)
+    assert np.abs(df - fd).max() <= 2e-3 * np.abs(fd).max()
+
+
+@pytest.mark.parametrize("kt", [1, 2])
+@pytest.mark.parametrize("nt", [1, 2, 3])
+@pytest.mark.parametrize("ck", ["potts", "matrix"])
+def test_kernel_gradient_central_differences(kt, nt, ck):
+    """DenseCRF::gradient's kernel_grad against FP64 central differences of the
+    cross-entropy over the kernel parameters (setKernelParameters rebuilds the lattices).
+    The reference differentiates with the transposed filter in place of the filter, so
+    the check uses 1-D features, where both blur axes are the same operator and the
+    lattice filter is symmetric; that pins the normaliser terms, the compatibility
+    product and the sum over iterations."""
+    rng = np.random.default_rng(8)
+    N, L, n = 80, 3, 3
+    U = (rng.standard_normal((L, N)) * 0.5).astype(np.float32)
+    gt = rng.integers(0, L, N).astype(np.int16)
+    f1 = (rng.standard_normal((1, N)) * 3).astype(np.float32)
+    f2 = (rng.standard_normal((1, N)) * 2).astype(np.float32)
+    o = po.OracleCRF(N=N, L=L)
+    o.set_unary(U)
+    o.add_pairwise(f1, 1.5 if ck == "potts" else rng.normal(0, 1, (L, L)).astype(np.float32), kt, nt)
+    o.add_pairwise(f2, 2.5, kt, nt)
+    o.set_kernel_params(np.array([1.1, 0.8], np.float32))
+    p0 = o.kernel_params().astype(np.float64)
+    _, _, _, kg = o.gradient(n, gt, kernel=True)
+    h = 1e-3
+    fd = np.zeros_like(p0)
+    for i in range(p0.size):
+        pp, pm = p0.copy(), p0.copy()
+        pp[i] += h
+        pm[i] -= h
+        o.set_kernel_params(pp.astype(np.float32))
+        cp = o.ce_fp64(n, gt)
+        o.set_kernel_params(pm.astype(np.float32))
+        cm = o.ce_fp64(n, gt)
+        fd[i] = -(cp - cm) / (2 * h)
+    np.testing.assert_allclose(kg, fd, rtol=2e-3, atol=2e-3 * np.abs(fd).max())
+
+
+def test_kernel_gradient_no_normalization_central_differences():
+    """NO_NORMALIZATION on 1-D features: the unnormalised messages make the objective
+    steep, so the check differentiates the float objective itself over one iteration."""
+    rng = np.random.default_rng(8)
+    N, L = 80, 3
+    U = (rng.standard_normal((L, N)) * 0.5).astype(np.float32)
+    gt = rng.integers(0, L, N).astype(np.int16)
+    f1 = (rng.standard_normal((1, N)) * 3).astype(np.float32)
+    for n in (1, 2):
+        o = po.OracleCRF(N=N, L=L)
+        o.set_unary(U)
+        o.add_pairwise(f1, 1.5, 1, po.NO_NORMALIZATION)
+        _, _, _, kg = o.gradient(n, gt, kernel=True)
+        h = 1e-3
+        o.set_kernel_params(np.array([1 + h], np.float32))
+        rp = o.gradient(n, gt)[0]
+        o.set_kernel_params(np.array([1 - h], np.float32))
+        rm = o.gradient(n, gt)[0]
+        assert abs(kg[0] - (rp - rm) / (2 * h)) <= 2e-3 * abs(kg[0])
+
+
 def test_exp_packet_accuracy():
     x = np.linspace(-87.0, 0.0, 20001).astype(np.float32)
     got = po.exp_packet(x).astype(np.float64)

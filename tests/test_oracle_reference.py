@@ -86,6 +86,47 @@ def test_crf_fixtures(i):
         assert np.abs(cg - z[p + "compat_grad"]).max() <= 2e-5 * cs
 
 
+def test_lattice_gradient_fixtures_bitexact():
+    """Permutohedral::gradient (permutohedral.cpp:612-695) on the reference's outputs."""
+    z = _load("kgrad_small.npz")
+    for i in range(int(z["nlat"])):
+        df = po.OracleLattice(z["l%d_feat" % i]).gradient(z["l%d_a" % i], z["l%d_b" % i])
+        assert np.array_equal(bits(df), bits(z["l%d_df" % i])), i
+
+
+@pytest.mark.parametrize("i", range(8))
+def test_kernel_gradient_fixtures(i):
+    """DenseCRF::gradient's kernel_grad (pairwise.cpp:152-163, 202-207) after
+    setKernelParameters: bit-exact for DIAG/CONST kernels with Potts or Diagonal
+    compatibilities; FULL kernels (the reference forms fg f^T with Eigen's GEMM) and
+    Matrix compatibilities within 2e-6 of the largest entry."""
+    z = _load("kgrad_small.npz")
+    W, H, L, ck, kt1, kt2, nt, n = [int(v) for v in z["c%d_meta" % i]]
+    p = "c%d_" % i
+    c1, c2 = z[p + "c1"], z[p + "c2"]
+    if ck == 0:
+        c1, c2 = float(c1[0]), float(c2[0])
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(z[p + "U"])
+    o.add_gaussian(3.0, 2.0, c1, ktype=kt1, ntype=nt)
+    o.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], c2, ktype=kt2, ntype=nt)
+    if len(z[p + "kparams"]):
+        o.set_kernel_params(z[p + "kparams"])
+    np.testing.assert_array_equal(o.kernel_params(), z[p + "kparams"])
+    r, ug, cg, kg = o.gradient(n, z[p + "gt"], kernel=True)
+    ref = z[p + "kernel_grad"]
+    assert kg.shape == ref.shape
+    if ck != 2 and 2 not in (kt1, kt2):
+        assert r == float(z[p + "obj"])
+        assert np.array_equal(bits(ug), bits(z[p + "unary_grad"]))
+        assert np.array_equal(bits(kg), bits(ref))
+    else:
+        assert abs(r - float(z[p + "obj"])) <= 2e-6 * max(1.0, abs(r))
+        assert np.abs(kg - ref).max() <= 2e-6 * np.abs(ref).max()
+    Q = o.inference(n)
+    assert np.abs(Q - z[p + "Q"]).max() <= 2e-6
+
+
 def test_reference_suite_fixture_10x10():
     """pydensecrf/tests/test_dcrf.py:29-83 on the oracle: the 2-D and the generic paths,
     bit-exact against the reference, and their MAPs agree (test_call_dcrf_eq_dcrf2d)."""
@@ -176,3 +217,31 @@ def test_generic_model_live_reference():
         c.add_pairwise(f2, w, ntype=po.NORMALIZE_AFTER)
         out.append(c.inference(7))
     assert np.array_equal(bits(out[0]), bits(out[1]))
+
+
+@ref
+@pytest.mark.parametrize("kt,nt", [(1, 3), (1, 0), (2, 1), (2, 2)])
+def test_kernel_gradient_live_reference(kt, nt):
+    """Generic features (DenseCRF::addPairwiseEnergy) with set kernel parameters."""
+    rng = np.random.default_rng(31 + kt * 4 + nt)
+    N, L, n = 400, 4, 3
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    gt = rng.integers(-1, L, N).astype(np.int16)
+    f1 = (rng.standard_normal((3, N)) * 2).astype(np.float32)
+    f2 = (rng.standard_normal((4, N)) * 2).astype(np.float32)
+    w = np.array([1, 2, 2, 3], np.float32)
+    out = []
+    for cls in (po.OracleCRF, po.RefCRF):
+        c = cls(N=N, L=L) if cls is po.OracleCRF else cls(N=N, L=L, identity_unary=True)
+        c.set_unary(U)
+        c.add_pairwise(f1, w, ktype=kt, ntype=nt)
+        c.add_pairwise(f2, 3.0, ktype=kt, ntype=nt)
+        p0 = c.kernel_params()
+        c.set_kernel_params(p0 * np.float32(0.9))
+        out.append(c.gradient(n, gt, kernel=True))
+    (ro, _, _, ko), (rr, _, _, kr) = out
+    assert ro == rr
+    if kt == 1:
+        assert np.array_equal(bits(ko), bits(kr))
+    else:
+        assert np.abs(ko - kr).max() <= 2e-6 * np.abs(kr).max()
