@@ -104,6 +104,8 @@ struct dcrf_model {
     // per-block partials and the accumulated gradient of all terms
     DevBuf<float> kgbuf, kgdf;
     DevBuf<double> kpartial, kacc;
+    DevBuf<int> id_vtx;  // [N] 0..N-1 and [N] ones: Q_t read as a one-vertex slice
+    DevBuf<float> id_sw;
 
     // captured inference (dcrf_capture_inference)
     hipGraph_t graph = nullptr;
@@ -408,89 +410,110 @@ RawPlan& raw_plan(dcrf_model& m, Term& t) {
         P->ones_rev = std::move(v1);
     else
         P->ones_rev = std::move(v0);
+    // the plan's own vertex rows: the zero rows V stay zero (the splat writes rows < V
+    // and the partial rows, the blur rows < V)
+    const size_t L = m.L;
+    auto zeroed = [&](DevBuf<float>& b, size_t n) {
+        b.reserve(n);
+        launch_fill(b.get(), 0.f, n, s);
+    };
+    zeroed(P->S2, (size_t)rows * 2 * L);
+    for (int i = 0; i < 2; i++) {
+        zeroed(P->F2[i], (size_t)(V + 1) * 2 * L);
+        zeroed(P->R2[i], (size_t)(V + 1) * 2 * L);
+    }
+    zeroed(P->S1, (size_t)rows * L);
+    zeroed(P->B1, (size_t)(V + 1) * L);
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
     t.raw = std::move(P);
     return *t.raw;
 }
 
-// Splat `in` ([N][L]) on the term's own lattice and blur it (reverse: axes d..0); the
-// rows are left in the model's vertex buffers, which the next global filter overwrites.
-const float* raw_lattice(dcrf_model& m, Term& t, const float* in, bool reverse, hipStream_t s) {
+// Splat `in` ([N][Lc], row stride Lc) on the term's own lattice into `dst` (stride Lc).
+void raw_splat(const Term& t, const float* in, int Lc, float* dst, hipStream_t s) {
     const RawPlan& P = *t.raw;
-    const int V = t.lat.V, L = m.L, ldv = m.ldv;
-    float* bufs[2] = {m.vals[0].get(), m.vals[1].get()};
-    launch_fill(bufs[0] + (size_t)V * ldv, 0.f, ldv, s);
-    launch_fill(bufs[1] + (size_t)V * ldv, 0.f, ldv, s);
-    launch_splat(P.ch.get(), P.nch, P.ent.get(), in, L, bufs[0], ldv, L, s);
-    launch_combine(P.cb.get(), P.ncb, bufs[0], ldv, L, s);
-    BlurTable bt{};
-    bt.nterms = 1;
-    bt.t[0] = BlurTerm{0, V, t.lat.d, P.nbr.get()};
-    for (int step = 0; step <= t.lat.d; step++)
-        launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, ldv, L, bt, step, reverse ? 1 : 0, s);
-    return bufs[(t.lat.d + 1) & 1];
+    launch_splat(P.ch.get(), P.nch, P.ent.get(), in, Lc, dst, Lc, Lc, s);
+    launch_combine(P.cb.get(), P.ncb, dst, Lc, Lc, s);
 }
 
-// out = K in (or K^T in): Permutohedral::compute without normaliser.
-void raw_filter(dcrf_model& m, Term& t, const float* in, float* out, bool reverse, hipStream_t s) {
-    const float* v = raw_lattice(m, t, in, reverse, s);
+// The d+1 blur steps of the term's lattice (reverse: axes d..0) from src through the
+// ping-pong pair buf (src itself is only read); returns the buffer holding the result.
+float* raw_blur(const Term& t, const float* src, float* const buf[2], int Lc, bool reverse, hipStream_t s) {
+    const int V = t.lat.V, d = t.lat.d;
+    BlurTable bt{};
+    bt.nterms = 1;
+    bt.t[0] = BlurTerm{0, V, d, t.raw->nbr.get()};
+    const float* in = src;
+    for (int step = 0; step <= d; step++) {
+        launch_blur(in, buf[step & 1], 0, V, Lc, Lc, bt, step, reverse ? 1 : 0, s);
+        in = buf[step & 1];
+    }
+    return buf[d & 1];
+}
+
+// out = slice of the blurred rows `rows` (row stride ldv, the first L labels used).
+void raw_slice(dcrf_model& m, const Term& t, const float* rows, int ldv, float* out, hipStream_t s) {
     SliceTable st{};
     st.nterms = 1;
-    st.t[0] = SliceTerm{t.lat.vtx.get(), t.raw->sw.get(), v, t.lat.d + 1, -1, nullptr, nullptr, nullptr, m.ldv};
+    st.t[0] = SliceTerm{t.lat.vtx.get(), t.raw->sw.get(), rows, t.lat.d + 1, -1, nullptr, nullptr, nullptr, ldv};
     launch_slice_term(out, m.N(), m.L, st, 0, s);
 }
 
-// acc += kernelGradient(b, Q_t) of one term (pairwise.cpp:202-207, 152-163, 88-114).
+// acc += kernelGradient(b, Q_t) of one term (pairwise.cpp:202-207, 152-163, 88-114):
+// fg = -G(x, 1) + G(a, c) with a, c and x per the normalisation type.  The inputs a
+// and c are splatted together as [a | c] and blurred in both orders: the forward rows
+// give K a (G(a, c), direction 0) and K c, the reverse rows K^T a and K^T c (G(a, c),
+// direction 1); x needs one more splat and forward blur, and G(x, 1)'s direction 1 is
+// the plan's reverse-blurred ones.
 void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, const float* kw, double* acc,
                       hipStream_t s) {
-    const int N = m.N(), L = m.L, d = t.lat.d;
+    const int N = m.N(), L = m.L, d = t.lat.d, L2 = 2 * L;
     const size_t NL = (size_t)N * L;
-    raw_plan(m, t);
+    RawPlan& P = raw_plan(m, t);
     float* lbl = m.kgbuf.get();
-    float* A = lbl + NL;
-    float* B = A + NL;
-    float* F1 = B + NL;
-    float* F2 = F1 + NL;
+    float* pk = lbl + NL;     // [N][2L]
+    float* F1 = pk + 2 * NL;  // fa, then x
+    float* F2 = F1 + NL;      // fb
     float* dfr = m.kgdf.get();
     float* dfg = dfr + (size_t)N * d;
     const float* nm = t.norm.get();
-    launch_compat_rows(Qt, N, L, t.ckind, t.d_compat.get(), kw, lbl, s);
-    // G(a, c) below; x the first argument of the normaliser term G(x, 1)
-    const float* a = b;
-    const float* c = lbl;
-    const float* x = nullptr;
-    if (t.ntype == DCRF_NORMALIZE_SYMMETRIC) {
-        launch_kg_scale(b, lbl, nm, N, L, A, B, s);
-        a = A;
-        c = B;
-        raw_filter(m, t, A, F1, true, s);   // fa = K^T (b n)
-        raw_filter(m, t, B, F2, false, s);  // fb = K (lbl n)
-        launch_kg_x(b, lbl, F1, F2, nm, N, L, t.ntype, F1, s);
-        x = F1;
-    } else if (t.ntype == DCRF_NORMALIZE_AFTER) {
-        launch_kg_scale(b, lbl, nm, N, L, A, nullptr, s);
-        a = A;
-        raw_filter(m, t, lbl, F2, false, s);  // fb = K lbl
-        launch_kg_x(b, lbl, nullptr, F2, nm, N, L, t.ntype, F1, s);
-        x = F1;
-    } else if (t.ntype == DCRF_NORMALIZE_BEFORE) {
-        launch_kg_scale(b, lbl, nm, N, L, nullptr, B, s);
-        c = B;
-        raw_filter(m, t, b, F1, true, s);  // fa = K^T b
-        launch_kg_x(b, lbl, F1, nullptr, nm, N, L, t.ntype, F1, s);
-        x = F1;
+    {
+        // lbl = C (w Q_t): the per-pixel compatibility kernel over a one-vertex "slice"
+        // that reads Q_t itself (vertex p = pixel p, weight 1)
+        if (!m.id_vtx.get() || m.id_vtx.n < (size_t)N) {
+            m.id_vtx.reserve(N);
+            m.id_sw.reserve(N);
+            launch_iota(m.id_vtx.get(), N, s);
+            launch_fill(m.id_sw.get(), 1.f, N, s);
+        }
+        SliceTable st{};
+        st.nterms = 1;
+        st.t[0] = SliceTerm{m.id_vtx.get(), m.id_sw.get(), Qt, 1, t.ckind, t.d_compat.get(), kw, nullptr, L};
+        launch_slice_term(lbl, N, L, st, 0, s);
     }
+    launch_kg_pack(b, lbl, nm, N, L, t.ntype, pk, s);
+    raw_splat(t, pk, L2, P.S2.get(), s);
+    float* const fbuf[2] = {P.F2[0].get(), P.F2[1].get()};
+    float* const rbuf[2] = {P.R2[0].get(), P.R2[1].get()};
+    const float* fwd = raw_blur(t, P.S2.get(), fbuf, L2, false, s);
+    const float* rev = raw_blur(t, P.S2.get(), rbuf, L2, true, s);
     const int* vtx = t.lat.vtx.get();
     const int* rank = t.lat.rank.get();
-    if (x) {
-        // r = G(x, 1): dir 0 pairs the forward-blurred x with ones, dir 1 the
-        // reverse-blurred ones with x
-        launch_slice_grad(raw_lattice(m, t, x, false, s), m.ldv, 1, vtx, rank, nullptr, N, L, d, 0, dfr, s);
-        launch_slice_grad(t.raw->ones_rev.get(), 1, 0, vtx, rank, x, N, L, d, 1, dfr, s);
+    const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
+    if (has_x) {
+        const bool need_fa = t.ntype != DCRF_NORMALIZE_AFTER, need_fb = t.ntype != DCRF_NORMALIZE_BEFORE;
+        if (need_fa) raw_slice(m, t, rev, L2, F1, s);      // fa = K^T a
+        if (need_fb) raw_slice(m, t, fwd + L, L2, F2, s);  // fb = K c
+        launch_kg_x(b, lbl, need_fa ? F1 : nullptr, need_fb ? F2 : nullptr, nm, N, L, t.ntype, F1, s);
+        raw_splat(t, F1, L, P.S1.get(), s);
+        float* const xbuf[2] = {P.B1.get(), P.S1.get()};
+        const float* xr = raw_blur(t, P.S1.get(), xbuf, L, false, s);
+        launch_slice_grad(xr, L, 1, vtx, rank, nullptr, 0, N, L, d, 0, dfr, s);
+        launch_slice_grad(P.ones_rev.get(), 1, 0, vtx, rank, F1, L, N, L, d, 1, dfr, s);
     }
-    launch_slice_grad(raw_lattice(m, t, a, false, s), m.ldv, 1, vtx, rank, c, N, L, d, 0, dfg, s);
-    launch_slice_grad(raw_lattice(m, t, c, true, s), m.ldv, 1, vtx, rank, a, N, L, d, 1, dfg, s);
-    launch_kparam_reduce(x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
+    launch_slice_grad(fwd, L2, 1, vtx, rank, pk + L, L2, N, L, d, 0, dfg, s);
+    launch_slice_grad(rev + L, L2, 1, vtx, rank, pk, L2, N, L, d, 1, dfg, s);
+    launch_kparam_reduce(has_x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
                          m.geo.n_inst, N, d, t.ktype == DCRF_FULL_KERNEL, m.kpartial.get(), acc, s);
 }
 
@@ -899,9 +922,6 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
                 if (m.terms[k]->num_kparams())
                     kernel_grad_term(m, *m.terms[k], b, Qc + (size_t)it * NL, m.train_kw ? m.train_kw + k : nullptr,
                                      m.kacc.get() + koff[k], s);
-            // the term-local filters may have written the global zero rows
-            launch_fill(m.vals[0].get() + (size_t)m.V_total * m.ldv, 0.f, m.ldv, s);
-            launch_fill(m.vals[1].get() + (size_t)m.V_total * m.ldv, 0.f, m.ldv, s);
         }
         run_filter(m, b, L, true, s);
         launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);

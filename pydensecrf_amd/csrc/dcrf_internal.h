@@ -117,6 +117,10 @@ struct RawPlan {
     DevBuf<int2> nbr;        // [(d+1)][V] neighbour rows, zero row V for an absent one
     DevBuf<float> sw;        // [N][d+1] slice weights bary / (1 + 2^-d)
     DevBuf<float> ones_rev;  // [V+1] splat of ones blurred in reverse order (row V zero)
+    // the plan's own vertex rows, zero rows V kept zero: the splat of the paired inputs
+    // [V+1+partials][2L], its forward and reverse blur chains [V+1][2L] x 2 each; the
+    // splat of one input [V+1+partials][L] and its blur partner [V+1][L]
+    DevBuf<float> S2, F2[2], R2[2], S1, B1;
 };
 
 struct Term {
@@ -329,24 +333,25 @@ void launch_compat_finish(const double* G, int ckind, int L, const float* cw, co
 void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s);
 
 // ---------------------------------------------------------------- kernel-parameter gradient
-// lbl = compat(w Q) [N][L] (w = kw[0], or 1 when kw is null).
-void launch_compat_rows(const float* Q, int N, int L, int ckind, const float* cw, const float* kw, float* out,
-                        hipStream_t s);
-// bn = b n, ln = lbl n per pixel (either output may be null).
-void launch_kg_scale(const float* b, const float* lbl, const float* norm, int N, int L, float* bn, float* ln,
-                     hipStream_t s);
+// P [N][2L] = [a | c], a = b (times n for SYMMETRIC / AFTER), c = lbl (times n for
+// SYMMETRIC / BEFORE).
+void launch_kg_pack(const float* b, const float* lbl, const float* norm, int N, int L, int ntype, float* P,
+                    hipStream_t s);
 // The normaliser-derivative input x of featureGradient for ntype (may overwrite fa).
 void launch_kg_x(const float* b, const float* lbl, const float* fa, const float* fb, const float* norm, int N, int L,
                  int ntype, float* x, hipStream_t s);
 // One direction of the lattice gradient: df (+)= slicing gradient of the blurred rows
-// vals (row stride ldv, label stride ks) paired with c [N][L] (null: ones).
-void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int N,
-                       int L, int d, int accumulate, float* df, hipStream_t s);
+// vals (row stride ldv, label stride ks) paired with c [N][L] (row stride ldc; null:
+// ones).
+void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int ldc,
+                       int N, int L, int d, int accumulate, float* df, hipStream_t s);
 // acc (d or d*d doubles) += parameter gradient of fg = (r ? -r + g : g) against the
 // padded feature table f; partial holds kparam_partial_doubles(d).
 size_t kparam_partial_doubles(int d);
 void launch_kparam_reduce(const float* r, const float* g, const float* f, const int* pix_base, const int* pad_base,
                           int n_inst, int N, int d, bool full, double* partial, double* acc, hipStream_t s);
+// out[i] = i.
+void launch_iota(int* out, int n, hipStream_t s);
 // out[i] = (float)acc[i].
 void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
 // Scaled features diag(P) f or P f (P column-major) over the padded table.

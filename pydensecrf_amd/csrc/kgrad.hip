@@ -36,40 +36,21 @@ namespace {
 constexpr int kThreads = 256;
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
-// lbl = w * compat(Q): Potts -c Q, Diagonal c_l Q_l, Matrix W Q (labelcompatibility.cpp:
-// 31-33, 46-48, 66-69).  One thread per (pixel, label).
-__global__ void k_compat_rows(const float* __restrict__ Q, int N, int L, int ckind, const float* __restrict__ cw,
-                              const float* __restrict__ kw, float* __restrict__ out) {
+// The two filter inputs of featureGradient (pairwise.cpp:88-114) side by side,
+// P[p] = [a | c] (2L floats): a = b n (SYMMETRIC, AFTER) or b, c = lbl n (SYMMETRIC,
+// BEFORE) or lbl.  One splat and the two blur orders of P give K a, K^T a, K c, K^T c.
+__global__ void k_kg_pack(const float* __restrict__ b, const float* __restrict__ lbl, const float* __restrict__ norm,
+                          int N, int L, int ntype, float* __restrict__ P) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)N * L) return;
-    const int l = (int)(i % L);
-    const size_t row = i - l;
-    const float w = kw ? kw[0] : 1.0f;
-    float v;
-    if (ckind == DCRF_COMPAT_POTTS) {
-        v = -cw[0] * (Q[i] * w);
-    } else if (ckind == DCRF_COMPAT_DIAGONAL) {
-        v = cw[l] * (Q[i] * w);
-    } else {
-        float s = 0.0f;
-        for (int j = 0; j < L; j++) {
-            const float m = cw[(size_t)l * L + j] * (Q[row + j] * w);
-            s = s + m;
-        }
-        v = s;
-    }
-    out[i] = v;
-}
-
-// Inputs of featureGradient (pairwise.cpp:88-114) before the filters: a_n = b n,
-// c_n = lbl n (either may be skipped: null output).
-__global__ void k_kg_scale(const float* __restrict__ b, const float* __restrict__ lbl, const float* __restrict__ norm,
-                           int N, int L, float* __restrict__ bn, float* __restrict__ ln) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * L) return;
-    const float n = norm[i / L];
-    if (bn) bn[i] = b[i] * n;
-    if (ln) ln[i] = lbl[i] * n;
+    const size_t p = i / L;
+    const int k = (int)(i - p * L);
+    const float n = norm[p];
+    const bool na = ntype == DCRF_NORMALIZE_SYMMETRIC || ntype == DCRF_NORMALIZE_AFTER;
+    const bool nc = ntype == DCRF_NORMALIZE_SYMMETRIC || ntype == DCRF_NORMALIZE_BEFORE;
+    float* row = P + p * 2 * L;
+    row[k] = na ? b[i] * n : b[i];
+    row[L + k] = nc ? lbl[i] * n : lbl[i];
 }
 
 // The first argument of the norm-derivative term, in place of fa (or into x):
@@ -110,12 +91,13 @@ struct GradScale {
 //   u_{r1} with r0 = d - rank[p][j], r1 = r0 + 1 mod d + 1; then for j = 1..d
 //   grad_j += c[p][k] * sf_{j-1} (sm - j ra_j), sm = ra_0 + .. + ra_{j-1}.
 // vals rows have stride ldv and label stride ks (0: one value per vertex for every
-// label); c null means all ones.  df[p][j-1] = grad_j (accumulate: += grad_j).
+// label); c (row stride ldc) null means all ones.  df[p][j-1] = grad_j (accumulate:
+// += grad_j).
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_slice_grad(const float* __restrict__ vals, int ldv, int ks,
                                                          const int* __restrict__ vtx, const int* __restrict__ rank,
-                                                         const float* __restrict__ c, int N, int L, GradScale gs,
-                                                         int accumulate, float* __restrict__ df) {
+                                                         const float* __restrict__ c, int ldc, int N, int L,
+                                                         GradScale gs, int accumulate, float* __restrict__ df) {
     constexpr int D1 = D + 1;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
@@ -142,7 +124,7 @@ __global__ void __launch_bounds__(kThreads) k_slice_grad(const float* __restrict
     float grad[D];
 #pragma unroll
     for (int j = 0; j < D; j++) grad[j] = 0.0f;
-    const float* crow = c ? c + (size_t)p * L : nullptr;
+    const float* crow = c ? c + (size_t)p * ldc : nullptr;
     for (int k = 0; k < L; k++) {
         const size_t col = (size_t)k * ks;
         float ra[D1];
@@ -168,6 +150,95 @@ __global__ void __launch_bounds__(kThreads) k_slice_grad(const float* __restrict
     for (int j = 0; j < D; j++) {
         float* o = df + (size_t)p * D + j;
         *o = accumulate ? *o + grad[j] : grad[j];
+    }
+}
+
+// The same with S lanes per point, R consecutive labels per lane (S <= 64: the lane
+// groups of one wave), for d <= 6: each lane loads its pieces of the d+1 vertex rows
+// once, forms the rotated differences by select, and the d gradients are summed over
+// the lanes of the group.  The label order of the sum differs from the reference's
+// sequential one (float tolerance).
+template <int R, int D>
+__global__ void __launch_bounds__(kWaveBlock) k_slice_grad_lanes(const float* __restrict__ vals, int ldv, int ks,
+                                                                 const int* __restrict__ vtx,
+                                                                 const int* __restrict__ rank,
+                                                                 const float* __restrict__ c, int ldc, int N, int L,
+                                                                 int S, int per, GradScale gs, int accumulate,
+                                                                 float* __restrict__ df) {
+    constexpr int D1 = D + 1;
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int pp = active ? p : 0;
+    int v[D1], r0[D1];
+#pragma unroll
+    for (int q = 0; q < D1; q++) {
+        v[q] = vtx[(size_t)pp * D1 + q];
+        r0[q] = D - rank[(size_t)pp * D1 + q];
+    }
+    // au[q][r] = alpha * vals[v[q]][k0 + r]
+    float au[D1][R];
+#pragma unroll
+    for (int q = 0; q < D1; q++) {
+        if (ks) {
+            load_piece<R>(vals + (size_t)v[q] * ldv, G.k0, L, au[q]);
+        } else {
+            const float x = vals[(size_t)v[q] * ldv];
+#pragma unroll
+            for (int r = 0; r < R; r++) au[q][r] = x;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) au[q][r] = gs.alpha * au[q][r];
+    }
+    float cv[R];
+    if (c) {
+        load_piece<R>(c + (size_t)pp * ldc, G.k0, L, cv);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) cv[r] = 1.0f;
+    }
+    float grad[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) grad[j] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float ra[D1];
+#pragma unroll
+        for (int j = 0; j < D1; j++) {
+            const int a = r0[j], b = a + 1 > D ? 0 : a + 1;
+            float x0 = au[0][r], x1 = au[0][r];
+#pragma unroll
+            for (int q = 1; q < D1; q++) {
+                x0 = a == q ? au[q][r] : x0;
+                x1 = b == q ? au[q][r] : x1;
+            }
+            ra[j] = x0 - x1;
+        }
+        float sm = ra[0];
+#pragma unroll
+        for (int j = 1; j <= D; j++) {
+            const float jr = (float)j * ra[j];
+            const float diff = sm - jr;
+            const float e = gs.sf[j - 1] * diff;
+            const float prod = cv[r] * e;
+            grad[j - 1] = grad[j - 1] + prod;
+            sm = sm + ra[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        float x[R];
+        x[0] = grad[j];
+#pragma unroll
+        for (int r = 1; r < R; r++) x[r] = 0.0f;
+        grad[j] = G.all_sum(x);  // every lane takes part in the shuffles
+    }
+    if (active && G.l == 0) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            float* o = df + (size_t)p * D + j;
+            *o = accumulate ? *o + grad[j] : grad[j];
+        }
     }
 }
 
@@ -212,13 +283,26 @@ __global__ void __launch_bounds__(kThreads) k_kparam_partial(const float* __rest
     }
 }
 
-// acc[q] += sum over blocks of partial[b][q], blocks in order.
-__global__ void k_kparam_final(const double* __restrict__ partial, int nb, int np, double* __restrict__ acc) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
+// acc[q] += sum over blocks of partial[b][q]: one workgroup per parameter, a fixed
+// strided assignment of blocks to threads and a fixed tree.
+__global__ void __launch_bounds__(kThreads) k_kparam_final(const double* __restrict__ partial, int nb, int np,
+                                                           double* __restrict__ acc) {
+    __shared__ double red[kThreads];
+    const int q = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nb; b++) s += partial[(size_t)b * np + q];
-    acc[q] += s;
+    for (int b = threadIdx.x; b < nb; b += kThreads) s += partial[(size_t)b * np + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[q] += red[0];
+}
+
+__global__ void k_iota(int* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
 }
 
 __global__ void k_kparam_store(const double* __restrict__ acc, int n, float* __restrict__ out) {
@@ -254,19 +338,11 @@ constexpr int kKparamBlocks = 512;
 
 size_t kparam_partial_doubles(int d) { return (size_t)kKparamBlocks * d * d; }
 
-void launch_compat_rows(const float* Q, int N, int L, int ckind, const float* cw, const float* kw, float* out,
-                        hipStream_t s) {
+void launch_kg_pack(const float* b, const float* lbl, const float* norm, int N, int L, int ntype, float* P,
+                    hipStream_t s) {
     const size_t n = (size_t)N * L;
     if (!n) return;
-    k_compat_rows<<<blocks_for(n), kThreads, 0, s>>>(Q, N, L, ckind, cw, kw, out);
-    DCRF_HIP_CHECK(hipGetLastError());
-}
-
-void launch_kg_scale(const float* b, const float* lbl, const float* norm, int N, int L, float* bn, float* ln,
-                     hipStream_t s) {
-    const size_t n = (size_t)N * L;
-    if (!n || (!bn && !ln)) return;
-    k_kg_scale<<<blocks_for(n), kThreads, 0, s>>>(b, lbl, norm, N, L, bn, ln);
+    k_kg_pack<<<blocks_for(n), kThreads, 0, s>>>(b, lbl, norm, N, L, ntype, P);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
@@ -278,18 +354,44 @@ void launch_kg_x(const float* b, const float* lbl, const float* fa, const float*
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
-void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int N,
-                       int L, int d, int accumulate, float* df, hipStream_t s) {
+void launch_iota(int* out, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_iota<<<blocks_for(n), kThreads, 0, s>>>(out, n);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int ldc,
+                       int N, int L, int d, int accumulate, float* df, hipStream_t s) {
     if (!N) return;
     GradScale gs{};
     // the reference's constants (permutohedral.cpp:620-628): double, stored as float
     const float inv_std_dev = (float)(std::sqrt(2.0 / 3.0) * (d + 1));
     for (int i = 0; i < d; i++) gs.sf[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * inv_std_dev);
     gs.alpha = 1.0f / (1 + std::pow(2.0f, (float)-d)) / (float)(d + 1);
+    const LaunchShape sh = shape_for(L);
+    if (sh.wave && d <= 6) {
+        const unsigned g = blocks_for(N, sh.items);
+#define DCRF_SGL(RR, DD)                                                                                       \
+    case RR * 8 + DD:                                                                                          \
+        k_slice_grad_lanes<RR, DD><<<g, kWaveBlock, 0, s>>>(vals, ldv, ks, vtx, rank, c, ldc, N, L, sh.S, sh.per, \
+                                                           gs, accumulate, df);                                \
+        break;
+        switch (sh.R * 8 + d) {
+            DCRF_SGL(1, 1) DCRF_SGL(1, 2) DCRF_SGL(1, 3) DCRF_SGL(1, 4) DCRF_SGL(1, 5) DCRF_SGL(1, 6)
+            DCRF_SGL(2, 1) DCRF_SGL(2, 2) DCRF_SGL(2, 3) DCRF_SGL(2, 4) DCRF_SGL(2, 5) DCRF_SGL(2, 6)
+            DCRF_SGL(3, 1) DCRF_SGL(3, 2) DCRF_SGL(3, 3) DCRF_SGL(3, 4) DCRF_SGL(3, 5) DCRF_SGL(3, 6)
+            DCRF_SGL(4, 1) DCRF_SGL(4, 2) DCRF_SGL(4, 3) DCRF_SGL(4, 4) DCRF_SGL(4, 5) DCRF_SGL(4, 6)
+            default:
+                throw Error{DCRF_ERR_STATE, "slice gradient: no lane-group kernel for this shape"};
+        }
+#undef DCRF_SGL
+        DCRF_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const unsigned nb = blocks_for(N);
 #define DCRF_SG(DD)                                                                                              \
     case DD:                                                                                                     \
-        k_slice_grad<DD><<<nb, kThreads, 0, s>>>(vals, ldv, ks, vtx, rank, c, N, L, gs, accumulate, df); \
+        k_slice_grad<DD><<<nb, kThreads, 0, s>>>(vals, ldv, ks, vtx, rank, c, ldc, N, L, gs, accumulate, df); \
         break;
     switch (d) {
         DCRF_SG(1) DCRF_SG(2) DCRF_SG(3) DCRF_SG(4) DCRF_SG(5) DCRF_SG(6) DCRF_SG(7) DCRF_SG(8)
@@ -308,7 +410,7 @@ void launch_kparam_reduce(const float* r, const float* g, const float* f, const 
     const int nb = std::max(1, std::min(kKparamBlocks, (int)blocks_for(N)));
     k_kparam_partial<<<nb, kThreads, 0, s>>>(r, g, f, pix_base, pad_base, n_inst, N, d, full ? 1 : 0, partial);
     DCRF_HIP_CHECK(hipGetLastError());
-    k_kparam_final<<<blocks_for(np, 64), 64, 0, s>>>(partial, nb, np, acc);
+    k_kparam_final<<<np, kThreads, 0, s>>>(partial, nb, np, acc);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
