@@ -57,6 +57,7 @@ SIGNATURES = {
     "dcrf_get_compat_params": (_c, [_P, _F]),
     "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
     "dcrf_num_kernel_params": (_c, [_P]),
+    "dcrf_set_blur_pairs": (_c, [_P, _c]),
     "dcrf_get_kernel_params": (_c, [_P, _F]),
     "dcrf_set_kernel_params": (_c, [_P, _F]),
     "dcrf_forward_device": (_c, [_P, _c, _P, _P, _P, _P, _P]),
@@ -286,6 +287,9 @@ class Model:
         p = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         check(lib().dcrf_get_compat_params(self.h, fptr(p)))
         return p[:lib().dcrf_num_compat_params(self.h)]
+
+    def set_blur_pairs(self, on):
+        check(lib().dcrf_set_blur_pairs(self.h, 1 if on else 0))
 
     def kernel_params(self):
         n = lib().dcrf_num_kernel_params(self.h)

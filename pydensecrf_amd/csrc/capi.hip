@@ -34,6 +34,7 @@ constexpr int kDefaultChunk = 32;
 
 struct TermLayout {
     DevBuf<int2> nbr;    // [(d+1)][V] global rows
+    DevBuf<int3> pairF, pairT;  // [blur_pairs(d)][V][3] two-step pass tables, forward / reverse
     DevBuf<int> vtx;     // [N][d+1] global rows
     DevBuf<float> swF;   // slice weights, forward
     DevBuf<float> swT;   // slice weights, transpose (unused when symmetric)
@@ -86,6 +87,9 @@ struct dcrf_model {
     DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
     bool runs = false;            // ... or through k_meanfield_runs (rows staged per pixel run)
     int mf_mode = 0;              // dcrf_set_meanfield_kernel: 0 auto (fused), 1 generic, 2 fused, 3 runs
+    bool pairs = true;            // blur two steps per pass (dcrf_set_blur_pairs)
+    int max_pairs = 0;
+    std::vector<int> prefixP;     // vertices of the terms with more than p passes
     int run_rows = 16;            // pixel rows of a run rectangle
     RunSlice rslice;
     int rows_lds = 0;
@@ -578,6 +582,10 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
 
 SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw = nullptr, float* mout = nullptr);
 
+// The vertex buffer holding a term's filtered rows after the global filter: one
+// buffer switch per blur step, or per two-step pass.
+int result_parity(const dcrf_model& m, int d) { return (m.pairs ? blur_pairs(d) : d + 1) & 1; }
+
 void ensure_layout(dcrf_model& m) {
     if (m.layout_valid) return;
     m.drop_graph();
@@ -689,6 +697,20 @@ void ensure_layout(dcrf_model& m) {
         auto lay = std::make_unique<TermLayout>();
         lay->nbr.reserve((size_t)d1 * lat.V);
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
+        if (m.pairs) {
+            // pass p: axes (2p, 2p+1) forward, (d-2p, d-2p-1) for the transpose
+            const int np = blur_pairs(lat.d);
+            lay->pairF.reserve((size_t)np * lat.V * 3);
+            lay->pairT.reserve((size_t)np * lat.V * 3);
+            for (int p = 0; p < np; p++) {
+                const int fa = 2 * p, fb = 2 * p + 1 <= lat.d ? 2 * p + 1 : -1;
+                const int ta = lat.d - 2 * p, tb = lat.d - 2 * p - 1 >= 0 ? lat.d - 2 * p - 1 : -1;
+                launch_pair_table(lay->nbr.get(), lat.V, m.vbase[k], V, fa, fb, lay->pairF.get() + (size_t)p * lat.V * 3,
+                                  s);
+                launch_pair_table(lay->nbr.get(), lat.V, m.vbase[k], V, ta, tb, lay->pairT.get() + (size_t)p * lat.V * 3,
+                                  s);
+            }
+        }
         lay->vtx.reserve(Ek);
         lay->swF.reserve(Ek);
         const float alpha = slice_alpha(lat.d);
@@ -715,7 +737,7 @@ void ensure_layout(dcrf_model& m) {
             launch_interleave_slice(m.tl[k]->vtx.get(), m.tl[k]->swF.get(), N, d1, ds, off, m.g_slc.get(), s);
             off += d1;
         }
-        // pixel runs (on request): entry j reads vals[(d+1) & 1] of its term
+        // pixel runs (on request): entry j reads vals[result_parity] of its term
         m.runs = false;
         if (m.mf_mode == 3 && m.V_total < (1 << 30)) {
             unsigned par = 0;
@@ -723,7 +745,7 @@ void ensure_layout(dcrf_model& m) {
             for (int k = 0; k < T; k++) {
                 const int d1 = m.terms[k]->lat.d + 1;
                 for (int q = 0; q < d1; q++, j++)
-                    if (d1 & 1) par |= 1u << j;
+                    if (result_parity(m, m.terms[k]->lat.d)) par |= 1u << j;
             }
             // rectangles of 2 waves' pixels per row x 16 rows (18 x 16 at 21 labels)
             // rectangles two waves of pixels wide, at most run_rows high and at most
@@ -743,6 +765,11 @@ void ensure_layout(dcrf_model& m) {
     for (int st = 0; st <= m.max_d; st++)
         for (int k = 0; k < T; k++)
             if (m.terms[k]->lat.d >= st) m.prefixV[st] += m.terms[k]->lat.V;
+    m.max_pairs = m.max_d >= 0 ? blur_pairs(m.max_d) : 0;
+    m.prefixP.assign(m.max_pairs, 0);
+    for (int p = 0; p < m.max_pairs; p++)
+        for (int k = 0; k < T; k++)
+            if (blur_pairs(m.terms[k]->lat.d) > p) m.prefixP[p] += m.terms[k]->lat.V;
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
     m.layout_valid = true;
 }
@@ -766,7 +793,7 @@ SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw, flo
         const Term& t = *m.terms[k];
         const TermLayout& lay = *m.tl[k];
         const float* sw = (transpose && !m.all_symmetric) ? lay.swT.get() : lay.swF.get();
-        st.t[k] = SliceTerm{lay.vtx.get(), sw, m.vals[(t.lat.d + 1) & 1].get(), t.lat.d + 1, t.ckind,
+        st.t[k] = SliceTerm{lay.vtx.get(), sw, m.vals[result_parity(m, t.lat.d)].get(), t.lat.d + 1, t.ckind,
                             t.d_compat.get(), kw ? kw + k : nullptr, mout ? mout + (size_t)k * NL : nullptr, m.ldv};
     }
     return st;
@@ -781,7 +808,7 @@ int compat_packed_len(int ckind, int L) {
 int stat_len(int ckind, int L) { return ckind == DCRF_COMPAT_MATRIX ? L * L : L; }
 
 // Splat, combine and blur every term's lattice: afterwards term k's filtered rows are
-// in vals[(d_k+1) & 1].  `transpose` selects the transposed filter (reverse blur
+// in vals[result_parity(m, d_k)].  `transpose` selects the transposed filter (reverse blur
 // order and the transpose's normaliser placement, pairwise.cpp:63-80).
 // Algorithmic bytes of the splat (input rows once; per term the (pixel, weight) pairs,
 // the normaliser and the vertex rows written) and of the per-pixel update (unary read,
@@ -811,6 +838,29 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
         Timed t(m, s, KC_COMBINE, 0.0);
         launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), m.ldv, L, s);
         t.done();
+    }
+    if (m.pairs) {
+        // two blur steps per pass; terms in layout order, pass p covers the terms with
+        // more than p passes (a prefix of the vertex space)
+        for (int p = 0; p < m.max_pairs; p++) {
+            PairTable pt{};
+            pt.nterms = (int)m.order.size();
+            double bytes = 0.0;
+            for (int i = 0; i < pt.nterms; i++) {
+                const int k = m.order[i];
+                const TermLattice& lat = m.terms[k]->lat;
+                const bool on = blur_pairs(lat.d) > p;
+                const TermLayout& lay = *m.tl[k];
+                pt.t[i] = PairTerm{m.vbase[k], lat.V,
+                                   on ? (transpose ? lay.pairT.get() : lay.pairF.get()) + (size_t)p * lat.V * 3
+                                      : nullptr};
+                if (on) bytes += (double)lat.V * (2 * p + 1 < lat.d + 1 ? 2 : 1) * (2.0 * L * 4 + 8);
+            }
+            Timed t(m, s, KC_BLUR, bytes);
+            launch_blur_pair(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s);
+            t.done();
+        }
+        return;
     }
     const BlurTable bt = blur_table(m);
     for (int step = 0; step <= m.max_d; step++) {
@@ -1558,6 +1608,17 @@ int dcrf_set_meanfield_kernel(dcrf_model* m, int mode, int run_rows) {
             m->mf_mode = mode;
             m->run_rows = rr;
             m->layout_valid = false;
+        }
+    });
+}
+
+int dcrf_set_blur_pairs(dcrf_model* m, int on) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        if (m->pairs != (on != 0)) {
+            m->pairs = on != 0;
+            m->layout_valid = false;
+            m->train_n = -1;
         }
     });
 }

@@ -308,6 +308,74 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
     store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
 }
 
+// Two blur steps in one pass (PairTable): y_u = x_u + 0.5 (x_{n1_a(u)} + x_{n2_a(u)})
+// for the three rows u of the pass, then out = y_v + 0.5 (y_{n1_b(v)} + y_{n2_b(v)}),
+// with the single step's operation order: the same bits as two k_blur launches (the
+// zero row gives y = 0 for an absent vertex, as the intermediate buffer's zero row).
+template <bool WAVE, int R>
+__global__ void __launch_bounds__(1024) k_blur_pair(const float* __restrict__ src, float* __restrict__ dst, int v0,
+                                                    int Vp, int ldv, int L, int S, int per, PairTable pt) {
+    LaneGroup<WAVE, R> G(L, S, per, nullptr);
+    const int v = v0 + G.item();  // vertices [v0, Vp)
+    if (!G.on || v >= Vp) return;
+    const int k0 = G.k0;
+    int vb = pt.t[0].vbase;
+    const int3* tab = pt.t[0].tab;
+#pragma unroll
+    for (int k = 1; k < DCRF_MAX_TERMS; k++) {
+        if (k < pt.nterms && v >= pt.t[k].vbase) {
+            vb = pt.t[k].vbase;
+            tab = pt.t[k].tab;
+        }
+    }
+    int3 e[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) e[i] = tab[(size_t)(v - vb) * 3 + i];
+    float x[3][3][R];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        load_piece<R>(src + (size_t)e[i].x * ldv, k0, L, x[i][0]);
+        load_piece<R>(src + (size_t)e[i].y * ldv, k0, L, x[i][1]);
+        load_piece<R>(src + (size_t)e[i].z * ldv, k0, L, x[i][2]);
+    }
+    float out[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float y[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float s2 = x[i][1][r] + x[i][2][r];
+            const float h = 0.5f * s2;
+            y[i] = x[i][0][r] + h;
+        }
+        const float s2 = y[1] + y[2];
+        const float h = 0.5f * s2;
+        out[r] = y[0] + h;
+    }
+    store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
+}
+
+__global__ void k_pair_table(const int2* __restrict__ nbr, int V, int vbase, int zero_row, int a, int b,
+                             int3* __restrict__ tab) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int u[3];
+    u[0] = vbase + v;
+    if (b >= 0) {
+        const int2 nb = nbr[(size_t)b * V + v];
+        u[1] = nb.x;
+        u[2] = nb.y;
+    } else {
+        u[1] = zero_row;
+        u[2] = zero_row;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int2 na = u[i] == zero_row ? make_int2(zero_row, zero_row) : nbr[(size_t)a * V + (u[i] - vbase)];
+        tab[(size_t)v * 3 + i] = make_int3(u[i], na.x, na.y);
+    }
+}
+
 // Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
 // (densecrf.cpp:122-129, with the per-term weight w_k = 1 unless set).
 template <bool WAVE, int R, bool CWL>
@@ -852,6 +920,19 @@ void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, c
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_blur, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, bt, step,
                   reverse);
+}
+
+void launch_blur_pair(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PairTable& pt,
+                      hipStream_t s) {
+    if (Vp <= v0) return;
+    const LaunchShape sh = shape_for(L);
+    DCRF_DISPATCH(sh, k_blur_pair, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, pt);
+}
+
+void launch_pair_table(const int2* nbr, int V, int vbase, int zero_row, int a, int b, int3* tab, hipStream_t s) {
+    if (V <= 0) return;
+    k_pair_table<<<blocks_for(V), kThreads, 0, s>>>(nbr, V, vbase, zero_row, a, b, tab);
+    DCRF_HIP_CHECK(hipGetLastError());
 }
 
 bool fused_meanfield_ok(int L, int nterms, int d1_0, int d1_1) {

@@ -274,3 +274,35 @@ def test_autotuned_chunk_keeps_results(tmp_path, monkeypatch):
     assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL
     again, t2 = tune.tune_splat_chunk(g)
     assert again == best and t2 is None  # cached
+
+
+@pytest.mark.parametrize("dims", [(2, 5), (1, 3), (4, 6), (7,)])
+def test_blur_pairs_bitwise(dims):
+    """Two blur steps per pass (the default) against one pass per step: the same bits for
+    inference, the transposed and forward messages, and the gradient (which runs the
+    transposed filter), for feature dimensions with an odd and an even number of steps,
+    mixed in one model."""
+    from pydensecrf_amd import capi
+    rng = np.random.default_rng(sum(dims))
+    N, L = 3000, 5
+    U = rng.standard_normal((L, N)).astype(np.float32)
+    gt = rng.integers(0, L, N).astype(np.int16)
+    feats = [(rng.standard_normal((d, N)) * 1.5).astype(np.float32) for d in dims]
+    out = []
+    for pairs in (True, False):
+        g = capi.Model.create(N, L)
+        g.set_unary(U)
+        for i, f in enumerate(feats):
+            g.add_pairwise(f, 2.0 + i, ntype=[SYM, AFTER, BEFORE, NONE][i % 4])
+        g.set_blur_pairs(pairs)
+        Q = g.inference(4)
+        msgs = [g.apply(k, Q, tr) for k in range(len(feats)) for tr in (False, True)]
+        grads = g.gradient(3, gt)
+        out.append((Q, msgs, grads))
+    (Qa, ma, ga), (Qb, mb, gb) = out
+    np.testing.assert_array_equal(Qa, Qb)
+    for x, y in zip(ma, mb):
+        np.testing.assert_array_equal(x, y)
+    assert ga[0] == gb[0]
+    for x, y in zip(ga[1:], gb[1:]):
+        np.testing.assert_array_equal(x, y)
