@@ -31,6 +31,7 @@ void set_error(int code, const std::string& msg) {
 namespace {
 
 constexpr int kDefaultChunk = 32;
+constexpr int kCombineSplit = 16;  // longest serial chain of partial rows in one combine
 
 struct TermLayout {
     DevBuf<int2> nbr;    // [(d+1)][V] global rows
@@ -72,12 +73,12 @@ struct dcrf_model {
 
     // global filter layout
     bool layout_valid = false;
-    int V_total = 0, rows0 = 0, E_total = 0, C_total = 0, CB_total = 0, max_d = 0;
+    int V_total = 0, rows0 = 0, E_total = 0, C_total = 0, CB_total = 0, CB2_total = 0, max_d = 0;
     std::vector<int> order, vbase;
     std::vector<int> prefixV;
     DevBuf<float> vals[2];
     DevBuf<Chunk> g_chunks;
-    DevBuf<Combine> g_comb;
+    DevBuf<Combine> g_comb, g_comb2;  // combine pieces, then the second level of split vertices
     DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
     bool tiled = false;                // splat through LDS tiles (else the global form)
     size_t lds_cap = 0;                // LDS bytes one workgroup may allocate
@@ -651,16 +652,35 @@ void ensure_layout(dcrf_model& m) {
     gte[NT] = e;
     if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
     require_adjoining(gch);
+    // Vertices split into more than kCombineSplit segments are recombined in two levels:
+    // pieces of about kCombineSplit partial rows into second-level rows (appended after
+    // the partial rows), then those into the vertex row, so that no lane group sums a
+    // long serial chain (the heaviest bilateral vertices at 1024x768 have ~245 rows).
+    std::vector<Combine> gcb2;
     for (int k : m.order)
         for (Combine c : m.terms[k]->lat.combines) {
             c.v += m.vbase[k];
             c.first += pbase[k];
-            gcb.push_back(c);
+            if (c.count <= kCombineSplit) {
+                gcb.push_back(c);
+                continue;
+            }
+            const int pieces = (c.count + kCombineSplit - 1) / kCombineSplit;
+            const int row2 = m.rows0;
+            m.rows0 += pieces;
+            for (int j = 0, off = 0; j < pieces; j++) {
+                const int n = (c.count - off) / (pieces - j);
+                gcb.push_back(Combine{row2 + j, c.first + off, n, 0});
+                off += n;
+            }
+            gcb2.push_back(Combine{c.v, row2, pieces, 0});
         }
     m.C_total = (int)gch.size();
     m.CB_total = (int)gcb.size();
+    m.CB2_total = (int)gcb2.size();
     upload(m.g_chunks, gch, s);
     upload(m.g_comb, gcb, s);
+    upload(m.g_comb2, gcb2, s);
     upload(m.g_tile_seg, gts, s);
     upload(m.g_tile_e, gte, s);
     const TileGeom tg = tile_geom(m);
@@ -837,6 +857,7 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
     if (m.CB_total) {
         Timed t(m, s, KC_COMBINE, 0.0);
         launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), m.ldv, L, s);
+        if (m.CB2_total) launch_combine(m.g_comb2.get(), m.CB2_total, m.vals[0].get(), m.ldv, L, s);
         t.done();
     }
     if (m.pairs) {
