@@ -872,10 +872,12 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
                 const TermLattice& lat = m.terms[k]->lat;
                 const bool on = blur_pairs(lat.d) > p;
                 const TermLayout& lay = *m.tl[k];
+                const int steps = 2 * p + 1 < lat.d + 1 ? 2 : 1;
                 pt.t[i] = PairTerm{m.vbase[k], lat.V,
                                    on ? (transpose ? lay.pairT.get() : lay.pairF.get()) + (size_t)p * lat.V * 3
-                                      : nullptr};
-                if (on) bytes += (double)lat.V * (2 * p + 1 < lat.d + 1 ? 2 : 1) * (2.0 * L * 4 + 8);
+                                      : nullptr,
+                                   steps == 1 ? 1 : 0};
+                if (on) bytes += (double)lat.V * steps * (2.0 * L * 4 + 8);
             }
             Timed t(m, s, KC_BLUR, bytes);
             launch_blur_pair(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s);

@@ -214,6 +214,7 @@ struct PairTerm {
     int vbase;        // first global vertex
     int V;
     const int3* tab;  // [V][3]
+    int single;       // the pass has one step: only tab[3v] is read
 };
 struct PairTable {
     int nterms;

@@ -312,6 +312,7 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
 // for the three rows u of the pass, then out = y_v + 0.5 (y_{n1_b(v)} + y_{n2_b(v)}),
 // with the single step's operation order: the same bits as two k_blur launches (the
 // zero row gives y = 0 for an absent vertex, as the intermediate buffer's zero row).
+// A term whose pass has one step (the last of an odd count) reads only its 3 rows.
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_blur_pair(const float* __restrict__ src, float* __restrict__ dst, int v0,
                                                     int Vp, int ldv, int L, int S, int per, PairTable pt) {
@@ -319,14 +320,31 @@ __global__ void __launch_bounds__(1024) k_blur_pair(const float* __restrict__ sr
     const int v = v0 + G.item();  // vertices [v0, Vp)
     if (!G.on || v >= Vp) return;
     const int k0 = G.k0;
-    int vb = pt.t[0].vbase;
+    int vb = pt.t[0].vbase, single = pt.t[0].single;
     const int3* tab = pt.t[0].tab;
 #pragma unroll
     for (int k = 1; k < DCRF_MAX_TERMS; k++) {
         if (k < pt.nterms && v >= pt.t[k].vbase) {
             vb = pt.t[k].vbase;
             tab = pt.t[k].tab;
+            single = pt.t[k].single;
         }
+    }
+    if (single) {
+        // one step: out = x_v + 0.5 (x_n1 + x_n2), k_blur's operation order
+        const int3 e = tab[(size_t)(v - vb) * 3];
+        float o[R], a[R], b[R], out[R];
+        load_piece<R>(src + (size_t)e.x * ldv, k0, L, o);
+        load_piece<R>(src + (size_t)e.y * ldv, k0, L, a);
+        load_piece<R>(src + (size_t)e.z * ldv, k0, L, b);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float s2 = a[r] + b[r];
+            const float h = 0.5f * s2;
+            out[r] = o[r] + h;
+        }
+        store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
+        return;
     }
     int3 e[3];
 #pragma unroll
