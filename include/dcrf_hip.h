@@ -213,8 +213,8 @@ int dcrf_set_splat_tile(dcrf_model *m, int side);
  * counterpart in the reference (a kernel choice for testing and benchmarking). */
 int dcrf_set_meanfield_kernel(dcrf_model *m, int mode, int run_rows);
 /* No counterpart in the reference (a scheduling choice; results are bit-identical):
- * on (default) blurs two lattice axes per pass through a 9-row table per vertex, off
- * runs one pass per axis. */
+ * on (default) blurs two lattice axes per pass through a table of the 9 rows each
+ * vertex's two steps read, off runs one pass per axis. */
 int dcrf_set_blur_pairs(dcrf_model *m, int on);
 /* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
  * (default 0.5): lower trades memory for shorter probe sequences. */

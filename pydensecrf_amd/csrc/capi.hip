@@ -35,7 +35,10 @@ constexpr int kCombineSplit = 16;  // longest serial chain of partial rows in on
 
 struct TermLayout {
     DevBuf<int2> nbr;    // [(d+1)][V] global rows
-    DevBuf<int3> pairF, pairT;  // [blur_pairs(d)][V][3] two-step pass tables, forward / reverse
+    // multi-step blur passes: steps per pass, table offsets (ints) per pass, tables
+    // forward / reverse
+    std::vector<int> pass_steps, pass_off;
+    DevBuf<int> passF, passT;
     DevBuf<int> vtx;     // [N][d+1] global rows
     DevBuf<float> swF;   // slice weights, forward
     DevBuf<float> swT;   // slice weights, transpose (unused when symmetric)
@@ -88,8 +91,8 @@ struct dcrf_model {
     DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
     bool runs = false;            // ... or through k_meanfield_runs (rows staged per pixel run)
     int mf_mode = 0;              // dcrf_set_meanfield_kernel: 0 auto (fused), 1 generic, 2 fused, 3 runs
-    bool pairs = true;            // blur two steps per pass (dcrf_set_blur_pairs)
-    int max_pairs = 0;
+    bool pairs = true;            // blur up to three steps per pass (dcrf_set_blur_pairs)
+    int n_passes = 0;             // launches per filter with multi-step passes
     std::vector<int> prefixP;     // vertices of the terms with more than p passes
     int run_rows = 16;            // pixel rows of a run rectangle
     RunSlice rslice;
@@ -583,9 +586,29 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
 
 SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw = nullptr, float* mout = nullptr);
 
-// The vertex buffer holding a term's filtered rows after the global filter: one
-// buffer switch per blur step, or per two-step pass.
-int result_parity(const dcrf_model& m, int d) { return (m.pairs ? blur_pairs(d) : d + 1) & 1; }
+// The vertex buffer holding term k's filtered rows after the global filter: one buffer
+// switch per blur step, or per multi-step pass.
+int result_parity(const dcrf_model& m, int k) {
+    return (m.pairs ? (int)m.tl[k]->pass_steps.size() : m.terms[k]->lat.d + 1) & 1;
+}
+
+// Multi-step blur schedule: term k runs its d_k+1 steps in ceil((d_k+1)/K) passes of at
+// most K steps, spread as evenly as possible (longer passes first); the number of passes
+// falls with d, so pass p covers a prefix of the vertex space.  K = 2: passes of three
+// steps (27 rows per vertex) measured slower at 1024x768x21 (49 us against 28 us of
+// blur per iteration), as the 27 gathers per vertex cost more than the launch saved.
+constexpr int kMaxPassSteps = 2;
+int blur_launches(const dcrf_model& m) {
+    int P = 0;
+    for (const auto& t : m.terms) P = std::max(P, (t->lat.d + kMaxPassSteps) / kMaxPassSteps);
+    return P;
+}
+std::vector<int> pass_schedule(int steps) {
+    const int n = (steps + kMaxPassSteps - 1) / kMaxPassSteps;
+    std::vector<int> out(n, steps / n);
+    for (int i = 0; i < steps % n; i++) out[i]++;
+    return out;
+}
 
 void ensure_layout(dcrf_model& m) {
     if (m.layout_valid) return;
@@ -718,17 +741,23 @@ void ensure_layout(dcrf_model& m) {
         lay->nbr.reserve((size_t)d1 * lat.V);
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
         if (m.pairs) {
-            // pass p: axes (2p, 2p+1) forward, (d-2p, d-2p-1) for the transpose
-            const int np = blur_pairs(lat.d);
-            lay->pairF.reserve((size_t)np * lat.V * 3);
-            lay->pairT.reserve((size_t)np * lat.V * 3);
-            for (int p = 0; p < np; p++) {
-                const int fa = 2 * p, fb = 2 * p + 1 <= lat.d ? 2 * p + 1 : -1;
-                const int ta = lat.d - 2 * p, tb = lat.d - 2 * p - 1 >= 0 ? lat.d - 2 * p - 1 : -1;
-                launch_pair_table(lay->nbr.get(), lat.V, m.vbase[k], V, fa, fb, lay->pairF.get() + (size_t)p * lat.V * 3,
-                                  s);
-                launch_pair_table(lay->nbr.get(), lat.V, m.vbase[k], V, ta, tb, lay->pairT.get() + (size_t)p * lat.V * 3,
-                                  s);
+            // forward passes walk axes 0..d, the transpose's d..0 (permutohedral.cpp:556)
+            lay->pass_steps = pass_schedule(lat.d + 1);
+            lay->pass_off.assign(1, 0);
+            for (int q : lay->pass_steps) lay->pass_off.push_back(lay->pass_off.back() + lat.V * (q == 1 ? 3 : q == 2 ? 9 : 27));
+            lay->passF.reserve(lay->pass_off.back());
+            lay->passT.reserve(lay->pass_off.back());
+            int first = 0;
+            for (size_t p = 0; p < lay->pass_steps.size(); p++) {
+                const int q = lay->pass_steps[p];
+                int fa[3], ta[3];
+                for (int j = 0; j < q; j++) {
+                    fa[j] = first + j;
+                    ta[j] = lat.d - first - j;
+                }
+                launch_pass_table(lay->nbr.get(), lat.V, m.vbase[k], V, q, fa, lay->passF.get() + lay->pass_off[p], s);
+                launch_pass_table(lay->nbr.get(), lat.V, m.vbase[k], V, q, ta, lay->passT.get() + lay->pass_off[p], s);
+                first += q;
             }
         }
         lay->vtx.reserve(Ek);
@@ -765,7 +794,7 @@ void ensure_layout(dcrf_model& m) {
             for (int k = 0; k < T; k++) {
                 const int d1 = m.terms[k]->lat.d + 1;
                 for (int q = 0; q < d1; q++, j++)
-                    if (result_parity(m, m.terms[k]->lat.d)) par |= 1u << j;
+                    if (result_parity(m, k)) par |= 1u << j;
             }
             // rectangles of 2 waves' pixels per row x 16 rows (18 x 16 at 21 labels)
             // rectangles two waves of pixels wide, at most run_rows high and at most
@@ -785,11 +814,11 @@ void ensure_layout(dcrf_model& m) {
     for (int st = 0; st <= m.max_d; st++)
         for (int k = 0; k < T; k++)
             if (m.terms[k]->lat.d >= st) m.prefixV[st] += m.terms[k]->lat.V;
-    m.max_pairs = m.max_d >= 0 ? blur_pairs(m.max_d) : 0;
-    m.prefixP.assign(m.max_pairs, 0);
-    for (int p = 0; p < m.max_pairs; p++)
+    m.n_passes = m.pairs ? blur_launches(m) : 0;
+    m.prefixP.assign(m.n_passes, 0);
+    for (int p = 0; p < m.n_passes; p++)
         for (int k = 0; k < T; k++)
-            if (blur_pairs(m.terms[k]->lat.d) > p) m.prefixP[p] += m.terms[k]->lat.V;
+            if ((int)m.tl[k]->pass_steps.size() > p) m.prefixP[p] += m.terms[k]->lat.V;
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
     m.layout_valid = true;
 }
@@ -813,7 +842,7 @@ SliceTable slice_table(const dcrf_model& m, bool transpose, const float* kw, flo
         const Term& t = *m.terms[k];
         const TermLayout& lay = *m.tl[k];
         const float* sw = (transpose && !m.all_symmetric) ? lay.swT.get() : lay.swF.get();
-        st.t[k] = SliceTerm{lay.vtx.get(), sw, m.vals[result_parity(m, t.lat.d)].get(), t.lat.d + 1, t.ckind,
+        st.t[k] = SliceTerm{lay.vtx.get(), sw, m.vals[result_parity(m, k)].get(), t.lat.d + 1, t.ckind,
                             t.d_compat.get(), kw ? kw + k : nullptr, mout ? mout + (size_t)k * NL : nullptr, m.ldv};
     }
     return st;
@@ -861,26 +890,25 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
         t.done();
     }
     if (m.pairs) {
-        // two blur steps per pass; terms in layout order, pass p covers the terms with
-        // more than p passes (a prefix of the vertex space)
-        for (int p = 0; p < m.max_pairs; p++) {
-            PairTable pt{};
+        // up to three blur steps per pass; terms in layout order, pass p covers the
+        // terms with more than p passes (a prefix of the vertex space)
+        for (int p = 0; p < m.n_passes; p++) {
+            PassTable pt{};
             pt.nterms = (int)m.order.size();
             double bytes = 0.0;
             for (int i = 0; i < pt.nterms; i++) {
                 const int k = m.order[i];
                 const TermLattice& lat = m.terms[k]->lat;
-                const bool on = blur_pairs(lat.d) > p;
                 const TermLayout& lay = *m.tl[k];
-                const int steps = 2 * p + 1 < lat.d + 1 ? 2 : 1;
-                pt.t[i] = PairTerm{m.vbase[k], lat.V,
-                                   on ? (transpose ? lay.pairT.get() : lay.pairF.get()) + (size_t)p * lat.V * 3
-                                      : nullptr,
-                                   steps == 1 ? 1 : 0};
+                const bool on = (int)lay.pass_steps.size() > p;
+                const int steps = on ? lay.pass_steps[p] : 1;
+                pt.t[i] = PassTerm{m.vbase[k], lat.V,
+                                   on ? (transpose ? lay.passT.get() : lay.passF.get()) + lay.pass_off[p] : nullptr,
+                                   steps};
                 if (on) bytes += (double)lat.V * steps * (2.0 * L * 4 + 8);
             }
             Timed t(m, s, KC_BLUR, bytes);
-            launch_blur_pair(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s);
+            launch_blur_pass(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s);
             t.done();
         }
         return;

@@ -206,22 +206,19 @@ struct BlurTable {
     BlurTerm t[DCRF_MAX_TERMS];
 };
 
-// Two consecutive blur steps (axis a, then axis b) as one pass: per vertex v the rows
-// u in {v, n1_b(v), n2_b(v)} and their axis-a neighbours, 9 global rows (zero row for
-// an absent vertex), as tab[3v + i] = (u_i, n1_a(u_i), n2_a(u_i)).  A pass with no
-// second step lists the zero row for u_1 and u_2.
-struct PairTerm {
+// A pass of K = 1..3 consecutive blur steps of one term: per vertex v the 3^K rows of
+// the step tree (v and its two neighbours along the pass's last axis, each expanded
+// along the axis before, ...), zero row for an absent vertex, at tab[v * 3^K ..].
+struct PassTerm {
     int vbase;        // first global vertex
     int V;
-    const int3* tab;  // [V][3]
-    int single;       // the pass has one step: only tab[3v] is read
+    const int* tab;   // [V][3^steps]
+    int steps;        // K
 };
-struct PairTable {
+struct PassTable {
     int nterms;
-    PairTerm t[DCRF_MAX_TERMS];
+    PassTerm t[DCRF_MAX_TERMS];
 };
-// Number of two-step passes of a (d+1)-step blur.
-inline int blur_pairs(int d) { return (d + 2) / 2; }
 
 // One term as seen by the per-pixel kernels (slice + compatibility).
 struct SliceTerm {
@@ -283,12 +280,13 @@ void launch_combine(const Combine* c, int n, float* vals, int ldv, int L, hipStr
 // One blur step over the vertices [v0, Vp) (a range of the terms with d >= step).
 void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);
-// One two-step pass over the vertices [v0, Vp): bit-identical to the two blur steps.
-void launch_blur_pair(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PairTable& pt,
+// One multi-step pass over the vertices [v0, Vp): bit-identical to its blur steps.
+void launch_blur_pass(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PassTable& pt,
                       hipStream_t s);
-// The 9-row table of pass (a, b) of one term (b < 0: a single step) from its global
-// neighbour rows nbr [(d+1)][V].
-void launch_pair_table(const int2* nbr, int V, int vbase, int zero_row, int a, int b, int3* tab, hipStream_t s);
+// The table of a K-step pass (axes[0] first) of one term from its global neighbour
+// rows nbr [(d+1)][V].
+void launch_pass_table(const int2* nbr, int V, int vbase, int zero_row, int K, const int* axes, int* tab,
+                       hipStream_t s);
 // Mean-field update: E = -U - sum_k compat_k(slice_k); Q = softmax(E).  U may be null.
 // tmp1/tmp2 optional outputs (energy, last term's message).
 void launch_meanfield(const float* U, float* Q, int N, int L, const SliceTable& st, float* tmp1,

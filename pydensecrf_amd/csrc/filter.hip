@@ -308,90 +308,97 @@ __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, fl
     store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
 }
 
-// Two blur steps in one pass (PairTable): y_u = x_u + 0.5 (x_{n1_a(u)} + x_{n2_a(u)})
-// for the three rows u of the pass, then out = y_v + 0.5 (y_{n1_b(v)} + y_{n2_b(v)}),
-// with the single step's operation order: the same bits as two k_blur launches (the
-// zero row gives y = 0 for an absent vertex, as the intermediate buffer's zero row).
-// A term whose pass has one step (the last of an odd count) reads only its 3 rows.
+// A pass of K = 1 or 2 consecutive blur steps (axes a_1 .. a_K) of one term.  Each step
+// is x' = x + 0.5 (x_n1 + x_n2) in k_blur's operation order; the pass table lists per
+// vertex v the 3^K rows of the step tree: level K holds v and its two neighbours along
+// a_K, each expanded along a_{K-1}, and so on, zero row for an absent vertex (whose
+// value is 0 at every level, as in the intermediate buffers of single steps).  The
+// kernel loads the 3^K rows together and folds them level by level: the same bits as
+// K launches of k_blur, with K-1 fewer launches and no intermediate rows written.
+__device__ __forceinline__ float blur_fold(float o, float a, float b) {
+    const float s2 = a + b;
+    const float h = 0.5f * s2;
+    return o + h;
+}
+
+template <int K, int R>
+__device__ __forceinline__ void blur_pass_vertex(const float* __restrict__ src, float* __restrict__ dst,
+                                                 const int* __restrict__ e, int v, int ldv, int k0, int L) {
+    constexpr int NR = K == 1 ? 3 : (K == 2 ? 9 : 27);
+    int idx[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) idx[i] = e[i];
+    float x[NR][R];
+#pragma unroll
+    for (int i = 0; i < NR; i++) load_piece<R>(src + (size_t)idx[i] * ldv, k0, L, x[i]);
+    float out[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float y[NR];
+#pragma unroll
+        for (int i = 0; i < NR; i++) y[i] = x[i][r];
+        int n = NR;
+#pragma unroll
+        for (int lev = 0; lev < K; lev++) {
+#pragma unroll
+            for (int i = 0; i < NR / 3; i++)
+                if (i < n / 3) y[i] = blur_fold(y[3 * i], y[3 * i + 1], y[3 * i + 2]);
+            n /= 3;
+        }
+        out[r] = y[0];
+    }
+    store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
+}
+
 template <bool WAVE, int R>
-__global__ void __launch_bounds__(1024) k_blur_pair(const float* __restrict__ src, float* __restrict__ dst, int v0,
-                                                    int Vp, int ldv, int L, int S, int per, PairTable pt) {
+__global__ void __launch_bounds__(1024) k_blur_pass(const float* __restrict__ src, float* __restrict__ dst, int v0,
+                                                    int Vp, int ldv, int L, int S, int per, PassTable pt) {
     LaneGroup<WAVE, R> G(L, S, per, nullptr);
     const int v = v0 + G.item();  // vertices [v0, Vp)
     if (!G.on || v >= Vp) return;
     const int k0 = G.k0;
-    int vb = pt.t[0].vbase, single = pt.t[0].single;
-    const int3* tab = pt.t[0].tab;
+    int vb = pt.t[0].vbase, steps = pt.t[0].steps;
+    const int* tab = pt.t[0].tab;
 #pragma unroll
     for (int k = 1; k < DCRF_MAX_TERMS; k++) {
         if (k < pt.nterms && v >= pt.t[k].vbase) {
             vb = pt.t[k].vbase;
             tab = pt.t[k].tab;
-            single = pt.t[k].single;
+            steps = pt.t[k].steps;
         }
     }
-    if (single) {
-        // one step: out = x_v + 0.5 (x_n1 + x_n2), k_blur's operation order
-        const int3 e = tab[(size_t)(v - vb) * 3];
-        float o[R], a[R], b[R], out[R];
-        load_piece<R>(src + (size_t)e.x * ldv, k0, L, o);
-        load_piece<R>(src + (size_t)e.y * ldv, k0, L, a);
-        load_piece<R>(src + (size_t)e.z * ldv, k0, L, b);
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const float s2 = a[r] + b[r];
-            const float h = 0.5f * s2;
-            out[r] = o[r] + h;
-        }
-        store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
-        return;
-    }
-    int3 e[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) e[i] = tab[(size_t)(v - vb) * 3 + i];
-    float x[3][3][R];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        load_piece<R>(src + (size_t)e[i].x * ldv, k0, L, x[i][0]);
-        load_piece<R>(src + (size_t)e[i].y * ldv, k0, L, x[i][1]);
-        load_piece<R>(src + (size_t)e[i].z * ldv, k0, L, x[i][2]);
-    }
-    float out[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        float y[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const float s2 = x[i][1][r] + x[i][2][r];
-            const float h = 0.5f * s2;
-            y[i] = x[i][0][r] + h;
-        }
-        const float s2 = y[1] + y[2];
-        const float h = 0.5f * s2;
-        out[r] = y[0] + h;
-    }
-    store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
+    // passes of three steps (27 rows) measured slower than two passes of up to two:
+    // the pass schedule never asks for them and the kernel does not instantiate them
+    const size_t lv = (size_t)(v - vb);
+    if (steps == 2)
+        blur_pass_vertex<2, R>(src, dst, tab + lv * 9, v, ldv, k0, L);
+    else
+        blur_pass_vertex<1, R>(src, dst, tab + lv * 3, v, ldv, k0, L);
 }
 
-__global__ void k_pair_table(const int2* __restrict__ nbr, int V, int vbase, int zero_row, int a, int b,
-                             int3* __restrict__ tab) {
+// Pass table of one term: entry (i_K, .., i_1) of vertex v at v*3^K + sum i_j 3^(j-1),
+// expanding along axis ax[j-1] at level j (ax[K-1] first, the pass's last step).
+__global__ void k_pass_table(const int2* __restrict__ nbr, int V, int vbase, int zero_row, int K, int ax0, int ax1,
+                             int ax2, int* __restrict__ tab) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    int u[3];
-    u[0] = vbase + v;
-    if (b >= 0) {
-        const int2 nb = nbr[(size_t)b * V + v];
-        u[1] = nb.x;
-        u[2] = nb.y;
-    } else {
-        u[1] = zero_row;
-        u[2] = zero_row;
+    const int ax[3] = {ax0, ax1, ax2};
+    int cur[27], nxt[27];
+    cur[0] = vbase + v;
+    int n = 1;
+    for (int lev = K - 1; lev >= 0; lev--) {
+        // expand every row of the current level along axis ax[lev]: (u, n1(u), n2(u))
+        for (int i = 0; i < n; i++) {
+            const int u = cur[i];
+            const int2 nb = u == zero_row ? make_int2(zero_row, zero_row) : nbr[(size_t)ax[lev] * V + (u - vbase)];
+            nxt[3 * i] = u;
+            nxt[3 * i + 1] = nb.x;
+            nxt[3 * i + 2] = nb.y;
+        }
+        n *= 3;
+        for (int i = 0; i < n; i++) cur[i] = nxt[i];
     }
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int2 na = u[i] == zero_row ? make_int2(zero_row, zero_row) : nbr[(size_t)a * V + (u[i] - vbase)];
-        tab[(size_t)v * 3 + i] = make_int3(u[i], na.x, na.y);
-    }
+    for (int i = 0; i < n; i++) tab[(size_t)v * n + i] = cur[i];
 }
 
 // Mean-field update of one pixel: E = -U - sum_k C_k(w_k slice_k), Q = softmax(-E)
@@ -940,16 +947,19 @@ void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, c
                   reverse);
 }
 
-void launch_blur_pair(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PairTable& pt,
+void launch_blur_pass(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PassTable& pt,
                       hipStream_t s) {
     if (Vp <= v0) return;
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH(sh, k_blur_pair, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, pt);
+    DCRF_DISPATCH(sh, k_blur_pass, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, pt);
 }
 
-void launch_pair_table(const int2* nbr, int V, int vbase, int zero_row, int a, int b, int3* tab, hipStream_t s) {
+void launch_pass_table(const int2* nbr, int V, int vbase, int zero_row, int K, const int* axes, int* tab,
+                       hipStream_t s) {
     if (V <= 0) return;
-    k_pair_table<<<blocks_for(V), kThreads, 0, s>>>(nbr, V, vbase, zero_row, a, b, tab);
+    if (K < 1 || K > 3) throw Error{DCRF_ERR_STATE, "blur pass: 1 to 3 steps"};
+    k_pass_table<<<blocks_for(V), kThreads, 0, s>>>(nbr, V, vbase, zero_row, K, axes[0], K > 1 ? axes[1] : 0,
+                                                    K > 2 ? axes[2] : 0, tab);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
