@@ -143,6 +143,18 @@ int dcrf_set_kernel_weights(dcrf_model *m, const float *w);
  * Diagonal L, Matrix upper triangle W(i, j>=i) row by row). */
 int dcrf_num_compat_params(const dcrf_model *m);
 int dcrf_get_compat_params(const dcrf_model *m, float *params);
+/* LogisticUnaryEnergy (unary.cpp:48-68; DenseCRF::setUnaryEnergy(L, f), densecrf.cpp:92-94):
+ * U = Lmat feat with Lmat (num_labels, F) and feat (F, N), both row-major; the product
+ * runs on the device (matrix cores) and is redone when the parameters change.  Its
+ * parameters are Lmat packed column-major (unaryParameters / setUnaryParameters,
+ * densecrf.cpp:298-306, unary.cpp:53-62); a plain unary (dcrf_set_unary) has none.  After dcrf_gradient,
+ * dcrf_unary_param_gradient returns the reference's unary_grad for this unary,
+ * sum over iterations of b feat^T, packed like the parameters. */
+int dcrf_set_logistic_unary(dcrf_model *m, const float *Lmat, int F, const float *feat);
+int dcrf_num_unary_params(const dcrf_model *m);
+int dcrf_get_unary_params(const dcrf_model *m, float *params);
+int dcrf_set_unary_params(dcrf_model *m, const float *params);
+int dcrf_unary_param_gradient(const dcrf_model *m, float *grad);
 /* kernelParameters / setKernelParameters (densecrf.cpp:334-360, pairwise.cpp:129-151):
  * per term none (CONST_KERNEL), d scales (DIAG_KERNEL, ones at first) or a d x d matrix
  * stored column-major (FULL_KERNEL, the identity at first), concatenated over terms.
@@ -161,6 +173,17 @@ int dcrf_set_kernel_params(dcrf_model *m, const float *params);
  * Q_t and the messages K~_k Q_t are kept from the forward pass (no re-filtering). */
 int dcrf_gradient(dcrf_model *m, int n_iterations, const int16_t *gt, float robust, double *objective,
                   float *unary_grad, float *compat_grad, float *kweight_grad, float *kernel_grad);
+/* The same with the reference's other learning objectives (objective.h:34-75,
+ * objective.cpp:37-108): DCRF_OBJECTIVE_LOG_LIKELIHOOD (param = robust),
+ * DCRF_OBJECTIVE_HAMMING (class_weight[n_class_weight], or NULL and param =
+ * class_weight_pow: weights from the label counts of gt), DCRF_OBJECTIVE_IOU (mean
+ * intersection over union of the soft labelling).  objective = the objective's value. */
+#define DCRF_OBJECTIVE_LOG_LIKELIHOOD 0
+#define DCRF_OBJECTIVE_HAMMING 1
+#define DCRF_OBJECTIVE_IOU 2
+int dcrf_gradient_objective(dcrf_model *m, int n_iterations, int objective_kind, const int16_t *gt, float param,
+                            const float *class_weight, int n_class_weight, double *objective, float *unary_grad,
+                            float *compat_grad, float *kweight_grad, float *kernel_grad);
 /* Training forward on device buffers: like dcrf_inference_device, but keeps Q_0..Q_n
  * and the messages for dcrf_backward_device.  compat_dev: raw compatibility parameters
  * of all terms concatenated (NULL = the model's); kw_dev: [num_terms] term weights (NULL

@@ -948,9 +948,91 @@ double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, f
     return orc_crf_gradient_k(c, n_iter, gt, robust, unary_grad, compat_grad, NULL);
 }
 
-/* DenseCRF::gradient (densecrf.cpp:238-297) with LogLikelihood (objective.cpp:37-50). */
 double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
                           float *compat_grad, float *kernel_grad)
+{
+    return orc_crf_gradient_obj(c, n_iter, ORC_LOG_LIKELIHOOD, gt, robust, NULL, 0, unary_grad, compat_grad,
+                                kernel_grad);
+}
+
+/* ObjectiveFunction::evaluate (objective.cpp:37-108): returns the objective, b = d_mul_Q
+ * (zeroed first).  LogLikelihood (:37-50, param = robust); Hamming (:51-80, class weights
+ * cw[ncw] if given, else from param = class_weight_pow as in the constructor :51-63);
+ * IntersectionOverUnion (:81-108). */
+static double objective_f(int kind, const short *gt, float param, const float *cw_in, int ncw, const float *Q,
+                          int N, int L, float *b)
+{
+    memset(b, 0, sizeof(float) * (size_t)N * L);
+    double r = 0;
+    if (kind == ORC_LOG_LIKELIHOOD) {
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0 && gt[i] < L) {
+                const float q = Q[(size_t)i * L + gt[i]];
+                const float qr = q + param;
+                const float QQ = qr > 1e-20f ? qr : 1e-20f;
+                r += log((double)QQ) / N;
+                const float dq = q / QQ;
+                b[(size_t)i * L + gt[i]] = b[(size_t)i * L + gt[i]] + dq / (float)N;
+            }
+    } else if (kind == ORC_HAMMING) {
+        float *cw;
+        int M;
+        if (cw_in) {
+            M = ncw;
+            cw = (float *)malloc(sizeof(float) * (size_t)(M > 0 ? M : 1));
+            memcpy(cw, cw_in, sizeof(float) * (size_t)M);
+        } else {
+            M = 0;
+            for (int i = 0; i < N; i++)
+                if (gt[i] >= M) M = gt[i] + 1;
+            float *cnt = (float *)calloc((size_t)(M > 0 ? M : 1), sizeof(float));
+            cw = (float *)malloc(sizeof(float) * (size_t)(M > 0 ? M : 1));
+            for (int i = 0; i < N; i++)
+                if (gt[i] >= 0) cnt[gt[i]] = cnt[gt[i]] + 1.0f;
+            const float total = orc_sum_packet(cnt, M);
+            for (int l = 0; l < M; l++) cw[l] = powf(cnt[l] / total, -param);
+            float *prod = (float *)malloc(sizeof(float) * (size_t)(M > 0 ? M : 1));
+            for (int l = 0; l < M; l++) prod[l] = cnt[l] * cw[l];
+            const float z = orc_sum_packet(prod, M);
+            for (int l = 0; l < M; l++) cw[l] = cw[l] / z;
+            free(cnt);
+            free(prod);
+        }
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0 && gt[i] < L) {
+                const float QQ = cw[gt[i]] * Q[(size_t)i * L + gt[i]];
+                r += QQ;
+                b[(size_t)i * L + gt[i]] = b[(size_t)i * L + gt[i]] + QQ;
+            }
+        free(cw);
+    } else {
+        double *in = (double *)calloc((size_t)L, sizeof(double)), *un = (double *)malloc(sizeof(double) * L);
+        for (int l = 0; l < L; l++) un[l] = 1e-20;
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0 && gt[i] < L) {
+                in[gt[i]] += Q[(size_t)i * L + gt[i]];
+                un[gt[i]] += 1;
+                for (int l = 0; l < L; l++)
+                    if (l != gt[i]) un[l] += Q[(size_t)i * L + l];
+            }
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0 && gt[i] < L)
+                for (int l = 0; l < L; l++) {
+                    const double q = Q[(size_t)i * L + l];
+                    b[(size_t)i * L + l] = l == gt[i] ? (float)(q / (un[l] * L)) : (float)(-q * in[l] / (un[l] * un[l] * L));
+                }
+        for (int l = 0; l < L; l++) r += in[l] / un[l];
+        r /= L;
+        free(in);
+        free(un);
+    }
+    return r;
+}
+
+/* DenseCRF::gradient (densecrf.cpp:238-297) with the objectives above. */
+double orc_crf_gradient_obj(orc_crf *c, int n_iter, int objective, const short *gt, float param,
+                            const float *class_weight, int n_class_weight, float *unary_grad, float *compat_grad,
+                            float *kernel_grad)
 {
     const int N = c->N, L = c->L;
     const size_t n = (size_t)N * L;
@@ -968,19 +1050,8 @@ double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust,
         }
         softmax_f(tmp1, Q[it + 1], N, L);
     }
-    /* objective: r = sum log(max(Q(gt)+robust, 1e-20))/N, d_mul_Q(gt) += Q/QQ/N */
     const float *Qn = Q[n_iter];
-    double r = 0;
-    memset(b, 0, sizeof(float) * n);
-    for (int i = 0; i < N; i++)
-        if (gt[i] >= 0 && gt[i] < L) {
-            const float q = Qn[(size_t)i * L + gt[i]];
-            const float qr = q + robust;
-            const float QQ = qr > 1e-20f ? qr : 1e-20f;
-            r += log((double)QQ) / N;
-            const float dq = q / QQ;
-            b[(size_t)i * L + gt[i]] = b[(size_t)i * L + gt[i]] + dq / (float)N;
-        }
+    const double r = objective_f(objective, gt, param, class_weight, n_class_weight, Qn, N, L, b);
     sum_and_normalize_f(b, Qn, b, N, L);
     if (unary_grad) memcpy(unary_grad, b, sizeof(float) * n);
     const int ncp = orc_crf_num_compat_params(c);

@@ -81,6 +81,13 @@ double orc_crf_gradient(orc_crf *c, int n_iter, const short *gt, float robust, f
 double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust, float *unary_grad,
                           float *compat_grad, float *kernel_grad);
 int orc_crf_num_kernel_params(const orc_crf *c);
+/* The same with another objective (objective.cpp:37-108): ORC_LOG_LIKELIHOOD (param =
+ * robust), ORC_HAMMING (class_weight[n_class_weight], or NULL and param =
+ * class_weight_pow), ORC_IOU. */
+enum { ORC_LOG_LIKELIHOOD = 0, ORC_HAMMING = 1, ORC_IOU = 2 };
+double orc_crf_gradient_obj(orc_crf *c, int n_iter, int objective, const short *gt, float param,
+                            const float *class_weight, int n_class_weight, float *unary_grad, float *compat_grad,
+                            float *kernel_grad);
 /* Kernel parameters of all terms, packed like the reference (ones / identity at first);
  * setting them rebuilds each DIAG/FULL term's lattice from the scaled features. */
 void orc_crf_get_kernel_params(const orc_crf *c, float *p);

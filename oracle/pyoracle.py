@@ -21,6 +21,7 @@ REF_SO = os.path.join(HERE, "_ref", "libdcrf_ref.so")
 
 NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC = 0, 1, 2, 3
 CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL = 0, 1, 2
+LOG_LIKELIHOOD, HAMMING, IOU = 0, 1, 2
 POTTS, DIAGONAL, MATRIX = 0, 1, 2
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -115,6 +116,9 @@ def _oracle():
         lib.orc_crf_gradient_k.argtypes = [_P, ctypes.c_int, _S, ctypes.c_float, _F, _F, _F]
         lib.orc_crf_gradient_k.restype = ctypes.c_double
         lib.orc_crf_num_kernel_params.argtypes = [_P]
+        lib.orc_crf_gradient_obj.argtypes = [_P, ctypes.c_int, ctypes.c_int, _S, ctypes.c_float, _F, ctypes.c_int,
+                                             _F, _F, _F]
+        lib.orc_crf_gradient_obj.restype = ctypes.c_double
         lib.orc_crf_get_kernel_params.argtypes = [_P, _F]
         lib.orc_crf_set_kernel_params.argtypes = [_P, _F]
         lib.orc_lattice_gradient.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_int]
@@ -325,6 +329,20 @@ class OracleCRF(_CRFBase):
         r = self._lib.orc_crf_gradient_k(self._h, n, gt.ctypes.data_as(_S), robust, _fp(ug), _fp(cg), _fp(kg))
         return r, ug.T, cg, kg
 
+    def gradient_obj(self, n, gt, objective, param=0.0, class_weight=None, kernel=False):
+        """DenseCRF::gradient with objective LOG_LIKELIHOOD / HAMMING / IOU: (objective,
+        dL/dU (L, N), compat gradient, kernel-parameter gradient)."""
+        gt = np.ascontiguousarray(gt, dtype=np.int16)
+        ug = self._out()
+        cg = np.empty(max(self._lib.orc_crf_num_compat_params(self._h), 1), dtype=np.float32)
+        kg = np.empty(max(self._lib.orc_crf_num_kernel_params(self._h), 1), dtype=np.float32)
+        cw = None if class_weight is None else _c32(class_weight)
+        r = self._lib.orc_crf_gradient_obj(self._h, n, objective, gt.ctypes.data_as(_S), param,
+                                           None if cw is None else _fp(cw), 0 if cw is None else cw.size,
+                                           _fp(ug), _fp(cg), _fp(kg) if kernel else None)
+        return (r, ug.T, cg[:self._lib.orc_crf_num_compat_params(self._h)],
+                kg[:self._lib.orc_crf_num_kernel_params(self._h)] if kernel else None)
+
     def ce_fp64(self, n, gt, U_LxN=None, compat_raw=None, kweights=None):
         gt = np.ascontiguousarray(gt, dtype=np.int16)
         U = None if U_LxN is None else np.ascontiguousarray(np.asarray(U_LxN, dtype=np.float64).T)
@@ -373,6 +391,13 @@ def _ref():
         lib.ref_crf_set_kernel_params.argtypes = [_P, _F, ctypes.c_int]
         lib.ref_crf_gradient.argtypes = [_P, ctypes.c_int, _S, ctypes.c_float, _F, _F, _F]
         lib.ref_crf_gradient.restype = ctypes.c_double
+        lib.ref_crf_gradient_obj.argtypes = [_P, ctypes.c_int, ctypes.c_int, _S, ctypes.c_float, _F, ctypes.c_int,
+                                             _F, _F, _F]
+        lib.ref_crf_gradient_obj.restype = ctypes.c_double
+        lib.ref_crf_set_unary_logistic.argtypes = [_P, _F, ctypes.c_int, _F]
+        lib.ref_crf_num_unary_params.argtypes = [_P]
+        lib.ref_crf_get_unary_params.argtypes = [_P, _F]
+        lib.ref_crf_set_unary_params.argtypes = [_P, _F, ctypes.c_int]
         lib._typed = True
     return lib
 
@@ -504,6 +529,34 @@ class RefCRF(_CRFBase):
     def set_kernel_params(self, p):
         p = _c32(p)
         self._lib.ref_crf_set_kernel_params(self._h, _fp(p), p.size)
+
+    def set_logistic_unary(self, Lm, f):
+        """DenseCRF::setUnaryEnergy(L, f): L (M, F), f (F, N)."""
+        Lm, f = _c32(Lm), _c32(f)
+        self._lib.ref_crf_set_unary_logistic(self._h, _fp(Lm), Lm.shape[1], _fp(f))
+
+    def unary_params(self):
+        p = np.empty(max(self._lib.ref_crf_num_unary_params(self._h), 1), dtype=np.float32)
+        self._lib.ref_crf_get_unary_params(self._h, _fp(p))
+        return p[:self._lib.ref_crf_num_unary_params(self._h)]
+
+    def set_unary_params(self, p):
+        p = _c32(p)
+        self._lib.ref_crf_set_unary_params(self._h, _fp(p), p.size)
+
+    def gradient_obj(self, n, gt, objective, param=0.0, class_weight=None, kernel=False, n_unary=None):
+        """DenseCRF::gradient with objective LOG_LIKELIHOOD / HAMMING / IOU.  The unary
+        gradient is the unary's own: (L, N) for the identity unary, else n_unary values."""
+        gt = np.ascontiguousarray(gt, dtype=np.int16)
+        ug = self._out() if n_unary is None else np.empty(max(n_unary, 1), np.float32)
+        cg = np.empty(max(self._lib.ref_crf_num_compat_params(self._h), 1), dtype=np.float32)
+        kg = np.empty(max(self._lib.ref_crf_num_kernel_params(self._h), 1), dtype=np.float32)
+        cw = None if class_weight is None else _c32(class_weight)
+        r = self._lib.ref_crf_gradient_obj(self._h, n, objective, gt.ctypes.data_as(_S), param,
+                                           None if cw is None else _fp(cw), 0 if cw is None else cw.size,
+                                           _fp(ug), _fp(cg), _fp(kg) if kernel else None)
+        return (r, ug.T if n_unary is None else ug[:n_unary], cg[:self._lib.ref_crf_num_compat_params(self._h)],
+                kg[:self._lib.ref_crf_num_kernel_params(self._h)] if kernel else None)
 
     def gradient(self, n, gt, robust=0.0, kernel=False):
         """(objective, dL/dU (L, N), compat gradient[, kernel-parameter gradient])."""

@@ -17,6 +17,7 @@
 #include "objective.h"
 #include "unary.h"
 #include <cstring>
+#include <memory>
 #include <vector>
 
 namespace {
@@ -185,6 +186,45 @@ double ref_crf_gradient(void* h, int n_iter, const short* gt, float robust, floa
     LogLikelihood obj(g, robust);
     VectorXf ug, cg, kg;
     double r = c->gradient(n_iter, obj, unary_grad ? &ug : nullptr, compat_grad ? &cg : nullptr,
+                           kernel_grad ? &kg : nullptr);
+    if (unary_grad) std::memcpy(unary_grad, ug.data(), sizeof(float) * ug.size());
+    if (compat_grad) std::memcpy(compat_grad, cg.data(), sizeof(float) * cg.size());
+    if (kernel_grad) std::memcpy(kernel_grad, kg.data(), sizeof(float) * kg.size());
+    return r;
+}
+
+// DenseCRF::setUnaryEnergy(L, f) (densecrf.cpp:92-94): LogisticUnaryEnergy with
+// L (M x F) and f (F x N), both row-major.
+void ref_crf_set_unary_logistic(void* h, const float* Lm, int F, const float* f) {
+    CRFProbe* c = static_cast<CRFProbe*>(h);
+    c->setUnaryEnergy(from_rowmajor(Lm, c->M_, F), from_rowmajor(f, F, c->N_));
+}
+int ref_crf_num_unary_params(void* h) { return (int)static_cast<CRFProbe*>(h)->unaryParameters().size(); }
+void ref_crf_get_unary_params(void* h, float* p) {
+    VectorXf v = static_cast<CRFProbe*>(h)->unaryParameters();
+    std::memcpy(p, v.data(), sizeof(float) * v.size());
+}
+void ref_crf_set_unary_params(void* h, const float* p, int n) {
+    static_cast<CRFProbe*>(h)->setUnaryParameters(Eigen::Map<const VectorXf>(p, n));
+}
+// DenseCRF::gradient with objective 0 LogLikelihood(gt, param), 1 Hamming(gt, param =
+// class_weight_pow) or Hamming(gt, class_weight) when given, 2 IntersectionOverUnion(gt).
+double ref_crf_gradient_obj(void* h, int n_iter, int objective, const short* gt, float param,
+                            const float* class_weight, int ncw, float* unary_grad, float* compat_grad,
+                            float* kernel_grad) {
+    CRFProbe* c = static_cast<CRFProbe*>(h);
+    VectorXs g = Eigen::Map<const VectorXs>(gt, c->N_);
+    std::unique_ptr<ObjectiveFunction> obj;
+    if (objective == 0)
+        obj.reset(new LogLikelihood(g, param));
+    else if (objective == 1 && class_weight)
+        obj.reset(new Hamming(g, VectorXf(Eigen::Map<const VectorXf>(class_weight, ncw))));
+    else if (objective == 1)
+        obj.reset(new Hamming(g, param));
+    else
+        obj.reset(new IntersectionOverUnion(g));
+    VectorXf ug, cg, kg;
+    double r = c->gradient(n_iter, *obj, unary_grad ? &ug : nullptr, compat_grad ? &cg : nullptr,
                            kernel_grad ? &kg : nullptr);
     if (unary_grad) std::memcpy(unary_grad, ug.data(), sizeof(float) * ug.size());
     if (compat_grad) std::memcpy(compat_grad, cg.data(), sizeof(float) * cg.size());

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libdcrf_hip.so")
 OK, ERR_VALUE, ERR_MEMORY, ERR_DEVICE, ERR_OVERFLOW, ERR_STATE = 0, -1, -2, -3, -4, -5
 NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC = 0, 1, 2, 3
 CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL = 0, 1, 2
+OBJECTIVE_LOG_LIKELIHOOD, OBJECTIVE_HAMMING, OBJECTIVE_IOU = 0, 1, 2
 COMPAT_POTTS, COMPAT_DIAGONAL, COMPAT_MATRIX = 0, 1, 2
 MAX_DIM, MAX_LABELS, MAX_TERMS = 15, 1024, 8
 
@@ -57,6 +58,12 @@ SIGNATURES = {
     "dcrf_get_compat_params": (_c, [_P, _F]),
     "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
     "dcrf_num_kernel_params": (_c, [_P]),
+    "dcrf_gradient_objective": (_c, [_P, _c, _c, _S, _f, _F, _c, _D, _F, _F, _F, _F]),
+    "dcrf_set_logistic_unary": (_c, [_P, _F, _c, _F]),
+    "dcrf_num_unary_params": (_c, [_P]),
+    "dcrf_get_unary_params": (_c, [_P, _F]),
+    "dcrf_set_unary_params": (_c, [_P, _F]),
+    "dcrf_unary_param_gradient": (_c, [_P, _F]),
     "dcrf_set_blur_pairs": (_c, [_P, _c]),
     "dcrf_get_kernel_params": (_c, [_P, _F]),
     "dcrf_set_kernel_params": (_c, [_P, _F]),
@@ -287,6 +294,52 @@ class Model:
         p = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         check(lib().dcrf_get_compat_params(self.h, fptr(p)))
         return p[:lib().dcrf_num_compat_params(self.h)]
+
+    def gradient_objective(self, n, gt, objective=0, param=0.0, class_weight=None, kernel=False):
+        """DenseCRF::gradient with LOG_LIKELIHOOD (param = robust), HAMMING (class_weight, or
+        param = class_weight_pow) or IOU: (objective, unary_grad (L, N), compat_grad packed,
+        kernel-weight grad [T], kernel-parameter grad or None)."""
+        g = np.ascontiguousarray(gt, dtype=np.int16)
+        obj = ctypes.c_double()
+        ug = self._buf()
+        cg = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
+        kg = np.empty(max(self.nterms, 1), np.float32)
+        nk = lib().dcrf_num_kernel_params(self.h)
+        kp = np.empty(max(nk, 1), np.float32)
+        cw = None if class_weight is None else np.ascontiguousarray(class_weight, dtype=np.float32).ravel()
+        check(lib().dcrf_gradient_objective(self.h, n, objective, g.ctypes.data_as(_S), param,
+                                            None if cw is None else fptr(cw), 0 if cw is None else cw.size,
+                                            ctypes.byref(obj), fptr(ug), fptr(cg), fptr(kg),
+                                            fptr(kp) if kernel else None))
+        return (obj.value, ug.T, cg[:lib().dcrf_num_compat_params(self.h)], kg[:self.nterms],
+                kp[:nk] if kernel else None)
+
+    def set_logistic_unary(self, Lmat, feat):
+        """LogisticUnaryEnergy: U = Lmat feat, Lmat (L, F), feat (F, N)."""
+        a = np.ascontiguousarray(Lmat, dtype=np.float32)
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        if a.ndim != 2 or f.ndim != 2 or a.shape[0] != self.L or f.shape != (a.shape[1], self.N):
+            raise ValueError("logistic unary: need L (%d, F) and f (F, %d), got %s and %s"
+                             % (self.L, self.N, a.shape, f.shape))
+        check(lib().dcrf_set_logistic_unary(self.h, fptr(a), a.shape[1], fptr(f)))
+
+    def unary_params(self):
+        n = lib().dcrf_num_unary_params(self.h)
+        p = np.empty(max(n, 1), np.float32)
+        check(lib().dcrf_get_unary_params(self.h, fptr(p)))
+        return p[:n]
+
+    def set_unary_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        if a.size != lib().dcrf_num_unary_params(self.h):
+            raise ValueError("expected %d unary parameters, got %d" % (lib().dcrf_num_unary_params(self.h), a.size))
+        check(lib().dcrf_set_unary_params(self.h, fptr(a)))
+
+    def unary_param_gradient(self):
+        n = lib().dcrf_num_unary_params(self.h)
+        g = np.empty(max(n, 1), np.float32)
+        check(lib().dcrf_unary_param_gradient(self.h, fptr(g)))
+        return g[:n]
 
     def set_blur_pairs(self, on):
         check(lib().dcrf_set_blur_pairs(self.h, 1 if on else 0))

@@ -64,6 +64,12 @@ struct dcrf_model {
     std::vector<float> unary_host;  // (L, N) row-major as given
     bool has_unary = false, unary_dirty = false;
     DevBuf<float> unary;  // [N][L]
+    // LogisticUnaryEnergy (U = L f): parameters L [L][F] row-major, features [N][F]
+    bool logistic = false;
+    int lF = 0;
+    std::vector<float> lL;
+    DevBuf<float> lL_dev, lf;
+    std::vector<float> ugrad_param;  // its parameter gradient from the last dcrf_gradient
     std::vector<std::unique_ptr<Term>> terms;
     DevBuf<float> Q, work, tmp1, tmp2, stage;
     DevBuf<float> Qp;  // inference iterate with the vertex-row stride (see run_inference)
@@ -107,7 +113,7 @@ struct dcrf_model {
     int train_n = -1;
     const float* train_kw = nullptr;
     DevBuf<float> qcache, mcache, bbuf, partial;
-    DevBuf<double> gacc;
+    DevBuf<double> gacc, gacc2;
     // kernel-parameter gradient: 5 [N][L] work slabs, 2 [N][d] feature gradients,
     // per-block partials and the accumulated gradient of all terms
     DevBuf<float> kgbuf, kgdf;
@@ -243,9 +249,25 @@ void ensure_device(dcrf_model& m) {
     m.dev_ready = true;
 }
 
+template <typename T>
+void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t s) {
+    dst.reserve(src.size());
+    if (!src.empty())
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
+}
+
 void ensure_unary(dcrf_model& m) {
     if (!m.has_unary || !m.unary_dirty) return;
     const size_t n = (size_t)m.N() * m.L;
+    if (m.logistic) {
+        // LogisticUnaryEnergy::get (unary.cpp:50-52) on the matrix cores
+        m.unary.reserve(n);
+        upload(m.lL_dev, m.lL, m.stream);
+        launch_logistic_unary(m.lf.get(), m.lL_dev.get(), m.N(), m.lF, m.L, m.unary.get(), m.stream);
+        DCRF_HIP_CHECK(hipStreamSynchronize(m.stream));
+        m.unary_dirty = false;
+        return;
+    }
     m.stage.reserve(n);
     m.unary.reserve(n);
     DCRF_HIP_CHECK(hipMemcpyAsync(m.stage.get(), m.unary_host.data(), sizeof(float) * n, hipMemcpyHostToDevice,
@@ -278,12 +300,6 @@ int max_run(const std::vector<int>& first) {
     return mx;
 }
 
-template <typename T>
-void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t s) {
-    dst.reserve(src.size());
-    if (!src.empty())
-        DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
-}
 
 // The splat's wave mode loads a wave's entries as one range: consecutive segments must
 // adjoin (segment c+1 begins where segment c ends), which the tile-major layout gives.
@@ -1039,6 +1055,78 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     }
 }
 
+// ObjectiveFunction::evaluate (objective.cpp:37-108) on the final Q (host [N][L]):
+// returns the objective and writes dQ = dL/dQ, which the adjoint multiplies by Q into
+// the reference's d_mul_Q.
+//   LogLikelihood(gt, robust)  r = sum log(max(Q_gt + robust, 1e-20)) / N; dQ_gt = 1/QQ/N
+//   Hamming(gt, w)             r = sum w_gt Q_gt;                      dQ_gt = w_gt
+//     (w given, or from the class counts c: (c/sum c)^-pow normalised by sum c w, :51-63)
+//   IntersectionOverUnion(gt)  r = sum_l in_l / un_l / M (in, un in double, un from 1e-20)
+//     dQ(l, i) = 1 / (un_l M) for l = gt_i, else -in_l / (un_l^2 M)
+double evaluate_objective(int kind, const int16_t* gt, float param, const float* cw_in, int ncw,
+                          const std::vector<float>& Q, int N, int L, std::vector<float>& dQ) {
+    double r = 0.0;
+    if (kind == DCRF_OBJECTIVE_LOG_LIKELIHOOD) {
+        for (int i = 0; i < N; i++) {
+            const int g = gt[i];
+            if (g < 0 || g >= L) continue;
+            const float q = Q[(size_t)i * L + g];
+            const float qr = q + param;
+            const float QQ = qr > 1e-20f ? qr : 1e-20f;
+            r += std::log((double)QQ) / N;
+            dQ[(size_t)i * L + g] = 1.0f / QQ / (float)N;
+        }
+        return r;
+    }
+    if (kind == DCRF_OBJECTIVE_HAMMING) {
+        std::vector<float> w;
+        if (cw_in) {
+            w.assign(cw_in, cw_in + ncw);
+        } else {
+            int M = 0;
+            for (int i = 0; i < N; i++) M = std::max(M, gt[i] + 1);
+            std::vector<float> cnt(M, 0.f);
+            for (int i = 0; i < N; i++)
+                if (gt[i] >= 0) cnt[gt[i]] += 1.0f;
+            float total = 0.f;
+            for (float c : cnt) total += c;
+            w.resize(M);
+            float z = 0.f;
+            for (int l = 0; l < M; l++) {
+                w[l] = std::pow(cnt[l] / total, -param);
+                z += cnt[l] * w[l];
+            }
+            for (int l = 0; l < M; l++) w[l] = w[l] / z;
+        }
+        for (int i = 0; i < N; i++) {
+            const int g = gt[i];
+            if (g < 0 || g >= L) continue;
+            require(g < (int)w.size(), "a ground-truth label has no class weight");
+            const float QQ = w[g] * Q[(size_t)i * L + g];
+            r += QQ;
+            dQ[(size_t)i * L + g] = w[g];
+        }
+        return r;
+    }
+    std::vector<double> in(L, 0.0), un(L, 1e-20);
+    for (int i = 0; i < N; i++) {
+        const int g = gt[i];
+        if (g < 0 || g >= L) continue;
+        in[g] += Q[(size_t)i * L + g];
+        un[g] += 1;
+        for (int l = 0; l < L; l++)
+            if (l != g) un[l] += Q[(size_t)i * L + l];
+    }
+    for (int i = 0; i < N; i++) {
+        const int g = gt[i];
+        if (g < 0 || g >= L) continue;
+        for (int l = 0; l < L; l++)
+            dQ[(size_t)i * L + l] = l == g ? (float)(1.0 / (un[l] * L)) : (float)(-in[l] / (un[l] * un[l] * L));
+    }
+    for (int l = 0; l < L; l++) r += in[l] / un[l];
+    return r / L;
+}
+
 dcrf_model* new_model(const std::vector<int>& N, const std::vector<int>* W, const std::vector<int>* H, int L) {
     require(!N.empty(), "a model needs at least one instance");
     require(L >= 1 && L <= DCRF_MAX_LABELS, "number of labels must be between 1 and " +
@@ -1130,12 +1218,71 @@ int dcrf_set_unary(dcrf_model* m, const float* U) {
         m->unary_host.assign(U, U + n);
         m->has_unary = true;
         m->unary_dirty = true;
+        m->logistic = false;
+    });
+}
+
+int dcrf_set_logistic_unary(dcrf_model* m, const float* Lmat, int F, const float* feat) {
+    return guard([&] {
+        require(m && Lmat && feat, "null argument");
+        require(F >= 1, "the logistic unary needs at least one feature");
+        const int N = m->N(), L = m->L;
+        require((long long)N * F < (1ll << 31), "logistic unary features too large for 32-bit indexing");
+        ensure_device(*m);
+        hipStream_t s = m->stream;
+        m->lL.assign(Lmat, Lmat + (size_t)L * F);
+        // features (F, N) row-major -> [N][F]
+        m->stage.reserve((size_t)N * F);
+        m->lf.reserve((size_t)N * F);
+        DCRF_HIP_CHECK(hipMemcpyAsync(m->stage.get(), feat, sizeof(float) * N * F, hipMemcpyHostToDevice, s));
+        launch_transpose(m->stage.get(), m->lf.get(), F, N, s);
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        m->lF = F;
+        m->logistic = true;
+        m->has_unary = true;
+        m->unary_dirty = true;
+        m->ugrad_param.clear();
+    });
+}
+
+int dcrf_num_unary_params(const dcrf_model* m) { return m && m->logistic ? m->L * m->lF : 0; }
+
+int dcrf_get_unary_params(const dcrf_model* m, float* params) {
+    return guard([&] {
+        require(m && params, "null argument");
+        // LogisticUnaryEnergy::parameters (unary.cpp:53-57): L packed column-major
+        const int L = m->L, F = m->lF;
+        if (!m->logistic) return;
+        for (int c = 0; c < F; c++)
+            for (int r = 0; r < L; r++) params[(size_t)c * L + r] = m->lL[(size_t)r * F + c];
+    });
+}
+
+int dcrf_set_unary_params(dcrf_model* m, const float* params) {
+    return guard([&] {
+        require(m && params, "null argument");
+        require(m->logistic, "the unary has no parameters (only a logistic unary does)");
+        const int L = m->L, F = m->lF;
+        for (int c = 0; c < F; c++)
+            for (int r = 0; r < L; r++) m->lL[(size_t)r * F + c] = params[(size_t)c * L + r];
+        m->unary_dirty = true;
+        m->train_n = -1;
+    });
+}
+
+int dcrf_unary_param_gradient(const dcrf_model* m, float* grad) {
+    return guard([&] {
+        require(m && grad, "null argument");
+        require(m->logistic, "the unary has no parameters (only a logistic unary does)");
+        require(!m->ugrad_param.empty(), "no dcrf_gradient call since the logistic unary was set");
+        std::copy(m->ugrad_param.begin(), m->ugrad_param.end(), grad);
     });
 }
 
 int dcrf_set_unary_instance(dcrf_model* m, int i, const float* U) {
     return guard([&] {
         require(m && U, "null argument");
+        require(!m->logistic, "the model has a logistic unary: set a full unary with dcrf_set_unary first");
         require(i >= 0 && i < m->geo.n_inst, "instance index out of range");
         const int N = m->N(), L = m->L, b = m->geo.pix_base[i], Ni = m->geo.N[i];
         if (m->unary_host.size() != (size_t)N * L) m->unary_host.assign((size_t)N * L, 0.f);
@@ -1456,9 +1603,19 @@ int dcrf_set_kernel_params(dcrf_model* m, const float* params) {
 
 int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, double* objective, float* unary_grad,
                   float* compat_grad, float* kweight_grad, float* kernel_grad) {
+    return dcrf_gradient_objective(m, n_iter, DCRF_OBJECTIVE_LOG_LIKELIHOOD, gt, robust, nullptr, 0, objective,
+                                   unary_grad, compat_grad, kweight_grad, kernel_grad);
+}
+
+int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const int16_t* gt, float param,
+                            const float* class_weight, int n_class_weight, double* objective, float* unary_grad,
+                            float* compat_grad, float* kweight_grad, float* kernel_grad) {
     return guard([&] {
         require(m && gt, "null argument");
         require(n_iter >= 0, "number of iterations must be non-negative");
+        require(objective_kind >= DCRF_OBJECTIVE_LOG_LIKELIHOOD && objective_kind <= DCRF_OBJECTIVE_IOU,
+                "unknown objective");
+        require(!class_weight || n_class_weight >= 0, "negative class-weight count");
         prepare(*m);
         const int N = m->N(), L = m->L, T = (int)m->terms.size();
         const size_t NL = (size_t)N * L;
@@ -1466,22 +1623,11 @@ int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, do
         const float* U = m->has_unary ? m->unary.get() : nullptr;
         const float* kw = m->kw_set ? m->kw.get() : nullptr;
         forward_train(*m, n_iter, U, nullptr, kw, s);
-        // LogLikelihood (objective.cpp:37-50): r = sum log(max(Q(gt)+robust, 1e-20)) / N,
-        // dL/dQ(gt) = 1 / QQ / N
         std::vector<float> Qn(NL), dQ(NL, 0.f);
         DCRF_HIP_CHECK(hipMemcpyAsync(Qn.data(), m->qcache.get() + (size_t)n_iter * NL, sizeof(float) * NL,
                                       hipMemcpyDeviceToHost, s));
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
-        double r = 0.0;
-        for (int i = 0; i < N; i++) {
-            const int g = gt[i];
-            if (g < 0 || g >= L) continue;
-            const float q = Qn[(size_t)i * L + g];
-            const float qr = q + robust;
-            const float QQ = qr > 1e-20f ? qr : 1e-20f;
-            r += std::log((double)QQ) / N;
-            dQ[(size_t)i * L + g] = 1.0f / QQ / (float)N;
-        }
+        const double r = evaluate_objective(objective_kind, gt, param, class_weight, n_class_weight, Qn, N, L, dQ);
         if (objective) *objective = r;
         m->work.reserve(NL);
         m->tmp1.reserve(NL);
@@ -1493,6 +1639,17 @@ int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, do
             DCRF_HIP_CHECK(hipMemcpyAsync(KG.data(), m->kacc.get(), sizeof(double) * nkp, hipMemcpyDeviceToHost, s));
         if (unary_grad)
             DCRF_HIP_CHECK(hipMemcpyAsync(unary_grad, m->tmp1.get(), sizeof(float) * NL, hipMemcpyDeviceToHost, s));
+        std::vector<double> UG;
+        if (m->logistic) {
+            // LogisticUnaryEnergy::gradient summed over the iterations (densecrf.cpp:261-262,
+            // 293-294; unary.cpp:64-68): sum_t b_t f^T = (dL/dU) f^T, one gram product
+            const int F = m->lF;
+            m->partial.reserve(std::max(m->partial.n, gram_partial_floats(L, F, N)));
+            m->gacc2.reserve((size_t)L * F);
+            launch_gram(m->tmp1.get(), L, m->lf.get(), F, N, m->partial.get(), m->gacc2.get(), s);
+            UG.resize((size_t)L * F);
+            DCRF_HIP_CHECK(hipMemcpyAsync(UG.data(), m->gacc2.get(), sizeof(double) * L * F, hipMemcpyDeviceToHost, s));
+        }
         size_t glen = 0;
         for (auto& t : m->terms) glen += stat_len(t->ckind, L);
         std::vector<double> G(glen);
@@ -1530,6 +1687,12 @@ int dcrf_gradient(dcrf_model* m, int n_iter, const int16_t* gt, float robust, do
         }
         if (kernel_grad)
             for (int i = 0; i < nkp; i++) kernel_grad[i] = (float)KG[i];
+        if (m->logistic) {
+            const int F = m->lF;
+            m->ugrad_param.resize((size_t)L * F);
+            for (int c = 0; c < F; c++)
+                for (int r = 0; r < L; r++) m->ugrad_param[(size_t)c * L + r] = (float)UG[(size_t)r * F + c];
+        }
         if (m->timing) collect_timing(*m);
     });
 }

@@ -378,6 +378,14 @@ void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
 // Scaled features diag(P) f or P f (P column-major) over the padded table.
 void launch_scale_features(const float* f, int npad, int d, bool full, const float* P, float* out, hipStream_t s);
 
+// ---------------------------------------------------------------- logistic unary
+// U [N][M] = lf [N][F] * Lm[M][F]^T on the matrix cores.
+void launch_logistic_unary(const float* lf, const float* Lm, int N, int F, int M, float* U, hipStream_t s);
+// out [LA][LB] (double) = sum_p A[p][a] B[p][c], A [N][LA], B [N][LB]; partial holds
+// gram_partial_floats(LA, LB, N) floats.
+size_t gram_partial_floats(int LA, int LB, int N);
+void launch_gram(const float* A, int LA, const float* B, int LB, int N, float* partial, double* out, hipStream_t s);
+
 // Global layout remaps.
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);
 // Entry arrays of one term: sorted entry j (csr_e[j] = p*(d+1)+r) goes to position

@@ -36,6 +36,7 @@ cdef extern from "dcrf_hip.h" nogil:
     int dcrf_start_inference(dcrf_model* m, float* Q)
     int dcrf_step_inference(dcrf_model* m, float* Q, float* tmp1, float* tmp2)
     int dcrf_kl_divergence(dcrf_model* m, const float* Q, double* kl)
+    int dcrf_set_logistic_unary(dcrf_model* m, const float* Lmat, int F, const float* feat)
 
 
 # pydensecrf/densecrf.pxd:43-44 (pairwise.h:32-42)
@@ -123,10 +124,11 @@ cdef class ConstUnary(Unary):
 
 
 cdef class LogisticUnary(Unary):
-    """LogisticUnaryEnergy (unary.cpp:48-68): U = L f, evaluated once on the host."""
+    """LogisticUnaryEnergy (unary.cpp:48-68): U = L f, L (labels, F), f (F, N); the
+    product runs on the GPU when the model takes the unary."""
     def __cinit__(self, float[:, ::1] L not None, float[:, ::1] f not None):
         import numpy as np
-        self._u = np.ascontiguousarray(np.asarray(L, dtype=np.float32) @ np.asarray(f, dtype=np.float32))
+        self._u = (np.array(L, dtype=np.float32, copy=True), np.array(f, dtype=np.float32, copy=True))
 
 
 cdef class DenseCRF:
@@ -165,23 +167,33 @@ cdef class DenseCRF:
         U = u.move()
         if U is None:
             raise ValueError("this unary object was already handed to a model")
-        self.setUnaryEnergy(U)
+        if isinstance(U, tuple):
+            self._set_logistic(U[0], U[1])
+        else:
+            self.setUnaryEnergy(U)
+
+    cdef _set_logistic(self, float[:, ::1] L, float[:, ::1] f):
+        # LogisticUnaryEnergy(L, f) (unary.cpp:48-52): U = L f on the device
+        if L.shape[0] != self._nlabel or f.shape[1] != self._nvar or f.shape[0] != L.shape[1]:
+            raise ValueError("Bad shape for the logistic unary (Need L ({}, F) and f (F, {}), got {} and {})".format(
+                self._nlabel, self._nvar, (L.shape[0], L.shape[1]), (f.shape[0], f.shape[1])))
+        cdef int st
+        cdef int F = <int>L.shape[1]
+        with nogil:
+            st = dcrf_set_logistic_unary(self._this, &L[0, 0], F, &f[0, 0])
+        _check(st)
 
     def setUnaryEnergy(self, float[:, ::1] u not None, float[:, ::1] f = None):
         if u.shape[0] != self._nlabel or u.shape[1] != self._nvar:
             raise ValueError("Bad shape for unary energy (Need {}, got {})".format(
                 (self._nlabel, self._nvar), (u.shape[0], u.shape[1])))
-        cdef float[:, ::1] U = u
         if f is not None:
-            # LogisticUnaryEnergy(u, f): U = u f (unary.cpp:54-56), evaluated on the host
-            import numpy as np
-            U = np.ascontiguousarray(np.asarray(u, dtype=np.float32) @ np.asarray(f, dtype=np.float32))
-            if U.shape[0] != self._nlabel or U.shape[1] != self._nvar:
-                raise ValueError("Bad shape for the logistic unary (u f must be {}, got {})".format(
-                    (self._nlabel, self._nvar), (U.shape[0], U.shape[1])))
+            # DenseCRF::setUnaryEnergy(L, f) (densecrf.cpp:92-94)
+            self._set_logistic(u, f)
+            return
         cdef int st
         with nogil:
-            st = dcrf_set_unary(self._this, &U[0, 0])
+            st = dcrf_set_unary(self._this, &u[0, 0])
         _check(st)
 
     def inference(self, int niter):

@@ -18,6 +18,9 @@ Fixtures
                      DenseCRF::gradient's kernel_grad (DIAG / FULL / CONST kernels, all
                      normalisations, default and set kernel parameters) with the
                      objective and unary / compatibility gradients of the same call.
+  learning_small.npz DenseCRF::gradient with the Hamming (counted and given class
+                     weights) and IntersectionOverUnion objectives; a LogisticUnaryEnergy
+                     model (its parameters, Q and its unary gradient sum_t b f^T).
   simple_10x10.npz   the reference test suite's own fixture (test_dcrf.py:8-26):
                      unary, image and Q of test_call_dcrf2d / test_call_dcrf.
   image_320x213.npz  the 320x213x21 5-iteration workload: V per term, SHA-256 of the
@@ -167,6 +170,53 @@ def kgrad_small():
     np.savez_compressed(os.path.join(HERE, "kgrad_small.npz"), **out)
 
 
+OBJ_CASES = [
+    # (objective, param, class weights or None, W, H, L, n_iter)
+    (1, 0.0, None, 15, 11, 3, 3),
+    (1, 1.0, None, 15, 11, 3, 3),
+    (1, 0.0, [1.0, 2.0, 0.5, 3.0, 1.5], 13, 9, 5, 2),
+    (2, 0.0, None, 15, 11, 3, 3),
+    (2, 0.0, None, 16, 12, 21, 2),
+    (0, 0.05, None, 15, 11, 3, 3),
+]
+
+
+def learning_small():
+    """DenseCRF::gradient with the Hamming and IntersectionOverUnion objectives, and
+    with a LogisticUnaryEnergy (its own unary gradient sum_t b f^T)."""
+    out = {}
+    for i, (obj, param, cw, W, H, L, n) in enumerate(OBJ_CASES):
+        img, U, gt = syn.make_image(W, H, L=L, sites=4, seed=900 + i)
+        crf = po.RefCRF(W=W, H=H, L=L, identity_unary=True)
+        crf.set_unary(U)
+        crf.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+        crf.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, img, 10.0, ntype=3)
+        cwa = None if cw is None else np.asarray(cw, np.float32)
+        r, ug, cg, kg = crf.gradient_obj(n, gt, obj, param, cwa, kernel=True)
+        p = "o%d_" % i
+        out.update({p + "img": img, p + "U": U, p + "gt": gt, p + "obj": np.float64(r), p + "unary_grad": ug,
+                    p + "compat_grad": cg, p + "kernel_grad": kg,
+                    p + "class_weight": np.zeros(0, np.float32) if cwa is None else cwa,
+                    p + "meta": np.array([obj, W, H, L, n], np.int32), p + "param": np.float32(param)})
+    out["nobj"] = np.int32(len(OBJ_CASES))
+    for i, (W, H, L, F, n) in enumerate([(14, 10, 3, 5, 3), (17, 9, 21, 13, 2)]):
+        rng = np.random.default_rng(950 + i)
+        img, _, gt = syn.make_image(W, H, L=L, sites=4, seed=950 + i)
+        Lm = (rng.standard_normal((L, F)) * 0.5).astype(np.float32)
+        f = rng.standard_normal((F, W * H)).astype(np.float32)
+        crf = po.RefCRF(W=W, H=H, L=L)
+        crf.set_logistic_unary(Lm, f)
+        crf.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+        crf.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, img, 10.0, ntype=3)
+        r, ugp, cg, _ = crf.gradient_obj(n, gt, 0, 0.0, None, n_unary=L * F)
+        p = "l%d_" % i
+        out.update({p + "img": img, p + "gt": gt, p + "Lm": Lm, p + "f": f, p + "obj": np.float64(r),
+                    p + "unary_param_grad": ugp, p + "compat_grad": cg, p + "params": crf.unary_params(),
+                    p + "Q": crf.inference(n), p + "meta": np.array([W, H, L, F, n], np.int32)})
+    out["nlog"] = np.int32(2)
+    np.savez_compressed(os.path.join(HERE, "learning_small.npz"), **out)
+
+
 def simple_10x10():
     # the reference suite's fixture, pydensecrf/tests/test_dcrf.py:8-26
     unary1 = np.zeros((10, 10), dtype=np.float32)
@@ -213,6 +263,7 @@ if __name__ == "__main__":
     if not po.ref_available():
         sys.exit("build the reference first: make -C oracle ref")
     writers = {"lattice_small": lattice_small, "crf_small": crf_small, "kgrad_small": kgrad_small,
+               "learning_small": learning_small,
                "simple_10x10": simple_10x10,
                "image_320x213": lambda: image_workload("320x213", syn.IMAGE_320x213, "image_320x213.npz"),
                "image_1024x768": lambda: image_workload("1024x768", syn.IMAGE_1024x768, "image_1024x768.npz")}

@@ -362,3 +362,93 @@ def test_autograd_kernel_params():
     fresh.set_unary(U)
     fresh.set_kernel_params(layer.kparams.detach().cpu().numpy())
     np.testing.assert_allclose(Q2, fresh.inference(n).T, atol=1e-6)
+
+
+# ---------------------------------------------------------------- learning objectives and LogisticUnary
+@pytest.mark.parametrize("i", range(6))
+def test_objectives_reference_fixtures(i):
+    """dcrf_gradient_objective with Hamming / IntersectionOverUnion / LogLikelihood
+    (objective.cpp:37-108) against the reference's DenseCRF::gradient outputs."""
+    z = np.load(os.path.join(GOLDEN, "learning_small.npz"), allow_pickle=False)
+    obj, W, H, L, n = [int(v) for v in z["o%d_meta" % i]]
+    p = "o%d_" % i
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(z[p + "U"])
+    g.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    g.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], 10.0, ntype=3)
+    cw = z[p + "class_weight"]
+    r, ug, cg, _, kg = g.gradient_objective(n, z[p + "gt"], obj, float(z[p + "param"]),
+                                            cw if cw.size else None, kernel=True)
+    ref = float(z[p + "obj"])
+    assert abs(r - ref) <= 1e-5 * max(1.0, abs(ref))
+    # sumAndNormalize (densecrf.cpp:107-114) forms b = (sum x) q - x with x = d_mul_Q; at
+    # a saturated pixel both terms are ~x and b is their small difference, so a 1e-7
+    # difference in Q leaves an absolute error of the order of 1e-7 max|x|.  max|x| is
+    # bounded per objective: the largest class weight (Hamming), 1/M (IoU), 1/N (LL).
+    gtv = z[p + "gt"]
+    if obj == 1:
+        if cw.size:
+            xmax = float(cw.max())
+        else:
+            cnt = np.bincount(gtv[gtv >= 0]).astype(np.float64)
+            w = (cnt / cnt.sum()) ** (-float(z[p + "param"]))
+            xmax = float((w / (cnt * w).sum()).max())
+    else:
+        xmax = 1.0 / L if obj == 2 else 1.0 / (W * H)
+    ref_ug = z[p + "unary_grad"]
+    assert np.abs(ug - ref_ug).max() <= 1e-4 * np.abs(ref_ug).max() + 1e-6 * xmax * n
+    # the compatibility and kernel gradients sum those b over all pixels: the same
+    # absolute error, gathered over the image (the order of the per-pixel label sum, not
+    # the lattice's, moves them: permuting the pixels changes the reference's own kernel
+    # gradient by 3e-6 only)
+    for got, want in ((cg, z[p + "compat_grad"]), (kg, z[p + "kernel_grad"])):
+        assert np.abs(got - want).max() <= KG_TOL * np.abs(want).max() + 1e-5 * xmax * n
+
+
+@pytest.mark.parametrize("i", range(2))
+def test_logistic_unary_reference_fixtures(i):
+    """LogisticUnaryEnergy on the matrix cores: parameters packed like the reference, Q,
+    the objective and the unary parameter gradient sum_t b f^T against the reference."""
+    z = np.load(os.path.join(GOLDEN, "learning_small.npz"), allow_pickle=False)
+    W, H, L, F, n = [int(v) for v in z["l%d_meta" % i]]
+    p = "l%d_" % i
+    g = capi.Model.create_2d(W, H, L)
+    g.set_logistic_unary(z[p + "Lm"], z[p + "f"])
+    g.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    g.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], 10.0, ntype=3)
+    np.testing.assert_array_equal(g.unary_params(), z[p + "params"])
+    np.testing.assert_allclose(g.inference(n), z[p + "Q"], atol=2e-5)
+    r, _, cg, _ = g.gradient(n, z[p + "gt"])
+    assert abs(r - float(z[p + "obj"])) <= 1e-5 * abs(r)
+    assert _rel_close(cg, z[p + "compat_grad"], 1e-4)
+    assert _rel_close(g.unary_param_gradient(), z[p + "unary_param_grad"], 1e-4)
+
+
+@pytest.mark.parametrize("N,L,F", [(1000, 21, 64), (777, 5, 13), (4096, 33, 9), (96, 1, 1)])
+def test_logistic_unary_products(N, L, F):
+    """U = L f and (dL/dU) f^T on the matrix cores against FP64 products: the start of
+    inference is softmax(-U); the parameter gradient equals the unary gradient times
+    f^T; setUnaryParameters (column-major) re-evaluates U."""
+    rng = np.random.default_rng(N + L + F)
+    Lm = (rng.standard_normal((L, F)) * 0.3).astype(np.float32)
+    f = rng.standard_normal((F, N)).astype(np.float32)
+    g = capi.Model.create(N, L)
+    g.set_logistic_unary(Lm, f)
+    feat = rng.standard_normal((3, N)).astype(np.float32)
+    g.add_pairwise(feat, 2.0)
+
+    def softmax_neg(U):
+        e = np.exp(-(U - U.min(0)))
+        return e / e.sum(0)
+
+    U = Lm.astype(np.float64) @ f.astype(np.float64)
+    np.testing.assert_allclose(g.start(), softmax_neg(U), atol=1e-5)
+    gt = rng.integers(0, L, N).astype(np.int16)
+    _, ug, _, _ = g.gradient(3, gt)
+    gp = g.unary_param_gradient()
+    expect = (ug.astype(np.float64) @ f.T.astype(np.float64)).T.ravel()
+    assert np.abs(gp - expect).max() <= 1e-4 * max(np.abs(expect).max(), 1e-12)
+    p2 = (Lm * 0.5).T.ravel()
+    g.set_unary_params(p2)
+    np.testing.assert_array_equal(g.unary_params(), p2)
+    np.testing.assert_allclose(g.start(), softmax_neg(U * 0.5), atol=1e-5)

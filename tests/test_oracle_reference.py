@@ -245,3 +245,51 @@ def test_kernel_gradient_live_reference(kt, nt):
         assert np.array_equal(bits(ko), bits(kr))
     else:
         assert np.abs(ko - kr).max() <= 2e-6 * np.abs(kr).max()
+
+
+def _learning_case(z, i):
+    obj, W, H, L, n = [int(v) for v in z["o%d_meta" % i]]
+    p = "o%d_" % i
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(z[p + "U"])
+    o.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    o.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], 10.0, ntype=3)
+    cw = z[p + "class_weight"]
+    return o, obj, n, p, (cw if cw.size else None)
+
+
+@pytest.mark.parametrize("i", range(6))
+def test_objective_fixtures_bitexact(i):
+    """Hamming and IntersectionOverUnion (objective.cpp:51-108) through DenseCRF::gradient:
+    objective, unary, compatibility and kernel gradients bit-exact."""
+    z = _load("learning_small.npz")
+    o, obj, n, p, cw = _learning_case(z, i)
+    r, ug, cg, kg = o.gradient_obj(n, z[p + "gt"], obj, float(z[p + "param"]), cw, kernel=True)
+    assert r == float(z[p + "obj"])
+    assert np.array_equal(bits(ug), bits(z[p + "unary_grad"]))
+    assert np.array_equal(bits(cg), bits(z[p + "compat_grad"]))
+    assert np.array_equal(bits(kg), bits(z[p + "kernel_grad"]))
+
+
+@pytest.mark.parametrize("i", range(2))
+def test_logistic_unary_fixture(i):
+    """LogisticUnaryEnergy (unary.cpp:48-68): U = L f and its unary gradient sum_t b f^T.
+    The oracle takes U as a plain unary (the FP64 product rounded to float; the reference's
+    Eigen product rounds in its own order), so Q agrees to 1e-6, and the unary gradient
+    is (dL/dU) f^T, the linearity of the sum over iterations, to 1e-5 of its largest
+    entry."""
+    z = _load("learning_small.npz")
+    W, H, L, F, n = [int(v) for v in z["l%d_meta" % i]]
+    p = "l%d_" % i
+    Lm, f = z[p + "Lm"], z[p + "f"]
+    np.testing.assert_array_equal(z[p + "params"], Lm.T.ravel())
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary((Lm.astype(np.float64) @ f.astype(np.float64)).astype(np.float32))
+    o.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    o.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z[p + "img"], 10.0, ntype=3)
+    assert np.abs(o.inference(n) - z[p + "Q"]).max() <= 1e-6
+    r, ug, cg, _ = o.gradient_obj(n, z[p + "gt"], po.LOG_LIKELIHOOD)
+    assert abs(r - float(z[p + "obj"])) <= 1e-6 * abs(r)
+    g = (ug.astype(np.float64) @ f.T.astype(np.float64)).T.ravel()
+    ref = z[p + "unary_param_grad"]
+    assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
