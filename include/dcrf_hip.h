@@ -143,6 +143,9 @@ int dcrf_set_kernel_weights(dcrf_model *m, const float *w);
  * Diagonal L, Matrix upper triangle W(i, j>=i) row by row). */
 int dcrf_num_compat_params(const dcrf_model *m);
 int dcrf_get_compat_params(const dcrf_model *m, float *params);
+/* setLabelCompatibilityParameters (densecrf.cpp:329-341): all terms from the packed
+ * vector of dcrf_get_compat_params (a Matrix entry sets w(i,j) and w(j,i)). */
+int dcrf_set_compat_params(dcrf_model *m, const float *params);
 /* LogisticUnaryEnergy (unary.cpp:48-68; DenseCRF::setUnaryEnergy(L, f), densecrf.cpp:92-94):
  * U = Lmat feat with Lmat (num_labels, F) and feat (F, N), both row-major; the product
  * runs on the device (matrix cores) and is redone when the parameters change.  Its

@@ -58,6 +58,7 @@ SIGNATURES = {
     "dcrf_get_compat_params": (_c, [_P, _F]),
     "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
     "dcrf_num_kernel_params": (_c, [_P]),
+    "dcrf_set_compat_params": (_c, [_P, _F]),
     "dcrf_gradient_objective": (_c, [_P, _c, _c, _S, _f, _F, _c, _D, _F, _F, _F, _F]),
     "dcrf_set_logistic_unary": (_c, [_P, _F, _c, _F]),
     "dcrf_num_unary_params": (_c, [_P]),
@@ -294,6 +295,13 @@ class Model:
         p = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         check(lib().dcrf_get_compat_params(self.h, fptr(p)))
         return p[:lib().dcrf_num_compat_params(self.h)]
+
+    def set_compat_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        if a.size != lib().dcrf_num_compat_params(self.h):
+            raise ValueError("expected %d compatibility parameters, got %d"
+                             % (lib().dcrf_num_compat_params(self.h), a.size))
+        check(lib().dcrf_set_compat_params(self.h, fptr(a)))
 
     def gradient_objective(self, n, gt, objective=0, param=0.0, class_weight=None, kernel=False):
         """DenseCRF::gradient with LOG_LIKELIHOOD (param = robust), HAMMING (class_weight, or

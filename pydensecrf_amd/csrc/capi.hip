@@ -1537,6 +1537,32 @@ int dcrf_get_compat_params(const dcrf_model* m, float* p) {
     });
 }
 
+int dcrf_set_compat_params(dcrf_model* m, const float* p) {
+    return guard([&] {
+        require(m && p, "null argument");
+        ensure_device(*m);
+        const int L = m->L;
+        // setLabelCompatibilityParameters (densecrf.cpp:329-341): Potts w, Diagonal w[L],
+        // Matrix upper triangle w(i, j>=i) written to both w(i,j) and w(j,i)
+        // (labelcompatibility.cpp:95-100)
+        for (auto& tp : m->terms) {
+            Term& t = *tp;
+            if (t.ckind == DCRF_COMPAT_MATRIX) {
+                for (int i = 0; i < L; i++)
+                    for (int j = i; j < L; j++) {
+                        const float v = *p++;
+                        t.compat[(size_t)i * L + j] = v;
+                        t.compat[(size_t)j * L + i] = v;
+                    }
+            } else {
+                for (float& v : t.compat) v = *p++;
+            }
+            DCRF_HIP_CHECK(hipMemcpy(t.d_compat.get(), t.compat.data(), sizeof(float) * t.compat.size(),
+                                     hipMemcpyHostToDevice));
+        }
+    });
+}
+
 int dcrf_num_kernel_params(const dcrf_model* m) { return m ? num_kernel_params(*m) : 0; }
 
 int dcrf_get_kernel_params(const dcrf_model* m, float* params) {
