@@ -384,6 +384,43 @@ void compute_norm(dcrf_model& m, Term& t) {
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Multi-step blur schedule: a lattice runs its d+1 steps in ceil((d+1)/K) passes of at
+// most K steps, spread as evenly as possible (longer passes first); the number of passes
+// falls with d, so in the global filter pass p covers a prefix of the vertex space.
+// K = 2: passes of three steps (27 rows per vertex) measured slower at 1024x768x21 (49
+// us against 28 us of blur per iteration), as the 27 gathers per vertex cost more than
+// the launch saved.
+constexpr int kMaxPassSteps = 2;
+std::vector<int> pass_schedule(int steps) {
+    const int n = (steps + kMaxPassSteps - 1) / kMaxPassSteps;
+    std::vector<int> out(n, steps / n);
+    for (int i = 0; i < steps % n; i++) out[i]++;
+    return out;
+}
+
+// Pass tables of one lattice (forward axes 0..d, reverse d..0) over its neighbour rows
+// nbr [(d+1)][V] (global rows from vbase, zero row zero_row).
+void build_pass_tables(const int2* nbr, int V, int d, int vbase, int zero_row, std::vector<int>& steps,
+                       std::vector<int>& off, DevBuf<int>& fwd, DevBuf<int>& rev, hipStream_t s) {
+    steps = pass_schedule(d + 1);
+    off.assign(1, 0);
+    for (int q : steps) off.push_back(off.back() + V * (q == 1 ? 3 : q == 2 ? 9 : 27));
+    fwd.reserve(off.back());
+    rev.reserve(off.back());
+    int first = 0;
+    for (size_t p = 0; p < steps.size(); p++) {
+        const int q = steps[p];
+        int fa[3], ta[3];
+        for (int j = 0; j < q; j++) {
+            fa[j] = first + j;
+            ta[j] = d - first - j;
+        }
+        launch_pass_table(nbr, V, vbase, zero_row, q, fa, fwd.get() + off[p], s);
+        launch_pass_table(nbr, V, vbase, zero_row, q, ta, rev.get() + off[p], s);
+        first += q;
+    }
+}
+
 // ---------------------------------------------------------------- kernel-parameter gradient
 // The term's own lattice filter without normaliser (the lattice_.compute calls of
 // DenseKernel::featureGradient, pairwise.cpp:88-114), built on first use: segments and
@@ -409,6 +446,7 @@ RawPlan& raw_plan(dcrf_model& m, Term& t) {
                         P->ent.get(), s);
     P->nbr.reserve((size_t)d1 * V);
     launch_remap_nbr(lat.nbr.get(), P->nbr.get(), d1 * V, 0, V, s);
+    build_pass_tables(P->nbr.get(), V, lat.d, 0, V, P->pass_steps, P->pass_off, P->passF, P->passT, s);
     DevBuf<int> gv;
     gv.reserve(E);
     P->sw.reserve(E);
@@ -463,16 +501,17 @@ void raw_splat(const Term& t, const float* in, int Lc, float* dst, hipStream_t s
 // The d+1 blur steps of the term's lattice (reverse: axes d..0) from src through the
 // ping-pong pair buf (src itself is only read); returns the buffer holding the result.
 float* raw_blur(const Term& t, const float* src, float* const buf[2], int Lc, bool reverse, hipStream_t s) {
-    const int V = t.lat.V, d = t.lat.d;
-    BlurTable bt{};
-    bt.nterms = 1;
-    bt.t[0] = BlurTerm{0, V, d, t.raw->nbr.get()};
+    const RawPlan& P = *t.raw;
+    const int V = t.lat.V, np = (int)P.pass_steps.size();
     const float* in = src;
-    for (int step = 0; step <= d; step++) {
-        launch_blur(in, buf[step & 1], 0, V, Lc, Lc, bt, step, reverse ? 1 : 0, s);
-        in = buf[step & 1];
+    for (int p = 0; p < np; p++) {
+        PassTable pt{};
+        pt.nterms = 1;
+        pt.t[0] = PassTerm{0, V, (reverse ? P.passT.get() : P.passF.get()) + P.pass_off[p], P.pass_steps[p]};
+        launch_blur_pass(in, buf[p & 1], 0, V, Lc, Lc, pt, s);
+        in = buf[p & 1];
     }
-    return buf[d & 1];
+    return buf[(np - 1) & 1];
 }
 
 // out = slice of the blurred rows `rows` (row stride ldv, the first L labels used).
@@ -608,22 +647,10 @@ int result_parity(const dcrf_model& m, int k) {
     return (m.pairs ? (int)m.tl[k]->pass_steps.size() : m.terms[k]->lat.d + 1) & 1;
 }
 
-// Multi-step blur schedule: term k runs its d_k+1 steps in ceil((d_k+1)/K) passes of at
-// most K steps, spread as evenly as possible (longer passes first); the number of passes
-// falls with d, so pass p covers a prefix of the vertex space.  K = 2: passes of three
-// steps (27 rows per vertex) measured slower at 1024x768x21 (49 us against 28 us of
-// blur per iteration), as the 27 gathers per vertex cost more than the launch saved.
-constexpr int kMaxPassSteps = 2;
 int blur_launches(const dcrf_model& m) {
     int P = 0;
     for (const auto& t : m.terms) P = std::max(P, (t->lat.d + kMaxPassSteps) / kMaxPassSteps);
     return P;
-}
-std::vector<int> pass_schedule(int steps) {
-    const int n = (steps + kMaxPassSteps - 1) / kMaxPassSteps;
-    std::vector<int> out(n, steps / n);
-    for (int i = 0; i < steps % n; i++) out[i]++;
-    return out;
 }
 
 void ensure_layout(dcrf_model& m) {
@@ -756,26 +783,9 @@ void ensure_layout(dcrf_model& m) {
         auto lay = std::make_unique<TermLayout>();
         lay->nbr.reserve((size_t)d1 * lat.V);
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
-        if (m.pairs) {
-            // forward passes walk axes 0..d, the transpose's d..0 (permutohedral.cpp:556)
-            lay->pass_steps = pass_schedule(lat.d + 1);
-            lay->pass_off.assign(1, 0);
-            for (int q : lay->pass_steps) lay->pass_off.push_back(lay->pass_off.back() + lat.V * (q == 1 ? 3 : q == 2 ? 9 : 27));
-            lay->passF.reserve(lay->pass_off.back());
-            lay->passT.reserve(lay->pass_off.back());
-            int first = 0;
-            for (size_t p = 0; p < lay->pass_steps.size(); p++) {
-                const int q = lay->pass_steps[p];
-                int fa[3], ta[3];
-                for (int j = 0; j < q; j++) {
-                    fa[j] = first + j;
-                    ta[j] = lat.d - first - j;
-                }
-                launch_pass_table(lay->nbr.get(), lat.V, m.vbase[k], V, q, fa, lay->passF.get() + lay->pass_off[p], s);
-                launch_pass_table(lay->nbr.get(), lat.V, m.vbase[k], V, q, ta, lay->passT.get() + lay->pass_off[p], s);
-                first += q;
-            }
-        }
+        if (m.pairs)  // forward passes walk axes 0..d, the transpose's d..0 (permutohedral.cpp:556)
+            build_pass_tables(lay->nbr.get(), lat.V, lat.d, m.vbase[k], V, lay->pass_steps, lay->pass_off,
+                              lay->passF, lay->passT, s);
         lay->vtx.reserve(Ek);
         lay->swF.reserve(Ek);
         const float alpha = slice_alpha(lat.d);

@@ -121,6 +121,9 @@ struct RawPlan {
     // [V+1+partials][2L], its forward and reverse blur chains [V+1][2L] x 2 each; the
     // splat of one input [V+1+partials][L] and its blur partner [V+1][L]
     DevBuf<float> S2, F2[2], R2[2], S1, B1;
+    // multi-step blur passes of the lattice: steps per pass, table offsets, tables
+    std::vector<int> pass_steps, pass_off;
+    DevBuf<int> passF, passT;
 };
 
 struct Term {
