@@ -68,7 +68,7 @@ struct dcrf_model {
     bool logistic = false;
     int lF = 0;
     std::vector<float> lL;
-    DevBuf<float> lL_dev, lf;
+    DevBuf<float> lL_dev, lf, lfFN;  // features [N][F] and as given [F][N]
     std::vector<float> ugrad_param;  // its parameter gradient from the last dcrf_gradient
     std::vector<std::unique_ptr<Term>> terms;
     DevBuf<float> Q, work, tmp1, tmp2, stage;
@@ -263,7 +263,7 @@ void ensure_unary(dcrf_model& m) {
         // LogisticUnaryEnergy::get (unary.cpp:50-52) on the matrix cores
         m.unary.reserve(n);
         upload(m.lL_dev, m.lL, m.stream);
-        launch_logistic_unary(m.lf.get(), m.lL_dev.get(), m.N(), m.lF, m.L, m.unary.get(), m.stream);
+        launch_logistic_unary(m.lfFN.get(), m.lL_dev.get(), m.N(), m.lF, m.L, m.unary.get(), m.stream);
         DCRF_HIP_CHECK(hipStreamSynchronize(m.stream));
         m.unary_dirty = false;
         return;
@@ -1241,11 +1241,11 @@ int dcrf_set_logistic_unary(dcrf_model* m, const float* Lmat, int F, const float
         ensure_device(*m);
         hipStream_t s = m->stream;
         m->lL.assign(Lmat, Lmat + (size_t)L * F);
-        // features (F, N) row-major -> [N][F]
-        m->stage.reserve((size_t)N * F);
+        // features (F, N) row-major, kept as given and as [N][F]
+        m->lfFN.reserve((size_t)N * F);
         m->lf.reserve((size_t)N * F);
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->stage.get(), feat, sizeof(float) * N * F, hipMemcpyHostToDevice, s));
-        launch_transpose(m->stage.get(), m->lf.get(), F, N, s);
+        DCRF_HIP_CHECK(hipMemcpyAsync(m->lfFN.get(), feat, sizeof(float) * N * F, hipMemcpyHostToDevice, s));
+        launch_transpose(m->lfFN.get(), m->lf.get(), F, N, s);
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
         m->lF = F;
         m->logistic = true;

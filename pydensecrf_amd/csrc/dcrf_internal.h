@@ -382,8 +382,8 @@ void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
 void launch_scale_features(const float* f, int npad, int d, bool full, const float* P, float* out, hipStream_t s);
 
 // ---------------------------------------------------------------- logistic unary
-// U [N][M] = lf [N][F] * Lm[M][F]^T on the matrix cores.
-void launch_logistic_unary(const float* lf, const float* Lm, int N, int F, int M, float* U, hipStream_t s);
+// U [N][M] = (Lm [M][F] fFN [F][N])^T on the matrix cores.
+void launch_logistic_unary(const float* fFN, const float* Lm, int N, int F, int M, float* U, hipStream_t s);
 // out [LA][LB] (double) = sum_p A[p][a] B[p][c], A [N][LA], B [N][LB]; partial holds
 // gram_partial_floats(LA, LB, N) floats.
 size_t gram_partial_floats(int LA, int LB, int N);

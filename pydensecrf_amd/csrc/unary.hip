@@ -1,14 +1,16 @@
 // LogisticUnaryEnergy on the matrix cores (pydensecrf/densecrf/src/unary.cpp:48-68):
 //   U = L f            L [M labels][F features], f [F][N]      (get, :50-52)
 //   dL/dL = b f^T      b = dL/dU summed over iterations        (gradient, :64-68)
-// Device layouts follow the rest of the library: U [N][M] label fastest, the features
-// transposed once to [N][F], L as given [M][F].
+// Device layouts: U [N][M] label fastest (the rest of the library's), L as given
+// [M][F]; the features both as given [F][N] (the forward's operand: a feature row is
+// contiguous over pixels) and transposed to [N][F] (the gradient's: a pixel row is
+// contiguous over features).
 //
 // Both products run on v_mfma_f32_32x32x2_f32 (fp32 in, fp32 accumulate, the precision
 // of the reference's Eigen product).  The forward is one wave per 32 pixels x 32
-// labels tile with K = F; a lane supplies four consecutive feature values per batch
-// (the K order inside an MFMA sum is free: A and B use the same assignment, here
-// k = c0 + 4 * (lane / 32) + j for instruction j).  The gradient is a tall-skinny GEMM
+// labels tile with K = F (the K order inside an MFMA sum is free as long as A and B
+// use the same assignment: here instruction j of a batch of 16 takes feature
+// c0 + 2j + lane/32).  The gradient is a tall-skinny GEMM
 // (K = N pixels): one wave per (32 x 32 output tile, pixel range), the ranges added in
 // double in a fixed order afterwards (deterministic).
 #include <algorithm>
@@ -22,7 +24,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int kThreads = 256;
 constexpr int kGramBatch = 8;  // pixel pairs of operands in flight per wave
 
-__global__ void __launch_bounds__(256) k_logistic_mfma(const float* __restrict__ lf, const float* __restrict__ Lm,
+__global__ void __launch_bounds__(256) k_logistic_mfma(const float* __restrict__ fFN, const float* __restrict__ Lm,
                                                        int N, int F, int M, int tilesM, int n_waves,
                                                        float* __restrict__ U) {
     const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -32,22 +34,24 @@ __global__ void __launch_bounds__(256) k_logistic_mfma(const float* __restrict__
     const int tn = w / tilesM, tm = w - tn * tilesM;
     const int n = tn * 32 + i, m = tm * 32 + i;
     const int nc = min(n, N - 1), mc = min(m, M - 1);  // clamped: every load stays in bounds
-    const float* arow = lf + (size_t)nc * F;
     const float* brow = Lm + (size_t)mc * F;
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    for (int c0 = 0; c0 < F; c0 += 8) {
-        float a[4], b[4];
+    // instruction j of a batch takes features c0 + 2j + h: the 32 lanes of a half read 32
+    // consecutive pixels of one feature row (one 128-byte access); a batch of 16
+    // features keeps 8 such loads in flight
+    for (int c0 = 0; c0 < F; c0 += 16) {
+        float a[8], b[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int c = c0 + 4 * h + j;
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 2 * j + h;
             const int cc = min(c, F - 1);
-            a[j] = arow[cc] * (c < F ? 1.0f : 0.0f);  // features past F add 0 (load unconditional)
+            a[j] = fFN[(size_t)cc * N + nc] * (c < F ? 1.0f : 0.0f);  // features past F add 0
             b[j] = brow[cc];
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j], b[j], acc, 0, 0, 0);
+        for (int j = 0; j < 8; j++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j], b[j], acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -115,13 +119,13 @@ __global__ void k_gram_reduce(const float* __restrict__ partial, int nranges, in
 
 }  // namespace
 
-void launch_logistic_unary(const float* lf, const float* Lm, int N, int F, int M, float* U, hipStream_t s) {
+void launch_logistic_unary(const float* fFN, const float* Lm, int N, int F, int M, float* U, hipStream_t s) {
     if (N <= 0 || M <= 0) return;
     if (F <= 0) throw Error{DCRF_ERR_VALUE, "logistic unary: no features"};
     const int tilesM = (M + 31) / 32;
     const long long waves = (long long)((N + 31) / 32) * tilesM;
     if (waves > 0x7fffffffll / 64) throw Error{DCRF_ERR_VALUE, "logistic unary too large"};
-    k_logistic_mfma<<<(unsigned)((waves + 3) / 4), kThreads, 0, s>>>(lf, Lm, N, F, M, tilesM, (int)waves, U);
+    k_logistic_mfma<<<(unsigned)((waves + 3) / 4), kThreads, 0, s>>>(fFN, Lm, N, F, M, tilesM, (int)waves, U);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

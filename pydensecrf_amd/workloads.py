@@ -1,0 +1,103 @@
+"""Fixed workloads for profiling the paths beside the inference benchmark.
+
+    python -m pydensecrf_amd.workloads kernel-gradient [--steps S]
+        DenseCRF2D 512x512, 21 labels, 5 iterations, Matrix compatibilities, learnable
+        DIAG kernel parameters: S forward+backward passes through the torch.autograd
+        layer (the kernel-parameter gradient in every backward).
+    python -m pydensecrf_amd.workloads logistic-unary [--reps R] [--features F]
+        1024x768 pixels, 21 labels, a logistic unary with F features (default 64):
+        R evaluations of U = L f on the matrix cores, with the parameters changed
+        between evaluations so that each one recomputes U.
+
+Each prints one line with its host-clock time per repetition; run under rocprofv3 for
+the per-kernel figures.
+"""
+import argparse
+import time
+
+import numpy as np
+
+
+def kernel_gradient(steps):
+    import torch
+
+    from . import capi, synthetic as syn
+    from .autograd import CRFLayer
+
+    cfg = syn.IMAGE_512x512
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, gt = syn.make_image(**cfg)
+    rng = np.random.default_rng(1003)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, c1)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    layer = CRFLayer(m, 5, [c1, c2], kweights=[3.0, 10.0], learn_kernel=True)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+
+    def step():
+        Q = layer(Ut)
+        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print("kernel-gradient 512x512x21, 5 iterations: %.3f ms per forward+backward" % ms, flush=True)
+
+
+def logistic_unary(reps, F):
+    from . import capi
+
+    W, H, L = 1024, 768, 21
+    N = W * H
+    rng = np.random.default_rng(77)
+    Lm = (rng.standard_normal((L, F)) * 0.2).astype(np.float32)
+    f = rng.standard_normal((F, N)).astype(np.float32)
+    m = capi.Model.create_2d(W, H, L)
+    m.set_logistic_unary(Lm, f)
+    Q = np.empty((N, L), np.float32)
+    Qd = None
+    try:
+        import torch
+
+        Qd = torch.empty((N, L), dtype=torch.float32, device="cuda")
+    except Exception:
+        pass
+    p = m.unary_params()
+    t = []
+    for r in range(reps + 1):
+        m.set_unary_params(p * np.float32(1.0 + 1e-3 * r))
+        t0 = time.perf_counter()
+        if Qd is not None:
+            m.inference_device(0, Qd.data_ptr())  # U on the device, softmax(-U) into Qd
+            torch.cuda.synchronize()
+        else:
+            Q[:] = m.inference(0).T
+        t.append(time.perf_counter() - t0)
+    ms = 1e3 * float(np.median(t[1:]))
+    print("logistic-unary %dx%d, %d labels, %d features: %.3f ms per evaluation (U = L f and softmax)"
+          % (W, H, L, F, ms), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary"])
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--features", type=int, default=64)
+    a = ap.parse_args()
+    if a.workload == "kernel-gradient":
+        kernel_gradient(a.steps)
+    else:
+        logistic_unary(a.reps, a.features)
+
+
+if __name__ == "__main__":
+    main()
