@@ -130,6 +130,23 @@ int dcrf_capture_inference(dcrf_model *m, int n_iterations, const float *U_dev, 
 /* Launch the captured graph on `stream` (NULL is the null stream). */
 int dcrf_launch_graph(dcrf_model *m, void *stream);
 
+/* utils.unary_from_softmax (pydensecrf/utils.py:55-79) on device buffers: sm [L][N]
+ * (class first, as the reference takes it) -> U [N][L] (the layout of
+ * dcrf_inference_device); scale <= 0: none, clip <= 0: none.  No model needed. */
+int dcrf_unary_from_softmax_device(const float *sm, int L, int N, double scale, double clip, float *U,
+                                   void *stream);
+/* utils.unary_from_labels (pydensecrf/utils.py:6-45) on device buffers: labels [N] int32
+ * -> U [N][L]; with zero_unsure label 0 means unsure and label l means class l-1 (a
+ * label past the classes gets the off-label energy everywhere, where numpy raises). */
+int dcrf_unary_from_labels_device(const int32_t *labels, int N, int L, double gt_prob, int zero_unsure, float *U,
+                                  void *stream);
+/* DenseCRF::unaryEnergy / pairwiseEnergy (densecrf.cpp:141-177), the per-variable
+ * energies of a labelling: labels [N] int16 (outside [0, L): 0); out [N].  The pairwise
+ * energy of term k is -0.5 (C_k K_k onehot(labels))(l_i, i) (times the term weight when
+ * one is set); term -1 sums all terms. */
+int dcrf_unary_energy(dcrf_model *m, const int16_t *labels, float *out);
+int dcrf_pairwise_energy(dcrf_model *m, const int16_t *labels, int term, float *out);
+
 /* ---- parameters and gradients ------------------------------------------------- */
 /* Replace the compatibility of one term; params in the raw layout of dcrf_add_pairwise
  * (Potts [1], Diagonal [L], Matrix [L*L] row-major, symmetrised as (m+m^T)/2).  The

@@ -395,6 +395,8 @@ def _ref():
                                              _F, _F, _F]
         lib.ref_crf_gradient_obj.restype = ctypes.c_double
         lib.ref_crf_set_unary_logistic.argtypes = [_P, _F, ctypes.c_int, _F]
+        lib.ref_crf_unary_energy.argtypes = [_P, _S, _F]
+        lib.ref_crf_pairwise_energy.argtypes = [_P, _S, ctypes.c_int, _F]
         lib.ref_crf_num_unary_params.argtypes = [_P]
         lib.ref_crf_get_unary_params.argtypes = [_P, _F]
         lib.ref_crf_set_unary_params.argtypes = [_P, _F, ctypes.c_int]
@@ -529,6 +531,18 @@ class RefCRF(_CRFBase):
     def set_kernel_params(self, p):
         p = _c32(p)
         self._lib.ref_crf_set_kernel_params(self._h, _fp(p), p.size)
+
+    def unary_energy(self, labels):
+        l = np.ascontiguousarray(labels, dtype=np.int16)
+        out = np.empty(self.N, np.float32)
+        self._lib.ref_crf_unary_energy(self._h, l.ctypes.data_as(_S), _fp(out))
+        return out
+
+    def pairwise_energy(self, labels, term=-1):
+        l = np.ascontiguousarray(labels, dtype=np.int16)
+        out = np.empty(self.N, np.float32)
+        self._lib.ref_crf_pairwise_energy(self._h, l.ctypes.data_as(_S), term, _fp(out))
+        return out
 
     def set_logistic_unary(self, Lm, f):
         """DenseCRF::setUnaryEnergy(L, f): L (M, F), f (F, N)."""

@@ -232,4 +232,16 @@ double ref_crf_gradient_obj(void* h, int n_iter, int objective, const short* gt,
     return r;
 }
 
+// DenseCRF::unaryEnergy / pairwiseEnergy (densecrf.cpp:141-177); labels int16 [N].
+void ref_crf_unary_energy(void* h, const short* l, float* out) {
+    CRFProbe* c = static_cast<CRFProbe*>(h);
+    VectorXf r = c->unaryEnergy(Eigen::Map<const VectorXs>(l, c->N_));
+    std::memcpy(out, r.data(), sizeof(float) * r.size());
+}
+void ref_crf_pairwise_energy(void* h, const short* l, int term, float* out) {
+    CRFProbe* c = static_cast<CRFProbe*>(h);
+    VectorXf r = c->pairwiseEnergy(Eigen::Map<const VectorXs>(l, c->N_), term);
+    std::memcpy(out, r.data(), sizeof(float) * r.size());
+}
+
 }  // extern "C"

@@ -29,6 +29,7 @@ _U8 = ctypes.POINTER(ctypes.c_uint8)
 _LL = ctypes.POINTER(ctypes.c_longlong)
 _c = ctypes.c_int
 _f = ctypes.c_float
+_D_val = ctypes.c_double
 
 # name -> (restype, argtypes); every symbol declared in include/dcrf_hip.h
 SIGNATURES = {
@@ -59,6 +60,10 @@ SIGNATURES = {
     "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
     "dcrf_num_kernel_params": (_c, [_P]),
     "dcrf_set_compat_params": (_c, [_P, _F]),
+    "dcrf_unary_energy": (_c, [_P, _S, _F]),
+    "dcrf_unary_from_softmax_device": (_c, [_P, _c, _c, _D_val, _D_val, _P, _P]),
+    "dcrf_unary_from_labels_device": (_c, [_P, _c, _c, _D_val, _c, _P, _P]),
+    "dcrf_pairwise_energy": (_c, [_P, _S, _c, _F]),
     "dcrf_gradient_objective": (_c, [_P, _c, _c, _S, _f, _F, _c, _D, _F, _F, _F, _F]),
     "dcrf_set_logistic_unary": (_c, [_P, _F, _c, _F]),
     "dcrf_num_unary_params": (_c, [_P]),
@@ -295,6 +300,24 @@ class Model:
         p = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         check(lib().dcrf_get_compat_params(self.h, fptr(p)))
         return p[:lib().dcrf_num_compat_params(self.h)]
+
+    def unary_energy(self, labels):
+        """DenseCRF::unaryEnergy: U(l_i, i) per variable."""
+        l = np.ascontiguousarray(labels, dtype=np.int16).ravel()
+        if l.size != self.N:
+            raise ValueError("need one label per variable (%d)" % self.N)
+        out = np.empty(self.N, np.float32)
+        check(lib().dcrf_unary_energy(self.h, l.ctypes.data_as(_S), fptr(out)))
+        return out
+
+    def pairwise_energy(self, labels, term=-1):
+        """DenseCRF::pairwiseEnergy: -0.5 (message of the one-hot labelling)(l_i, i)."""
+        l = np.ascontiguousarray(labels, dtype=np.int16).ravel()
+        if l.size != self.N:
+            raise ValueError("need one label per variable (%d)" % self.N)
+        out = np.empty(self.N, np.float32)
+        check(lib().dcrf_pairwise_energy(self.h, l.ctypes.data_as(_S), term, fptr(out)))
+        return out
 
     def set_compat_params(self, p):
         a = np.ascontiguousarray(p, dtype=np.float32).ravel()

@@ -1484,6 +1484,80 @@ int dcrf_apply_pairwise(dcrf_model* m, int term, const float* Q, float* out, int
     });
 }
 
+int dcrf_unary_from_softmax_device(const float* sm, int L, int N, double scale, double clip, float* U, void* stream) {
+    return guard([&] {
+        require(sm && U, "null argument");
+        require(L >= 1 && N >= 0, "bad shape");
+        require(!(scale > 1.0), "scale needs to be in (0, 1]");
+        launch_unary_from_softmax(sm, L, N, scale, scale > 0.0, clip, clip > 0.0, U, (hipStream_t)stream);
+    });
+}
+
+int dcrf_unary_from_labels_device(const int32_t* labels, int N, int L, double gt_prob, int zero_unsure, float* U,
+                                  void* stream) {
+    return guard([&] {
+        require(labels && U, "null argument");
+        require(gt_prob > 0.0 && gt_prob < 1.0, "gt_prob must be in (0, 1)");
+        require(L >= 2, "at least two labels");
+        // the energies as numpy forms them: float64 logs stored to float32
+        const double n_e = -std::log((1.0 - gt_prob) / (L - 1));
+        const double p_e = -std::log(gt_prob);
+        const double u_e = -std::log(1.0 / L);
+        launch_unary_from_labels(labels, N, L, (float)p_e, (float)n_e, (float)u_e, zero_unsure != 0, U,
+                                 (hipStream_t)stream);
+    });
+}
+
+int dcrf_unary_energy(dcrf_model* m, const int16_t* labels, float* out) {
+    return guard([&] {
+        require(m && labels && out, "null argument");
+        const int N = m->N(), L = m->L;
+        // DenseCRF::unaryEnergy (densecrf.cpp:141-152): U(l_i, i), 0 for a label outside
+        // [0, L) or without a unary
+        std::fill(out, out + N, 0.0f);
+        if (!m->has_unary) return;
+        ensure_device(*m);
+        ensure_unary(*m);
+        std::vector<float> U((size_t)N * L);
+        DCRF_HIP_CHECK(hipMemcpy(U.data(), m->unary.get(), sizeof(float) * U.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++)
+            if (labels[i] >= 0 && labels[i] < L) out[i] = U[(size_t)i * L + labels[i]];
+    });
+}
+
+int dcrf_pairwise_energy(dcrf_model* m, const int16_t* labels, int term, float* out) {
+    return guard([&] {
+        require(m && labels && out, "null argument");
+        const int T = (int)m->terms.size();
+        require(term == -1 || (term >= 0 && term < T), "pairwise term index out of range");
+        const int N = m->N(), L = m->L;
+        const size_t n = (size_t)N * L;
+        std::fill(out, out + N, 0.0f);
+        if (!T) return;
+        // DenseCRF::pairwiseEnergy (densecrf.cpp:153-177): the one-hot labelling Q
+        // through the term's message, -0.5 Q'(l_i, i); term -1 sums the terms in order
+        prepare(*m);
+        std::vector<float> Q(n, 0.0f), x(n);
+        for (int i = 0; i < N; i++)
+            if (labels[i] >= 0 && labels[i] < L) Q[(size_t)i * L + labels[i]] = 1.0f;
+        m->work.reserve(n);
+        m->tmp1.reserve(n);
+        hipStream_t s = m->stream;
+        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+        run_filter(*m, m->work.get(), L, false, s);
+        const SliceTable st = slice_table(*m, false, m->kw_set ? m->kw.get() : nullptr);
+        for (int k = term < 0 ? 0 : term; k < (term < 0 ? T : term + 1); k++) {
+            launch_slice_term(m->tmp1.get(), N, L, st, k, s);
+            DCRF_HIP_CHECK(hipMemcpyAsync(x.data(), m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            DCRF_HIP_CHECK(hipStreamSynchronize(s));
+            for (int i = 0; i < N; i++) {
+                const float e = labels[i] >= 0 && labels[i] < L ? -0.5f * x[(size_t)i * L + labels[i]] : 0.0f;
+                out[i] = out[i] + e;
+            }
+        }
+    });
+}
+
 int dcrf_set_compat(dcrf_model* m, int term, const float* params) {
     return guard([&] {
         require(m && params, "null argument");

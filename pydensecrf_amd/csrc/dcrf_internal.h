@@ -384,6 +384,11 @@ void launch_scale_features(const float* f, int npad, int d, bool full, const flo
 // ---------------------------------------------------------------- logistic unary
 // U [N][M] = (Lm [M][F] fFN [F][N])^T on the matrix cores.
 void launch_logistic_unary(const float* fFN, const float* Lm, int N, int F, int M, float* U, hipStream_t s);
+// Unary helpers on the device, output [N][L].
+void launch_unary_from_softmax(const float* sm, int L, int N, double scale, bool use_scale, double clip,
+                               bool use_clip, float* U, hipStream_t s);
+void launch_unary_from_labels(const int* labels, int N, int L, float p_energy, float n_energy, float u_energy,
+                              bool zero_unsure, float* U, hipStream_t s);
 // out [LA][LB] (double) = sum_p A[p][a] B[p][c], A [N][LA], B [N][LB]; partial holds
 // gram_partial_floats(LA, LB, N) floats.
 size_t gram_partial_floats(int LA, int LB, int N);

@@ -14,6 +14,7 @@
 // (K = N pixels): one wave per (32 x 32 output tile, pixel range), the ranges added in
 // double in a fixed order afterwards (deterministic).
 #include <algorithm>
+#include <cmath>
 
 #include "dcrf_internal.h"
 
@@ -117,7 +118,68 @@ __global__ void k_gram_reduce(const float* __restrict__ partial, int nranges, in
     if (t == 0) out[j] = red[0];
 }
 
+// utils.unary_from_softmax (pydensecrf/utils.py:55-79) fused with the transpose into
+// the library's layout: U[n][l] = -log(clip(scale sm[l][n] + (1 - scale) / L)), in
+// double like numpy's float64 path, rounded once to float.  A 32 x 32 tile per block
+// through LDS: reads coalesced along pixels, writes along labels.
+__global__ void k_unary_from_softmax(const float* __restrict__ sm, int L, int N, double scale, int use_scale,
+                                     double clip, int use_clip, float* __restrict__ U) {
+    __shared__ float tile[32][33];
+    const int n0 = blockIdx.x * 32, l0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int l = l0 + k, n = n0 + tx;
+        float v = 0.0f;
+        if (l < L && n < N) {
+            double x = sm[(size_t)l * N + n];
+            if (use_scale) x = scale * x + (1.0 - scale) * (1.0 / L);
+            if (use_clip) x = x < clip ? clip : (x > 1.0 ? 1.0 : x);
+            v = (float)(-log(x));
+        }
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int n = n0 + k, l = l0 + tx;
+        if (n < N && l < L) U[(size_t)n * L + l] = tile[tx][k];
+    }
+}
+
+// utils.unary_from_labels (pydensecrf/utils.py:6-45): U[n][l] = p_energy for the
+// annotated label (label - 1 with zero_unsure), n_energy elsewhere, and -log(1/L) for
+// every label of an unsure (0) pixel.
+__global__ void k_unary_from_labels(const int* __restrict__ labels, int N, int L, float p_energy, float n_energy,
+                                    float u_energy, int zero_unsure, float* __restrict__ U) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * L) return;
+    const int n = (int)(i / L), l = (int)(i - (size_t)n * L);
+    const int lab = labels[n];
+    float v;
+    if (zero_unsure && lab == 0)
+        v = u_energy;
+    else
+        v = (zero_unsure ? lab - 1 : lab) == l ? p_energy : n_energy;
+    U[i] = v;
+}
+
 }  // namespace
+
+void launch_unary_from_softmax(const float* sm, int L, int N, double scale, bool use_scale, double clip,
+                               bool use_clip, float* U, hipStream_t s) {
+    if (N <= 0 || L <= 0) return;
+    const dim3 grid((unsigned)((N + 31) / 32), (unsigned)((L + 31) / 32));
+    k_unary_from_softmax<<<grid, kThreads, 0, s>>>(sm, L, N, scale, use_scale ? 1 : 0, clip, use_clip ? 1 : 0, U);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_unary_from_labels(const int* labels, int N, int L, float p_energy, float n_energy, float u_energy,
+                              bool zero_unsure, float* U, hipStream_t s) {
+    const size_t n = (size_t)N * L;
+    if (!n) return;
+    k_unary_from_labels<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, s>>>(
+        labels, N, L, p_energy, n_energy, u_energy, zero_unsure ? 1 : 0, U);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
 
 void launch_logistic_unary(const float* fFN, const float* Lm, int N, int F, int M, float* U, hipStream_t s) {
     if (N <= 0 || M <= 0) return;

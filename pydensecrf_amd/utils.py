@@ -84,3 +84,46 @@ def create_pairwise_bilateral(sdims, schan, img, chdim=-1):
     mesh = _scaled_mesh(im_feat.shape[1:], sdims)
     feats = np.concatenate([mesh, im_feat])
     return feats.reshape([feats.shape[0], -1])
+
+
+# ---------------------------------------------------------------- device forms
+# The same helpers on torch tensors that live on the GPU, written straight into the
+# (N, L) label-fastest layout that CRFLayer and Model.inference_device take.
+
+def unary_from_softmax_device(sm, scale=None, clip=1e-5):
+    """unary_from_softmax for a float32 CUDA tensor sm (n_classes, ...): returns the
+    (N, n_classes) unary tensor, computed in double and rounded once to float32."""
+    import torch
+
+    from . import capi
+    if not (sm.is_cuda and sm.dtype == torch.float32):
+        raise TypeError("sm must be a float32 CUDA tensor")
+    if scale is not None and not 0 < scale <= 1:
+        raise ValueError("`scale` needs to be in (0,1]")
+    L = sm.shape[0]
+    s = sm.reshape(L, -1).contiguous()
+    N = s.shape[1]
+    U = torch.empty((N, L), dtype=torch.float32, device=sm.device)
+    capi.check(capi.lib().dcrf_unary_from_softmax_device(s.data_ptr(), L, N, -1.0 if scale is None else float(scale),
+                                                         -1.0 if clip is None else float(clip), U.data_ptr(),
+                                                         torch.cuda.current_stream(sm.device).cuda_stream))
+    return U
+
+
+def unary_from_labels_device(labels, n_labels, gt_prob, zero_unsure=True):
+    """unary_from_labels for an integer CUDA tensor of labels: returns the (N, n_labels)
+    unary tensor."""
+    import torch
+
+    from . import capi
+    if not labels.is_cuda:
+        raise TypeError("labels must be a CUDA tensor")
+    if not 0 < gt_prob < 1:
+        raise ValueError("`gt_prob must be in (0,1).")
+    lab = labels.reshape(-1).to(torch.int32).contiguous()
+    N = lab.shape[0]
+    U = torch.empty((N, n_labels), dtype=torch.float32, device=labels.device)
+    capi.check(capi.lib().dcrf_unary_from_labels_device(lab.data_ptr(), N, n_labels, float(gt_prob),
+                                                        1 if zero_unsure else 0, U.data_ptr(),
+                                                        torch.cuda.current_stream(labels.device).cuda_stream))
+    return U

@@ -20,7 +20,8 @@ Fixtures
                      objective and unary / compatibility gradients of the same call.
   learning_small.npz DenseCRF::gradient with the Hamming (counted and given class
                      weights) and IntersectionOverUnion objectives; a LogisticUnaryEnergy
-                     model (its parameters, Q and its unary gradient sum_t b f^T).
+                     model (its parameters, Q and its unary gradient sum_t b f^T);
+                     unaryEnergy / pairwiseEnergy of a labelling.
   simple_10x10.npz   the reference test suite's own fixture (test_dcrf.py:8-26):
                      unary, image and Q of test_call_dcrf2d / test_call_dcrf.
   image_320x213.npz  the 320x213x21 5-iteration workload: V per term, SHA-256 of the
@@ -214,6 +215,17 @@ def learning_small():
                     p + "unary_param_grad": ugp, p + "compat_grad": cg, p + "params": crf.unary_params(),
                     p + "Q": crf.inference(n), p + "meta": np.array([W, H, L, F, n], np.int32)})
     out["nlog"] = np.int32(2)
+    # DenseCRF::unaryEnergy / pairwiseEnergy of a labelling (labels outside [0, L) give 0)
+    W, H, L = 19, 13, 5
+    img, U, gt = syn.make_image(W, H, L=L, sites=5, seed=977)
+    crf = po.RefCRF(W=W, H=H, L=L)
+    crf.set_unary(U)
+    crf.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    crf.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, img, np.arange(1, L + 1, dtype=np.float32), ntype=1)
+    lab = np.random.default_rng(978).integers(-1, L + 1, W * H).astype(np.int16)
+    out.update({"e_img": img, "e_U": U, "e_labels": lab, "e_unary": crf.unary_energy(lab),
+                "e_pair0": crf.pairwise_energy(lab, 0), "e_pair1": crf.pairwise_energy(lab, 1),
+                "e_pair": crf.pairwise_energy(lab, -1), "e_meta": np.array([W, H, L], np.int32)})
     np.savez_compressed(os.path.join(HERE, "learning_small.npz"), **out)
 
 

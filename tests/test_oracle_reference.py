@@ -293,3 +293,28 @@ def test_logistic_unary_fixture(i):
     g = (ug.astype(np.float64) @ f.T.astype(np.float64)).T.ravel()
     ref = z[p + "unary_param_grad"]
     assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_debug_energies_fixture():
+    """DenseCRF::unaryEnergy / pairwiseEnergy (densecrf.cpp:141-177) from the oracle's
+    pairwise messages of the one-hot labelling: bit-exact."""
+    z = _load("learning_small.npz")
+    W, H, L = [int(v) for v in z["e_meta"]]
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(z["e_U"])
+    o.add_gaussian(3.0, 2.0, 3.0, ntype=3)
+    o.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, z["e_img"], np.arange(1, L + 1, dtype=np.float32), ntype=1)
+    lab = z["e_labels"].astype(np.int64)
+    ok = (lab >= 0) & (lab < L)
+    idx = np.arange(W * H)
+    ue = np.where(ok, z["e_U"][np.clip(lab, 0, L - 1), idx], 0).astype(np.float32)
+    assert np.array_equal(bits(ue), bits(z["e_unary"]))
+    onehot = np.zeros((L, W * H), np.float32)
+    onehot[lab[ok], idx[ok]] = 1
+    total = np.zeros(W * H, np.float32)
+    for k in range(2):
+        m = o.apply(k, onehot)
+        e = np.where(ok, np.float32(-0.5) * m[np.clip(lab, 0, L - 1), idx], np.float32(0)).astype(np.float32)
+        assert np.array_equal(bits(e), bits(z["e_pair%d" % k]))
+        total = (total + e).astype(np.float32)
+    assert np.array_equal(bits(total), bits(z["e_pair"]))
