@@ -61,6 +61,9 @@ typedef struct dcrf_model dcrf_model;
 const char *dcrf_last_error(void);
 /* Number of visible HIP devices (0 without a GPU); never fails. */
 int dcrf_device_count(void);
+/* No counterpart in the reference: the library keeps freed device buffers for reuse
+ * (up to 4 GiB per process); this returns them to the runtime.  bytes may be NULL. */
+int dcrf_release_cached_memory(size_t *bytes);
 
 /* ---- construction --------------------------------------------------------------- */
 /* DenseCRF(N, M): densecrf.cpp:38-39, densecrf.pyx:53-64.  Touches no device. */

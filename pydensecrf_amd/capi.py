@@ -60,6 +60,7 @@ SIGNATURES = {
     "dcrf_gradient": (_c, [_P, _c, _S, _f, _D, _F, _F, _F, _F]),
     "dcrf_num_kernel_params": (_c, [_P]),
     "dcrf_set_compat_params": (_c, [_P, _F]),
+    "dcrf_release_cached_memory": (_c, [ctypes.POINTER(ctypes.c_size_t)]),
     "dcrf_unary_energy": (_c, [_P, _S, _F]),
     "dcrf_unary_from_softmax_device": (_c, [_P, _c, _c, _D_val, _D_val, _P, _P]),
     "dcrf_unary_from_labels_device": (_c, [_P, _c, _c, _D_val, _c, _P, _P]),

@@ -1173,6 +1173,13 @@ extern "C" {
 
 const char* dcrf_last_error(void) { return g_last_error.c_str(); }
 
+int dcrf_release_cached_memory(size_t* bytes) {
+    return guard([&] {
+        const size_t b = pool_release_all();
+        if (bytes) *bytes = b;
+    });
+}
+
 int dcrf_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -40,6 +40,12 @@ void set_error(int code, const std::string& msg);
     } while (0)
 
 // ---------------------------------------------------------------- device buffers
+// Caching pool over hipMalloc (pool.hip): a freed block is reused after the next device
+// synchronisation the pool performs, so that no stream can still be using it.
+void* pool_alloc(size_t bytes);
+void pool_free(void* p);
+size_t pool_release_all();
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -54,7 +60,7 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool_free(p);
         p = nullptr;
         n = 0;
     }
@@ -63,7 +69,7 @@ struct DevBuf {
         if (count <= n && p) return;
         release();
         size_t bytes = sizeof(T) * (count ? count : 1);
-        DCRF_HIP_CHECK(hipMalloc(&p, bytes));
+        p = static_cast<T*>(pool_alloc(bytes));
         n = count;
     }
     T* get() const { return p; }

@@ -263,7 +263,10 @@ __global__ void __launch_bounds__(kThreads) k_hash_insert(const Key<D>* __restri
             cur = old;
         }
         if (key_eq<D>(keys[cur], k)) {
-            atomicMin(&table[h], e);
+            // the slot keeps the smallest entry of its key (first occurrence); the value
+            // only falls, so an entry above the one just read cannot lower it: most
+            // entries of a popular vertex skip the atomic
+            if (e < cur) atomicMin(&table[h], e);
             slot[e] = (int)h;
             return;
         }
