@@ -152,3 +152,43 @@ def test_batched_volume_gradients_match_single():
         dk_sum = dk_sum + dks.astype(np.float64)
     np.testing.assert_allclose(dcb, dc_sum, rtol=1e-5, atol=1e-6 * np.abs(dc_sum).max())
     np.testing.assert_allclose(dkb, dk_sum, rtol=1e-5, atol=1e-6 * np.abs(dk_sum).max())
+
+
+def test_batched_kernel_gradients_match_single():
+    """Kernel-parameter gradients (learnable DIAG feature scales, set away from one so the
+    lattices are rebuilt) of a batch of volumes equal the sum of the per-volume ones: the
+    per-instance feature tables and lattices of the batch give each instance its own
+    parameter-gradient sums."""
+    import torch
+    from pydensecrf_amd.autograd import CRFLayer
+    L, n = 4, 4
+    dims = [(6, 9, 10), (8, 7, 6), (5, 10, 12)]
+    feats_g, feats_b, Us, gts, sizes = _volume_case(dims, L, 37)
+    w1 = np.array([1, 2, 2, 3], np.float32)
+    w2 = np.array([2, 1, 3, 1], np.float32)
+    kp = np.array([1.1, 0.9, 1.05, 0.8, 1.2, 0.95, 1.0], np.float32)
+
+    def run(model, U, gt):
+        layer = CRFLayer(model, n, [w1, w2], kparams=kp, learn_kernel=True)
+        Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+        Q = layer(Ut)
+        gtt = torch.tensor(gt, device="cuda")
+        loss = -torch.log(Q.gather(1, gtt[:, None])).sum()
+        loss.backward()
+        return Q.detach().cpu().numpy(), layer.kparams.grad.cpu().numpy().astype(np.float64)
+
+    b = BatchedDenseCRF(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseEnergy(feats_g, compat=w1)
+    b.addPairwiseEnergy(feats_b, compat=w2)
+    Qb, dkb = run(b.model, np.concatenate(Us, axis=1), np.concatenate(gts))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    dk_sum = 0.0
+    for i in range(len(dims)):
+        m = capi.Model.create(sizes[i], L)
+        m.add_pairwise(feats_g[i], w1)
+        m.add_pairwise(feats_b[i], w2)
+        Qs, dks = run(m, Us[i], gts[i])
+        np.testing.assert_array_equal(Qb[off[i]:off[i + 1]], Qs)
+        dk_sum = dk_sum + dks
+    np.testing.assert_allclose(dkb, dk_sum, rtol=1e-4, atol=1e-5 * np.abs(dk_sum).max())
