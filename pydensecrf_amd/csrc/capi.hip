@@ -1002,8 +1002,11 @@ void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compa
         const float* Qt = m.qcache.get() + (size_t)it * NL;
         float* Qn = m.qcache.get() + (size_t)(it + 1) * NL;
         run_filter(m, Qt, L, false, s);
-        launch_meanfield(U, Qn, N, L, slice_table(m, false, kw, m.mcache.get() + (size_t)it * T * NL), nullptr,
-                         nullptr, s);
+        const SliceTable st = slice_table(m, false, kw, m.mcache.get() + (size_t)it * T * NL);
+        if (m.fused)  // same arithmetic as k_meanfield, gathers of all entries up front
+            launch_meanfield_fused(U, Qn, L, N, L, st, m.g_slc.get(), s);
+        else
+            launch_meanfield(U, Qn, N, L, st, nullptr, nullptr, s);
     }
     m.train_n = n_iter;
     m.train_kw = kw;

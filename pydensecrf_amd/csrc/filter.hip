@@ -467,11 +467,13 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
 
 // The per-pixel tail of the fused mean-field kernels: per term the weighted sum of the
 // gathered rows (entry order), the term weight and the compatibility, subtracted from
-// -U; then the softmax over the lane group, stored when `active`.
+// -U; then the softmax over the lane group, stored when `active`.  A term with `mout`
+// also stores its raw message (the sum before the term weight) at pixel p, as
+// k_meanfield does.
 template <int R, int D0, int D1>
 __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceTable& st, const int2 (&e)[D0 + D1],
                                              const float (&x)[D0 + D1][R], const float (&u)[R], bool active,
-                                             float* __restrict__ qrow, const float* shc) {
+                                             float* __restrict__ qrow, const float* shc, int p) {
     constexpr int DS = D0 + D1;
     float en[R];
 #pragma unroll
@@ -492,6 +494,7 @@ __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceT
                 m[r] = m[r] + prod;
             }
         }
+        if (T.mout && active) store_piece<R>(T.mout + (size_t)p * G.L, G.k0, G.L, m);
         if (T.kw) {
             const float kw = T.kw[0];
 #pragma unroll
@@ -570,7 +573,7 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
         const SliceTerm& T = j < D0 ? st.t[0] : st.t[1];
         load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
     }
-    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc);
+    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc, p);
 }
 
 // The same update over pixel runs (2-D rectangles whose rows hold whole waves of
@@ -652,7 +655,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_runs(const float* __re
         float x[DS][R];
 #pragma unroll
         for (int j = 0; j < DS; j++) load_piece<R>(tq + (size_t)e[j].x * L, k0, L, x[j]);
-        finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc);
+        finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc, p);
     }
 }
 

@@ -213,6 +213,29 @@ def test_gradient_label_shapes(L):
     assert _rel_close(cg, cgo, 1e-4)
 
 
+@pytest.mark.parametrize("L,terms", [(21, 2), (4, 2), (9, 1), (33, 2)])
+def test_training_forward_fused_equals_generic(L, terms):
+    """The training forward runs the fused mean-field kernel (which also stores the raw
+    messages) where the model has one; it performs k_meanfield's arithmetic in the same
+    order, so every gradient output is bit-identical to the generic kernel's."""
+    W, H, n = 41, 29, 4
+    img, U, gt = image_case(W, H, L, seed=100 + L)
+    rng = np.random.default_rng(L)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = (-2 * np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    outs = []
+    for mode in (1, 0):
+        g = capi.Model.create_2d(W, H, L)
+        g.set_unary(U)
+        g.add_gaussian(3.0, 3.0, c1)
+        if terms > 1:
+            g.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c2)
+        g.set_meanfield_kernel(mode)
+        outs.append(g.gradient(n, gt, kernel=True))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
 # ---------------------------------------------------------------- kernel parameters
 # DenseCRF::gradient's kernel_grad (pairwise.cpp:152-163, 202-207, permutohedral.cpp:
 # 612-695) and kernelParameters / setKernelParameters (densecrf.cpp:334-360).  The GPU
