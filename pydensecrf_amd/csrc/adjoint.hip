@@ -24,21 +24,6 @@ namespace {
 constexpr int kThreads = 256;
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
-// sumAndNormalize of x = Q .* g over the lane group: b = sum(x) q - x.
-template <bool WAVE, int R>
-__device__ __forceinline__ void sum_normalize(const float (&q)[R], const float (&gv)[R], float (&b)[R],
-                                              LaneGroup<WAVE, R>& G) {
-    float x[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) x[r] = q[r] * gv[r];
-    const float s = G.all_sum(x);
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const float sq = s * q[r];
-        b[r] = sq - x[r];
-    }
-}
-
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, const float* __restrict__ dQ,
                                                    float* __restrict__ b, float* __restrict__ dU, int N, int L,

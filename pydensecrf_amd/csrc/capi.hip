@@ -95,6 +95,8 @@ struct dcrf_model {
     DevBuf<int2> g_entF, g_entT;  // splat entries (pixel, weight bits), forward / transposed
     bool fused = false;           // inference through k_meanfield_fused
     DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
+    DevBuf<int2> g_slcT;          // the same with the transposed filter's slice weights (adjoint;
+    bool slcT_valid = false;      // built on first use when they differ from the forward's)
     bool runs = false;            // ... or through k_meanfield_runs (rows staged per pixel run)
     int mf_mode = 0;              // dcrf_set_meanfield_kernel: 0 auto (fused), 1 generic, 2 fused, 3 runs
     bool pairs = true;            // blur up to three steps per pass (dcrf_set_blur_pairs)
@@ -800,6 +802,7 @@ void ensure_layout(dcrf_model& m) {
     }
     // interleaved slice pairs for the fused mean-field kernel
     m.runs = false;
+    m.slcT_valid = false;
     m.fused = m.mf_mode != 1 &&
               fused_meanfield_ok(L, T, T > 0 ? m.terms[0]->lat.d + 1 : 0, T > 1 ? m.terms[1]->lat.d + 1 : 0);
     if (m.fused) {
@@ -1015,6 +1018,27 @@ void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compa
 // Adjoint of forward_train.  dQ: dL/dQ_n.  Outputs dU (always), per-term statistics in
 // m.gacc, and optionally raw compatibility / term-weight gradients on the device.
 // kernel: also accumulate the kernel-parameter gradient of every term into m.kacc.
+// The interleaved slice pairs of the adjoint's fused kernel: the forward's when every
+// term's transposed slice weights equal its forward ones (SYMMETRIC / NO_NORMALIZATION),
+// else pairs with the transposed weights, built once per layout.
+const int2* adjoint_pairs(dcrf_model& m, hipStream_t s) {
+    if (m.all_symmetric) return m.g_slc.get();
+    if (!m.slcT_valid) {
+        const int T = (int)m.terms.size(), N = m.N();
+        int ds = 0;
+        for (int k = 0; k < T; k++) ds += m.terms[k]->lat.d + 1;
+        m.g_slcT.reserve((size_t)N * ds);
+        int off = 0;
+        for (int k = 0; k < T; k++) {
+            const int d1 = m.terms[k]->lat.d + 1;
+            launch_interleave_slice(m.tl[k]->vtx.get(), m.tl[k]->swT.get(), N, d1, ds, off, m.g_slcT.get(), s);
+            off += d1;
+        }
+        m.slcT_valid = true;
+    }
+    return m.g_slcT.get();
+}
+
 void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* dkw, hipStream_t s,
               bool kernel = false) {
     if (m.train_n < 0) throw Error{DCRF_ERR_STATE, "backward needs a preceding training forward"};
@@ -1054,7 +1078,11 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
                                      m.kacc.get() + koff[k], s);
         }
         run_filter(m, b, L, true, s);
-        launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);
+        if (m.fused)
+            launch_adjoint_fused(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw),
+                                 adjoint_pairs(m, s), s);
+        else
+            launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);
     }
     if (dcompat || dkw) {
         size_t coff = 0;

@@ -244,6 +244,28 @@ __device__ __forceinline__ void stage_compat(const SliceTable& st, int L, float*
     if (any) __syncthreads();
 }
 
+// c[r] = sum_b cw[k0 + r][b] m_b over the pixel's whole message vector (read from its
+// group in label order), one fused multiply-add per weight.
+template <bool WAVE, int R>
+__device__ __forceinline__ void matrix_piece(const float* __restrict__ cw, const float (&m)[R], float (&c)[R],
+                                             LaneGroup<WAVE, R>& G) {
+    const int L = G.L;
+#pragma unroll
+    for (int r = 0; r < R; r++) c[r] = 0.0f;
+    for (int s = 0; s < G.S; s++) {
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+            const float mb = G.lane_value(m[rr], s);
+            const int b = s * R + rr;
+            if (b < L) {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (G.has(r)) c[r] = fmaf(cw[(size_t)(G.k0 + r) * L + b], mb, c[r]);
+            }
+        }
+    }
+}
+
 // Compatibility transform of the lane's message values (labelcompatibility.cpp:46-48,
 // 66-69, 82-84).  Matrix reads the pixel's whole message vector from its group, in
 // label order, and its weights from the LDS copy when there is one.  All lanes must
@@ -262,24 +284,27 @@ __device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m
         for (int r = 0; r < R; r++) c[r] = G.has(r) ? T.cw[G.k0 + r] * m[r] : 0.0f;
         return;
     }
-    const int L = G.L;
-    const float* cw = T.cw_lds >= 0 ? shc + T.cw_lds : T.cw;
+    // two copies of the loop, one per weight source, so that each pointer keeps its
+    // address space (one select between them would make every weight read a flat load)
+    if (T.cw_lds >= 0)
+        matrix_piece<WAVE, R>(shc + T.cw_lds, m, c, G);
+    else
+        matrix_piece<WAVE, R>(T.cw, m, c, G);
+}
+
+// sumAndNormalize (densecrf.cpp:107-115) of x = Q .* g over the lane group:
+// b = sum(x) q - x.
+template <bool WAVE, int R>
+__device__ __forceinline__ void sum_normalize(const float (&q)[R], const float (&gv)[R], float (&b)[R],
+                                              LaneGroup<WAVE, R>& G) {
+    float x[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) c[r] = 0.0f;
-    for (int s = 0; s < G.S; s++) {
+    for (int r = 0; r < R; r++) x[r] = q[r] * gv[r];
+    const float s = G.all_sum(x);
 #pragma unroll
-        for (int rr = 0; rr < R; rr++) {
-            const float mb = G.lane_value(m[rr], s);
-            const int b = s * R + rr;
-            if (b < L) {
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if (G.has(r)) {
-                        const float prod = cw[(size_t)(G.k0 + r) * L + b] * mb;
-                        c[r] = c[r] + prod;
-                    }
-            }
-        }
+    for (int r = 0; r < R; r++) {
+        const float sq = s * q[r];
+        b[r] = sq - x[r];
     }
 }
 

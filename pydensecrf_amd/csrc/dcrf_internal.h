@@ -351,6 +351,10 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s);
 // b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) in place; dU += b.
 void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s);
+// The same over interleaved slice pairs (those of launch_meanfield_fused, with the
+// transposed filter's slice weights); models that fused_meanfield_ok accepts.
+void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, const int2* slc,
+                          hipStream_t s);
 // acc += sum_p b_p M_p^T (full L*L when `full`, else its diagonal [L]); partial holds
 // max(grad_partial_floats(), grad_partial_blocks(N, L) * L) floats.
 int grad_partial_blocks(int N, int L);

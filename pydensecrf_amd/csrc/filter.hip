@@ -465,42 +465,50 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
     }
 }
 
-// The per-pixel tail of the fused mean-field kernels: per term the weighted sum of the
-// gathered rows (entry order), the term weight and the compatibility, subtracted from
-// -U; then the softmax over the lane group, stored when `active`.  A term with `mout`
-// also stores its raw message (the sum before the term weight) at pixel p, as
-// k_meanfield does.
+// Term k's part of a fused pixel update: the weighted sum of its gathered rows (entry
+// order, as slice_piece), stored raw to `mout` at pixel p when the term has one (as
+// k_meanfield does), then the term weight and the compatibility into c.
+template <int R, int D0, int D1>
+__device__ __forceinline__ void term_compat(LaneGroup<true, R>& G, const SliceTerm& T, int k,
+                                            const int2 (&e)[D0 + D1], const float (&x)[D0 + D1][R], bool active,
+                                            int p, const float* shc, float (&c)[R]) {
+    constexpr int DS = D0 + D1;
+    const int jb = k ? D0 : 0, je = k ? DS : D0;
+    float m[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) m[r] = 0.0f;
+#pragma unroll
+    for (int j = jb; j < je; j++) {
+        const float w = __int_as_float(e[j].y);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float prod = w * x[j][r];
+            m[r] = m[r] + prod;
+        }
+    }
+    if (T.mout && active) store_piece<R>(T.mout + (size_t)p * G.L, G.k0, G.L, m);
+    if (T.kw) {
+        const float kw = T.kw[0];
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+    }
+    compat_piece<true, R>(T, m, c, G, shc);
+}
+
+// The per-pixel tail of the fused mean-field kernels: the terms' compatibility
+// messages subtracted from -U, then the softmax over the lane group, stored when
+// `active`.
 template <int R, int D0, int D1>
 __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceTable& st, const int2 (&e)[D0 + D1],
                                              const float (&x)[D0 + D1][R], const float (&u)[R], bool active,
                                              float* __restrict__ qrow, const float* shc, int p) {
-    constexpr int DS = D0 + D1;
     float en[R];
 #pragma unroll
     for (int r = 0; r < R; r++) en[r] = -u[r];
 #pragma unroll
     for (int k = 0; k < (D1 ? 2 : 1); k++) {
-        const SliceTerm& T = st.t[k];
-        const int jb = k ? D0 : 0, je = k ? DS : D0;
-        float m[R], c[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) m[r] = 0.0f;
-#pragma unroll
-        for (int j = jb; j < je; j++) {
-            const float w = __int_as_float(e[j].y);
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const float prod = w * x[j][r];
-                m[r] = m[r] + prod;
-            }
-        }
-        if (T.mout && active) store_piece<R>(T.mout + (size_t)p * G.L, G.k0, G.L, m);
-        if (T.kw) {
-            const float kw = T.kw[0];
-#pragma unroll
-            for (int r = 0; r < R; r++) m[r] = m[r] * kw;
-        }
-        compat_piece<true, R>(T, m, c, G, shc);
+        float c[R];
+        term_compat<R, D0, D1>(G, st.t[k], k, e, x, active, p, shc, c);
 #pragma unroll
         for (int r = 0; r < R; r++) en[r] = en[r] - c[r];
     }
@@ -515,6 +523,30 @@ __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceT
         for (int r = 0; r < R; r++) q[r] = y[r] / sum;
         store_piece<R>(qrow, G.k0, G.L, q);
     }
+}
+
+// Stage the slice pairs of the wave's pixels (consecutive: one contiguous run of
+// per * DS pairs, 64 pairs per load instruction) into the wave's LDS slice wp; the
+// wave's groups then read their own.  `p` is the group's pixel.
+template <int R, int DS>
+__device__ __forceinline__ void stage_wave_pairs(const LaneGroup<true, R>& G, int p, int N,
+                                                 const int2* __restrict__ slc, int2* wp) {
+    const int per = G.per;
+    const int pw0 = p - G.g;  // the wave's first pixel
+    const int np = max(0, min(per, N - pw0)) * DS;
+    const int lane = threadIdx.x & 63;
+    // two loads per lane per round (clamped, unconditional), then the stores
+    const size_t base = (size_t)pw0 * DS;
+    const int last = max(np - 1, 0);
+    for (int i0 = 0; i0 < np; i0 += 128) {
+        const int2 t0 = slc[base + min(i0 + lane, last)];
+        const int2 t1 = slc[base + min(i0 + 64 + lane, last)];
+        if (i0 + lane < np) wp[i0 + lane] = t0;
+        if (i0 + 64 + lane < np) wp[i0 + 64 + lane] = t1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Mean-field update for models of one or two terms with d+1 = D0 (and D1) lattice
@@ -532,31 +564,13 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
     const bool active = G.on && p < N;
     const int k0 = G.k0;
     const int pc = active ? p : 0;  // idle lanes read pixel 0 (in bounds; results unused)
-    // the wave's pixels are consecutive: load their slice pairs as one contiguous run
-    // (64 pairs per load instruction) into LDS, then each group reads its own
     extern __shared__ int2 pairs_all[];
     int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
     // CWL: some Matrix term has an LDS copy of its weights (staging costs a barrier, so
     // it is compiled in only then)
     float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
     if (CWL) stage_compat(st, L, shc);
-    {
-        const int pw0 = p - G.g;  // the wave's first pixel
-        const int np = max(0, min(per, N - pw0)) * DS;
-        const int lane = threadIdx.x & 63;
-        // two loads per lane per round (clamped, unconditional), then the stores
-        const size_t base = (size_t)pw0 * DS;
-        const int last = max(np - 1, 0);
-        for (int i0 = 0; i0 < np; i0 += 128) {
-            const int2 t0 = slc[base + min(i0 + lane, last)];
-            const int2 t1 = slc[base + min(i0 + 64 + lane, last)];
-            if (i0 + lane < np) wp[i0 + lane] = t0;
-            if (i0 + 64 + lane < np) wp[i0 + 64 + lane] = t1;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    stage_wave_pairs<R, DS>(G, p, N, slc, wp);
     int2 e[DS];
     float u[R];
 #pragma unroll
@@ -574,6 +588,58 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
         load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
     }
     finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc, p);
+}
+
+// The adjoint step of k_adjoint (adjoint.hip) over the same interleaved slice pairs:
+// g = sum_k C_k(w_k slice_k(filtered b)), b = sumAndNormalize(Q .* g, Q), dU += b.  The
+// pairs carry the transposed filter's slice weights.  Same arithmetic, in the same
+// order, as k_adjoint.
+template <int R, int D0, int D1, bool CWL>
+__global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __restrict__ dU, int N, int L, int S,
+                int per, SliceTable st, const int2* __restrict__ slc) {
+    constexpr int DS = D0 + D1;
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const int k0 = G.k0;
+    const int pc = active ? p : 0;
+    extern __shared__ int2 pairs_all[];
+    int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
+    float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
+    if (CWL) stage_compat(st, L, shc);
+    stage_wave_pairs<R, DS>(G, p, N, slc, wp);
+    int2 e[DS];
+#pragma unroll
+    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
+    float q[R];
+    load_piece<R>(Q + (size_t)pc * L, k0, L, q);
+    float x[DS][R];
+#pragma unroll
+    for (int j = 0; j < DS; j++) {
+        const SliceTerm& T = j < D0 ? st.t[0] : st.t[1];
+        load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
+    }
+    float gsum[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) gsum[r] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < (D1 ? 2 : 1); k++) {
+        float c[R];
+        term_compat<R, D0, D1>(G, st.t[k], k, e, x, active, p, shc, c);
+#pragma unroll
+        for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[r];
+    }
+    float bn[R];
+    sum_normalize<true, R>(q, gsum, bn, G);
+    if (active) {
+        float du[R];
+        load_piece<R>(dU + (size_t)p * L, k0, L, du);
+#pragma unroll
+        for (int r = 0; r < R; r++) du[r] = du[r] + bn[r];
+        store_piece<R>(b + (size_t)p * L, k0, L, bn);
+        store_piece<R>(dU + (size_t)p * L, k0, L, du);
+    }
 }
 
 // The same update over pixel runs (2-D rectangles whose rows hold whole waves of
@@ -1001,6 +1067,25 @@ void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, in
 }
 #undef DCRF_FUSED
 
+#define DCRF_ADJ_FUSED(R_, D0_, D1_)                                                                      \
+    (cwf ? k_adjoint_fused<R_, D0_, D1_, true> : k_adjoint_fused<R_, D0_, D1_, false>)<<<                   \
+        blocks_for(N, sh.items), kWaveBlock,                                                                  \
+        (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) + (size_t)cwf * sizeof(float), s>>>( \
+        Q, b, dU, N, L, sh.S, sh.per, st, slc)
+
+template <int R>
+void launch_adj_fused_r(const LaunchShape& sh, const float* Q, float* b, float* dU, int N, int L,
+                        const SliceTable& st, const int2* slc, int cwf, hipStream_t s) {
+    const int a = st.t[0].d1, c = st.nterms > 1 ? st.t[1].d1 : 0;
+    if (a == 3 && c == 0) DCRF_ADJ_FUSED(R, 3, 0);
+    else if (a == 6 && c == 0) DCRF_ADJ_FUSED(R, 6, 0);
+    else if (a == 3 && c == 6) DCRF_ADJ_FUSED(R, 3, 6);
+    else if (a == 6 && c == 3) DCRF_ADJ_FUSED(R, 6, 3);
+    else throw Error{DCRF_ERR_VALUE, "no fused adjoint kernel for these terms"};
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+#undef DCRF_ADJ_FUSED
+
 #define DCRF_RUNS(R_, D0_, D1_)                                                                           \
     k_meanfield_runs<R_, D0_, D1_><<<rs.n_runs, kWaveBlock, lds, s>>>(U, Q, N, L, sh.S, sh.per, st, rs)
 
@@ -1066,6 +1151,21 @@ void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, con
         case 2: launch_fused_r<2>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
         case 3: launch_fused_r<3>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
         default: launch_fused_r<4>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
+    }
+}
+
+void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st_in,
+                          const int2* slc, hipStream_t s) {
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
+    const LaunchShape sh = shape_for(L);
+    if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
+        throw Error{DCRF_ERR_VALUE, "no fused adjoint kernel for these terms"};
+    switch (sh.R) {
+        case 1: launch_adj_fused_r<1>(sh, Q, b, dU, N, L, st, slc, cwf, s); break;
+        case 2: launch_adj_fused_r<2>(sh, Q, b, dU, N, L, st, slc, cwf, s); break;
+        case 3: launch_adj_fused_r<3>(sh, Q, b, dU, N, L, st, slc, cwf, s); break;
+        default: launch_adj_fused_r<4>(sh, Q, b, dU, N, L, st, slc, cwf, s); break;
     }
 }
 

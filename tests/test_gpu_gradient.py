@@ -213,11 +213,15 @@ def test_gradient_label_shapes(L):
     assert _rel_close(cg, cgo, 1e-4)
 
 
-@pytest.mark.parametrize("L,terms", [(21, 2), (4, 2), (9, 1), (33, 2)])
-def test_training_forward_fused_equals_generic(L, terms):
-    """The training forward runs the fused mean-field kernel (which also stores the raw
-    messages) where the model has one; it performs k_meanfield's arithmetic in the same
-    order, so every gradient output is bit-identical to the generic kernel's."""
+@pytest.mark.parametrize("L,terms,nt", [(21, 2, capi.NORMALIZE_SYMMETRIC), (4, 2, capi.NORMALIZE_AFTER),
+                                         (9, 1, capi.NORMALIZE_BEFORE), (33, 2, capi.NO_NORMALIZATION),
+                                         (21, 2, capi.NORMALIZE_BEFORE)])
+def test_training_forward_fused_equals_generic(L, terms, nt):
+    """The training forward and the adjoint run the fused kernels (the forward also
+    storing the raw messages; the adjoint over slice pairs with the transposed filter's
+    weights) where the model has them; they perform k_meanfield's and k_adjoint's
+    arithmetic in the same order, so every gradient output is bit-identical to the
+    generic kernels'."""
     W, H, n = 41, 29, 4
     img, U, gt = image_case(W, H, L, seed=100 + L)
     rng = np.random.default_rng(L)
@@ -227,9 +231,9 @@ def test_training_forward_fused_equals_generic(L, terms):
     for mode in (1, 0):
         g = capi.Model.create_2d(W, H, L)
         g.set_unary(U)
-        g.add_gaussian(3.0, 3.0, c1)
+        g.add_gaussian(3.0, 3.0, c1, ntype=nt)
         if terms > 1:
-            g.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c2)
+            g.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c2, ntype=nt)
         g.set_meanfield_kernel(mode)
         outs.append(g.gradient(n, gt, kernel=True))
     for a, b in zip(outs[0], outs[1]):
