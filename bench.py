@@ -225,13 +225,23 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
     ms_rebuild = (time.perf_counter() - t0) * 1e3 / 3
     m.set_kernel_params(kp)
     log("fwd+bwd with kernel gradient %.3f ms; lattice rebuild %.2f ms" % (ms_fbk, ms_rebuild))
+    # one energy evaluation of the learning loop (dcrf_gradient_objective, LogLikelihood,
+    # compatibility gradient only): ground truth uploaded, objective on the device, small
+    # results copied back
+    gt16 = gt.astype(np.int16)
+    m.gradient_objective(iters, gt16, unary_grad=False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.gradient_objective(iters, gt16, unary_grad=False)
+    ms_energy = (time.perf_counter() - t0) * 1e3 / reps
+    log("learning energy evaluation %.3f ms" % ms_energy)
     return {"workload": "DenseCRF2D 512x512, 21 labels, %d iterations, MatrixCompatibility + term weights, "
                         "cross-entropy, torch.autograd" % iters,
             "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
             "fwd_Mpixel_iters_per_s": round(W * H * iters / (ms_f / 1e3) / 1e6, 2),
             "fwd_bwd_Mpixel_iters_per_s": round(W * H * iters / (ms_fb / 1e3) / 1e6, 2),
             "fwd_bwd_kernel_ms": round(ms_fbk, 4), "kernel_ratio": round(ms_fbk / ms_f, 3),
-            "kernel_param_rebuild_ms": round(ms_rebuild, 3)}
+            "kernel_param_rebuild_ms": round(ms_rebuild, 3), "learning_energy_eval_ms": round(ms_energy, 3)}
 
 
 def bench_volumes(torch, capi, syn, iters, reps, log):

@@ -200,7 +200,9 @@ int dcrf_gradient(dcrf_model *m, int n_iterations, const int16_t *gt, float robu
  * objective.cpp:37-108): DCRF_OBJECTIVE_LOG_LIKELIHOOD (param = robust),
  * DCRF_OBJECTIVE_HAMMING (class_weight[n_class_weight], or NULL and param =
  * class_weight_pow: weights from the label counts of gt), DCRF_OBJECTIVE_IOU (mean
- * intersection over union of the soft labelling).  objective = the objective's value. */
+ * intersection over union of the soft labelling).  objective = the objective's value.
+ * The objective and dL/dQ are evaluated on the device (gt is uploaded; Q_n stays in
+ * device memory); sums over pixels are double in a fixed order. */
 #define DCRF_OBJECTIVE_LOG_LIKELIHOOD 0
 #define DCRF_OBJECTIVE_HAMMING 1
 #define DCRF_OBJECTIVE_IOU 2

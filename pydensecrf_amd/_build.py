@@ -23,7 +23,8 @@ INCLUDE = os.path.join(ROOT, "include")
 ORACLE = os.path.join(ROOT, "oracle")
 REFERENCE = "/root/reference"
 
-HIP_SOURCES = ["pool.hip", "lattice.hip", "filter.hip", "adjoint.hip", "kgrad.hip", "unary.hip", "capi.hip"]
+HIP_SOURCES = ["pool.hip", "lattice.hip", "filter.hip", "adjoint.hip", "kgrad.hip", "unary.hip", "objective.hip",
+               "capi.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # keep the compiler from forming packed-FP32 instructions

@@ -327,13 +327,16 @@ class Model:
                              % (lib().dcrf_num_compat_params(self.h), a.size))
         check(lib().dcrf_set_compat_params(self.h, fptr(a)))
 
-    def gradient_objective(self, n, gt, objective=0, param=0.0, class_weight=None, kernel=False):
+    def gradient_objective(self, n, gt, objective=0, param=0.0, class_weight=None, kernel=False, unary_grad=True):
         """DenseCRF::gradient with LOG_LIKELIHOOD (param = robust), HAMMING (class_weight, or
-        param = class_weight_pow) or IOU: (objective, unary_grad (L, N), compat_grad packed,
-        kernel-weight grad [T], kernel-parameter grad or None)."""
+        param = class_weight_pow) or IOU: (objective, unary_grad (L, N) or None when not
+        asked for, compat_grad packed, kernel-weight grad [T], kernel-parameter grad or
+        None)."""
         g = np.ascontiguousarray(gt, dtype=np.int16)
+        if g.shape != (self.N,):
+            raise ValueError("gt must have one label per variable (%d), got %s" % (self.N, g.shape))
         obj = ctypes.c_double()
-        ug = self._buf()
+        ug = self._buf() if unary_grad else None
         cg = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)
         kg = np.empty(max(self.nterms, 1), np.float32)
         nk = lib().dcrf_num_kernel_params(self.h)
@@ -341,10 +344,10 @@ class Model:
         cw = None if class_weight is None else np.ascontiguousarray(class_weight, dtype=np.float32).ravel()
         check(lib().dcrf_gradient_objective(self.h, n, objective, g.ctypes.data_as(_S), param,
                                             None if cw is None else fptr(cw), 0 if cw is None else cw.size,
-                                            ctypes.byref(obj), fptr(ug), fptr(cg), fptr(kg),
+                                            ctypes.byref(obj), fptr(ug) if unary_grad else None, fptr(cg), fptr(kg),
                                             fptr(kp) if kernel else None))
-        return (obj.value, ug.T, cg[:lib().dcrf_num_compat_params(self.h)], kg[:self.nterms],
-                kp[:nk] if kernel else None)
+        return (obj.value, ug.T if unary_grad else None, cg[:lib().dcrf_num_compat_params(self.h)],
+                kg[:self.nterms], kp[:nk] if kernel else None)
 
     def set_logistic_unary(self, Lmat, feat):
         """LogisticUnaryEnergy: U = Lmat feat, Lmat (L, F), feat (F, N)."""
@@ -392,6 +395,8 @@ class Model:
         """DenseCRF::gradient with LogLikelihood: (objective, unary_grad (L, N),
         compat_grad packed, kernel-weight grad [T][, kernel-parameter grad])."""
         g = np.ascontiguousarray(gt, dtype=np.int16)
+        if g.shape != (self.N,):
+            raise ValueError("gt must have one label per variable (%d), got %s" % (self.N, g.shape))
         obj = ctypes.c_double()
         ug = self._buf()
         cg = np.empty(max(lib().dcrf_num_compat_params(self.h), 1), np.float32)

@@ -116,6 +116,10 @@ struct dcrf_model {
     const float* train_kw = nullptr;
     DevBuf<float> qcache, mcache, bbuf, partial;
     DevBuf<double> gacc, gacc2;
+    // objective on the device: ground truth, Hamming's class weights, partial sums
+    DevBuf<int16_t> gt_dev;
+    DevBuf<float> obj_w;
+    DevBuf<double> obj_partial;
     // kernel-parameter gradient: 5 [N][L] work slabs, 2 [N][d] feature gradients,
     // per-block partials and the accumulated gradient of all terms
     DevBuf<float> kgbuf, kgdf;
@@ -1096,76 +1100,48 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     }
 }
 
-// ObjectiveFunction::evaluate (objective.cpp:37-108) on the final Q (host [N][L]):
-// returns the objective and writes dQ = dL/dQ, which the adjoint multiplies by Q into
-// the reference's d_mul_Q.
-//   LogLikelihood(gt, robust)  r = sum log(max(Q_gt + robust, 1e-20)) / N; dQ_gt = 1/QQ/N
-//   Hamming(gt, w)             r = sum w_gt Q_gt;                      dQ_gt = w_gt
-//     (w given, or from the class counts c: (c/sum c)^-pow normalised by sum c w, :51-63)
-//   IntersectionOverUnion(gt)  r = sum_l in_l / un_l / M (in, un in double, un from 1e-20)
-//     dQ(l, i) = 1 / (un_l M) for l = gt_i, else -in_l / (un_l^2 M)
-double evaluate_objective(int kind, const int16_t* gt, float param, const float* cw_in, int ncw,
-                          const std::vector<float>& Q, int N, int L, std::vector<float>& dQ) {
+// Hamming's class weights (objective.cpp:51-63): the given ones, or from the class
+// counts c of gt as (c / sum c)^-pow normalised by sum c w.  Checks that every label of
+// gt inside [0, L) has a weight.
+std::vector<float> hamming_weights(const int16_t* gt, int N, int L, float pow_, const float* cw_in, int ncw) {
+    std::vector<float> w;
+    if (cw_in) {
+        w.assign(cw_in, cw_in + ncw);
+    } else {
+        int M = 0;
+        for (int i = 0; i < N; i++) M = std::max(M, gt[i] + 1);
+        std::vector<float> cnt(M, 0.f);
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0) cnt[gt[i]] += 1.0f;
+        float total = 0.f;
+        for (float c : cnt) total += c;
+        w.resize(M);
+        float z = 0.f;
+        for (int l = 0; l < M; l++) {
+            w[l] = std::pow(cnt[l] / total, -pow_);
+            z += cnt[l] * w[l];
+        }
+        for (int l = 0; l < M; l++) w[l] = w[l] / z;
+    }
+    for (int i = 0; i < N; i++) {
+        const int g = gt[i];
+        if (g >= 0 && g < L) require(g < (int)w.size(), "a ground-truth label has no class weight");
+    }
+    if (w.empty()) w.assign(1, 0.f);  // no labelled pixel: the kernel reads no weight
+    return w;
+}
+
+// The objective's value from launch_objective's partial sums (OP: its partial array):
+// LogLikelihood / Hamming add the block sums in order; IoU is sum_l in_l / un_l / L.
+double objective_value(int kind, const std::vector<double>& OP, int L) {
     double r = 0.0;
-    if (kind == DCRF_OBJECTIVE_LOG_LIKELIHOOD) {
-        for (int i = 0; i < N; i++) {
-            const int g = gt[i];
-            if (g < 0 || g >= L) continue;
-            const float q = Q[(size_t)i * L + g];
-            const float qr = q + param;
-            const float QQ = qr > 1e-20f ? qr : 1e-20f;
-            r += std::log((double)QQ) / N;
-            dQ[(size_t)i * L + g] = 1.0f / QQ / (float)N;
-        }
-        return r;
+    if (kind == DCRF_OBJECTIVE_IOU) {
+        const double* st = OP.data() + OP.size() - 2 * (size_t)L;
+        for (int l = 0; l < L; l++) r += st[2 * l] / st[2 * l + 1];
+        return r / L;
     }
-    if (kind == DCRF_OBJECTIVE_HAMMING) {
-        std::vector<float> w;
-        if (cw_in) {
-            w.assign(cw_in, cw_in + ncw);
-        } else {
-            int M = 0;
-            for (int i = 0; i < N; i++) M = std::max(M, gt[i] + 1);
-            std::vector<float> cnt(M, 0.f);
-            for (int i = 0; i < N; i++)
-                if (gt[i] >= 0) cnt[gt[i]] += 1.0f;
-            float total = 0.f;
-            for (float c : cnt) total += c;
-            w.resize(M);
-            float z = 0.f;
-            for (int l = 0; l < M; l++) {
-                w[l] = std::pow(cnt[l] / total, -param);
-                z += cnt[l] * w[l];
-            }
-            for (int l = 0; l < M; l++) w[l] = w[l] / z;
-        }
-        for (int i = 0; i < N; i++) {
-            const int g = gt[i];
-            if (g < 0 || g >= L) continue;
-            require(g < (int)w.size(), "a ground-truth label has no class weight");
-            const float QQ = w[g] * Q[(size_t)i * L + g];
-            r += QQ;
-            dQ[(size_t)i * L + g] = w[g];
-        }
-        return r;
-    }
-    std::vector<double> in(L, 0.0), un(L, 1e-20);
-    for (int i = 0; i < N; i++) {
-        const int g = gt[i];
-        if (g < 0 || g >= L) continue;
-        in[g] += Q[(size_t)i * L + g];
-        un[g] += 1;
-        for (int l = 0; l < L; l++)
-            if (l != g) un[l] += Q[(size_t)i * L + l];
-    }
-    for (int i = 0; i < N; i++) {
-        const int g = gt[i];
-        if (g < 0 || g >= L) continue;
-        for (int l = 0; l < L; l++)
-            dQ[(size_t)i * L + l] = l == g ? (float)(1.0 / (un[l] * L)) : (float)(-in[l] / (un[l] * un[l] * L));
-    }
-    for (int l = 0; l < L; l++) r += in[l] / un[l];
-    return r / L;
+    for (double v : OP) r += v;
+    return r;
 }
 
 dcrf_model* new_model(const std::vector<int>& N, const std::vector<int>* W, const std::vector<int>* H, int L) {
@@ -1770,16 +1746,27 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
         hipStream_t s = m->stream;
         const float* U = m->has_unary ? m->unary.get() : nullptr;
         const float* kw = m->kw_set ? m->kw.get() : nullptr;
+        // Hamming's class weights come from the host's ground truth (objective.cpp:51-63)
+        std::vector<float> hw;
+        if (objective_kind == DCRF_OBJECTIVE_HAMMING) hw = hamming_weights(gt, N, L, param, class_weight,
+                                                                              n_class_weight);
         forward_train(*m, n_iter, U, nullptr, kw, s);
-        std::vector<float> Qn(NL), dQ(NL, 0.f);
-        DCRF_HIP_CHECK(hipMemcpyAsync(Qn.data(), m->qcache.get() + (size_t)n_iter * NL, sizeof(float) * NL,
-                                      hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
-        const double r = evaluate_objective(objective_kind, gt, param, class_weight, n_class_weight, Qn, N, L, dQ);
-        if (objective) *objective = r;
         m->work.reserve(NL);
         m->tmp1.reserve(NL);
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), dQ.data(), sizeof(float) * NL, hipMemcpyHostToDevice, s));
+        m->gt_dev.reserve(N);
+        DCRF_HIP_CHECK(hipMemcpyAsync(m->gt_dev.get(), gt, sizeof(int16_t) * N, hipMemcpyHostToDevice, s));
+        if (!hw.empty()) {
+            m->obj_w.reserve(hw.size());
+            DCRF_HIP_CHECK(hipMemcpyAsync(m->obj_w.get(), hw.data(), sizeof(float) * hw.size(),
+                                          hipMemcpyHostToDevice, s));
+        }
+        const int npart = objective_partials(objective_kind, N, L);
+        m->obj_partial.reserve(npart);
+        launch_objective(objective_kind, m->qcache.get() + (size_t)n_iter * NL, m->gt_dev.get(), N, L, param,
+                         hw.empty() ? nullptr : m->obj_w.get(), m->work.get(), m->obj_partial.get(), s);
+        std::vector<double> OP(npart);
+        DCRF_HIP_CHECK(hipMemcpyAsync(OP.data(), m->obj_partial.get(), sizeof(double) * npart,
+                                      hipMemcpyDeviceToHost, s));
         const int nkp = num_kernel_params(*m);
         backward(*m, m->work.get(), m->tmp1.get(), nullptr, nullptr, s, kernel_grad && nkp > 0);
         std::vector<double> KG(nkp, 0.0);
@@ -1805,6 +1792,7 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
         std::vector<float> w(T, 1.0f);
         if (kw) DCRF_HIP_CHECK(hipMemcpyAsync(w.data(), kw, sizeof(float) * T, hipMemcpyDeviceToHost, s));
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        if (objective) *objective = objective_value(objective_kind, OP, L);
         // pack like DenseCRF::gradient (labelcompatibility.cpp:57-61, 76-78, 101-108)
         size_t go = 0;
         float* cp = compat_grad;

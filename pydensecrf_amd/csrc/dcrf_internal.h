@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -402,6 +403,16 @@ void launch_unary_from_labels(const int* labels, int N, int L, float p_energy, f
 // out [LA][LB] (double) = sum_p A[p][a] B[p][c], A [N][LA], B [N][LB]; partial holds
 // gram_partial_floats(LA, LB, N) floats.
 size_t gram_partial_floats(int LA, int LB, int N);
+// ---------------------------------------------------------------- objectives
+// ObjectiveFunction::evaluate on the device (DCRF_OBJECTIVE_*): dQ [N][L] written whole;
+// partial holds objective_partials(kind, N, L) doubles: for LogLikelihood / Hamming the
+// per-block sums of r (the first objective_partials entries, added in order), for IoU
+// the block statistics followed by the final [L][2] (in_l, un_l) at offset
+// objective_partials - 2 L.  w: Hamming's class weights on the device (one per label
+// that occurs in gt).
+int objective_partials(int kind, int N, int L);
+void launch_objective(int kind, const float* Q, const int16_t* gt, int N, int L, float param, const float* w,
+                      float* dQ, double* partial, hipStream_t s);
 void launch_gram(const float* A, int LA, const float* B, int LB, int N, float* partial, double* out, hipStream_t s);
 
 // Global layout remaps.

@@ -85,7 +85,7 @@ class CRFEnergy:
         """(E(x), dE/dx), both float64."""
         self.set(x)
         r, _, cg, _, kg = self.model.gradient_objective(self.n_iter, self.gt, self.objective, self.param,
-                                                        self.class_weight, kernel=self.kernel)
+                                                        self.class_weight, kernel=self.kernel, unary_grad=False)
         self.evaluations += 1
         parts = []
         if self.unary:

@@ -432,6 +432,50 @@ def test_objectives_reference_fixtures(i):
         assert np.abs(got - want).max() <= KG_TOL * np.abs(want).max() + 1e-5 * xmax * n
 
 
+@pytest.mark.parametrize("obj,param,weights,L,n", [
+    (0, 0.0, False, 21, 0), (0, 0.05, False, 21, 2), (1, 0.5, False, 21, 2), (1, 0.0, True, 7, 1),
+    (2, 0.0, False, 21, 0), (2, 0.0, False, 21, 2), (2, 0.0, False, 200, 1), (0, 0.0, False, 200, 1)])
+def test_device_objectives_vs_oracle(obj, param, weights, L, n):
+    """The objectives evaluated on the device (value and dL/dQ; objective.cpp:37-108)
+    against the oracle's host restatement, on images whose ground truth holds labels
+    outside [0, L) (ignored) and whose pixel count spans many reduction blocks."""
+    W, H = 173, 61
+    N = W * H
+    img, U, gt = image_case(W, H, L, seed=500 + L + n)
+    rng = np.random.default_rng(obj * 10 + n)
+    gt = gt.astype(np.int16)
+    gt[rng.random(N) < 0.05] = -1
+    gt[N // 2:N // 2 + L] = np.arange(L)  # every label occurs (Hamming's counts stay positive)
+    if obj != 1 or weights:
+        gt[rng.random(N) < 0.02] = L  # a label past L: counted for Hamming's weights only
+    cw = rng.uniform(0.5, 2.0, L + 1).astype(np.float32) if weights else None
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0)
+        m.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, 10.0)
+    r, ug, cg, _, _ = g.gradient_objective(n, gt, obj, param, cw)
+    ro, ugo, cgo, _ = o.gradient_obj(n, gt, obj, param, cw)
+    assert abs(r - ro) <= 1e-5 * max(1.0, abs(ro)), (r, ro)
+    # the error model of the reference-fixture test: a 1e-7 difference in Q moves b by
+    # ~1e-7 max|dQ Q| at saturated pixels
+    if obj == 1:
+        if weights:
+            xmax = float(cw.max())
+        else:
+            cnt = np.bincount(gt[gt >= 0]).astype(np.float64)
+            w = (cnt / cnt.sum()) ** (-param)
+            xmax = float((w / (cnt * w).sum()).max())
+    else:
+        xmax = 1.0 / L if obj == 2 else 1.0 / N
+    assert np.abs(ug - ugo).max() <= 1e-4 * np.abs(ugo).max() + 1e-6 * xmax * max(n, 1)
+    assert np.abs(cg - cgo).max() <= 1e-4 * np.abs(cgo).max() + 1e-5 * xmax * max(n, 1)
+    # without the unary gradient the rest is unchanged
+    r2, ug2, cg2, _, _ = g.gradient_objective(n, gt, obj, param, cw, unary_grad=False)
+    assert ug2 is None and r2 == r and np.array_equal(cg2, cg)
+
+
 @pytest.mark.parametrize("i", range(2))
 def test_logistic_unary_reference_fixtures(i):
     """LogisticUnaryEnergy on the matrix cores: parameters packed like the reference, Q,
