@@ -219,9 +219,14 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
     layer = CRFLayer(m, iters, [c1, c2], kweights=[3.0, 10.0], learn_kernel=True)
     ms_fbk = timed(fwd_bwd)
     kp = m.kernel_params()
+    Qd = torch.empty((W * H, L), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in range(3):
+        # the lattices, then the layout (built by the next call: a 0-iteration inference)
         m.set_kernel_params(kp * np.float32(1.0 + 0.01 * (r + 1)))
+        m.inference_device(0, Qd.data_ptr())
+    torch.cuda.synchronize()
     ms_rebuild = (time.perf_counter() - t0) * 1e3 / 3
     m.set_kernel_params(kp)
     log("fwd+bwd with kernel gradient %.3f ms; lattice rebuild %.2f ms" % (ms_fbk, ms_rebuild))

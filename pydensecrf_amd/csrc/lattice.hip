@@ -253,7 +253,12 @@ __global__ void __launch_bounds__(kThreads) k_hash_insert(const Key<D>* __restri
     const Key<D> k = keys[e];
     uint32_t h = key_hash<D>(k) & mask;
     for (;;) {
-        int cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // a workgroup-scope read may return a stale value from the CU's cache, which is
+        // safe: a slot only goes from empty to entries of one key, whose value then only
+        // falls, so a stale non-empty value names the right key and is >= the current one
+        // (the e < cur test below stays exact), and a stale empty value ends in the CAS,
+        // which reads the slot at device scope
+        int cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (cur == kEmpty) {
             const int old = atomicCAS(&table[h], kEmpty, e);
             if (old == kEmpty) {

@@ -9,6 +9,11 @@
         R evaluations of U = L f on the matrix cores, with the parameters changed
         between evaluations so that each one recomputes U.
 
+    python -m pydensecrf_amd.workloads kernel-rebuild [--reps R]
+        DenseCRF2D 512x512, 21 labels, Gaussian (d = 2) + bilateral (d = 5) with DIAG
+        kernel parameters: R changes of the parameter vector, each rebuilding both
+        lattices (dcrf_set_kernel_params) and the model's layout.
+
 Each prints one line with its host-clock time per repetition; run under rocprofv3 for
 the per-kernel figures.
 """
@@ -86,15 +91,48 @@ def logistic_unary(reps, F):
           % (W, H, L, F, ms), flush=True)
 
 
+def kernel_rebuild(reps):
+    from . import capi, synthetic as syn
+
+    cfg = syn.IMAGE_512x512
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, _ = syn.make_image(**cfg)
+    m = capi.Model.create_2d(W, H, L)
+    m.set_unary(U)
+    m.add_gaussian(3.0, 3.0, 3.0)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+    import torch
+
+    Qd = torch.empty((W * H, L), dtype=torch.float32, device="cuda")
+    kp = m.kernel_params()
+
+    def change(f):
+        # new lattices, then the layout (built by the next call on the model: here a
+        # 0-iteration inference, Q = softmax(-U))
+        m.set_kernel_params(kp * np.float32(f))
+        m.inference_device(0, Qd.data_ptr())
+        torch.cuda.synchronize()
+
+    change(1.001)
+    t0 = time.perf_counter()
+    for r in range(reps):
+        change(1.0 + 0.01 * (r + 1))
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    print("kernel-rebuild 512x512x21, d = 2 + 5: %.3f ms per parameter change (lattices and layout)" % ms,
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary"])
+    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary", "kernel-rebuild"])
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--features", type=int, default=64)
     a = ap.parse_args()
     if a.workload == "kernel-gradient":
         kernel_gradient(a.steps)
+    elif a.workload == "kernel-rebuild":
+        kernel_rebuild(a.reps)
     else:
         logistic_unary(a.reps, a.features)
 
