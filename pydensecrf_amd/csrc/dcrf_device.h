@@ -239,7 +239,15 @@ __device__ __forceinline__ void stage_compat(const SliceTable& st, int L, float*
             any = true;
             const float* src = st.t[k].cw;
             float* dst = shc + st.t[k].cw_lds;
-            for (int i = threadIdx.x; i < L * L; i += blockDim.x) dst[i] = src[i];
+            const int ldw = st.t[k].ldw;
+            if (ldw == L) {
+                for (int i = threadIdx.x; i < L * L; i += blockDim.x) dst[i] = src[i];
+            } else {  // padded rows; the pad columns read as 0
+                for (int i = threadIdx.x; i < L * ldw; i += blockDim.x) {
+                    const int row = i / ldw, col = i - row * ldw;
+                    dst[i] = col < L ? src[row * L + col] : 0.0f;
+                }
+            }
         }
     if (any) __syncthreads();
 }
@@ -247,8 +255,8 @@ __device__ __forceinline__ void stage_compat(const SliceTable& st, int L, float*
 // c[r] = sum_b cw[k0 + r][b] m_b over the pixel's whole message vector (read from its
 // group in label order), one fused multiply-add per weight.
 template <bool WAVE, int R>
-__device__ __forceinline__ void matrix_piece(const float* __restrict__ cw, const float (&m)[R], float (&c)[R],
-                                             LaneGroup<WAVE, R>& G) {
+__device__ __forceinline__ void matrix_piece(const float* __restrict__ cw, int ldw, const float (&m)[R],
+                                             float (&c)[R], LaneGroup<WAVE, R>& G) {
     const int L = G.L;
 #pragma unroll
     for (int r = 0; r < R; r++) c[r] = 0.0f;
@@ -260,7 +268,7 @@ __device__ __forceinline__ void matrix_piece(const float* __restrict__ cw, const
             if (b < L) {
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    if (G.has(r)) c[r] = fmaf(cw[(size_t)(G.k0 + r) * L + b], mb, c[r]);
+                    if (G.has(r)) c[r] = fmaf(cw[(size_t)(G.k0 + r) * ldw + b], mb, c[r]);
             }
         }
     }
@@ -287,9 +295,39 @@ __device__ __forceinline__ void compat_piece(const SliceTerm& T, const float (&m
     // two copies of the loop, one per weight source, so that each pointer keeps its
     // address space (one select between them would make every weight read a flat load)
     if (T.cw_lds >= 0)
-        matrix_piece<WAVE, R>(shc + T.cw_lds, m, c, G);
+        matrix_piece<WAVE, R>(shc + T.cw_lds, T.ldw, m, c, G);
     else
-        matrix_piece<WAVE, R>(T.cw, m, c, G);
+        matrix_piece<WAVE, R>(T.cw, G.L, m, c, G);
+}
+
+// matrix_piece with the pixel's message vector already in LDS (mrow, 16-byte aligned)
+// and the weights in LDS rows of stride ldw (a multiple of 4, 16-byte aligned): 16-byte
+// reads of both, no cross-lane traffic; the same fused multiply-adds in the same (label)
+// order, so the results are bit-identical to matrix_piece's.
+template <int R>
+__device__ __forceinline__ void matrix_piece_rows(const float* __restrict__ w, int ldw,
+                                                  const float* __restrict__ mrow, int L, int k0, float (&c)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; r++) c[r] = 0.0f;
+    int b = 0;
+#pragma unroll 1
+    for (; b + 4 <= L; b += 4) {
+        const float4 mv = *reinterpret_cast<const float4*>(mrow + b);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float4 wv = *reinterpret_cast<const float4*>(w + (size_t)(k0 + r) * ldw + b);
+            c[r] = fmaf(wv.x, mv.x, c[r]);
+            c[r] = fmaf(wv.y, mv.y, c[r]);
+            c[r] = fmaf(wv.z, mv.z, c[r]);
+            c[r] = fmaf(wv.w, mv.w, c[r]);
+        }
+    }
+#pragma unroll 1
+    for (; b < L; b++) {
+        const float mv = mrow[b];
+#pragma unroll
+        for (int r = 0; r < R; r++) c[r] = fmaf(w[(size_t)(k0 + r) * ldw + b], mv, c[r]);
+    }
 }
 
 // sumAndNormalize (densecrf.cpp:107-115) of x = Q .* g over the lane group:

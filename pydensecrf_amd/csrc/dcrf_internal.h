@@ -243,15 +243,17 @@ struct SliceTerm {
     int ldv = 0;        // row stride of vals (floats)
     int cw_lds = -1;    // Matrix: offset (floats) of the LDS copy of cw in the kernel's compat
                         // region, or -1 to read cw from global memory (set by the launchers)
+    int ldw = 0;        // row stride (floats) of that LDS copy: L, or L rounded up to 4
 };
 struct SliceTable {
     int nterms;
     SliceTerm t[DCRF_MAX_TERMS];
 };
 // Give each Matrix term an LDS copy of its L x L compatibility (cw_lds) while they fit
-// kCompatLdsFloats in total; returns the floats of the compat LDS region.
+// kCompatLdsFloats in total; returns the floats of the compat LDS region.  pad: rows of
+// (L + 3) & ~3 floats (16-byte aligned, for vector reads), else of L.
 constexpr int kCompatLdsFloats = 8192;
-int assign_compat_lds(SliceTable& st, int L);
+int assign_compat_lds(SliceTable& st, int L, bool pad = false);
 
 // Lane-group geometry of the per-item kernels (see dcrf_device.h).
 struct LaunchShape {

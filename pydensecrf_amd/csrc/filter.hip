@@ -465,16 +465,15 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
     }
 }
 
-// Term k's part of a fused pixel update: the weighted sum of its gathered rows (entry
+// Term k's message in a fused pixel update: the weighted sum of its gathered rows (entry
 // order, as slice_piece), stored raw to `mout` at pixel p when the term has one (as
-// k_meanfield does), then the term weight and the compatibility into c.
+// k_meanfield does), then times the term weight.
 template <int R, int D0, int D1>
-__device__ __forceinline__ void term_compat(LaneGroup<true, R>& G, const SliceTerm& T, int k,
-                                            const int2 (&e)[D0 + D1], const float (&x)[D0 + D1][R], bool active,
-                                            int p, const float* shc, float (&c)[R]) {
+__device__ __forceinline__ void term_message(LaneGroup<true, R>& G, const SliceTerm& T, int k,
+                                             const int2 (&e)[D0 + D1], const float (&x)[D0 + D1][R], bool active,
+                                             int p, float (&m)[R]) {
     constexpr int DS = D0 + D1;
     const int jb = k ? D0 : 0, je = k ? DS : D0;
-    float m[R];
 #pragma unroll
     for (int r = 0; r < R; r++) m[r] = 0.0f;
 #pragma unroll
@@ -492,26 +491,64 @@ __device__ __forceinline__ void term_compat(LaneGroup<true, R>& G, const SliceTe
 #pragma unroll
         for (int r = 0; r < R; r++) m[r] = m[r] * kw;
     }
-    compat_piece<true, R>(T, m, c, G, shc);
+}
+
+// The compatibility transforms c[k] of the pixel's NT term messages.  ROWS: the Matrix
+// terms whose weights have an LDS copy (16-byte aligned rows) first write the group's
+// message vector into the wave's message region `msg` (NT rows of (L + 3) & ~3 floats
+// per group), then read it back whole with matrix_piece_rows: no cross-lane shuffles.
+// The other terms go through compat_piece.  Same results either way.
+template <int R, int NT, bool ROWS>
+__device__ __forceinline__ void compat_terms(LaneGroup<true, R>& G, const SliceTable& st, const float (&mk)[NT][R],
+                                             float (&c)[NT][R], const float* shc, float* msg) {
+    const int ldm = (G.L + 3) & ~3;
+    if (ROWS) {
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+            if (st.t[k].ckind == DCRF_COMPAT_MATRIX && st.t[k].cw_lds >= 0) {
+                any = true;
+                if (G.on) {  // idle lanes shadow group 0: they must not write its row
+                    float* row = msg + ((size_t)k * G.per + G.g) * ldm + G.k0;
+#pragma unroll
+                    for (int r = 0; r < R; r++) row[r] = mk[k][r];
+                }
+            }
+        if (any) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const SliceTerm& T = st.t[k];
+        if (ROWS && T.ckind == DCRF_COMPAT_MATRIX && T.cw_lds >= 0)
+            matrix_piece_rows<R>(shc + T.cw_lds, T.ldw, msg + ((size_t)k * G.per + G.g) * ldm, G.L, G.k0, c[k]);
+        else
+            compat_piece<true, R>(T, mk[k], c[k], G, shc);
+    }
 }
 
 // The per-pixel tail of the fused mean-field kernels: the terms' compatibility
 // messages subtracted from -U, then the softmax over the lane group, stored when
 // `active`.
-template <int R, int D0, int D1>
+template <int R, int D0, int D1, bool ROWS>
 __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceTable& st, const int2 (&e)[D0 + D1],
                                              const float (&x)[D0 + D1][R], const float (&u)[R], bool active,
-                                             float* __restrict__ qrow, const float* shc, int p) {
+                                             float* __restrict__ qrow, const float* shc, int p, float* msg) {
+    constexpr int NT = D1 ? 2 : 1;
+    float mk[NT][R], c[NT][R];
+#pragma unroll
+    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(G, st.t[k], k, e, x, active, p, mk[k]);
+    compat_terms<R, NT, ROWS>(G, st, mk, c, shc, msg);
     float en[R];
 #pragma unroll
     for (int r = 0; r < R; r++) en[r] = -u[r];
 #pragma unroll
-    for (int k = 0; k < (D1 ? 2 : 1); k++) {
-        float c[R];
-        term_compat<R, D0, D1>(G, st.t[k], k, e, x, active, p, shc, c);
+    for (int k = 0; k < NT; k++)
 #pragma unroll
-        for (int r = 0; r < R; r++) en[r] = en[r] - c[r];
-    }
+        for (int r = 0; r < R; r++) en[r] = en[r] - c[k][r];
     const float mx = G.all_max(en);
     float y[R];
 #pragma unroll
@@ -557,14 +594,15 @@ template <int R, int D0, int D1, bool CWL>
 __global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))  // <= 64 VGPRs
 k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
                                                                 int ldq, int N, int L, int S, int per,
-                                                                SliceTable st, const int2* __restrict__ slc) {
+                                                                SliceTable st, const int2* __restrict__ slc,
+                                                                int cwf) {
     constexpr int DS = D0 + D1;
     LaneGroup<true, R> G(L, S, per, nullptr);
     const int p = G.item();
     const bool active = G.on && p < N;
     const int k0 = G.k0;
     const int pc = active ? p : 0;  // idle lanes read pixel 0 (in bounds; results unused)
-    extern __shared__ int2 pairs_all[];
+    extern __shared__ __align__(16) int2 pairs_all[];
     int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
     // CWL: some Matrix term has an LDS copy of its weights (staging costs a barrier, so
     // it is compiled in only then)
@@ -587,7 +625,9 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
         const SliceTerm& T = j < D0 ? st.t[0] : st.t[1];
         load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
     }
-    finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc, p);
+    // CWL: the wave's message rows follow the compat region
+    float* msg = shc + cwf + (size_t)(threadIdx.x >> 6) * (D1 ? 2 : 1) * per * ((L + 3) & ~3);
+    finish_pixel<R, D0, D1, CWL>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc, p, msg);
 }
 
 // The adjoint step of k_adjoint (adjoint.hip) over the same interleaved slice pairs:
@@ -597,14 +637,14 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
 template <int R, int D0, int D1, bool CWL>
 __global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __restrict__ dU, int N, int L, int S,
-                int per, SliceTable st, const int2* __restrict__ slc) {
+                int per, SliceTable st, const int2* __restrict__ slc, int cwf) {
     constexpr int DS = D0 + D1;
     LaneGroup<true, R> G(L, S, per, nullptr);
     const int p = G.item();
     const bool active = G.on && p < N;
     const int k0 = G.k0;
     const int pc = active ? p : 0;
-    extern __shared__ int2 pairs_all[];
+    extern __shared__ __align__(16) int2 pairs_all[];
     int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
     float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
     if (CWL) stage_compat(st, L, shc);
@@ -620,16 +660,19 @@ k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __res
         const SliceTerm& T = j < D0 ? st.t[0] : st.t[1];
         load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
     }
+    constexpr int NT = D1 ? 2 : 1;
+    float mk[NT][R], c[NT][R];
+#pragma unroll
+    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(G, st.t[k], k, e, x, active, p, mk[k]);
+    float* msg = shc + cwf + (size_t)(threadIdx.x >> 6) * NT * per * ((L + 3) & ~3);
+    compat_terms<R, NT, CWL>(G, st, mk, c, shc, msg);
     float gsum[R];
 #pragma unroll
     for (int r = 0; r < R; r++) gsum[r] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < (D1 ? 2 : 1); k++) {
-        float c[R];
-        term_compat<R, D0, D1>(G, st.t[k], k, e, x, active, p, shc, c);
+    for (int k = 0; k < NT; k++)
 #pragma unroll
-        for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[r];
-    }
+        for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[k][r];
     float bn[R];
     sum_normalize<true, R>(q, gsum, bn, G);
     if (active) {
@@ -721,7 +764,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_meanfield_runs(const float* __re
         float x[DS][R];
 #pragma unroll
         for (int j = 0; j < DS; j++) load_piece<R>(tq + (size_t)e[j].x * L, k0, L, x[j]);
-        finish_pixel<R, D0, D1>(G, st, e, x, u, active, Q + (size_t)p * L, shc, p);
+        finish_pixel<R, D0, D1, false>(G, st, e, x, u, active, Q + (size_t)p * L, shc, p, nullptr);
     }
 }
 
@@ -1046,13 +1089,18 @@ void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
+// dynamic LDS of the fused kernels: the wave's slice pairs, the compat region (cwf
+// floats) and, with a compat region, the message rows (NT per group, (L + 3) & ~3 floats)
+inline size_t fused_lds_bytes(const LaunchShape& sh, int ds, int nt, int L, int cwf) {
+    const size_t waves = kWaveBlock / 64;
+    return waves * sh.per * ds * sizeof(int2) + (size_t)cwf * sizeof(float) +
+           (cwf ? waves * nt * sh.per * (size_t)((L + 3) & ~3) * sizeof(float) : 0);
+}
+
 #define DCRF_FUSED(R_, D0_, D1_)                                                                          \
-    (cwf ? k_meanfield_fused<R_, D0_, D1_, true> : k_meanfield_fused<R_, D0_, D1_, false>)<<<                   \
-        blocks_for(N, sh.items), kWaveBlock,                                                                    \
-                                      (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) +            \
-                                          (size_t)cwf * sizeof(float), s>>>(                                          \
-        U, Q, ldq, N, L, sh.S, sh.per, st, \
-                                                                                slc)
+    (cwf ? k_meanfield_fused<R_, D0_, D1_, true> : k_meanfield_fused<R_, D0_, D1_, false>)<<<                 \
+        blocks_for(N, sh.items), kWaveBlock, fused_lds_bytes(sh, D0_ + D1_, D1_ ? 2 : 1, L, cwf), s>>>(       \
+        U, Q, ldq, N, L, sh.S, sh.per, st, slc, cwf)
 
 template <int R>
 void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
@@ -1069,9 +1117,8 @@ void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, in
 
 #define DCRF_ADJ_FUSED(R_, D0_, D1_)                                                                      \
     (cwf ? k_adjoint_fused<R_, D0_, D1_, true> : k_adjoint_fused<R_, D0_, D1_, false>)<<<                   \
-        blocks_for(N, sh.items), kWaveBlock,                                                                  \
-        (size_t)(kWaveBlock / 64) * sh.per * (D0_ + D1_) * sizeof(int2) + (size_t)cwf * sizeof(float), s>>>( \
-        Q, b, dU, N, L, sh.S, sh.per, st, slc)
+        blocks_for(N, sh.items), kWaveBlock, fused_lds_bytes(sh, D0_ + D1_, D1_ ? 2 : 1, L, cwf), s>>>(     \
+        Q, b, dU, N, L, sh.S, sh.per, st, slc, cwf)
 
 template <int R>
 void launch_adj_fused_r(const LaunchShape& sh, const float* Q, float* b, float* dU, int N, int L,
@@ -1142,7 +1189,7 @@ void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTa
 void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st_in,
                             const int2* slc, hipStream_t s) {
     SliceTable st = st_in;
-    const int cwf = assign_compat_lds(st, L);
+    const int cwf = assign_compat_lds(st, L, true);  // 16-byte rows for matrix_piece_rows
     const LaunchShape sh = shape_for(L);
     if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
         throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
@@ -1157,7 +1204,7 @@ void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, con
 void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st_in,
                           const int2* slc, hipStream_t s) {
     SliceTable st = st_in;
-    const int cwf = assign_compat_lds(st, L);
+    const int cwf = assign_compat_lds(st, L, true);  // 16-byte rows for matrix_piece_rows
     const LaunchShape sh = shape_for(L);
     if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
         throw Error{DCRF_ERR_VALUE, "no fused adjoint kernel for these terms"};
@@ -1169,13 +1216,17 @@ void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, con
     }
 }
 
-int assign_compat_lds(SliceTable& st, int L) {
+int assign_compat_lds(SliceTable& st, int L, bool pad) {
+    const int ldw = pad ? (L + 3) & ~3 : L;
     int total = 0;
-    for (int k = 0; k < st.nterms; k++) st.t[k].cw_lds = -1;
+    for (int k = 0; k < st.nterms; k++) {
+        st.t[k].cw_lds = -1;
+        st.t[k].ldw = ldw;
+    }
     for (int k = 0; k < st.nterms; k++)
-        if (st.t[k].ckind == DCRF_COMPAT_MATRIX && total + L * L <= kCompatLdsFloats) {
+        if (st.t[k].ckind == DCRF_COMPAT_MATRIX && total + L * ldw <= kCompatLdsFloats) {
             st.t[k].cw_lds = total;
-            total += L * L;
+            total += L * ldw;
         }
     return total;
 }
