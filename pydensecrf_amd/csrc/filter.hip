@@ -466,26 +466,29 @@ __global__ void __launch_bounds__(1024) k_meanfield(const float* __restrict__ U,
 }
 
 // Term k's message in a fused pixel update: the weighted sum of its gathered rows (entry
-// order, as slice_piece), stored raw to `mout` at pixel p when the term has one (as
-// k_meanfield does), then times the term weight.
+// order, as slice_piece) into raw, and raw times the term weight into m.  The caller
+// stores raw to the term's `mout` (as k_meanfield does) only after its last read of
+// the terms' weights: once a kernel may have stored to global memory, the compiler
+// reads those weights through the vector memory path and waits for every load in
+// flight (the row gathers among them).
 template <int R, int D0, int D1>
-__device__ __forceinline__ void term_message(LaneGroup<true, R>& G, const SliceTerm& T, int k,
-                                             const int2 (&e)[D0 + D1], const float (&x)[D0 + D1][R], bool active,
-                                             int p, float (&m)[R]) {
+__device__ __forceinline__ void term_message(const SliceTerm& T, int k, const int2 (&e)[D0 + D1],
+                                             const float (&x)[D0 + D1][R], float (&raw)[R], float (&m)[R]) {
     constexpr int DS = D0 + D1;
     const int jb = k ? D0 : 0, je = k ? DS : D0;
 #pragma unroll
-    for (int r = 0; r < R; r++) m[r] = 0.0f;
+    for (int r = 0; r < R; r++) raw[r] = 0.0f;
 #pragma unroll
     for (int j = jb; j < je; j++) {
         const float w = __int_as_float(e[j].y);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const float prod = w * x[j][r];
-            m[r] = m[r] + prod;
+            raw[r] = raw[r] + prod;
         }
     }
-    if (T.mout && active) store_piece<R>(T.mout + (size_t)p * G.L, G.k0, G.L, m);
+#pragma unroll
+    for (int r = 0; r < R; r++) m[r] = raw[r];
     if (T.kw) {
         const float kw = T.kw[0];
 #pragma unroll
@@ -538,9 +541,9 @@ __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceT
                                              const float (&x)[D0 + D1][R], const float (&u)[R], bool active,
                                              float* __restrict__ qrow, const float* shc, int p, float* msg) {
     constexpr int NT = D1 ? 2 : 1;
-    float mk[NT][R], c[NT][R];
+    float raw[NT][R], mk[NT][R], c[NT][R];
 #pragma unroll
-    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(G, st.t[k], k, e, x, active, p, mk[k]);
+    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(st.t[k], k, e, x, raw[k], mk[k]);
     compat_terms<R, NT, ROWS>(G, st, mk, c, shc, msg);
     float en[R];
 #pragma unroll
@@ -559,6 +562,9 @@ __device__ __forceinline__ void finish_pixel(LaneGroup<true, R>& G, const SliceT
 #pragma unroll
         for (int r = 0; r < R; r++) q[r] = y[r] / sum;
         store_piece<R>(qrow, G.k0, G.L, q);
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+            if (st.t[k].mout) store_piece<R>(st.t[k].mout + (size_t)p * G.L, G.k0, G.L, raw[k]);
     }
 }
 
@@ -661,9 +667,9 @@ k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __res
         load_piece<R>(T.vals + (size_t)e[j].x * T.ldv, k0, L, x[j]);
     }
     constexpr int NT = D1 ? 2 : 1;
-    float mk[NT][R], c[NT][R];
+    float raw[NT][R], mk[NT][R], c[NT][R];  // (the adjoint stores no raw messages)
 #pragma unroll
-    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(G, st.t[k], k, e, x, active, p, mk[k]);
+    for (int k = 0; k < NT; k++) term_message<R, D0, D1>(st.t[k], k, e, x, raw[k], mk[k]);
     float* msg = shc + cwf + (size_t)(threadIdx.x >> 6) * NT * per * ((L + 3) & ~3);
     compat_terms<R, NT, CWL>(G, st, mk, c, shc, msg);
     float gsum[R];
