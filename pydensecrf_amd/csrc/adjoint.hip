@@ -126,14 +126,16 @@ __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b,
 #pragma unroll
     for (int r = 0; r < R; r++) sh[t * R + r] = G.on ? acc[r] : 0.0f;
     __syncthreads();
-    if (t < L) {
-        const int ln = t / R, rr = t - ln * R;  // lane and slot holding label t
+    // labels strided over the block's threads (a block-mode group may hold more labels
+    // than the block has threads)
+    for (int lab = t; lab < L; lab += blockDim.x) {
+        const int ln = lab / R, rr = lab - ln * R;  // lane and slot holding label lab
         float s = 0.0f;
         for (int gi = 0; gi < items; gi++) {
             const int lane0 = WAVE ? (gi / per) * 64 + (gi % per) * S : gi * S;
             s = s + sh[(lane0 + ln) * R + rr];
         }
-        partial[(size_t)blockIdx.x * L + t] = s;
+        partial[(size_t)blockIdx.x * L + lab] = s;
     }
 }
 

@@ -50,19 +50,22 @@ __global__ void __launch_bounds__(kThreads) k_obj_pixel(const float* __restrict_
     if (t == 0) partial[blockIdx.x] = red[0];
 }
 
-// IoU statistics: thread t of a block keeps label t % L for the pixels t / L,
-// t / L + P, ... of the block's range (P = 256 / L pixels per round, reads coalesced over
-// [N][L]); the threads of one label are then added in thread order into
-// partial[block][2 l] (in_l) and [2 l + 1] (un_l without its 1e-20 start).
+// IoU statistics: labels in chunks of LB = min(L, 256) (blockIdx.y); thread t of a
+// block keeps label chunk * LB + t % LB for the pixels t / LB, t / LB + P, ... of the
+// block's range (P = 256 / LB pixels per round, reads coalesced over [N][L] when the
+// chunk is the whole label range); the threads of one label are then added in thread
+// order into partial[block][2 l] (in_l) and [2 l + 1] (un_l without its 1e-20 start).
 __global__ void __launch_bounds__(kThreads) k_iou_stats(const float* __restrict__ Q, const int16_t* __restrict__ gt,
                                                         int N, int L, int ppb, double* __restrict__ partial) {
     __shared__ double s_in[kThreads], s_un[kThreads];
     const int t = threadIdx.x;
-    const int P = kThreads / L;  // pixels per round (L <= 256)
-    const int l = t % L, po = t / L;
+    const int LB = min(L, kThreads);
+    const int P = kThreads / LB;  // pixels per round
+    const int lc = t % LB, po = t / LB;
+    const int l = blockIdx.y * LB + lc;
     const int p0 = blockIdx.x * ppb, p1 = min(N, p0 + ppb);
     double in = 0.0, un = 0.0;
-    if (po < P) {
+    if (po < P && l < L) {
         for (int i = p0 + po; i < p1; i += P) {
             const int g = gt[i];
             if (g < 0 || g >= L) continue;
@@ -78,14 +81,14 @@ __global__ void __launch_bounds__(kThreads) k_iou_stats(const float* __restrict_
     s_in[t] = in;
     s_un[t] = un;
     __syncthreads();
-    if (t < L) {
+    if (t < LB && l < L) {
         double a = 0.0, b = 0.0;
         for (int k = 0; k < P; k++) {
-            a += s_in[k * L + t];
-            b += s_un[k * L + t];
+            a += s_in[k * LB + t];
+            b += s_un[k * LB + t];
         }
-        partial[(size_t)blockIdx.x * 2 * L + 2 * t] = a;
-        partial[(size_t)blockIdx.x * 2 * L + 2 * t + 1] = b;
+        partial[(size_t)blockIdx.x * 2 * L + 2 * l] = a;
+        partial[(size_t)blockIdx.x * 2 * L + 2 * l + 1] = b;
     }
 }
 
@@ -125,14 +128,14 @@ int objective_partials(int kind, int N, int L) {
 void launch_objective(int kind, const float* Q, const int16_t* gt, int N, int L, float param, const float* w,
                       float* dQ, double* partial, hipStream_t s) {
     if (N <= 0) return;
-    if (L > kThreads) throw Error{DCRF_ERR_VALUE, "objective: at most 256 labels"};
     if (kind == DCRF_OBJECTIVE_IOU) {
         const int nb = std::max(1, std::min(1024, (N + 4095) / 4096));
         const int ppb = (N + nb - 1) / nb;
-        k_iou_stats<<<nb, kThreads, 0, s>>>(Q, gt, N, L, ppb, partial);
+        const int chunks = (L + kThreads - 1) / kThreads;
+        k_iou_stats<<<dim3((unsigned)nb, (unsigned)chunks), kThreads, 0, s>>>(Q, gt, N, L, ppb, partial);
         DCRF_HIP_CHECK(hipGetLastError());
         double* stats = partial + (size_t)nb * 2 * L;
-        k_iou_final<<<1, 2 * L <= 64 ? 64 : 512, 0, s>>>(partial, nb, L, stats);
+        k_iou_final<<<(unsigned)((2 * L + kThreads - 1) / kThreads), kThreads, 0, s>>>(partial, nb, L, stats);
         DCRF_HIP_CHECK(hipGetLastError());
         const size_t n = (size_t)N * L;
         k_iou_grad<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, s>>>(gt, stats, N, L, dQ);

@@ -193,7 +193,7 @@ def test_autograd_repeatable_and_guarded():
         Q1.sum().backward()
 
 
-@pytest.mark.parametrize("L", [4, 7, 130])
+@pytest.mark.parametrize("L", [4, 7, 130, 300])
 def test_gradient_label_shapes(L):
     W, H, n = 19, 13, 3
     img, U, gt = image_case(W, H, L, seed=L)
@@ -432,14 +432,19 @@ def test_objectives_reference_fixtures(i):
         assert np.abs(got - want).max() <= KG_TOL * np.abs(want).max() + 1e-5 * xmax * n
 
 
-@pytest.mark.parametrize("obj,param,weights,L,n", [
-    (0, 0.0, False, 21, 0), (0, 0.05, False, 21, 2), (1, 0.5, False, 21, 2), (1, 0.0, True, 7, 1),
-    (2, 0.0, False, 21, 0), (2, 0.0, False, 21, 2), (2, 0.0, False, 200, 1), (0, 0.0, False, 200, 1)])
-def test_device_objectives_vs_oracle(obj, param, weights, L, n):
+@pytest.mark.parametrize("obj,param,weights,L,n,W,H", [
+    (0, 0.0, False, 21, 0, 173, 61), (0, 0.05, False, 21, 2, 173, 61), (1, 0.5, False, 21, 2, 173, 61),
+    (1, 0.0, True, 7, 1, 173, 61), (2, 0.0, False, 21, 0, 173, 61), (2, 0.0, False, 21, 2, 173, 61),
+    (2, 0.0, False, 200, 1, 173, 61), (0, 0.0, False, 200, 1, 173, 61),
+    (2, 0.0, False, 300, 1, 40, 25), (1, 0.0, True, 300, 1, 40, 25)])
+def test_device_objectives_vs_oracle(obj, param, weights, L, n, W, H):
     """The objectives evaluated on the device (value and dL/dQ; objective.cpp:37-108)
     against the oracle's host restatement, on images whose ground truth holds labels
-    outside [0, L) (ignored) and whose pixel count spans many reduction blocks."""
-    W, H = 173, 61
+    outside [0, L) (ignored); the 173x61 images span many reduction blocks, the
+    300-label cases two label chunks of the IoU statistics.  (The 300-label images are
+    small because the oracle forms the Potts gradient as the reference does, one float
+    sum over all N L products in packet order: at 3.2 M products under Hamming weights its
+    own rounding reaches 1e-3 of the result.)"""
     N = W * H
     img, U, gt = image_case(W, H, L, seed=500 + L + n)
     rng = np.random.default_rng(obj * 10 + n)
@@ -474,6 +479,25 @@ def test_device_objectives_vs_oracle(obj, param, weights, L, n):
     # without the unary gradient the rest is unchanged
     r2, ug2, cg2, _, _ = g.gradient_objective(n, gt, obj, param, cw, unary_grad=False)
     assert ug2 is None and r2 == r and np.array_equal(cg2, cg)
+
+
+def test_objective_errors():
+    """Bad objective arguments are refused before any device work, as ValueError."""
+    W, H, L = 12, 9, 3
+    img, U, gt = image_case(W, H, L, seed=9)
+    gt = gt.astype(np.int16)
+    gt[:L] = np.arange(L)
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(U)
+    g.add_gaussian(3.0, 3.0, 3.0)
+    with pytest.raises(ValueError):
+        g.gradient_objective(2, gt, 7)  # no such objective
+    with pytest.raises(ValueError):
+        g.gradient_objective(2, gt, capi.OBJECTIVE_HAMMING, class_weight=np.ones(L - 1, np.float32))
+    with pytest.raises(ValueError):
+        g.gradient_objective(2, gt[:-1], capi.OBJECTIVE_IOU)
+    r, _, _, _, _ = g.gradient_objective(2, gt, capi.OBJECTIVE_HAMMING, class_weight=np.ones(L, np.float32))
+    assert np.isfinite(r)
 
 
 @pytest.mark.parametrize("i", range(2))
