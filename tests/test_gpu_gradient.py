@@ -320,6 +320,27 @@ def test_kernel_gradient_vs_oracle_l21(kt, nt):
     assert err <= KG_TOL_L21
 
 
+@pytest.mark.parametrize("L,nt", [(300, 3), (130, 2)])
+def test_kernel_gradient_many_labels(L, nt):
+    """The kernel-parameter gradient with more labels than a wave's lanes hold (the
+    block-mode kernels and the thread-per-pixel slicing gradient)."""
+    W, H, n = 24, 16, 2
+    img, U, gt = image_case(W, H, L, seed=L + nt)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0, ktype=capi.DIAG_KERNEL, ntype=nt)
+        m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.DIAG_KERNEL, ntype=nt)
+    obj, ug, cg, _, kg = g.gradient(n, gt, kernel=True)
+    r, ugo, cgo, kgo = o.gradient(n, gt, kernel=True)
+    assert abs(obj - r) <= 1e-5 * abs(r)
+    assert _rel_close(ug, ugo, 1e-4)
+    assert _rel_close(cg, cgo, 1e-4)
+    err = np.abs(kg - kgo).max() / np.abs(kgo).max()
+    assert err <= KG_TOL_L21, (err, kg, kgo)
+
+
 def test_kernel_params_rebuild_and_reset():
     """setKernelParameters rebuilds the lattices: inference follows the oracle at the new
     parameters, and setting the initial parameters again restores the first result."""
