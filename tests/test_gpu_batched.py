@@ -48,6 +48,33 @@ def test_batched_2d_matches_single():
         assert np.abs(Qb[i] - o.inference(n)).max() <= 1e-5
 
 
+@pytest.mark.parametrize("L", [3, 130])
+def test_batched_tiny_and_wide_instances(L):
+    """A batch mixing a one-pixel image, a one-row image and ordinary ones, at a label
+    count that fits a wave's lanes (3) and one that needs block-wide groups (130): each
+    instance bit-identical to its image run alone, and within 1e-5 of the oracle."""
+    shapes = [(1, 1), (29, 1), (12, 9), (40, 31)]
+    n = 4
+    cases = [image_case(w, h, L, seed=900 + i, sites=3) for i, (w, h) in enumerate(shapes)]
+    b = BatchedDenseCRF2D(shapes, L)
+    b.setUnaryEnergy([U for _, U, _ in cases])
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=30, srgb=13, rgbims=[img for img, _, _ in cases], compat=10)
+    Qb = b.inference(n)
+    for i, ((w, h), (img, U, _)) in enumerate(zip(shapes, cases)):
+        s = capi.Model.create_2d(w, h, L)
+        s.set_unary(U)
+        s.add_gaussian(3.0, 3.0, 3.0)
+        s.add_bilateral(30.0, 30.0, 13.0, 13.0, 13.0, img, 10.0)
+        Qs = s.inference(n)
+        assert np.array_equal(Qb[i].view(np.int32), np.ascontiguousarray(Qs).view(np.int32)), "instance %d" % i
+        o = po.OracleCRF(W=w, H=h, L=L)
+        o.set_unary(U)
+        o.add_gaussian(3.0, 3.0, 3.0)
+        o.add_bilateral(30.0, 30.0, 13.0, 13.0, 13.0, img, 10.0)
+        assert np.abs(Qb[i] - o.inference(n)).max() <= 1e-5
+
+
 def test_batched_generic_volumes():
     L, n = 4, 6
     rng = np.random.default_rng(9)
