@@ -139,7 +139,7 @@ struct LaneGroup {
         return WAVE ? (b * (kWaveBlock / 64) + (t >> 6)) * per + g : b * per + g;
     }
     __device__ static __forceinline__ int items_per_block(int per) { return WAVE ? (kWaveBlock / 64) * per : per; }
-    __device__ __forceinline__ bool has(int r) const { return true; }  // R divides L: every slot is a label
+    __device__ __forceinline__ bool has(int) const { return true; }  // R divides L: every slot is a label
 
     // Value of lane s of this group (all lanes must call).
     __device__ __forceinline__ float lane_value(float x, int s) {

@@ -700,6 +700,7 @@ template <int R, int D0, int D1>
 __global__ void __launch_bounds__(kWaveBlock) k_meanfield_runs(const float* __restrict__ U, float* __restrict__ Q,
                                                                int N, int L, int S, int per, SliceTable st,
                                                                RunSliceDev rs) {
+    (void)N;  // the runs cover the pixels exactly (rs.pix_base)
     constexpr int DS = D0 + D1;
     extern __shared__ float lds[];
     float* tq = lds;  // rs.rows_lds floats: the run's rows
