@@ -613,18 +613,20 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
     // CWL: some Matrix term has an LDS copy of its weights (staging costs a barrier, so
     // it is compiled in only then)
     float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
-    if (CWL) stage_compat(st, L, shc);
-    stage_wave_pairs<R, DS>(G, p, N, slc, wp);
-    int2 e[DS];
+    // the unary row does not depend on the slice pairs: its load is in flight while they
+    // are staged
     float u[R];
-#pragma unroll
-    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
     if (U) {
         load_piece<R>(U + (size_t)pc * L, k0, L, u);
     } else {
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] = 0.0f;
     }
+    if (CWL) stage_compat(st, L, shc);
+    stage_wave_pairs<R, DS>(G, p, N, slc, wp);
+    int2 e[DS];
+#pragma unroll
+    for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
     float x[DS][R];
 #pragma unroll
     for (int j = 0; j < DS; j++) {
@@ -653,13 +655,13 @@ k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __res
     extern __shared__ __align__(16) int2 pairs_all[];
     int2* wp = pairs_all + (threadIdx.x >> 6) * (per * DS);
     float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
+    float q[R];
+    load_piece<R>(Q + (size_t)pc * L, k0, L, q);  // in flight while the pairs are staged
     if (CWL) stage_compat(st, L, shc);
     stage_wave_pairs<R, DS>(G, p, N, slc, wp);
     int2 e[DS];
 #pragma unroll
     for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
-    float q[R];
-    load_piece<R>(Q + (size_t)pc * L, k0, L, q);
     float x[DS][R];
 #pragma unroll
     for (int j = 0; j < DS; j++) {
