@@ -614,19 +614,24 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
     // it is compiled in only then)
     float* shc = reinterpret_cast<float*>(pairs_all + (blockDim.x >> 6) * (per * DS));
     // the unary row does not depend on the slice pairs: its load is in flight while they
-    // are staged
+    // are staged (not at four labels per lane, where the longer-lived row would spill)
+    constexpr bool kEarlyU = R <= 3;
     float u[R];
-    if (U) {
-        load_piece<R>(U + (size_t)pc * L, k0, L, u);
-    } else {
+    auto load_u = [&]() {
+        if (U) {
+            load_piece<R>(U + (size_t)pc * L, k0, L, u);
+        } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) u[r] = 0.0f;
-    }
+            for (int r = 0; r < R; r++) u[r] = 0.0f;
+        }
+    };
+    if (kEarlyU) load_u();
     if (CWL) stage_compat(st, L, shc);
     stage_wave_pairs<R, DS>(G, p, N, slc, wp);
     int2 e[DS];
 #pragma unroll
     for (int j = 0; j < DS; j++) e[j] = active ? wp[G.g * DS + j] : make_int2(0, 0);
+    if (!kEarlyU) load_u();
     float x[DS][R];
 #pragma unroll
     for (int j = 0; j < DS; j++) {
