@@ -329,8 +329,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the mean-field path runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not built); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the mean-field path runs on a single MI355X "
+              "(sharding over several GPUs is not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
