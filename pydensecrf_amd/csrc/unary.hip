@@ -61,6 +61,49 @@ __global__ void __launch_bounds__(256) k_logistic_mfma(const float* __restrict__
     }
 }
 
+// k_logistic_mfma for M <= 32 labels (one label tile): the block first copies L into
+// LDS (rows padded to F + 1 floats: the 32 lanes of a half read 32 different rows of one
+// column without bank conflicts), so the B operand comes from LDS instead of a 32-row
+// gather per instruction.  Same products in the same order as k_logistic_mfma.
+__global__ void __launch_bounds__(256) k_logistic_mfma_lds(const float* __restrict__ fFN,
+                                                           const float* __restrict__ Lm, int N, int F, int M,
+                                                           int n_waves, float* __restrict__ U) {
+    extern __shared__ float sL[];  // M rows of F + 1 floats
+    const int ldl = F + 1;
+    for (int q = threadIdx.x; q < M * F; q += blockDim.x) {
+        const int r = q / F, c = q - r * F;
+        sL[r * ldl + c] = Lm[q];
+    }
+    __syncthreads();
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= n_waves) return;  // after the block's only barrier
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 31, h = lane >> 5;
+    const int n = w * 32 + i;
+    const int nc = min(n, N - 1), mc = min(i, M - 1);
+    const float* brow = sL + mc * ldl;
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    for (int c0 = 0; c0 < F; c0 += 16) {
+        float a[8], b[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = c0 + 2 * j + h;
+            const int cc = min(c, F - 1);
+            a[j] = fFN[(size_t)cc * N + nc] * (c < F ? 1.0f : 0.0f);  // features past F add 0
+            b[j] = brow[cc];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j], b[j], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = w * 32 + 8 * (r >> 2) + 4 * h + (r & 3), col = i;
+        if (row < N && col < M) U[(size_t)row * M + col] = acc[r];
+    }
+}
+
 // G[a][c] = sum_p A[p][a] B[p][c] over the pixel range of each wave: lane l supplies
 // A[l % 32][l / 32] = A[p + l/32][a] and B[l / 32][l % 32] = B[p + l/32][c] (columns past
 // LA / LB and pixels past the range read as 0); partial[range][a * LB + c].
@@ -187,7 +230,12 @@ void launch_logistic_unary(const float* fFN, const float* Lm, int N, int F, int 
     const int tilesM = (M + 31) / 32;
     const long long waves = (long long)((N + 31) / 32) * tilesM;
     if (waves > 0x7fffffffll / 64) throw Error{DCRF_ERR_VALUE, "logistic unary too large"};
-    k_logistic_mfma<<<(unsigned)((waves + 3) / 4), kThreads, 0, s>>>(fFN, Lm, N, F, M, tilesM, (int)waves, U);
+    const size_t lds = (size_t)M * (F + 1) * sizeof(float);
+    if (tilesM == 1 && lds <= 32 * 1024) {
+        k_logistic_mfma_lds<<<(unsigned)((waves + 3) / 4), kThreads, lds, s>>>(fFN, Lm, N, F, M, (int)waves, U);
+    } else {
+        k_logistic_mfma<<<(unsigned)((waves + 3) / 4), kThreads, 0, s>>>(fFN, Lm, N, F, M, tilesM, (int)waves, U);
+    }
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
