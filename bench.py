@@ -16,6 +16,8 @@ Also reported on the same line:
   kernels    the same figures for every kernel class (splat, combine, blur, meanfield)
   hipgraph   the same steps replayed as one captured hipGraph
   batched    64 images of uneven size in one batched model vs 64 plain per-image calls
+  fwd_bwd_1024x768  the inference workload as a torch.autograd layer (Potts compatibilities
+             and term weights learnable, cross-entropy): forward and forward+backward
   fwd_bwd    512x512x21, 5 iterations, learnable Matrix compatibility: forward and
              forward+backward through the torch.autograd layer, also with learnable
              kernel parameters (their gradient and the lattice rebuild)
@@ -99,13 +101,8 @@ def bench_batched(torch, capi, syn, n_images, iters, reps, log):
     Lattices are built beforehand in both cases; images/s over `reps` passes."""
     import numpy as np  # noqa: F401
     from pydensecrf_amd.batched import BatchedDenseCRF2D
-    sizes = syn.batch_image_sizes(n_images)
     L = 21
-    imgs, Us = [], []
-    for i, (w, h) in enumerate(sizes):
-        img, U, _ = syn.make_image(w, h, L=L, sites=12, seed=3000 + i)
-        imgs.append(img)
-        Us.append(U)
+    sizes, imgs, Us = syn.image_batch(n_images, L)
     total_px = sum(w * h for w, h in sizes)
     stream = torch.cuda.current_stream()
     # plain per-image path
@@ -167,24 +164,11 @@ def bench_batched(torch, capi, syn, n_images, iters, reps, log):
     }
 
 
-def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
-    """512x512x21, 5 iterations, MatrixCompatibility (-I + N(0, 0.1), learnable) and
-    learnable term weights, cross-entropy loss: forward vs forward+backward through
-    the torch.autograd layer (pydensecrf_amd.autograd.CRFLayer)."""
-    import numpy as np
-    from pydensecrf_amd.autograd import CRFLayer
-    cfg = syn.IMAGE_512x512
-    W, H, L = cfg["W"], cfg["H"], cfg["L"]
-    img, U, gt = syn.make_image(**cfg)
-    rng = np.random.default_rng(1003)
-    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
-    c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
-    m = capi.Model.create_2d(W, H, L)
-    m.add_gaussian(3.0, 3.0, c1)
-    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
-    layer = CRFLayer(m, iters, [c1, c2], kweights=[3.0, 10.0])
-    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
-    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+def time_layer(torch, layer, Ut, gtt, reps):
+    """Forward (no autograd graph) and forward+backward of a CRFLayer with the mean
+    cross-entropy of the true labels, -mean_p log Q[p, gt_p] (the value of
+    nll_loss(log Q, gt); gather + mean, as torch's 2-D nll_loss kernel reduces in one
+    workgroup and would dominate the step), in ms per pass on the current stream."""
     stream = torch.cuda.current_stream()
 
     def fwd():
@@ -192,9 +176,6 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
             layer(Ut)
 
     def fwd_bwd():
-        # mean cross-entropy of the true labels, -mean_p log Q[p, gt_p] (the value of
-        # nll_loss(log Q, gt); gather + mean, as torch's 2-D nll_loss kernel reduces in
-        # one workgroup and would dominate the step)
         Q = layer(Ut)
         loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
         loss.backward()
@@ -210,13 +191,74 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    ms_f = timed(fwd)
-    ms_fb = timed(fwd_bwd)
+    return timed(fwd), timed(fwd_bwd)
+
+
+def bench_fwd_bwd_image(torch, capi, syn, iters, reps, log):
+    """The inference workload (1024x768x21, 10 iterations, Gaussian + bilateral) as a
+    differentiable layer: Potts compatibilities (3, 10) and term weights learnable,
+    cross-entropy of the generator's true labels; forward vs forward+backward."""
+    import numpy as np
+    from pydensecrf_amd.autograd import CRFLayer
+    cfg = syn.IMAGE_1024x768
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, gt = syn.make_image(**cfg)
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, 3.0)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+    layer = CRFLayer(m, iters, [3.0, 10.0])
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+    ms_f, ms_fb = time_layer(torch, layer, Ut, gtt, reps)
+    log("fwd+bwd %dx%dx%d: fwd %.3f ms, fwd+bwd %.3f ms" % (W, H, L, ms_f, ms_fb))
+    N = W * H
+    m.close()
+    return {"workload": "DenseCRF2D %dx%d, %d labels, %d iterations, Gaussian + bilateral, Potts compatibilities "
+                        "and term weights learnable, cross-entropy, torch.autograd" % (W, H, L, iters),
+            "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
+            "fwd_Mpixel_iters_per_s": round(N * iters / (ms_f / 1e3) / 1e6, 2),
+            "fwd_bwd_Mpixel_iters_per_s": round(N * iters / (ms_fb / 1e3) / 1e6, 2)}
+
+
+def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
+    """512x512x21, 5 iterations, MatrixCompatibility (-I + N(0, 0.1), learnable) and
+    learnable term weights, cross-entropy loss: forward vs forward+backward through
+    the torch.autograd layer (pydensecrf_amd.autograd.CRFLayer)."""
+    import numpy as np
+    from pydensecrf_amd.autograd import CRFLayer
+    cfg = syn.IMAGE_512x512
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, gt, c1, c2, kw = syn.gradient_problem_512()
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, c1)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    layer = CRFLayer(m, iters, [c1, c2], kweights=kw)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def fwd_bwd():
+        Q = layer(Ut)
+        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+        loss.backward()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f, ms_fb = time_layer(torch, layer, Ut, gtt, reps)
     log("fwd+bwd 512x512x21: fwd %.3f ms, fwd+bwd %.3f ms" % (ms_f, ms_fb))
     # the same with the kernel parameters (DIAG feature scales, 2 + 5) learnable as well:
     # the backward adds the kernel-parameter gradient; a changed parameter vector
     # rebuilds both lattices before the next forward (timed on its own)
-    layer = CRFLayer(m, iters, [c1, c2], kweights=[3.0, 10.0], learn_kernel=True)
+    layer = CRFLayer(m, iters, [c1, c2], kweights=kw, learn_kernel=True)
     ms_fbk = timed(fwd_bwd)
     kp = m.kernel_params()
     Qd = torch.empty((W * H, L), dtype=torch.float32, device="cuda")
@@ -257,19 +299,9 @@ def bench_volumes(torch, capi, syn, iters, reps, log):
     import numpy as np
     from pydensecrf_amd.autograd import CRFLayer
     from pydensecrf_amd.batched import BatchedDenseCRF
-    depths = syn.batch_volume_depths(8)
     L = 4
-    feats_g, feats_b, Us, gts, sizes = [], [], [], [], []
-    for i, Z in enumerate(depths):
-        vol, U, gt = syn.make_volume(Z, 96, 96, L=L, seed=2000 + i)
-        zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(96), np.arange(96), indexing="ij")
-        pos = np.stack([zz, yy, xx]).reshape(3, -1).astype(np.float32)
-        feats_g.append(pos / np.float32(3))
-        feats_b.append(np.concatenate([pos / np.float32(40), vol.reshape(1, -1) / np.float32(0.1)]))
-        Us.append(U)
-        gts.append(gt)
-        sizes.append(Z * 96 * 96)
-    w = np.array([1, 2, 2, 3], np.float32)
+    sizes, feats_g, feats_b, Us, gts = syn.volume_batch(8, L)
+    w = syn.VOLUME_COMPAT
     t0 = time.perf_counter()
     b = BatchedDenseCRF(sizes, L)
     b.setUnaryEnergy(Us)
@@ -280,30 +312,7 @@ def bench_volumes(torch, capi, syn, iters, reps, log):
     layer = CRFLayer(b.model, iters, [w, w])
     Ut = torch.tensor(np.ascontiguousarray(np.concatenate(Us, axis=1).T), device="cuda", requires_grad=True)
     gtt = torch.tensor(np.concatenate(gts).astype(np.int64), device="cuda")
-    stream = torch.cuda.current_stream()
-
-    def fwd():
-        with torch.no_grad():
-            layer(Ut)
-
-    def fwd_bwd():
-        Q = layer(Ut)
-        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
-        loss.backward()
-
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
-
-    ms_f = timed(fwd)
-    ms_fb = timed(fwd_bwd)
+    ms_f, ms_fb = time_layer(torch, layer, Ut, gtt, reps)
     nvox = sum(sizes)
     log("volumes: %d voxels, fwd %.3f ms, fwd+bwd %.3f ms" % (nvox, ms_f, ms_fb))
     return {"workload": "8 volumes 96x96xZ, Z in [32,64] (seeded), 4 labels, %d iterations, Gaussian d=3 + "
@@ -483,6 +492,7 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(torch, capi, syn, 64, iters, 3, log)
     if not args.no_fwd_bwd:
+        out["fwd_bwd_1024x768"] = bench_fwd_bwd_image(torch, capi, syn, iters, 5, log)
         out["fwd_bwd"] = bench_fwd_bwd(torch, capi, syn, 5, 5, log)
     if not args.no_volumes:
         out["volumes"] = bench_volumes(torch, capi, syn, iters, 3, log)

@@ -400,8 +400,32 @@ def _ref():
         lib.ref_crf_num_unary_params.argtypes = [_P]
         lib.ref_crf_get_unary_params.argtypes = [_P, _F]
         lib.ref_crf_set_unary_params.argtypes = [_P, _F, ctypes.c_int]
+        lib.ref_minimize_lbfgs.argtypes = [ctypes.c_int, _F, _ENERGY_CB, _P, ctypes.c_int, _F]
         lib._typed = True
     return lib
+
+
+# double energy(const float* x, float* g, int n, void* ctx)
+_ENERGY_CB = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                              ctypes.c_int, ctypes.c_void_p)
+
+
+def ref_minimize_lbfgs(x0, fun, restart=0):
+    """The reference's minimizeLBFGS (optimization.cpp:68-103, liblbfgs 1.10) from x0 on
+    fun(x float32 (n,)) -> (value, gradient (n,)).  Returns the float32 result."""
+    x0 = _c32(x0).ravel()
+    n = x0.size
+
+    def cb(xp, gp, nn, _ctx):
+        x = np.ctypeslib.as_array(xp, shape=(nn,)).copy()
+        f, g = fun(x)
+        np.ctypeslib.as_array(gp, shape=(nn,))[:] = np.asarray(g, np.float32)
+        return float(f)
+
+    c = _ENERGY_CB(cb)
+    out = np.empty(n, np.float32)
+    _ref().ref_minimize_lbfgs(n, _fp(x0), c, None, int(restart), _fp(out))
+    return out
 
 
 class RefLattice:

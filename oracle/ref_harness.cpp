@@ -11,6 +11,7 @@
 //   unary     (L, N) row-major  -> Eigen MatrixXf L x N
 //   Q / msgs  [N][L] memory (Eigen col-major L x N), i.e. the reference's storage.
 #include "densecrf.h"
+#include "optimization.h"
 #include "permutohedral.h"
 #include "pairwise.h"
 #include "labelcompatibility.h"
@@ -70,9 +71,34 @@ void copy_out(const MatrixXf& m, float* dst) {
     std::memcpy(dst, m.data(), sizeof(float) * m.size());
 }
 
+// An EnergyFunction (optimization.h:31-35) whose value and gradient come from a C
+// callback, so that minimizeLBFGS can be driven on any energy from Python.
+typedef double (*energy_cb)(const float* x, float* g, int n, void* ctx);
+struct CallbackEnergy : public EnergyFunction {
+    VectorXf x0_;
+    energy_cb cb_;
+    void* ctx_;
+    CallbackEnergy(const float* x0, int n, energy_cb cb, void* ctx) : x0_(n), cb_(cb), ctx_(ctx) {
+        std::memcpy(x0_.data(), x0, sizeof(float) * n);
+    }
+    VectorXf initialValue() override { return x0_; }
+    double gradient(const VectorXf& x, VectorXf& dx) override {
+        dx.resize(x.rows());
+        return cb_(x.data(), dx.data(), (int)x.rows(), ctx_);
+    }
+};
+
 }  // namespace
 
 extern "C" {
+
+// minimizeLBFGS (optimization.cpp:68-103) from x0 on the callback's energy; x_out gets
+// the result (n floats).
+void ref_minimize_lbfgs(int n, const float* x0, energy_cb cb, void* ctx, int restart, float* x_out) {
+    CallbackEnergy e(x0, n, cb, ctx);
+    VectorXf r = minimizeLBFGS(e, restart, false);
+    std::memcpy(x_out, r.data(), sizeof(float) * n);
+}
 
 // ---------------- lattice probe ----------------
 void* ref_lattice_new(const float* feat_dxN, int d, int N) {

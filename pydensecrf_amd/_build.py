@@ -10,6 +10,7 @@ Object files and generated C++ go to _build/ at the repository root.
 """
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -32,6 +33,11 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 CY_MODULES = ["eigen", "densecrf"]
+# Sources whose kernels must not use scratch (per-lane private memory): the per-pixel
+# and per-vertex filter and adjoint kernels keep every array in registers or LDS.  The
+# compiler's resource-usage remarks are read at every build and the build fails on a
+# kernel with ScratchSize > 0.
+NO_SCRATCH_SOURCES = ["filter.hip", "adjoint.hip"]
 
 
 def _hipcc():
@@ -50,19 +56,38 @@ def _run(cmd, cwd=None, verbose=False):
     return r.stdout
 
 
+def scratch_kernels(remarks):
+    """(kernel, bytes per lane) of every kernel with scratch in hipcc's
+    -Rpass-analysis=kernel-resource-usage remarks."""
+    out, name = [], None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0:
+            out.append((name, int(m.group(1))))
+    return out
+
+
 def build_hip(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
     objs = []
     with cf.ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
-        futs = []
+        futs = {}
         for src in HIP_SOURCES:
             obj = os.path.join(BUILD, src.replace(".hip", ".o"))
             objs.append(obj)
-            futs.append(ex.submit(_run, [hipcc] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj],
-                                  None, verbose))
-        for f in futs:
-            f.result()
+            extra = ["-Rpass-analysis=kernel-resource-usage"] if src in NO_SCRATCH_SOURCES else []
+            futs[src] = ex.submit(_run, [hipcc] + HIPCC_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj],
+                                  None, verbose)
+        for src, f in futs.items():
+            remarks = f.result()
+            if src in NO_SCRATCH_SOURCES:
+                bad = scratch_kernels(remarks)
+                if bad:
+                    raise RuntimeError("%s: kernels with scratch memory: %s" % (src, bad))
     out = os.path.join(PKG, "libdcrf_hip.so")
     tmp = out + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose=verbose)

@@ -40,7 +40,7 @@ class _MeanField(torch.autograd.Function):
         unary = unary.contiguous()
         compat = compat.detach().contiguous()
         kweights = kweights.detach().contiguous()
-        layer._sync_kernel_params(kparams.detach())
+        layer._sync_kernel_params(kparams)
         Q = torch.empty_like(unary)
         layer._version += 1
         layer.model.forward_device(layer.n_iter, unary.data_ptr(), Q.data_ptr(), compat.data_ptr(),
@@ -101,16 +101,23 @@ class CRFLayer(torch.nn.Module):
             raise ValueError("need %d kernel parameters, got %d" % (kp0.size, kp.numel()))
         self.kparams = torch.nn.Parameter(kp.cuda(), requires_grad=learn_kernel)
         self._kp_model = torch.as_tensor(kp0, dtype=torch.float32)  # what the lattices were built with
+        self._kp_seen = None  # (storage, in-place version) of the kernel parameters last compared
         self._version = 0
 
     def _sync_kernel_params(self, kp):
-        """Rebuild the lattices when the kernel parameters moved (a host round trip)."""
+        """Rebuild the lattices when the kernel parameters moved.  Reading them back is a
+        host round trip (a stream synchronisation), so it happens only when the tensor
+        was replaced or written in place (an optimiser step) since the last forward."""
         if not kp.numel():
+            return
+        seen = (kp.data_ptr(), kp._version)
+        if seen == self._kp_seen:
             return
         host = kp.float().cpu()
         if not torch.equal(host, self._kp_model):
             self.model.set_kernel_params(host.numpy())
             self._kp_model = host.clone()
+        self._kp_seen = seen
 
     def compat_of(self, k):
         """Raw compatibility parameters of term k (a view of the flat parameter)."""

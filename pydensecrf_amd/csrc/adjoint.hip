@@ -100,18 +100,24 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
 }
 
 // partial[block][label] = sum over the block's pixels of b .* M (groups stride over the
-// pixels; the block's groups are then added in group order per label).
+// pixels; the block's groups are then added in group order per label).  The grid covers
+// `iters` iterations of nb blocks each: block i * nb + j takes iteration i's operands
+// b + i * bstride and M + i * mstride.
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b, const float* __restrict__ M, int N,
-                                                    int L, int S, int per, float* __restrict__ partial) {
+                                                    int L, int S, int per, int nb, size_t bstride, size_t mstride,
+                                                    float* __restrict__ partial) {
     extern __shared__ float sh[];  // block * R floats
+    const int it = blockIdx.x / nb, blk = blockIdx.x - it * nb;
+    b += (size_t)it * bstride;
+    M += (size_t)it * mstride;
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int items = LaneGroup<WAVE, R>::items_per_block(per);
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
     if (G.on) {
-        for (int p = G.item(); p < N; p += gridDim.x * items) {
+        for (int p = G.item_in(blk); p < N; p += nb * items) {
             float x[R], y[R];
             load_piece<R>(b + (size_t)p * L, G.k0, L, x);
             load_piece<R>(M + (size_t)p * L, G.k0, L, y);
@@ -139,23 +145,27 @@ __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b,
     }
 }
 
-// The L x L statistic G[a][c] = sum_p b[p][a] M[p][c] is a tall-skinny GEMM (K = N
-// pixels): one wave per (32 x 32 output tile, pixel range) accumulates it on the
-// matrix cores with v_mfma_f32_32x32x2_f32, two pixels per instruction, kMfmaBatch
-// pixel pairs of operands in flight; lane l supplies A[l % 32][l / 32] = b[p + l/32][a]
-// and B[l / 32][l % 32] = M[p + l/32][c] (labels past L read as 0).  The range's tile
-// goes to partial[range][a * L + c]; k_reduce_cols adds the ranges in double.
+// The L x L statistic G[a][c] = sum_t sum_p b_t[p][a] M_t[p][c] is a tall-skinny GEMM
+// (K = iterations x pixels): one wave per (iteration, pixel range, 32 x 32 output tile)
+// accumulates it on the matrix cores with v_mfma_f32_32x32x2_f32, two pixels per
+// instruction, kMfmaBatch pixel pairs of operands in flight; lane l supplies
+// A[l % 32][l / 32] = b[p + l/32][a] and B[l / 32][l % 32] = M[p + l/32][c] (labels past
+// L read as 0).  Iteration t's operands are b + t * bstride and M + t * mstride.  The
+// tile goes to partial[t * ranges + range][a * L + c]; k_reduce_cols adds them in double.
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int kMfmaBatch = 8;
 
 __global__ void __launch_bounds__(256) k_grad_mfma(const float* __restrict__ b, const float* __restrict__ M, int N,
-                                                   int L, int tiles1, int ppw, int n_waves,
-                                                   float* __restrict__ partial) {
+                                                   int L, int tiles1, int ppw, int ranges, size_t bstride,
+                                                   size_t mstride, int n_waves, float* __restrict__ partial) {
     const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= n_waves) return;  // whole waves only; no barrier in this kernel
     const int lane = threadIdx.x & 63;
     const int tiles2 = tiles1 * tiles1;
-    const int range = w / tiles2, tile = w - range * tiles2;
+    const int it_range = w / tiles2, tile = w - it_range * tiles2;
+    const int it = it_range / ranges, range = it_range - it * ranges;
+    b += (size_t)it * bstride;
+    M += (size_t)it * mstride;
     const int ta = tile / tiles1, tc = tile - ta * tiles1;
     const int i = lane & 31, k = lane >> 5;
     const int a = ta * 32 + i, c = tc * 32 + i;
@@ -180,7 +190,7 @@ __global__ void __launch_bounds__(256) k_grad_mfma(const float* __restrict__ b, 
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u] * (ma * live), bv[u] * mc, acc, 0, 0, 0);
         }
     }
-    float* out = partial + (size_t)range * L * L;
+    float* out = partial + (size_t)it_range * L * L;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int row = ta * 32 + 8 * (r >> 2) + 4 * k + (r & 3), col = tc * 32 + i;
@@ -325,32 +335,58 @@ int grad_partial_blocks(int N, int L) {
     return need < 1024 ? need : 1024;
 }
 
-size_t grad_partial_floats() { return (size_t)1 << 20; }
+namespace {
+// Pixel ranges per iteration of the matrix-core statistic: about kStatWaves waves over
+// all iterations, at least 16 pixels per range; an even range length keeps the MFMA
+// pixel pairs aligned.
+constexpr long long kStatWaves = 4096;
+void stat_ranges(int N, int L, int iters, int* ppw_out, long long* ranges_out) {
+    const int tiles1 = (L + 31) / 32;
+    const long long per_iter = std::max<long long>(1, kStatWaves / ((long long)iters * tiles1 * tiles1));
+    long long ranges = std::max<long long>(1, std::min<long long>(per_iter, (N + 15) / 16));
+    int ppw = (int)((N + ranges - 1) / ranges);
+    ppw = (ppw + 1) & ~1;
+    *ppw_out = ppw;
+    *ranges_out = (N + ppw - 1) / ppw;
+}
+}  // namespace
 
-void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
-                      hipStream_t s) {
+size_t grad_partial_floats(int N, int L, int iters, bool full) {
+    if (!full) return (size_t)iters * grad_partial_blocks(N, L) * L;
+    int ppw;
+    long long ranges;
+    stat_ranges(N, L, iters, &ppw, &ranges);
+    return (size_t)iters * ranges * L * L;
+}
+
+void launch_grad_stat(const float* b, size_t bstride, const float* M, size_t mstride, int iters, int N, int L,
+                      bool full, float* partial, double* acc, hipStream_t s) {
+    if (iters < 1) return;
     if (!full) {
         const int nb = grad_partial_blocks(N, L);
         const LaunchShape sh = shape_for(L);
-        DCRF_DISPATCH(sh, k_grad_diag, nb, sizeof(float) * sh.block * sh.R, s, b, M, N, L, sh.S, sh.per, partial);
-        k_reduce_cols<<<L, 256, 0, s>>>(partial, nb, L, acc, 0);
+        DCRF_DISPATCH(sh, k_grad_diag, (unsigned)iters * nb, sizeof(float) * sh.block * sh.R, s, b, M, N, L, sh.S,
+                      sh.per, nb, bstride, mstride, partial);
+        const long long nblk = (long long)iters * nb;
+        const int threads = nblk >= 256 ? 256 : (nblk >= 64 ? 64 : 32);
+        k_reduce_cols<<<L, threads, 0, s>>>(partial, (int)nblk, L, acc, 0);
         DCRF_HIP_CHECK(hipGetLastError());
         return;
     }
-    // pixel ranges: as many as the partial buffer holds (L*L floats each), at least 16
-    // pixels per range; an even range length keeps the MFMA pixel pairs aligned
     const long long LL = (long long)L * L;
-    long long ranges = std::max<long long>(1, std::min<long long>((long long)grad_partial_floats() / LL, (N + 15) / 16));
-    int ppw = (int)((N + ranges - 1) / ranges);
-    ppw = (ppw + 1) & ~1;
-    ranges = (N + ppw - 1) / ppw;
+    int ppw;
+    long long ranges;
+    stat_ranges(N, L, iters, &ppw, &ranges);
     const int tiles1 = (L + 31) / 32;
-    const long long waves = ranges * tiles1 * tiles1;
-    if (waves > 0x7fffffffll / 64) throw Error{DCRF_ERR_VALUE, "gradient statistic too large"};
-    k_grad_mfma<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(b, M, N, L, tiles1, ppw, (int)waves, partial);
+    const long long waves = (long long)iters * ranges * tiles1 * tiles1;
+    if (waves > 0x7fffffffll / 64 || (long long)iters * ranges > 0x7fffffffll)
+        throw Error{DCRF_ERR_VALUE, "gradient statistic too large"};
+    k_grad_mfma<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(b, M, N, L, tiles1, ppw, (int)ranges, bstride, mstride,
+                                                            (int)waves, partial);
     DCRF_HIP_CHECK(hipGetLastError());
-    const int threads = ranges >= 256 ? 256 : (ranges >= 64 ? 64 : 32);
-    k_reduce_cols<<<(unsigned)LL, threads, 0, s>>>(partial, (int)ranges, (int)LL, acc, 0);
+    const long long nblk = (long long)iters * ranges;
+    const int threads = nblk >= 256 ? 256 : (nblk >= 64 ? 64 : 32);
+    k_reduce_cols<<<(unsigned)LL, threads, 0, s>>>(partial, (int)nblk, (int)LL, acc, 0);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

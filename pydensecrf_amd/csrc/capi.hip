@@ -149,6 +149,12 @@ struct dcrf_model {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    // what a captured graph reads: the model's own unary buffer (nullptr: the caller's
+    // unary, or none) and the term-weight buffer (nullptr: no weights)
+    bool graph_own_unary = false;
+    const float* graph_unary = nullptr;
+    const float* graph_kw = nullptr;
+
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1048,15 +1054,19 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     if (m.train_n < 0) throw Error{DCRF_ERR_STATE, "backward needs a preceding training forward"};
     const int N = m.N(), L = m.L, T = (int)m.terms.size(), n = m.train_n;
     const size_t NL = (size_t)N * L;
-    m.bbuf.reserve(NL);
-    float* b = m.bbuf.get();
+    // b_t, the adjoint entering reverse iteration t - 1, in slot t (t = n .. 0): every
+    // iteration's b stays for the compatibility statistics, taken once at the end
+    m.bbuf.reserve((size_t)(n + 1) * NL);
+    auto bslot = [&](int t) { return m.bbuf.get() + (size_t)t * NL; };
     const float* Qc = m.qcache.get();
-    launch_sn_first(Qc + (size_t)n * NL, dQ, b, dU, N, L, s);
+    launch_sn_first(Qc + (size_t)n * NL, dQ, bslot(n), dU, N, L, s);
     std::vector<size_t> goff(T + 1, 0);
     for (int k = 0; k < T; k++) goff[k + 1] = goff[k] + stat_len(m.terms[k]->ckind, L);
     m.gacc.reserve(std::max<size_t>(goff[T], 1));
     launch_fill_double(m.gacc.get(), goff[T], s);
-    m.partial.reserve(std::max(grad_partial_floats(), (size_t)grad_partial_blocks(N, L) * L));
+    size_t pf = 1;
+    for (auto& t : m.terms) pf = std::max(pf, grad_partial_floats(N, L, n, t->ckind == DCRF_COMPAT_MATRIX));
+    m.partial.reserve(pf);
     std::vector<int> koff(T + 1, 0);
     for (int k = 0; k < T; k++) koff[k + 1] = koff[k] + m.terms[k]->num_kparams();
     kernel = kernel && koff[T] > 0;
@@ -1070,9 +1080,7 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
         launch_fill_double(m.kacc.get(), koff[T], s);
     }
     for (int it = n - 1; it >= 0; it--) {
-        for (int k = 0; k < T; k++)
-            launch_grad_stat(b, m.mcache.get() + ((size_t)it * T + k) * NL, N, L,
-                             m.terms[k]->ckind == DCRF_COMPAT_MATRIX, m.partial.get(), m.gacc.get() + goff[k], s);
+        float* b = bslot(it + 1);
         if (kernel) {
             // DenseCRF::gradient takes the kernel gradient with b before its update
             // (densecrf.cpp:280-284)
@@ -1083,11 +1091,16 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
         }
         run_filter(m, b, L, true, s);
         if (m.fused)
-            launch_adjoint_fused(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw),
+            launch_adjoint_fused(Qc + (size_t)it * NL, bslot(it), dU, N, L, slice_table(m, true, m.train_kw),
                                  adjoint_pairs(m, s), s);
         else
-            launch_adjoint(Qc + (size_t)it * NL, b, dU, N, L, slice_table(m, true, m.train_kw), s);
+            launch_adjoint(Qc + (size_t)it * NL, bslot(it), dU, N, L, slice_table(m, true, m.train_kw), s);
     }
+    // G_k = sum_t b_{t+1} (K~_k Q_t)^T over all iterations: b_{t+1} in slot t + 1, the
+    // messages of iteration t at mcache[t][k]
+    for (int k = 0; k < T; k++)
+        launch_grad_stat(bslot(1), NL, m.mcache.get() + (size_t)k * NL, (size_t)T * NL, n, N, L,
+                         m.terms[k]->ckind == DCRF_COMPAT_MATRIX, m.partial.get(), m.gacc.get() + goff[k], s);
     if (dcompat || dkw) {
         size_t coff = 0;
         for (int k = 0; k < T; k++) {
@@ -1403,6 +1416,9 @@ int dcrf_capture_inference(dcrf_model* m, int n_iter, const float* U_dev, float*
         }
         DCRF_HIP_CHECK(hipStreamEndCapture(m->stream, &m->graph));
         DCRF_HIP_CHECK(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+        m->graph_own_unary = U_dev == nullptr;
+        m->graph_unary = U_dev ? nullptr : U;
+        m->graph_kw = m->kw_set ? m->kw.get() : nullptr;
         m->graph_valid = true;
     });
 }
@@ -1412,6 +1428,20 @@ int dcrf_launch_graph(dcrf_model* m, void* stream) {
         require(m != nullptr, "null model");
         require(m->graph_valid && m->layout_valid, "no captured inference (or the model changed since)",
                 DCRF_ERR_STATE);
+        // The graph reads the unary and term-weight buffers it was captured with: new
+        // values in the same buffers are picked up (a unary set since the capture is
+        // uploaded first), a buffer that appeared, went or moved needs a new capture.
+        if (m->graph_own_unary) {
+            require(m->has_unary == (m->graph_unary != nullptr),
+                    "the unary was set or removed since the capture: capture again", DCRF_ERR_STATE);
+            if (m->has_unary) {
+                ensure_unary(*m);
+                require(m->unary.get() == m->graph_unary, "the unary buffer moved since the capture: capture again",
+                        DCRF_ERR_STATE);
+            }
+        }
+        require((m->kw_set ? m->kw.get() : nullptr) == m->graph_kw,
+                "term weights were set or removed since the capture: capture again", DCRF_ERR_STATE);
         DCRF_HIP_CHECK(hipGraphLaunch(m->graph_exec, (hipStream_t)stream));
     });
 }
@@ -1791,6 +1821,15 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
         DCRF_HIP_CHECK(hipMemcpyAsync(G.data(), m->gacc.get(), sizeof(double) * glen, hipMemcpyDeviceToHost, s));
         std::vector<float> w(T, 1.0f);
         if (kw) DCRF_HIP_CHECK(hipMemcpyAsync(w.data(), kw, sizeof(float) * T, hipMemcpyDeviceToHost, s));
+        // the compatibilities the forward used: the device copies (a training forward with
+        // raw compatibilities, dcrf_forward_device, writes them there only)
+        std::vector<std::vector<float>> cw(T);
+        for (int k = 0; k < T; k++) {
+            const Term& t = *m->terms[k];
+            cw[k].resize(compat_raw_len(t.ckind, L));
+            DCRF_HIP_CHECK(hipMemcpyAsync(cw[k].data(), t.d_compat.get(), sizeof(float) * cw[k].size(),
+                                          hipMemcpyDeviceToHost, s));
+        }
         DCRF_HIP_CHECK(hipStreamSynchronize(s));
         if (objective) *objective = objective_value(objective_kind, OP, L);
         // pack like DenseCRF::gradient (labelcompatibility.cpp:57-61, 76-78, 101-108)
@@ -1804,11 +1843,11 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
                 double sd = 0.0;
                 for (int a = 0; a < L; a++) sd += g[a];
                 if (cp) *cp++ = (float)(-w[k] * sd);
-                kg = -(double)t.compat[0] * sd;
+                kg = -(double)cw[k][0] * sd;
             } else if (t.ckind == DCRF_COMPAT_DIAGONAL) {
                 for (int a = 0; a < L; a++) {
                     if (cp) *cp++ = (float)(w[k] * g[a]);
-                    kg += (double)t.compat[a] * g[a];
+                    kg += (double)cw[k][a] * g[a];
                 }
             } else {
                 for (int i = 0; i < L; i++)
@@ -1816,7 +1855,7 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
                         const double v = i == j ? g[i * L + i] : g[i * L + j] + g[j * L + i];
                         if (cp) *cp++ = (float)(w[k] * v);
                     }
-                for (int j = 0; j < L * L; j++) kg += (double)t.compat[j] * g[j];
+                for (int j = 0; j < L * L; j++) kg += (double)cw[k][j] * g[j];
             }
             if (kweight_grad) kweight_grad[k] = (float)kg;
             go += stat_len(t.ckind, L);

@@ -133,11 +133,12 @@ struct LaneGroup {
         k0 = l * R;
         base = WAVE ? (t & ~63) + g * S : g * S;
     }
-    // Index of this group's item.
-    __device__ __forceinline__ int item() const {
-        const int b = xcd_block(blockIdx.x, gridDim.x);
+    // Index of this group's item in block b of a grid.
+    __device__ __forceinline__ int item_in(int b) const {
         return WAVE ? (b * (kWaveBlock / 64) + (t >> 6)) * per + g : b * per + g;
     }
+    // Index of this group's item (blocks taken in XCD-contiguous order).
+    __device__ __forceinline__ int item() const { return item_in(xcd_block(blockIdx.x, gridDim.x)); }
     __device__ static __forceinline__ int items_per_block(int per) { return WAVE ? (kWaveBlock / 64) * per : per; }
     __device__ __forceinline__ bool has(int) const { return true; }  // R divides L: every slot is a label
 

@@ -352,18 +352,20 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
 // ---------------------------------------------------------------- adjoint kernels
 // b = sumAndNormalize(Q .* dQ, Q); dU = b.
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s);
-// b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) in place; dU += b.
+// b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) (b is written, not read); dU += b.
 void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s);
 // The same over interleaved slice pairs (those of launch_meanfield_fused, with the
 // transposed filter's slice weights); models that fused_meanfield_ok accepts.
 void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, const int2* slc,
                           hipStream_t s);
-// acc += sum_p b_p M_p^T (full L*L when `full`, else its diagonal [L]); partial holds
-// max(grad_partial_floats(), grad_partial_blocks(N, L) * L) floats.
+// acc += sum_{t < iters} sum_p b_t[p] M_t[p]^T (full L*L when `full`, else its diagonal
+// [L]), with b_t = b + t * bstride and M_t = M + t * mstride; one launch and one
+// reduction for all iterations.  partial holds grad_partial_floats(N, L, iters, full)
+// floats.
 int grad_partial_blocks(int N, int L);
-size_t grad_partial_floats();
-void launch_grad_stat(const float* b, const float* M, int N, int L, bool full, float* partial, double* acc,
-                      hipStream_t s);
+size_t grad_partial_floats(int N, int L, int iters, bool full);
+void launch_grad_stat(const float* b, size_t bstride, const float* M, size_t mstride, int iters, int N, int L,
+                      bool full, float* partial, double* acc, hipStream_t s);
 void launch_fill_double(double* p, size_t n, hipStream_t s);
 void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
                           float* dkw, hipStream_t s);

@@ -82,3 +82,59 @@ def batch_image_sizes(n=64, lo=256, hi=1024, seed=1002):
 def batch_volume_depths(n=8, lo=32, hi=64, seed=1004):
     rng = np.random.default_rng(seed)
     return [int(z) for z in rng.integers(lo, hi + 1, n)]
+
+
+def image_batch(n=64, L=21):
+    """The uneven image batch: sides U[256, 1024] (batch_image_sizes), 12 Voronoi sites,
+    image i seeded 3000 + i.  Returns (sizes [(W, H)], images, unaries)."""
+    sizes = batch_image_sizes(n)
+    imgs, Us = [], []
+    for i, (w, h) in enumerate(sizes):
+        img, U, _ = make_image(w, h, L=L, sites=12, seed=3000 + i)
+        imgs.append(img)
+        Us.append(U)
+    return sizes, imgs, Us
+
+
+VOLUME_COMPAT = np.array([1, 2, 2, 3], np.float32)  # DiagonalCompatibility of both terms
+
+
+def volume_features(vol):
+    """Gaussian features (z, y, x) / 3 (d = 3) and bilateral features (z, y, x) / 40 with
+    intensity / 0.1 (d = 4), each (d, Z*Y*X) float32 in (z, y, x) raster order."""
+    Z, Y, X = vol.shape
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    pos = np.stack([zz, yy, xx]).reshape(3, -1).astype(np.float32)
+    fg = pos / np.float32(3)
+    fb = np.concatenate([pos / np.float32(40), vol.reshape(1, -1) / np.float32(0.1)])
+    return np.ascontiguousarray(fg), np.ascontiguousarray(fb)
+
+
+def volume_batch(n=8, L=4):
+    """The volume batch: n volumes of 96x96xZ, Z ~ U{32..64} (batch_volume_depths), volume
+    i seeded 2000 + i.  Returns (voxel counts, Gaussian features, bilateral features,
+    unaries, true labels)."""
+    sizes, fgs, fbs, Us, gts = [], [], [], [], []
+    for i, Z in enumerate(batch_volume_depths(n)):
+        vol, U, gt = make_volume(Z, 96, 96, L=L, seed=2000 + i)
+        fg, fb = volume_features(vol)
+        sizes.append(Z * 96 * 96)
+        fgs.append(fg)
+        fbs.append(fb)
+        Us.append(U)
+        gts.append(gt)
+    return sizes, fgs, fbs, Us, gts
+
+
+def gradient_problem_512():
+    """The differentiable 512x512x21 workload (5 iterations): image, unary and true labels
+    of IMAGE_512x512, MatrixCompatibility per term initialised to -I + N(0, 0.1) (seed
+    1003; Gaussian term first), term weights (3, 10) for the Gaussian (sxy = 3) and the
+    bilateral (sxy = 80, srgb = 13) term.  Returns (img, U, gt, c1, c2, kweights)."""
+    cfg = IMAGE_512x512
+    L = cfg["L"]
+    img, U, gt = make_image(**cfg)
+    rng = np.random.default_rng(1003)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    return img, U, gt, c1, c2, np.array([3.0, 10.0], np.float32)

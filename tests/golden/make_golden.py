@@ -27,6 +27,13 @@ Fixtures
   image_320x213.npz  the 320x213x21 5-iteration workload: V per term, SHA-256 of the
                      lattice offsets, Q on a seeded sample of 4096 pixels.
   image_1024x768.npz the 1024x768x21 10-iteration workload: V per term, Q sample.
+  batch_64.npz       the 64-image batch: Q samples and MAP hashes of three instances.
+  volumes_8.npz      the 8-volume batch: Q samples, MAP hashes and DenseCRF::gradient
+                     of two volumes.
+  grad_512x512.npz   the 512x512x21 Matrix-compatibility workload: DenseCRF::gradient,
+                     and FP64 central differences of the cross-entropy from the oracle.
+  lbfgs_small.npz    minimizeLBFGS (liblbfgs 1.10): evaluated points and results on
+                     Rosenbrock and on a small CRF learning energy.
 """
 import hashlib
 import os
@@ -271,6 +278,172 @@ def image_workload(name, cfg, fname):
                         V_gaussian=np.int32(latg.V), offset_gaussian_sha=sha(latg.offset[:W * H]))
 
 
+def batch_64():
+    """The 64-image batch (synthetic.image_batch, 10 iterations, Gaussian sxy=3 w=3 +
+    bilateral sxy=80 srgb=13 w=10, Potts): the reference's Q on a 4096-pixel sample and
+    its MAP hash for three instances, the smallest, the median and the largest."""
+    sizes, imgs, Us = syn.image_batch(64)
+    Ns = np.array([w * h for w, h in sizes])
+    order = np.argsort(Ns, kind="stable")
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    out = {"sizes": np.array(sizes, np.int32), "picks": np.array(picks, np.int32), "iters": np.int32(10)}
+    for j, i in enumerate(picks):
+        w, h = sizes[i]
+        crf = po.RefCRF(W=w, H=h, L=21)
+        crf.set_unary(Us[i])
+        crf.add_gaussian(3.0, 3.0, 3.0)
+        crf.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, imgs[i], 10.0)
+        Q = crf.inference(10)
+        idx = np.sort(np.random.default_rng(11 + j).choice(w * h, 4096, replace=False))
+        p = "p%d_" % j
+        out.update({p + "idx": idx, p + "Q_sample": Q[:, idx], p + "map_sha": sha(Q.argmax(0).astype(np.int16)),
+                    p + "unary_sha": sha(Us[i]), p + "image_sha": sha(imgs[i])})
+    np.savez_compressed(os.path.join(HERE, "batch_64.npz"), **out)
+
+
+def volumes_8():
+    """The 8-volume batch (synthetic.volume_batch, L = 4, DiagonalCompatibility [1,2,2,3]
+    per term, 10 iterations): for the first and the largest volume, the reference's Q on
+    a 4096-voxel sample and its MAP hash, and DenseCRF::gradient (LogLikelihood of the
+    true labels, identity-gradient unary) on that volume alone: objective, unary gradient
+    on the sample, its largest magnitude over all entries, compatibility gradient."""
+    sizes, fgs, fbs, Us, gts = syn.volume_batch(8)
+    picks = [0, int(np.argmax(sizes))]
+    out = {"sizes": np.array(sizes, np.int64), "picks": np.array(picks, np.int32), "iters": np.int32(10)}
+    for j, i in enumerate(picks):
+        crf = po.RefCRF(N=sizes[i], L=4, identity_unary=True)
+        crf.set_unary(Us[i])
+        crf.add_pairwise(fgs[i], syn.VOLUME_COMPAT)
+        crf.add_pairwise(fbs[i], syn.VOLUME_COMPAT)
+        Q = crf.inference(10)
+        r, ug, cg = crf.gradient(10, gts[i])
+        idx = np.sort(np.random.default_rng(21 + j).choice(sizes[i], 4096, replace=False))
+        p = "p%d_" % j
+        out.update({p + "idx": idx, p + "Q_sample": Q[:, idx], p + "map_sha": sha(Q.argmax(0).astype(np.int16)),
+                    p + "unary_sha": sha(Us[i]), p + "obj": np.float64(r), p + "unary_grad_sample": ug[:, idx],
+                    p + "unary_grad_absmax": np.float32(np.abs(ug).max()), p + "compat_grad": cg})
+    np.savez_compressed(os.path.join(HERE, "volumes_8.npz"), **out)
+
+
+# FP64 central differences of the 512x512 cross-entropy (workers of a process pool)
+_FD = {}
+FD_ITERS, FD_H = 5, 1e-3
+
+
+def _fd_init():
+    img, U, gt, c1, c2, kw = syn.gradient_problem_512()
+    o = po.OracleCRF(W=512, H=512, L=21)
+    o.set_unary(U)
+    o.add_gaussian(3.0, 3.0, c1)
+    o.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    _FD.update(o=o, U=U.astype(np.float64), gt=gt, raw=np.concatenate([c1.ravel(), c2.ravel()]).astype(np.float64),
+               kw=kw.astype(np.float64), L=21)
+
+
+def _fd_ce(kind, idx, delta):
+    c = _FD
+    U, raw, kw = c["U"], c["raw"], c["kw"]
+    if kind == "u":
+        U = U.copy()
+        U.flat[idx] += delta
+    elif kind == "c":  # one unordered pair (a, b) of term t: m_ab and m_ba move together
+        t, a, b = idx
+        L = c["L"]
+        raw = raw.copy()
+        raw[t * L * L + a * L + b] += delta
+        if a != b:
+            raw[t * L * L + b * L + a] += delta
+    elif kind == "k":
+        kw = kw.copy()
+        kw[idx] += delta
+    return c["o"].ce_fp64(FD_ITERS, c["gt"], U, raw, kw)
+
+
+def _fd_job(job):
+    kind, idx = job
+    if kind == "0":
+        return _fd_ce("0", None, 0.0)
+    return (_fd_ce(kind, idx, FD_H) - _fd_ce(kind, idx, -FD_H)) / (2 * FD_H)
+
+
+def grad_512x512(jobs=None):
+    """The differentiable 512x512x21 workload (synthetic.gradient_problem_512, 5
+    iterations).
+
+    Reference: DenseCRF::gradient (LogLikelihood of the true labels, identity-gradient
+    unary) with MatrixCompatibility w_k m_k (the term weight folded into the matrix):
+    objective, dL/dU on 64 entries and its largest magnitude over all entries, the
+    packed compatibility gradient of both terms.  The 64 unary entries are drawn (seed 7)
+    32 among those whose reference gradient is at least 1% of the largest and 32 among
+    those between 1e-4 and 1% of it: most pixels are saturated, and their entries (~1e-11
+    of a largest ~6e-6) would only test rounding.
+
+    FP64 central differences (h = 1e-3) of the mean cross-entropy with the oracle on the
+    same lattices, over the parameters of the autograd layer (raw m_k and w_k): the 64
+    unary entries, every unordered compatibility pair (a, b) of each term (m_ab and m_ba
+    moved together, so the quotient is dCE/dm_ab + dCE/dm_ba off the diagonal), and the
+    two term weights; plus the FP64 cross-entropy itself."""
+    import multiprocessing as mp
+    img, U, gt, c1, c2, kw = syn.gradient_problem_512()
+    L = 21
+    crf = po.RefCRF(W=512, H=512, L=L, identity_unary=True)
+    crf.set_unary(U)
+    crf.add_gaussian(3.0, 3.0, (kw[0] * c1).astype(np.float32))
+    crf.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, (kw[1] * c2).astype(np.float32))
+    r, ug, cg = crf.gradient(FD_ITERS, gt)
+    mag = np.abs(ug.ravel()) / np.abs(ug).max()
+    rng = np.random.default_rng(7)
+    sel = np.sort(np.concatenate([rng.choice(np.flatnonzero(mag >= 1e-2), 32, replace=False),
+                                  rng.choice(np.flatnonzero((mag >= 1e-4) & (mag < 1e-2)), 32, replace=False)]))
+    pairs = [(t, a, b) for t in range(2) for a in range(L) for b in range(a, L)]
+    work = [("0", None)] + [("u", int(i)) for i in sel] + [("c", p) for p in pairs] + [("k", 0), ("k", 1)]
+    with mp.Pool(jobs or os.cpu_count() or 1, initializer=_fd_init) as pool:
+        vals = np.array(pool.map(_fd_job, work, chunksize=4))
+    nu = len(sel)
+    np.savez_compressed(os.path.join(HERE, "grad_512x512.npz"), sel=sel, ref_obj=np.float64(r),
+                        ref_unary_grad_sel=ug.ravel()[sel], ref_unary_grad_absmax=np.float32(np.abs(ug).max()),
+                        ref_compat_grad=cg, ce_fp64=np.float64(vals[0]), fd_unary=vals[1:1 + nu],
+                        fd_pairs=vals[1 + nu:1 + nu + len(pairs)], pairs=np.array(pairs, np.int32),
+                        fd_kweights=vals[1 + nu + len(pairs):], unary_sha=sha(U), image_sha=sha(img),
+                        iters=np.int32(FD_ITERS), h=np.float64(FD_H))
+
+
+def lbfgs_small():
+    """minimizeLBFGS (optimization.cpp:68-103 over liblbfgs 1.10) from the reference
+    build: every point it evaluates and its result, on the extended Rosenbrock function
+    from two starts, and on a CRF learning energy (tests/helpers.py crf_learning_energy:
+    -LogLikelihood + l2 / 2 ||x||^2 over Diagonal compatibilities and DIAG kernel
+    parameters, evaluated with the reference's DenseCRF::gradient) with one restart."""
+    from tests.helpers import crf_learning_energy, crf_learning_model, rosenbrock
+    out = {}
+
+    def traced(fun, trace):
+        def f(x):
+            trace.append(x.copy())
+            return fun(x)
+        return f
+
+    for i, x0 in enumerate([np.array([-1.2, 1.0] * 3, np.float32),
+                            np.array([3.0, -2.0, 0.5, 4.0, -1.0, 1.0, 2.0, 2.0], np.float32)]):
+        tr = []
+        out["r%d_x" % i] = po.ref_minimize_lbfgs(x0, traced(rosenbrock, tr), 0)
+        out["r%d_x0" % i], out["r%d_trace" % i] = x0, np.array(tr)
+    crf, gt = crf_learning_model(po.RefCRF, identity_unary=True)
+    x0 = np.concatenate([crf.compat_params(), crf.kernel_params()]).astype(np.float32)
+    tr, vals = [], []
+    fun = crf_learning_energy(crf, gt)
+
+    def rec(x):
+        tr.append(x.copy())
+        f, g = fun(x)
+        vals.append(f)
+        return f, g
+
+    out["crf_x"] = po.ref_minimize_lbfgs(x0, rec, 1)
+    out["crf_x0"], out["crf_trace"], out["crf_values"] = x0, np.array(tr), np.array(vals)
+    np.savez_compressed(os.path.join(HERE, "lbfgs_small.npz"), **out)
+
+
 if __name__ == "__main__":
     if not po.ref_available():
         sys.exit("build the reference first: make -C oracle ref")
@@ -278,7 +451,9 @@ if __name__ == "__main__":
                "learning_small": learning_small,
                "simple_10x10": simple_10x10,
                "image_320x213": lambda: image_workload("320x213", syn.IMAGE_320x213, "image_320x213.npz"),
-               "image_1024x768": lambda: image_workload("1024x768", syn.IMAGE_1024x768, "image_1024x768.npz")}
+               "image_1024x768": lambda: image_workload("1024x768", syn.IMAGE_1024x768, "image_1024x768.npz"),
+               "batch_64": batch_64, "volumes_8": volumes_8, "grad_512x512": grad_512x512,
+               "lbfgs_small": lbfgs_small}
     # python tests/golden/make_golden.py [fixture ...]: all of them by default
     for name in sys.argv[1:] or list(writers):
         writers[name]()

@@ -84,6 +84,91 @@ def test_pairwise_features_order_and_scale():
     np.testing.assert_array_equal(g[3], 2.0)
 
 
+@pytest.mark.parametrize("sxy,srgb", [((3.0, 3.0), (13.0, 13.0, 13.0)), ((80.0, 80.0), (13.0, 13.0, 13.0)),
+                                      ((2.5, 7.3), (13.0, 7.1, 0.3))])
+def test_pairwise_features_match_densecrf2d(sxy, srgb):
+    """The utils features are DenseCRF2D's own with the two coordinate rows swapped
+    ((y, x) against (x, y), densecrf.cpp:61-81, float32 divisions): the oracle (bit-exact
+    with the reference's lattice) builds the identical lattice from either."""
+    from oracle import pyoracle as po
+    W, H = 23, 17
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (H, W, 3)).astype(np.uint8)
+    sx, sy = sxy
+    fb = utils.create_pairwise_bilateral(sdims=(sy, sx), schan=srgb, img=img, chdim=2)
+    fg = utils.create_pairwise_gaussian(sdims=(sy, sx), shape=(H, W))
+    assert fb.shape == (5, W * H) and fg.shape == (2, W * H)
+    assert np.array_equal(fb[:2], fg)
+    swap = [1, 0, 2, 3, 4]
+    a = po.OracleCRF(W=W, H=H, L=2)
+    a.add_gaussian(sx, sy, 1.0)
+    a.add_bilateral(sx, sy, srgb[0], srgb[1], srgb[2], img, 1.0)
+    b = po.OracleCRF(N=W * H, L=2)
+    b.add_pairwise(np.ascontiguousarray(fg[[1, 0]]), 1.0)
+    b.add_pairwise(np.ascontiguousarray(fb[swap]), 1.0)
+    for k in range(2):
+        la, lb = a.lattice(k), b.lattice(k)
+        assert la.V == lb.V
+        assert np.array_equal(la.offset, lb.offset)
+        assert np.array_equal(la.bary.view(np.int32), lb.bary.view(np.int32))
+
+
+def test_pairwise_features_nd_and_channel_axes():
+    """A 3-D grid with the channel axis first, last, or absent; scales given for fewer
+    channels than there are leave the rest unscaled."""
+    vol = np.random.default_rng(1).integers(0, 9, (4, 5, 6)).astype(np.float32)
+    f = utils.create_pairwise_bilateral(sdims=(1, 2, 3), schan=0.5, img=vol, chdim=-1)
+    assert f.shape == (4, 120) and f.dtype == np.float32
+    zz, yy, xx = np.meshgrid(np.arange(4), np.arange(5), np.arange(6), indexing="ij")
+    np.testing.assert_array_equal(f[0], zz.ravel())
+    np.testing.assert_array_equal(f[2], xx.ravel().astype(np.float32) / np.float32(3))
+    np.testing.assert_array_equal(f[3], vol.ravel() / np.float32(0.5))
+    two = np.stack([vol, vol + 1])  # channel axis 0
+    g = utils.create_pairwise_bilateral(sdims=(1, 1, 1), schan=(2.0,), img=two, chdim=0)
+    assert g.shape == (5, 120)
+    np.testing.assert_array_equal(g[3], vol.ravel() / np.float32(2))
+    np.testing.assert_array_equal(g[4], vol.ravel() + 1)
+    h = utils.create_pairwise_bilateral(sdims=(1, 1, 1), schan=(2.0, 4.0), img=np.moveaxis(two, 0, -1), chdim=3)
+    np.testing.assert_array_equal(h, np.concatenate([g[:4], (vol.ravel()[None] + 1) / np.float32(4)]))
+    with pytest.raises(IndexError):
+        utils.create_pairwise_gaussian(sdims=(1, 1, 1), shape=(3, 4))
+
+
+def test_unary_from_labels_cases():
+    """zero_unsure on and off, label maps of any shape, float32 energies of the double
+    -log values, and the errors of out-of-range or non-integer labels."""
+    lab = np.array([[0, 2], [3, 1]])
+    U = utils.unary_from_labels(lab, 3, 0.7)
+    assert U.dtype == np.float32 and U.shape == (3, 4)
+    e_pos, e_neg, e_uni = np.float32(-np.log(0.7)), np.float32(-np.log(0.15)), np.float32(-np.log(1 / 3))
+    np.testing.assert_array_equal(U[:, 0], [e_uni] * 3)
+    np.testing.assert_array_equal(U[:, 1], [e_neg, e_pos, e_neg])
+    np.testing.assert_array_equal(U[:, 2], [e_neg, e_neg, e_pos])
+    V = utils.unary_from_labels(np.array([0, 2]), 3, 0.7, zero_unsure=False)
+    np.testing.assert_array_equal(V[:, 0], [e_pos, e_neg, e_neg])
+    with pytest.raises(IndexError):
+        utils.unary_from_labels(np.array([4]), 3, 0.7)
+    with pytest.raises(IndexError):
+        utils.unary_from_labels(np.array([1.0]), 3, 0.7)
+    with pytest.raises(AssertionError):
+        utils.unary_from_labels(lab, 3, 1.0)
+
+
+def test_unary_from_softmax_precision():
+    """float32 probabilities: the log in float32 (clipped at float32(clip)); mixed with the
+    uniform share: in double, rounded once."""
+    sm = np.random.default_rng(2).dirichlet(np.ones(4), size=(3, 5)).transpose(2, 0, 1).astype(np.float32)
+    sm[0, 0, 0] = 0.0
+    u = utils.unary_from_softmax(sm)
+    assert u.shape == (4, 15) and u.dtype == np.float32
+    np.testing.assert_array_equal(u, -np.log(np.maximum(sm, np.float32(1e-5))).reshape(4, -1))
+    v = utils.unary_from_softmax(sm, scale=0.8, clip=None)
+    exp = -np.log((0.8 * sm).astype(np.float64) + 0.2 / 4)
+    np.testing.assert_array_equal(v, exp.reshape(4, -1).astype(np.float32))
+    with pytest.raises(AssertionError):
+        utils.unary_from_softmax(sm, scale=1.5)
+
+
 # ---------------------------------------------------------------- API errors
 def test_compat_wrong_dtype():
     """pydensecrf/tests/test_dcrf.py:86-99: a float64 compatibility raises ValueError."""

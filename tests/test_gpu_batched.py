@@ -219,3 +219,33 @@ def test_batched_kernel_gradients_match_single():
         np.testing.assert_array_equal(Qb[off[i]:off[i + 1]], Qs)
         dk_sum = dk_sum + dks
     np.testing.assert_allclose(dkb, dk_sum, rtol=1e-4, atol=1e-5 * np.abs(dk_sum).max())
+
+
+def test_graph_replay_follows_unary_and_weights():
+    """A captured graph replays with the model's current unary and term-weight values: a
+    new unary (uploaded before the replay) and new weights in the same buffer give the
+    bits of an eager inference; weights removed since the capture are an error."""
+    import torch
+    L, n = 21, 4
+    cases = _cases(L)
+    shapes = SHAPES[:3]
+    b = BatchedDenseCRF2D(shapes, L)
+    b.setUnaryEnergy([U for _, U, _ in cases[:3]])
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=30, srgb=13, rgbims=[img for img, _, _ in cases[:3]], compat=10)
+    b.model.set_kernel_weights(np.array([1.0, 1.0], np.float32))
+    Q = torch.zeros(b.N * L, device="cuda", dtype=torch.float32)
+    b.capture(n, Q.data_ptr())
+    rng = np.random.default_rng(5)
+    newU = [(U + rng.normal(0, 0.5, U.shape)).astype(np.float32) for _, U, _ in cases[:3]]
+    b.setUnaryEnergy(newU)
+    b.model.set_kernel_weights(np.array([0.5, 2.0], np.float32))
+    Q.fill_(-1.0)
+    b.replay()
+    torch.cuda.synchronize()
+    got = Q.view(b.N, L).cpu().numpy().T
+    ref = np.concatenate(b.inference(n), axis=1)
+    assert np.array_equal(got.view(np.int32), ref.view(np.int32))
+    b.model.set_kernel_weights(None)
+    with pytest.raises(RuntimeError, match="capture again"):
+        b.replay()

@@ -568,3 +568,33 @@ def test_logistic_unary_products(N, L, F):
     g.set_unary_params(p2)
     np.testing.assert_array_equal(g.unary_params(), p2)
     np.testing.assert_allclose(g.start(), softmax_neg(U * 0.5), atol=1e-5)
+
+
+def test_term_weight_gradient_after_layer_forward():
+    """dcrf_gradient after a CRFLayer forward with other compatibilities uses the ones
+    that forward put on the device, for the compatibility and the term-weight gradients:
+    it equals the gradient of a model created with those compatibilities."""
+    import torch
+    from pydensecrf_amd.autograd import CRFLayer
+    W, H, L, n = 30, 22, 4, 3
+    img, U, gt = image_case(W, H, L, seed=41)
+    rng = np.random.default_rng(41)
+    c_init = [(-np.eye(L) + rng.normal(0, 0.2, (L, L))).astype(np.float32) for _ in range(2)]
+    c_new = [(-2 * np.eye(L) + rng.normal(0, 0.2, (L, L))).astype(np.float32) for _ in range(2)]
+    a = capi.Model.create_2d(W, H, L)
+    a.set_unary(U)
+    a.add_gaussian(3.0, 3.0, c_init[0])
+    a.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c_init[1])
+    layer = CRFLayer(a, n, c_new, learn_kweights=False)
+    with torch.no_grad():
+        layer(torch.tensor(np.ascontiguousarray(U.T), device="cuda"))
+    torch.cuda.synchronize()
+    b = capi.Model.create_2d(W, H, L)
+    b.set_unary(U)
+    b.add_gaussian(3.0, 3.0, c_new[0])
+    b.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c_new[1])
+    oa, uga, cga, kga = a.gradient(n, gt)
+    ob, ugb, cgb, kgb = b.gradient(n, gt)
+    assert abs(oa - ob) <= 1e-6 * abs(ob)
+    assert _rel_close(cga, cgb, 1e-5)
+    assert _rel_close(kga, kgb, 1e-5)

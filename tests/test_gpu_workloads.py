@@ -1,0 +1,286 @@
+"""GPU parity at the benchmark workload sizes, through the C ABI.
+
+Workloads (pydensecrf_amd.synthetic, seeded):
+  320x213x21, 5 iterations           Gaussian sxy=3 w=3 + bilateral sxy=80 srgb=13 w=10, Potts
+  1024x768x21, 10 iterations         the same terms (the benchmark's workload)
+  64 images, sides U[256, 1024]      the same terms, one batched model, 10 iterations
+  512x512x21, 5 iterations           MatrixCompatibility per term, term weights, gradients
+  8 volumes 96x96xZ, L = 4           d = 3 and d = 4 features, DiagonalCompatibility, 10 iterations
+
+Bars: Q within 1e-5 absolute of the reference (tests/golden/*.npz, written from the
+reference's own C++) and of the oracle over every entry; MAP labels identical except at
+ties (the two largest probabilities within TIE = 2e-6, listed per test); batched
+instances bit-identical to the same image run alone; gradients within 1e-4 of the
+largest entry of the reference's DenseCRF::gradient and within 1e-3 relative of FP64
+central differences (floored per parameter group, tests/helpers.py FD_FLOOR).  Each
+test also prints the error of the GPU's Q against the oracle's FP64 re-run on the same
+lattices.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from pydensecrf_amd import capi
+from pydensecrf_amd import synthetic as syn
+from tests.helpers import fd_relative_errors
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+Q_TOL = 1e-5
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _potts_2d(model, img):
+    model.add_gaussian(3.0, 3.0, 3.0)
+    model.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+
+
+def _oracle_2d(W, H, L, U, img, c1=3.0, c2=10.0):
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(U)
+    o.add_gaussian(3.0, 3.0, c1)
+    o.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    return o
+
+
+# MAP labels may differ from the reference's only at ties: where the two largest
+# probabilities of the reference's Q (the oracle's, bit-exact) are within TIE of each
+# other, FP32 rounding decides the label (FP32 and FP64 runs of one lattice already
+# disagree at such pixels).
+TIE = 2e-6
+
+
+def _check_image_fixture(z, Q, label, map_exact=True):
+    err = float(np.abs(Q[:, z["idx"]] - z["Q_sample"]).max())
+    _say("%s: max |Q_gpu - Q_ref| on the %d-pixel sample = %.3g" % (label, z["idx"].size, err))
+    assert err <= Q_TOL
+    same = _sha(Q.argmax(0).astype(np.int16)) == str(z["map_sha"])
+    if map_exact:
+        assert same, "%s: MAP differs from the reference" % label
+    return same
+
+
+def _check_map_ties(Q, Qo, label):
+    """MAP of Q against the oracle's Q: every differing pixel is a tie (TIE)."""
+    diff = np.flatnonzero(Q.argmax(0) != Qo.argmax(0))
+    top2 = np.sort(Qo[:, diff], axis=0)[-2:] if diff.size else np.zeros((2, 0), np.float32)
+    gaps = top2[1] - top2[0]
+    _say("%s: MAP differs from the reference at %d pixels, all ties (largest top-two gap %.3g <= %g)"
+         % (label, diff.size, float(gaps.max()) if diff.size else 0.0, TIE))
+    assert np.all(gaps <= TIE), "%s: a MAP difference that is not a tie" % label
+
+
+def _fp64_report(label, Q, Q64):
+    err = float(np.abs(Q - Q64).max())
+    _say("%s: max |Q_gpu - Q_fp64| = %.3g (FP64 re-run of the same lattices)" % (label, err))
+    assert err <= Q_TOL
+    return err
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name,cfg", [("image_320x213.npz", syn.IMAGE_320x213),
+                                      ("image_1024x768.npz", syn.IMAGE_1024x768)])
+def test_image_workload(name, cfg):
+    """The single-image workloads at full size: lattice fingerprint (Gaussian V and the
+    SHA-256 of its vertex offsets), Q sample and MAP hash against the reference; all of Q
+    and MAP against the oracle; the FP64 error."""
+    z = _load(name)
+    W, H, L, n = cfg["W"], cfg["H"], cfg["L"], cfg["iters"]
+    label = "%dx%dx%d, %d iterations" % (W, H, L, n)
+    img, U, _ = syn.make_image(**cfg)
+    assert _sha(U) == str(z["unary_sha"]) and _sha(img) == str(z["image_sha"]), "generator drifted"
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(U)
+    _potts_2d(g, img)
+    lat = g.lattice(0)
+    assert lat["V"] == int(z["V_gaussian"])
+    assert _sha(lat["offset"]) == str(z["offset_gaussian_sha"]), "Gaussian lattice offsets differ"
+    Q = np.ascontiguousarray(g.inference(n))
+    _check_image_fixture(z, Q, label)
+    o = _oracle_2d(W, H, L, U, img)
+    Qo = o.inference(n)
+    err = float(np.abs(Q - Qo).max())
+    _say("%s: max |Q_gpu - Q_oracle| over all %d entries = %.3g" % (label, Q.size, err))
+    assert err <= Q_TOL
+    assert np.array_equal(Q.argmax(0), Qo.argmax(0))
+    _fp64_report(label, Q, o.inference_fp64(n))
+
+
+@pytest.mark.timeout(900)
+def test_batch_64_images():
+    """64 images of uneven size in one batched model: every instance bit-identical to the
+    image run alone (plain per-image models), the captured graph bit-identical to the
+    eager batch, three instances (smallest, median, largest) against the reference, and
+    the FP64 error of those three."""
+    import torch
+
+    from pydensecrf_amd.batched import BatchedDenseCRF2D
+    z = _load("batch_64.npz")
+    L, n = 21, int(z["iters"])
+    sizes, imgs, Us = syn.image_batch(64)
+    assert np.array_equal(np.array(sizes), z["sizes"])
+    _say("batch: %d images, %.2f Mpixel generated" % (len(sizes), sum(w * h for w, h in sizes) / 1e6))
+    b = BatchedDenseCRF2D(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=80, srgb=13, rgbims=imgs, compat=10)
+    Qb = torch.empty(b.N * L, device="cuda", dtype=torch.float32)
+    b.inference_device(n, Qb.data_ptr())
+    torch.cuda.synchronize()
+    _say("batch: batched inference done")
+    Qg = torch.full_like(Qb, -1.0)
+    b.capture(n, Qg.data_ptr())
+    b.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(Qg, Qb), "graph replay differs from the eager batch"
+    del Qg
+    for i, ((w, h), img, U) in enumerate(zip(sizes, imgs, Us)):
+        m = capi.Model.create_2d(w, h, L)
+        m.set_unary(U)
+        _potts_2d(m, img)
+        q = torch.empty(w * h * L, device="cuda", dtype=torch.float32)
+        m.inference_device(n, q.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(Qb[b.offsets[i] * L:b.offsets[i + 1] * L], q), "instance %d (%dx%d)" % (i, w, h)
+        m.close()
+    _say("batch: all 64 instances bit-identical to their plain per-image runs")
+    for j, i in enumerate(z["picks"]):
+        w, h = sizes[i]
+        Q = Qb[b.offsets[i] * L:b.offsets[i + 1] * L].view(w * h, L).cpu().numpy().T
+        zj = {k[3:]: z[k] for k in z.files if k.startswith("p%d_" % j)}
+        assert _sha(Us[i]) == str(zj["unary_sha"]) and _sha(imgs[i]) == str(zj["image_sha"])
+        label = "batch instance %d (%dx%d)" % (i, w, h)
+        _check_image_fixture(zj, Q, label)
+        o = _oracle_2d(w, h, L, Us[i], imgs[i])
+        _fp64_report(label, Q, o.inference_fp64(n))
+
+
+def _pair_sum(dc, pairs, L):
+    """dCE/dm_ab + dCE/dm_ba per unordered pair (t, a, b) (the diagonal once)."""
+    out = []
+    for t, a, b in pairs:
+        v, w = dc[t * L * L + a * L + b], dc[t * L * L + b * L + a]
+        out.append(v + w if a != b else v)
+    return np.array(out, np.float64)
+
+
+@pytest.mark.timeout(300)
+def test_gradient_512x512():
+    """The differentiable workload at 512x512x21: DenseCRF::gradient (dcrf_gradient with
+    the term weights folded into the matrices) against the reference's, and the autograd
+    layer (raw matrices, term weights) against FP64 central differences of the
+    cross-entropy on 64 unary entries, every compatibility pair and both term weights
+    (bars: tests/helpers.py FD_FLOOR)."""
+    import torch
+
+    from pydensecrf_amd.autograd import CRFLayer
+    z = _load("grad_512x512.npz")
+    W = H = 512
+    L, n = 21, int(z["iters"])
+    img, U, gt, c1, c2, kw = syn.gradient_problem_512()
+    assert _sha(U) == str(z["unary_sha"]) and _sha(img) == str(z["image_sha"])
+    C1, C2 = (kw[0] * c1).astype(np.float32), (kw[1] * c2).astype(np.float32)
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(U)
+    g.add_gaussian(3.0, 3.0, C1)
+    g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, C2)
+    obj, ug, cg, _ = g.gradient(n, gt)
+    ref_obj = float(z["ref_obj"])
+    _say("512x512: objective %.8f, reference %.8f" % (obj, ref_obj))
+    assert abs(obj - ref_obj) <= 1e-5 * abs(ref_obj)
+    scale_u = float(z["ref_unary_grad_absmax"])
+    err_u = float(np.abs(ug.ravel()[z["sel"]] - z["ref_unary_grad_sel"]).max()) / scale_u
+    assert abs(float(np.abs(ug).max()) - scale_u) <= 1e-4 * scale_u
+    err_c = float(np.abs(cg - z["ref_compat_grad"]).max()) / float(np.abs(z["ref_compat_grad"]).max())
+    _say("512x512: vs the reference, unary gradient %.3g, compatibility gradient %.3g of the largest entry"
+         % (err_u, err_c))
+    assert err_u <= 1e-4 and err_c <= 1e-4
+    Q = np.ascontiguousarray(g.inference(n))
+    o = _oracle_2d(W, H, L, U, img, C1, C2)
+    _fp64_report("512x512x21 Matrix", Q, o.inference_fp64(n))
+    g.close()
+
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, c1)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    layer = CRFLayer(m, n, [c1, c2], kweights=kw)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+    Qt = layer(Ut)
+    loss = -torch.log(Qt.gather(1, gtt[:, None])).mean()
+    loss.backward()
+    ce = float(loss)
+    _say("512x512: cross-entropy %.8f, FP64 %.8f" % (ce, float(z["ce_fp64"])))
+    assert abs(ce - float(z["ce_fp64"])) <= 1e-5 * abs(float(z["ce_fp64"]))
+    an = np.concatenate([Ut.grad.cpu().numpy().T.ravel()[z["sel"]].astype(np.float64),
+                         _pair_sum(layer.compat.grad.cpu().numpy().astype(np.float64), z["pairs"], L),
+                         layer.kweights.grad.cpu().numpy().astype(np.float64)])
+    nu, npair = z["fd_unary"].size, z["fd_pairs"].size
+    rel = fd_relative_errors(an[:nu], an[nu:nu + npair], an[nu + npair:], z)
+    _say("512x512: vs FP64 central differences, max relative error: unary %.3g, compatibility %.3g, "
+         "term weights %.3g" % tuple(r.max() for r in rel))
+    assert max(r.max() for r in rel) <= 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_volumes_8():
+    """8 volumes 96x96xZ in one batched model: every volume bit-identical to its own
+    model, two volumes against the reference (Q sample, MAP hash, and DenseCRF::gradient
+    of that volume alone), and the FP64 error of those two."""
+    from pydensecrf_amd.batched import BatchedDenseCRF
+    z = _load("volumes_8.npz")
+    L, n = 4, int(z["iters"])
+    sizes, fgs, fbs, Us, gts = syn.volume_batch(8)
+    assert np.array_equal(np.array(sizes), z["sizes"])
+    w = syn.VOLUME_COMPAT
+    b = BatchedDenseCRF(sizes, L)
+    b.setUnaryEnergy(Us)
+    b.addPairwiseEnergy(fgs, compat=w)
+    b.addPairwiseEnergy(fbs, compat=w)
+    Qb = b.inference(n)
+    _say("volumes: batched inference over %d voxels done" % sum(sizes))
+    for i in range(len(sizes)):
+        m = capi.Model.create(sizes[i], L)
+        m.set_unary(Us[i])
+        m.add_pairwise(fgs[i], w)
+        m.add_pairwise(fbs[i], w)
+        Qs = np.ascontiguousarray(m.inference(n))
+        assert np.array_equal(Qb[i].view(np.int32), Qs.view(np.int32)), "volume %d" % i
+        j = [k for k, p in enumerate(z["picks"]) if p == i]
+        if j:
+            zj = {k[3:]: z[k] for k in z.files if k.startswith("p%d_" % j[0])}
+            assert _sha(Us[i]) == str(zj["unary_sha"])
+            label = "volume %d (%d voxels)" % (i, sizes[i])
+            _check_image_fixture(zj, Qs, label, map_exact=False)
+            obj, ug, cg, _ = m.gradient(n, gts[i])
+            scale = float(zj["unary_grad_absmax"])
+            err_u = float(np.abs(ug[:, zj["idx"]] - zj["unary_grad_sample"]).max()) / scale
+            err_c = float(np.abs(cg - zj["compat_grad"]).max()) / float(np.abs(zj["compat_grad"]).max())
+            _say("%s: objective %.8f (reference %.8f); gradient vs the reference: unary %.3g, compatibility %.3g "
+                 "of the largest entry" % (label, obj, float(zj["obj"]), err_u, err_c))
+            assert abs(obj - float(zj["obj"])) <= 1e-5 * abs(float(zj["obj"]))
+            assert err_u <= 1e-4 and err_c <= 1e-4
+            o = po.OracleCRF(N=sizes[i], L=L)
+            o.set_unary(Us[i])
+            o.add_pairwise(fgs[i], w)
+            o.add_pairwise(fbs[i], w)
+            Qo = o.inference(n)  # the reference's Q bit for bit (tests/test_oracle_reference.py)
+            assert float(np.abs(Qs - Qo).max()) <= Q_TOL
+            _check_map_ties(Qs, Qo, label)
+            _fp64_report(label, Qs, o.inference_fp64(n))
+        m.close()
+    _say("volumes: all 8 bit-identical to their own models")

@@ -1,10 +1,78 @@
-"""The learning driver (pydensecrf_amd.learning) on a known energy, without a GPU:
-minimize_lbfgs finds the minimum of a convex quadratic and stops its restarts once the
-energy no longer falls; numeric_gradient, gradient, grad_check and compute_function
-follow optimization.cpp:104-134 on a closed-form gradient."""
+"""The learning driver (pydensecrf_amd.learning) without a GPU.
+
+* lbfgs / minimize_lbfgs against the reference's minimizeLBFGS (liblbfgs 1.10, built
+  from the reference sources): tests/golden/lbfgs_small.npz holds every point the
+  reference evaluated and its result, on Rosenbrock and on a CRF learning energy whose
+  values the oracle reproduces bit for bit; the restatement must evaluate the same
+  points and return the same result.
+* minimize_lbfgs finds the minimum of a convex quadratic and stops its restarts once
+  the energy no longer falls; numeric_gradient, gradient, grad_check and
+  compute_function follow optimization.cpp:104-134 on a closed-form gradient.
+"""
+import os
+
 import numpy as np
+import pytest
 
 from pydensecrf_amd import learning
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _traced(fun, trace):
+    def f(x):
+        trace.append(x.copy())
+        return fun(x)
+    return f
+
+
+@pytest.mark.parametrize("i", [0, 1])
+def test_lbfgs_rosenbrock_matches_reference(i):
+    from tests.helpers import rosenbrock
+    z = np.load(os.path.join(GOLDEN, "lbfgs_small.npz"), allow_pickle=False)
+    tr = []
+    x, _, status = learning.lbfgs(_traced(rosenbrock, tr), z["r%d_x0" % i])
+    ref = z["r%d_trace" % i]
+    assert len(tr) == len(ref)
+    assert np.array_equal(np.array(tr), ref)
+    assert np.array_equal(x.astype(np.float32), z["r%d_x" % i])
+    assert status in (learning.LBFGS_SUCCESS, learning.LBFGSERR_MAXIMUMITERATION)
+
+
+def test_minimize_lbfgs_crf_energy_matches_reference():
+    """minimizeLBFGS with one restart on -LogLikelihood + l2 of a small CRF (Diagonal
+    compatibilities, DIAG kernel parameters), the energy evaluated by the oracle (whose
+    DenseCRF::gradient is the reference's, bit for bit, for these terms)."""
+    from oracle import pyoracle as po
+    from tests.helpers import crf_learning_energy, crf_learning_model
+    z = np.load(os.path.join(GOLDEN, "lbfgs_small.npz"), allow_pickle=False)
+    crf, gt = crf_learning_model(po.OracleCRF)
+    x0 = np.concatenate([crf.compat_params(), crf.kernel_params()]).astype(np.float32)
+    assert np.array_equal(x0, z["crf_x0"])
+    fun = crf_learning_energy(crf, gt)
+
+    class Energy:
+        def __init__(self):
+            self.trace, self.values, self.last = [], [], None
+
+        def initial_value(self):
+            return x0
+
+        def gradient(self, x):
+            self.trace.append(np.asarray(x, np.float32).copy())
+            f, g = fun(np.asarray(x, np.float32))
+            self.values.append(f)
+            return f, g
+
+        def set(self, x):
+            self.last = x
+
+    e = Energy()
+    x = learning.minimize_lbfgs(e, restart=1)
+    assert len(e.trace) == len(z["crf_trace"])
+    assert np.array_equal(np.array(e.trace), z["crf_trace"])
+    np.testing.assert_array_equal(np.array(e.values), z["crf_values"])
+    assert np.array_equal(x, z["crf_x"]) and np.array_equal(e.last, x)
 
 
 class Quadratic:

@@ -318,3 +318,42 @@ def test_debug_energies_fixture():
         assert np.array_equal(bits(e), bits(z["e_pair%d" % k]))
         total = (total + e).astype(np.float32)
     assert np.array_equal(bits(total), bits(z["e_pair"]))
+
+
+def test_gradient_512x512_fixture_reference_vs_fd():
+    """The reference's DenseCRF::gradient at 512x512x21 (Matrix compatibilities, term
+    weights folded in) against the oracle's FP64 central differences in the same fixture,
+    under the bars the GPU is held to: LogLikelihood = -CE, so dCE/dU = -g_U, a pair
+    (a, b) of term k moves W_ab = w_k (m_ab + m_ba) / 2 by w_k h, dCE = -w_k g_ab."""
+    from pydensecrf_amd import synthetic as syn
+    from tests.helpers import fd_relative_errors
+    z = _load("grad_512x512.npz")
+    _, _, _, c1, c2, kw = syn.gradient_problem_512()
+    L = 21
+    assert abs(-float(z["ref_obj"]) - float(z["ce_fp64"])) <= 1e-6 * float(z["ce_fp64"])
+    npk = L * (L + 1) // 2
+    cg = z["ref_compat_grad"].astype(np.float64)
+    g_pairs = np.concatenate([-kw[0] * cg[:npk], -kw[1] * cg[npk:]])
+    # term weight k: W_k = w_k (m_k + m_k^T) / 2, so dCE/dw_k = -sum_pairs g_ab (m_ab + m_ba) / 2
+    iu = np.triu_indices(L)
+    g_kw = np.array([-(cg[k * npk:(k + 1) * npk] * (0.5 * (c + c.T))[iu].astype(np.float64)).sum()
+                     for k, c in enumerate((c1, c2))])
+    rel = fd_relative_errors(-z["ref_unary_grad_sel"].astype(np.float64), g_pairs, g_kw, z)
+    assert max(r.max() for r in rel) <= 1e-3, [r.max() for r in rel]
+
+
+def test_volume_fixture_oracle_bitexact():
+    """The oracle reproduces the reference on the volume workload (d = 3 and d = 4
+    features, Diagonal compatibilities, 10 iterations): the Q sample bit for bit and
+    the MAP hash of the whole volume, for the first volume (the smaller pick)."""
+    from pydensecrf_amd import synthetic as syn
+    z = _load("volumes_8.npz")
+    sizes, fgs, fbs, Us, _ = syn.volume_batch(8)
+    i = int(z["picks"][0])
+    o = po.OracleCRF(N=sizes[i], L=4)
+    o.set_unary(Us[i])
+    o.add_pairwise(fgs[i], syn.VOLUME_COMPAT)
+    o.add_pairwise(fbs[i], syn.VOLUME_COMPAT)
+    Q = o.inference(int(z["iters"]))
+    assert np.array_equal(bits(Q[:, z["p0_idx"]]), bits(z["p0_Q_sample"]))
+    assert sha(Q.argmax(0).astype(np.int16)) == str(z["p0_map_sha"])
