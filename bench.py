@@ -15,7 +15,8 @@ Also reported on the same line:
              bytes per launch from the committed rocprofv3 counter summary
   kernels    the same figures for every kernel class (splat, combine, blur, meanfield)
   hipgraph   the same steps replayed as one captured hipGraph
-  batched    64 images of uneven size in one batched model vs 64 plain per-image calls
+  batched    64 images of uneven size in one batched model vs 64 plain per-image calls,
+             and the reference CPU path on the same images with every host core
   fwd_bwd_1024x768  the inference workload as a torch.autograd layer (Potts compatibilities
              and term weights learnable, cross-entropy): forward and forward+backward
   fwd_bwd    512x512x21, 5 iterations, learnable Matrix compatibility: forward and
@@ -93,6 +94,67 @@ def cpu_baseline(img, U, W, H, L, iters):
         "sample": "full %dx%dx%d image, %d-iteration inference run %d times (%.1f s); lattice build %.2f s "
                   "excluded" % (W, H, L, iters, runs, spent, t1 - t0),
     }
+
+
+def _cpu_batch_worker(args):
+    """One worker of the all-core CPU baseline: builds the reference models of its images
+    (untimed), waits at the barrier, runs their inferences, waits again."""
+    idx, iters, barrier = args
+    from oracle import pyoracle as po
+    from pydensecrf_amd import synthetic as syn
+    sizes = syn.batch_image_sizes(64)
+    models = []
+    for i in idx:
+        w, h = sizes[i]
+        img, U, _ = syn.make_image(w, h, L=21, sites=12, seed=3000 + i)
+        crf = po.RefCRF(W=w, H=h, L=21) if po.ref_available() else po.OracleCRF(W=w, H=h, L=21)
+        crf.set_unary(U)
+        crf.add_gaussian(3.0, 3.0, 3.0)
+        crf.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+        models.append(crf)
+    barrier.wait()
+    t0 = time.perf_counter()
+    for crf in models:
+        crf.inference(iters)
+    spent = time.perf_counter() - t0
+    barrier.wait()
+    return spent
+
+
+def cpu_baseline_batched(syn, n_images, iters, log):
+    """The reference CPU path on the uneven image batch with every host core (up to 16, the
+    box's share), one process per core, each running the inferences of its images in turn
+    (images dealt longest-first to the least-loaded process); lattice builds excluded.
+    images/s = images / wall time between two barriers around the inferences."""
+    import multiprocessing as mp
+    from oracle import pyoracle as po
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    sizes = syn.batch_image_sizes(n_images)
+    order = sorted(range(n_images), key=lambda i: -sizes[i][0] * sizes[i][1])
+    groups, load = [[] for _ in range(cores)], [0] * cores
+    for i in order:
+        j = load.index(min(load))
+        groups[j].append(i)
+        load[j] += sizes[i][0] * sizes[i][1]
+    ctx = mp.get_context("fork")
+    with ctx.Manager() as mgr:
+        barrier = mgr.Barrier(cores + 1)
+        with ctx.Pool(cores) as pool:
+            res = pool.map_async(_cpu_batch_worker, [(g, iters, barrier) for g in groups], chunksize=1)
+            barrier.wait()
+            t0 = time.perf_counter()
+            barrier.wait()
+            wall = time.perf_counter() - t0
+            spent = res.get()
+    log("cpu baseline (batch): %d images on %d processes, %.2f s" % (n_images, cores, wall))
+    return {"value": round(n_images / wall, 4), "unit": "images/s", "cores": cores,
+            "kind": "reference" if po.ref_available() else "port",
+            "sample": "all %d images, one process per core (%d), longest-first assignment; per-process inference "
+                      "time %.2f-%.2f s; lattice builds excluded" % (n_images, cores, min(spent), max(spent))}
 
 
 def bench_batched(torch, capi, syn, n_images, iters, reps, log):
@@ -491,6 +553,8 @@ def main():
                            "note": "dcrf_inference into a host (L, N) buffer: device work plus the Q download"}
     if not args.no_batched:
         out["batched"] = bench_batched(torch, capi, syn, 64, iters, 3, log)
+        if not args.no_cpu_baseline:
+            out["batched"]["cpu_baseline"] = cpu_baseline_batched(syn, 64, iters, log)
     if not args.no_fwd_bwd:
         out["fwd_bwd_1024x768"] = bench_fwd_bwd_image(torch, capi, syn, iters, 5, log)
         out["fwd_bwd"] = bench_fwd_bwd(torch, capi, syn, 5, 5, log)

@@ -36,6 +36,14 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Position of pixel p (row-major, image W pixels wide) when the image is stored in
+// 8 x 8 pixel blocks, blocks row-major, pixels row-major inside a block.
+__device__ __forceinline__ int q_block_pos(int p, int W) {
+    const int y = p / W, x = p - y * W;
+    const int nbx = (W + 7) >> 3;
+    return (((y >> 3) * nbx + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
+}
+
 // Last i in [0, n) with base[i] <= q (base ascending, base[0] <= q).
 __device__ __forceinline__ int last_le(const int* __restrict__ base, int n, int q) {
     int lo = 0, hi = n - 1;

@@ -333,8 +333,13 @@ void build_run_slice(const Geometry& g, const GeometryDev& gd, const int2* slc, 
 size_t meanfield_runs_lds(int L, int ds, int P, int rows_lds, int cwf);
 void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTable& st, const RunSliceDev& rs,
                            hipStream_t s);
+// bw > 0: Q is written in 8 x 8 pixel blocks of an image bw pixels wide (q_block_pos,
+// rows of ldq floats), for the splat over block-position entries (launch_entry_block).
 void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
-                            const int2* slc, hipStream_t s);
+                            const int2* slc, hipStream_t s, int bw = 0);
+void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s);
+// rows of a Q held in 8 x 8 pixel blocks for an image of W x H pixels
+inline size_t q_block_rows(int W, int H) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64; }
 void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,
                              hipStream_t s);
 // out = compat_k(slice_k) for term index `which` of st.

@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu
 k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
                                                                 int ldq, int N, int L, int S, int per,
                                                                 SliceTable st, const int2* __restrict__ slc,
-                                                                int cwf) {
+                                                                int cwf, int bw) {
     constexpr int DS = D0 + D1;
     LaneGroup<true, R> G(L, S, per, nullptr);
     const int p = G.item();
@@ -640,7 +640,9 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
     }
     // CWL: the wave's message rows follow the compat region
     float* msg = shc + cwf + (size_t)(threadIdx.x >> 6) * (D1 ? 2 : 1) * per * ((L + 3) & ~3);
-    finish_pixel<R, D0, D1, CWL>(G, st, e, x, u, active, Q + (size_t)p * ldq, shc, p, msg);
+    // bw > 0: Q is written in 8 x 8 pixel blocks of an image bw pixels wide (q_block_pos)
+    const int qpos = bw > 0 ? q_block_pos(pc, bw) : p;
+    finish_pixel<R, D0, D1, CWL>(G, st, e, x, u, active, Q + (size_t)qpos * ldq, shc, p, msg);
 }
 
 // The adjoint step of k_adjoint (adjoint.hip) over the same interleaved slice pairs:
@@ -1114,11 +1116,11 @@ inline size_t fused_lds_bytes(const LaunchShape& sh, int ds, int nt, int L, int 
 #define DCRF_FUSED(R_, D0_, D1_)                                                                          \
     (cwf ? k_meanfield_fused<R_, D0_, D1_, true> : k_meanfield_fused<R_, D0_, D1_, false>)<<<                 \
         blocks_for(N, sh.items), kWaveBlock, fused_lds_bytes(sh, D0_ + D1_, D1_ ? 2 : 1, L, cwf), s>>>(       \
-        U, Q, ldq, N, L, sh.S, sh.per, st, slc, cwf)
+        U, Q, ldq, N, L, sh.S, sh.per, st, slc, cwf, bw)
 
 template <int R>
 void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
-                    const int2* slc, int cwf, hipStream_t s) {
+                    const int2* slc, int cwf, int bw, hipStream_t s) {
     const int a = st.t[0].d1, b = st.nterms > 1 ? st.t[1].d1 : 0;
     if (a == 3 && b == 0) DCRF_FUSED(R, 3, 0);
     else if (a == 6 && b == 0) DCRF_FUSED(R, 6, 0);
@@ -1201,18 +1203,35 @@ void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTa
 }
 
 void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st_in,
-                            const int2* slc, hipStream_t s) {
+                            const int2* slc, hipStream_t s, int bw) {
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L, true);  // 16-byte rows for matrix_piece_rows
     const LaunchShape sh = shape_for(L);
     if (!fused_meanfield_ok(L, st.nterms, st.t[0].d1, st.nterms > 1 ? st.t[1].d1 : 0))
         throw Error{DCRF_ERR_VALUE, "no fused mean-field kernel for these terms"};
     switch (sh.R) {
-        case 1: launch_fused_r<1>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
-        case 2: launch_fused_r<2>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
-        case 3: launch_fused_r<3>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
-        default: launch_fused_r<4>(sh, U, Q, ldq, N, L, st, slc, cwf, s); break;
+        case 1: launch_fused_r<1>(sh, U, Q, ldq, N, L, st, slc, cwf, bw, s); break;
+        case 2: launch_fused_r<2>(sh, U, Q, ldq, N, L, st, slc, cwf, bw, s); break;
+        case 3: launch_fused_r<3>(sh, U, Q, ldq, N, L, st, slc, cwf, bw, s); break;
+        default: launch_fused_r<4>(sh, U, Q, ldq, N, L, st, slc, cwf, bw, s); break;
     }
+}
+
+namespace {
+// Entry (pixel, weight) -> (its 8 x 8 block position, weight) for an image W pixels wide.
+__global__ void k_entry_block(const int2* __restrict__ ent, size_t E, int W, int2* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < E) {
+        const int2 e = ent[i];
+        out[i] = make_int2(q_block_pos(e.x, W), e.y);
+    }
+}
+}  // namespace
+
+void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s) {
+    if (!E) return;
+    k_entry_block<<<blocks_for(E), kThreads, 0, s>>>(ent, E, W, out);
+    DCRF_HIP_CHECK(hipGetLastError());
 }
 
 void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st_in,

@@ -77,6 +77,18 @@ def main(argv):
             valu = cs.get("SQ_INSTS_VALU", 0) / cs["SQ_WAVES"] if cs.get("SQ_WAVES") else float("nan")
             print("%-60s %14.1f %14.1f %14.1f" % (_short(k), cs.get("FETCH_SIZE", float("nan")),
                                                   cs.get("WRITE_SIZE", float("nan")), valu))
+        # every other counter of the passes, per dispatch, for each mean-field kernel class
+        # (the kernel of the class with the most dispatches)
+        shown = {"FETCH_SIZE", "WRITE_SIZE", "SQ_WAVES", "SQ_INSTS_VALU"}
+        others = sorted({c for cs in merged.values() for c in cs} - shown)
+        if others:
+            print("%-40s %s" % ("counter (avg per dispatch)", " ".join("%16s" % c for c in CLASSES)))
+            rep = {}
+            for cls, prefix in CLASSES.items():
+                names = [k for k in merged if _short(k).startswith(prefix)]
+                rep[cls] = merged[max(names, key=lambda n: calls[n])] if names else {}
+            for c in others:
+                print("%-40s %s" % (c, " ".join("%16.4g" % rep[cls].get(c, float("nan")) for cls in CLASSES)))
         if copy_kib and copy_max.get("FETCH_SIZE") and copy_max.get("WRITE_SIZE"):
             fs, ws = copy_kib / copy_max["FETCH_SIZE"], copy_kib / copy_max["WRITE_SIZE"]
             print("copy calibration: %.0f KiB copy reads FETCH_SIZE %.1f, WRITE_SIZE %.1f -> scales %.3f, %.3f"

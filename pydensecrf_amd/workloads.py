@@ -9,6 +9,11 @@
         R evaluations of U = L f on the matrix cores, with the parameters changed
         between evaluations so that each one recomputes U.
 
+    python -m pydensecrf_amd.workloads fwd-bwd [--steps S] [--size 512|1024]
+        The differentiable workloads through the torch.autograd layer: 512 = 512x512, 21
+        labels, 5 iterations, Matrix compatibilities and term weights learnable; 1024 =
+        1024x768, 21 labels, 10 iterations, Potts compatibilities and term weights
+        learnable.  S forward+backward passes (cross-entropy of the true labels).
     python -m pydensecrf_amd.workloads kernel-rebuild [--reps R]
         DenseCRF2D 512x512, 21 labels, Gaussian (d = 2) + bilateral (d = 5) with DIAG
         kernel parameters: R changes of the parameter vector, each rebuilding both
@@ -55,6 +60,43 @@ def kernel_gradient(steps):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     print("kernel-gradient 512x512x21, 5 iterations: %.3f ms per forward+backward" % ms, flush=True)
+
+
+def fwd_bwd(steps, size):
+    import torch
+
+    from . import capi, synthetic as syn
+    from .autograd import CRFLayer
+
+    if size == 512:
+        cfg = syn.IMAGE_512x512
+        img, U, gt, c1, c2, kw = syn.gradient_problem_512()
+        compat, n = [c1, c2], cfg["iters"]
+    else:
+        cfg = syn.IMAGE_1024x768
+        img, U, gt = syn.make_image(**cfg)
+        compat, kw, n = [3.0, 10.0], None, cfg["iters"]
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, compat[0])
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, compat[1])
+    layer = CRFLayer(m, n, compat, kweights=kw)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+
+    def step():
+        Q = layer(Ut)
+        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print("fwd-bwd %dx%dx%d, %d iterations: %.3f ms per forward+backward" % (W, H, L, n, ms), flush=True)
 
 
 def logistic_unary(reps, F):
@@ -124,13 +166,16 @@ def kernel_rebuild(reps):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary", "kernel-rebuild"])
+    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary", "kernel-rebuild", "fwd-bwd"])
+    ap.add_argument("--size", type=int, default=512, choices=[512, 1024])
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--features", type=int, default=64)
     a = ap.parse_args()
     if a.workload == "kernel-gradient":
         kernel_gradient(a.steps)
+    elif a.workload == "fwd-bwd":
+        fwd_bwd(a.steps, a.size)
     elif a.workload == "kernel-rebuild":
         kernel_rebuild(a.reps)
     else:

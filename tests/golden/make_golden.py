@@ -306,7 +306,8 @@ def volumes_8():
     per term, 10 iterations): for the first and the largest volume, the reference's Q on
     a 4096-voxel sample and its MAP hash, and DenseCRF::gradient (LogLikelihood of the
     true labels, identity-gradient unary) on that volume alone: objective, unary gradient
-    on the sample, its largest magnitude over all entries, compatibility gradient."""
+    on the sample, its largest magnitude over all entries, compatibility gradient; and
+    FP64 central differences of the cross-entropy over the compatibility entries."""
     sizes, fgs, fbs, Us, gts = syn.volume_batch(8)
     picks = [0, int(np.argmax(sizes))]
     out = {"sizes": np.array(sizes, np.int64), "picks": np.array(picks, np.int32), "iters": np.int32(10)}
@@ -317,11 +318,25 @@ def volumes_8():
         crf.add_pairwise(fbs[i], syn.VOLUME_COMPAT)
         Q = crf.inference(10)
         r, ug, cg = crf.gradient(10, gts[i])
+        # FP64 central differences (h = 1e-3) of the cross-entropy over the 8 Diagonal
+        # compatibility entries, from the oracle's FP64 twin on the same lattices
+        o = po.OracleCRF(N=sizes[i], L=4)
+        o.set_unary(Us[i])
+        o.add_pairwise(fgs[i], syn.VOLUME_COMPAT)
+        o.add_pairwise(fbs[i], syn.VOLUME_COMPAT)
+        raw = np.concatenate([syn.VOLUME_COMPAT, syn.VOLUME_COMPAT]).astype(np.float64)
+        U64, fd = Us[i].astype(np.float64), []
+        for k in range(raw.size):
+            a, b = raw.copy(), raw.copy()
+            a[k] += 1e-3
+            b[k] -= 1e-3
+            fd.append((o.ce_fp64(10, gts[i], U64, a, None) - o.ce_fp64(10, gts[i], U64, b, None)) / 2e-3)
         idx = np.sort(np.random.default_rng(21 + j).choice(sizes[i], 4096, replace=False))
         p = "p%d_" % j
         out.update({p + "idx": idx, p + "Q_sample": Q[:, idx], p + "map_sha": sha(Q.argmax(0).astype(np.int16)),
                     p + "unary_sha": sha(Us[i]), p + "obj": np.float64(r), p + "unary_grad_sample": ug[:, idx],
-                    p + "unary_grad_absmax": np.float32(np.abs(ug).max()), p + "compat_grad": cg})
+                    p + "unary_grad_absmax": np.float32(np.abs(ug).max()), p + "compat_grad": cg,
+                    p + "compat_fd": np.array(fd)})
     np.savez_compressed(os.path.join(HERE, "volumes_8.npz"), **out)
 
 
@@ -411,9 +426,10 @@ def grad_512x512(jobs=None):
 def lbfgs_small():
     """minimizeLBFGS (optimization.cpp:68-103 over liblbfgs 1.10) from the reference
     build: every point it evaluates and its result, on the extended Rosenbrock function
-    from two starts, and on a CRF learning energy (tests/helpers.py crf_learning_energy:
-    -LogLikelihood + l2 / 2 ||x||^2 over Diagonal compatibilities and DIAG kernel
-    parameters, evaluated with the reference's DenseCRF::gradient) with one restart."""
+    from two starts, and on CRF learning energies (tests/helpers.py crf_learning_energy:
+    -LogLikelihood + l2 / 2 ||x||^2 evaluated with the reference's DenseCRF::gradient)
+    with one restart: "crf" over Diagonal compatibilities and DIAG kernel parameters,
+    "crfc" over the compatibilities alone (an energy smooth in its parameters)."""
     from tests.helpers import crf_learning_energy, crf_learning_model, rosenbrock
     out = {}
 
@@ -428,19 +444,20 @@ def lbfgs_small():
         tr = []
         out["r%d_x" % i] = po.ref_minimize_lbfgs(x0, traced(rosenbrock, tr), 0)
         out["r%d_x0" % i], out["r%d_trace" % i] = x0, np.array(tr)
-    crf, gt = crf_learning_model(po.RefCRF, identity_unary=True)
-    x0 = np.concatenate([crf.compat_params(), crf.kernel_params()]).astype(np.float32)
-    tr, vals = [], []
-    fun = crf_learning_energy(crf, gt)
+    for name, kernel in (("crf", True), ("crfc", False)):
+        crf, gt = crf_learning_model(po.RefCRF, identity_unary=True)
+        x0 = np.concatenate([crf.compat_params()] + ([crf.kernel_params()] if kernel else [])).astype(np.float32)
+        tr, vals = [], []
+        fun = crf_learning_energy(crf, gt, kernel=kernel)
 
-    def rec(x):
-        tr.append(x.copy())
-        f, g = fun(x)
-        vals.append(f)
-        return f, g
+        def rec(x, fun=fun, tr=tr, vals=vals):
+            tr.append(x.copy())
+            f, g = fun(x)
+            vals.append(f)
+            return f, g
 
-    out["crf_x"] = po.ref_minimize_lbfgs(x0, rec, 1)
-    out["crf_x0"], out["crf_trace"], out["crf_values"] = x0, np.array(tr), np.array(vals)
+        out[name + "_x"] = po.ref_minimize_lbfgs(x0, rec, 1)
+        out[name + "_x0"], out[name + "_trace"], out[name + "_values"] = x0, np.array(tr), np.array(vals)
     np.savez_compressed(os.path.join(HERE, "lbfgs_small.npz"), **out)
 
 

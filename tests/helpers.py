@@ -90,17 +90,18 @@ def crf_learning_problem():
     return img, U, gt
 
 
-def crf_learning_energy(crf, gt, n_iter=3, l2=1e-2):
-    """E(x) = -LogLikelihood(Q_n) + l2 / 2 ||x||^2 over x = (compatibility parameters,
-    kernel parameters) of an oracle or reference CRF (set, then DenseCRF::gradient)."""
+def crf_learning_energy(crf, gt, n_iter=3, l2=1e-2, kernel=True):
+    """E(x) = -LogLikelihood(Q_n) + l2 / 2 ||x||^2 over x = (compatibility parameters[,
+    kernel parameters]) of an oracle or reference CRF (set, then DenseCRF::gradient)."""
     nc = crf.compat_params().size
 
     def fun(x):
         crf.set_compat_params(x[:nc])
-        crf.set_kernel_params(x[nc:])
-        r, _, cg, kg = crf.gradient_obj(n_iter, gt, 0, 0.0, None, kernel=True)
+        if kernel:
+            crf.set_kernel_params(x[nc:])
+        r, _, cg, kg = crf.gradient_obj(n_iter, gt, 0, 0.0, None, kernel=kernel)
         x64 = x.astype(np.float64)
-        g = -np.concatenate([cg, kg]).astype(np.float64) + l2 * x64
+        g = -np.concatenate([cg, kg] if kernel else [cg]).astype(np.float64) + l2 * x64
         return -r + 0.5 * l2 * float(x64 @ x64), g.astype(np.float32)
     return fun
 

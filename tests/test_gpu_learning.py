@@ -52,11 +52,13 @@ def test_minimize_lbfgs_lowers_energy(objective, param):
 
 
 def test_minimize_lbfgs_reaches_reference_optimum():
-    """The learning problem of tests/golden/lbfgs_small.npz (the reference's minimizeLBFGS
-    with one restart on -LogLikelihood + l2 over Diagonal compatibilities and DIAG kernel
-    parameters) run with the GPU's energy: the GPU's values differ from the reference's
-    in the last float bits, so the iterates drift apart; the optimum reached agrees to
-    1e-3 of the parameters' scale and its energy to 1e-5."""
+    """The compatibility-learning problem of tests/golden/lbfgs_small.npz ("crfc": the
+    reference's minimizeLBFGS with one restart on -LogLikelihood + l2 over Diagonal
+    compatibilities, an energy smooth in its parameters) run with the GPU's energy.  The
+    GPU's values differ from the reference's in the last float bits, so the iterates
+    differ a little; the optimum reached agrees to 1e-4 of the parameters' scale and its
+    energy to 1e-6.  (With the bilateral term's kernel parameters the energy is
+    discontinuous, d = 5, and runs that differ in rounding end at different points.)"""
     import os
 
     from tests.helpers import crf_learning_problem
@@ -67,14 +69,14 @@ def test_minimize_lbfgs_reaches_reference_optimum():
     g.set_unary(U)
     g.add_gaussian(3.0, 2.0, np.array([1.0, 2.0, 1.5], np.float32))
     g.add_bilateral(20.0, 30.0, 13.0, 17.0, 11.0, img, np.array([3.0, 4.0, 5.0], np.float32))
-    e = learning.CRFEnergy(g, gt, 3, pairwise=True, kernel=True, l2=1e-2)
-    assert np.array_equal(e.initial_value().astype(np.float32), z["crf_x0"])
+    e = learning.CRFEnergy(g, gt, 3, pairwise=True, kernel=False, l2=1e-2)
+    assert np.array_equal(e.initial_value().astype(np.float32), z["crfc_x0"])
     x = learning.minimize_lbfgs(e, restart=1)
-    ref = z["crf_x"]
+    ref = z["crfc_x"]
     err = float(np.abs(x - ref).max()) / float(np.abs(ref).max())
     f_gpu = e.gradient(x)[0]
-    f_ref = float(z["crf_values"].min())
-    print("GPU L-BFGS: %d evaluations (reference %d), max |x - x_ref| / max |x_ref| = %.3g, E %.8f (reference "
-          "best %.8f)" % (e.evaluations, len(z["crf_trace"]), err, f_gpu, f_ref), flush=True)
-    assert err <= 1e-3
-    assert abs(f_gpu - f_ref) <= 1e-5 * abs(f_ref)
+    f_ref = float(z["crfc_values"].min())
+    print("GPU L-BFGS: %d evaluations (reference %d), max |x - x_ref| / max |x_ref| = %.3g, E %.9f (reference "
+          "best %.9f)" % (e.evaluations, len(z["crfc_trace"]), err, f_gpu, f_ref), flush=True)
+    assert err <= 1e-4
+    assert abs(f_gpu - f_ref) <= 1e-6 * abs(f_ref)

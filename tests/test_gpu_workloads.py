@@ -270,10 +270,16 @@ def test_volumes_8():
             scale = float(zj["unary_grad_absmax"])
             err_u = float(np.abs(ug[:, zj["idx"]] - zj["unary_grad_sample"]).max()) / scale
             err_c = float(np.abs(cg - zj["compat_grad"]).max()) / float(np.abs(zj["compat_grad"]).max())
+            # the compatibility gradient sums ~5e5 voxels x 10 iterations: the reference
+            # sums in float (1.6e-4 of its largest entry from the FP64 differences), the
+            # GPU in double; the FP64 central differences decide (dCE = -dLogLikelihood)
+            fd = zj["compat_fd"]
+            err_fd = float(np.abs(-cg.astype(np.float64) - fd).max()) / float(np.abs(fd).max())
             _say("%s: objective %.8f (reference %.8f); gradient vs the reference: unary %.3g, compatibility %.3g "
-                 "of the largest entry" % (label, obj, float(zj["obj"]), err_u, err_c))
+                 "of the largest entry; compatibility vs FP64 central differences %.3g of the largest"
+                 % (label, obj, float(zj["obj"]), err_u, err_c, err_fd))
             assert abs(obj - float(zj["obj"])) <= 1e-5 * abs(float(zj["obj"]))
-            assert err_u <= 1e-4 and err_c <= 1e-4
+            assert err_u <= 1e-4 and err_c <= 3e-4 and err_fd <= 1e-4
             o = po.OracleCRF(N=sizes[i], L=L)
             o.set_unary(Us[i])
             o.add_pairwise(fgs[i], w)

@@ -39,17 +39,19 @@ def test_lbfgs_rosenbrock_matches_reference(i):
     assert status in (learning.LBFGS_SUCCESS, learning.LBFGSERR_MAXIMUMITERATION)
 
 
-def test_minimize_lbfgs_crf_energy_matches_reference():
+@pytest.mark.parametrize("name,kernel", [("crf", True), ("crfc", False)])
+def test_minimize_lbfgs_crf_energy_matches_reference(name, kernel):
     """minimizeLBFGS with one restart on -LogLikelihood + l2 of a small CRF (Diagonal
-    compatibilities, DIAG kernel parameters), the energy evaluated by the oracle (whose
-    DenseCRF::gradient is the reference's, bit for bit, for these terms)."""
+    compatibilities, with or without the DIAG kernel parameters), the energy evaluated by
+    the oracle (whose DenseCRF::gradient is the reference's, bit for bit, for these
+    terms)."""
     from oracle import pyoracle as po
     from tests.helpers import crf_learning_energy, crf_learning_model
     z = np.load(os.path.join(GOLDEN, "lbfgs_small.npz"), allow_pickle=False)
     crf, gt = crf_learning_model(po.OracleCRF)
-    x0 = np.concatenate([crf.compat_params(), crf.kernel_params()]).astype(np.float32)
-    assert np.array_equal(x0, z["crf_x0"])
-    fun = crf_learning_energy(crf, gt)
+    x0 = np.concatenate([crf.compat_params()] + ([crf.kernel_params()] if kernel else [])).astype(np.float32)
+    assert np.array_equal(x0, z[name + "_x0"])
+    fun = crf_learning_energy(crf, gt, kernel=kernel)
 
     class Energy:
         def __init__(self):
@@ -69,10 +71,10 @@ def test_minimize_lbfgs_crf_energy_matches_reference():
 
     e = Energy()
     x = learning.minimize_lbfgs(e, restart=1)
-    assert len(e.trace) == len(z["crf_trace"])
-    assert np.array_equal(np.array(e.trace), z["crf_trace"])
-    np.testing.assert_array_equal(np.array(e.values), z["crf_values"])
-    assert np.array_equal(x, z["crf_x"]) and np.array_equal(e.last, x)
+    assert len(e.trace) == len(z[name + "_trace"])
+    assert np.array_equal(np.array(e.trace), z[name + "_trace"])
+    np.testing.assert_array_equal(np.array(e.values), z[name + "_values"])
+    assert np.array_equal(x, z[name + "_x"]) and np.array_equal(e.last, x)
 
 
 class Quadratic:

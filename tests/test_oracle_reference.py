@@ -345,7 +345,8 @@ def test_gradient_512x512_fixture_reference_vs_fd():
 def test_volume_fixture_oracle_bitexact():
     """The oracle reproduces the reference on the volume workload (d = 3 and d = 4
     features, Diagonal compatibilities, 10 iterations): the Q sample bit for bit and
-    the MAP hash of the whole volume, for the first volume (the smaller pick)."""
+    the MAP hash of the whole volume, for the first volume (the smaller pick); the
+    fixture's reference gradient against its FP64 central differences."""
     from pydensecrf_amd import synthetic as syn
     z = _load("volumes_8.npz")
     sizes, fgs, fbs, Us, _ = syn.volume_batch(8)
@@ -357,3 +358,8 @@ def test_volume_fixture_oracle_bitexact():
     Q = o.inference(int(z["iters"]))
     assert np.array_equal(bits(Q[:, z["p0_idx"]]), bits(z["p0_Q_sample"]))
     assert sha(Q.argmax(0).astype(np.int16)) == str(z["p0_map_sha"])
+    # the reference's compatibility gradient (float sums over ~5e5 voxels) against the
+    # FP64 central differences in the same fixture: within 3e-4 of the largest entry
+    for p in ("p0_", "p1_"):
+        fd = z[p + "compat_fd"]
+        assert np.abs(-z[p + "compat_grad"].astype(np.float64) - fd).max() <= 3e-4 * np.abs(fd).max()
