@@ -133,6 +133,18 @@ int dcrf_capture_inference(dcrf_model *m, int n_iterations, const float *U_dev, 
 /* Launch the captured graph on `stream` (NULL is the null stream). */
 int dcrf_launch_graph(dcrf_model *m, void *stream);
 
+/* dcrf_inference_device without any device allocation, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The filter's vertex rows live in one
+ * static device array of 64 MiB per GPU (held until the process ends) instead of the
+ * model's buffers; the model must have run dcrf_inference_device (or dcrf_inference)
+ * once, so that its layout and unary are on the device, and its vertex rows must fit
+ * the array (DCRF_ERR_VALUE otherwise).  Every call on one GPU shares that array, which
+ * keeps the last call's contents: calls that overlap on one GPU race.  Same kernels and
+ * results as dcrf_inference_device. */
+int dcrf_inference_device_noalloc(dcrf_model *m, int n_iterations, const float *U_dev, float *Q_dev,
+                                  void *stream);
+
 /* utils.unary_from_softmax (pydensecrf/utils.py:55-79) on device buffers: sm [L][N]
  * (class first, as the reference takes it) -> U [N][L] (the layout of
  * dcrf_inference_device); scale <= 0: none, clip <= 0: none.  No model needed. */

@@ -52,6 +52,7 @@ SIGNATURES = {
     "dcrf_apply_pairwise": (_c, [_P, _c, _F, _F, _c]),
     "dcrf_inference_device": (_c, [_P, _c, _P, _P, _P]),
     "dcrf_capture_inference": (_c, [_P, _c, _P, _P]),
+    "dcrf_inference_device_noalloc": (_c, [_P, _c, _P, _P, _P]),
     "dcrf_launch_graph": (_c, [_P, _P]),
     "dcrf_set_compat": (_c, [_P, _c, _F]),
     "dcrf_set_kernel_weights": (_c, [_P, _F]),
@@ -276,6 +277,11 @@ class Model:
 
     def inference_device(self, n, Q_ptr, U_ptr=None, stream=None):
         check(lib().dcrf_inference_device(self.h, n, U_ptr, Q_ptr, stream))
+
+    def inference_device_noalloc(self, n, Q_ptr, U_ptr=None, stream=None):
+        """inference_device with the vertex rows in the library's static scratch array
+        (no device allocation; the model must have run inference_device once)."""
+        check(lib().dcrf_inference_device_noalloc(self.h, n, U_ptr, Q_ptr, stream))
 
     def capture(self, n, Q_ptr, U_ptr=None):
         check(lib().dcrf_capture_inference(self.h, n, U_ptr, Q_ptr))

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 
@@ -973,11 +974,11 @@ void prepare(dcrf_model& m) {
 // With the fused kernel (and the global splat), the iterations between the first and
 // the last keep Q in m.Qp with the vertex-row stride: every row the splat gathers then
 // starts a cache line.  The first splat reads the caller's Q, the last update writes it.
-void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStream_t s) {
+void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStream_t s, bool allow_pad = true) {
     const int L = m.L, N = m.N();
     launch_softmax_neg(U, Q, N, L, s);
     const SliceTable st = slice_table(m, false, m.kw_set ? m.kw.get() : nullptr);
-    const bool pad = m.fused && !m.runs && !m.tiled && n_iter > 1 && m.ldv != L && m.Qp.get();
+    const bool pad = allow_pad && m.fused && !m.runs && !m.tiled && n_iter > 1 && m.ldv != L && m.Qp.get();
     for (int it = 0; it < n_iter; it++) {
         const bool in_p = pad && it > 0, out_p = pad && it + 1 < n_iter;
         run_filter(m, in_p ? m.Qp.get() : Q, in_p ? m.ldv : L, false, s);
@@ -1156,6 +1157,57 @@ double objective_value(int kind, const std::vector<double>& OP, int L) {
     for (double v : OP) r += v;
     return r;
 }
+
+// ---- allocation-free inference (dcrf_inference_device_noalloc)
+// The filter's vertex rows (splat output, partial rows, both blur buffers) in one static
+// device array per GPU instead of the model's buffers.  Host code must not use the
+// array's name as an address: the address comes from hipGetSymbolAddress, once per
+// device, and is kept in scratch_dev.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+float* scratch_dev[kMaxDevices] = {};
+std::mutex scratch_mu;
+
+float* scratch_for_current_device() {
+    int dev = 0;
+    DCRF_HIP_CHECK(hipGetDevice(&dev));
+    require(dev >= 0 && dev < kMaxDevices, "device index out of range for the allocation-free inference");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        DCRF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "the scratch array has no device address");
+        scratch_dev[dev] = static_cast<float*>(p);
+    }
+    return scratch_dev[dev];
+}
+
+// Points the model's two vertex-row buffers at other memory for its lifetime (nothing
+// may reserve or release them meanwhile), then gives the model its own back.
+struct BorrowedRows {
+    dcrf_model& m;
+    float* own_p[2];
+    size_t own_n[2];
+    BorrowedRows(dcrf_model& m_, float* r0, size_t n0, float* r1, size_t n1) : m(m_) {
+        for (int i = 0; i < 2; i++) {
+            own_p[i] = m.vals[i].p;
+            own_n[i] = m.vals[i].n;
+        }
+        m.vals[0].p = r0;
+        m.vals[0].n = n0;
+        m.vals[1].p = r1;
+        m.vals[1].n = n1;
+    }
+    ~BorrowedRows() {
+        for (int i = 0; i < 2; i++) {
+            m.vals[i].p = own_p[i];
+            m.vals[i].n = own_n[i];
+        }
+    }
+    BorrowedRows(const BorrowedRows&) = delete;
+    BorrowedRows& operator=(const BorrowedRows&) = delete;
+};
 
 dcrf_model* new_model(const std::vector<int>& N, const std::vector<int>* W, const std::vector<int>* H, int L) {
     require(!N.empty(), "a model needs at least one instance");
@@ -1443,6 +1495,37 @@ int dcrf_launch_graph(dcrf_model* m, void* stream) {
         require((m->kw_set ? m->kw.get() : nullptr) == m->graph_kw,
                 "term weights were set or removed since the capture: capture again", DCRF_ERR_STATE);
         DCRF_HIP_CHECK(hipGraphLaunch(m->graph_exec, (hipStream_t)stream));
+    });
+}
+
+int dcrf_inference_device_noalloc(dcrf_model* m, int n_iter, const float* U_dev, float* Q_dev, void* stream) {
+    return guard([&] {
+        require(m && Q_dev, "null argument");
+        require(n_iter >= 0, "number of iterations must be non-negative");
+        require(m->dev_ready && m->layout_valid,
+                "the model's layout is not built: run dcrf_inference_device (or dcrf_inference) on it once first",
+                DCRF_ERR_STATE);
+        require(!m->timing, "disable kernel timing for the allocation-free inference", DCRF_ERR_STATE);
+        require(!m->tiled && !m->runs,
+                "the allocation-free inference runs the default splat and mean-field kernels only", DCRF_ERR_STATE);
+        require(U_dev || !m->has_unary || !m->unary_dirty,
+                "the unary changed since the last call: upload it with dcrf_inference_device first, or pass U_dev",
+                DCRF_ERR_STATE);
+        const size_t n0 = (size_t)m->rows0 * m->ldv, n1 = (size_t)(m->V_total + 1) * m->ldv;
+        const size_t off1 = (n0 + 63) / 64 * 64;  // second buffer 256-byte aligned
+        const size_t need = off1 + n1;
+        if (need > kNoAllocScratchBytes / sizeof(float))
+            throw Error{DCRF_ERR_VALUE, "the model needs " + std::to_string(need * sizeof(float)) +
+                                            " bytes of vertex rows; the allocation-free inference holds at most " +
+                                            std::to_string(kNoAllocScratchBytes) + " (use dcrf_inference_device)"};
+        float* base = scratch_for_current_device();
+        hipStream_t s = (hipStream_t)stream;
+        // the rows start at zero, as the layout leaves the model's own buffers (the
+        // missing-neighbour row is read as zero); the array keeps the last call's rows
+        launch_fill(base, 0.f, need, s);
+        BorrowedRows rows(*m, base, n0, base + off1, n1);
+        const float* U = U_dev ? U_dev : (m->has_unary ? m->unary.get() : nullptr);
+        run_inference(*m, n_iter, U, Q_dev, s, false);
     });
 }
 
