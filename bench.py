@@ -395,6 +395,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-image batched comparison")
     ap.add_argument("--no-fwd-bwd", action="store_true", help="skip the 512x512 forward+backward timing")
     ap.add_argument("--no-volumes", action="store_true", help="skip the 8-volume batched forward+backward timing")
+    ap.add_argument("--q-blocks", action="store_true",
+                    help="keep Q between iterations in 8x8 pixel blocks (dcrf_set_q_blocks; opt-in layout)")
+    ap.add_argument("--blur-lanes", type=int, nargs=2, metavar=("R", "BELOW_V"), default=None,
+                    help="blur passes below BELOW_V vertices with R labels per lane (dcrf_set_blur_lanes)")
+    ap.add_argument("--splat-sort", type=int, default=None, metavar="RUN",
+                    help="sort the splat segments by length inside runs of RUN segments (dcrf_set_splat_sort)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy three times (a known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -432,6 +438,12 @@ def main():
     warm.close()
     m = capi.Model.create_2d(W, H, L)
     m.set_unary(U)
+    if args.q_blocks:
+        m.set_q_blocks(True)
+    if args.blur_lanes:
+        m.set_blur_lanes(*args.blur_lanes)
+    if args.splat_sort is not None:
+        m.set_splat_sort(args.splat_sort)
     t0 = time.perf_counter()
     m.add_gaussian(3.0, 3.0, 3.0)
     t1 = time.perf_counter()
@@ -519,6 +531,9 @@ def main():
         "data": "synthetic (seeded Voronoi image and unary, pydensecrf_amd.synthetic)",
         "config": {"workload": WORKLOAD, "width": W, "height": H, "labels": L, "iterations": iters,
                    "launch": "stream-ordered kernels on the torch stream",
+                   "q_blocks": bool(args.q_blocks),
+                   "blur_lanes": args.blur_lanes,
+                   "splat_sort": args.splat_sort,
                    "vertices": {"gaussian": Vg, "bilateral": Vb}},
         "roofline": {
             "bound": "hbm",

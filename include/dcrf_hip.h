@@ -276,6 +276,21 @@ int dcrf_set_meanfield_kernel(dcrf_model *m, int mode, int run_rows);
  * on (default) blurs two lattice axes per pass through a table of the 9 rows each
  * vertex's two steps read, off runs one pass per axis. */
 int dcrf_set_blur_pairs(dcrf_model *m, int on);
+/* Inference only, 2-D single-image models with the fused mean-field kernel: keep Q
+ * between iterations in 8 x 8 pixel blocks (the splat then gathers the rows of a
+ * vertex's contributors from a few contiguous blocks instead of several image rows).
+ * Same results, bit for bit; 0 (off) by default.  No reference counterpart. */
+int dcrf_set_q_blocks(dcrf_model *m, int on);
+/* Order the splat's segments by length (longest first) inside runs of `run` segments of
+ * one pixel tile and term (0 or 1: layout order), so that the lane groups of a wave sum
+ * segments of similar length.  Each segment keeps its entries and output row: the same
+ * bits for any setting.  No reference counterpart. */
+int dcrf_set_splat_sort(dcrf_model *m, int run);
+/* Blur passes over fewer than below_vertices vertices (all passes when <= 0) with
+ * labels_per_lane labels per lane (0: the default for the label count), i.e. more lanes
+ * per vertex for small, latency-bound passes.  The blur is elementwise per label: the
+ * same bits for any setting.  No reference counterpart. */
+int dcrf_set_blur_lanes(dcrf_model *m, int labels_per_lane, int below_vertices);
 /* Hash-table load factor of later lattice builds, entries / slots in (0, 0.5]
  * (default 0.5): lower trades memory for shorter probe sequences. */
 int dcrf_set_hash_load(dcrf_model *m, float load);

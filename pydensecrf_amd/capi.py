@@ -73,6 +73,9 @@ SIGNATURES = {
     "dcrf_set_unary_params": (_c, [_P, _F]),
     "dcrf_unary_param_gradient": (_c, [_P, _F]),
     "dcrf_set_blur_pairs": (_c, [_P, _c]),
+    "dcrf_set_q_blocks": (_c, [_P, _c]),
+    "dcrf_set_splat_sort": (_c, [_P, _c]),
+    "dcrf_set_blur_lanes": (_c, [_P, _c, _c]),
     "dcrf_get_kernel_params": (_c, [_P, _F]),
     "dcrf_set_kernel_params": (_c, [_P, _F]),
     "dcrf_forward_device": (_c, [_P, _c, _P, _P, _P, _P, _P]),
@@ -155,27 +158,43 @@ def compat_args(compat, L):
 class Model:
     """Owner of a dcrf_model handle (plain, 2-D or batched)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owner=True):
         self.h = handle
+        self.owner = owner
+        self._tuned = False
+
+    @classmethod
+    def borrow(cls, address):
+        """A view of a model owned elsewhere (e.g. the drop-in module's, by its address):
+        closing it does not destroy the model."""
+        return cls(_P(address), owner=False)
+
+    @classmethod
+    def _created(cls, h):
+        m = cls(h)
+        from . import tune
+        if tune.enabled():
+            tune.apply_build_knobs(m)  # cached splat tile / hash load, before any term
+        return m
 
     @classmethod
     def create(cls, N, L):
         h = _P()
         check(lib().dcrf_create(N, L, ctypes.byref(h)))
-        return cls(h)
+        return cls._created(h)
 
     @classmethod
     def create_2d(cls, W, H, L):
         h = _P()
         check(lib().dcrf_create_2d(W, H, L, ctypes.byref(h)))
-        return cls(h)
+        return cls._created(h)
 
     @classmethod
     def create_batch(cls, Ns, L):
         a = np.ascontiguousarray(Ns, dtype=np.int32)
         h = _P()
         check(lib().dcrf_create_batch(len(a), a.ctypes.data_as(_I), L, ctypes.byref(h)))
-        return cls(h)
+        return cls._created(h)
 
     @classmethod
     def create_batch_2d(cls, Ws, Hs, L):
@@ -183,11 +202,22 @@ class Model:
         hh = np.ascontiguousarray(Hs, dtype=np.int32)
         h = _P()
         check(lib().dcrf_create_batch_2d(len(w), w.ctypes.data_as(_I), hh.ctypes.data_as(_I), L, ctypes.byref(h)))
-        return cls(h)
+        return cls._created(h)
+
+    def tuned(self):
+        """Apply the cached scheduling knobs of this model's shape (pydensecrf_amd.tune),
+        once, before its first computation."""
+        if not self._tuned and self.nterms:
+            self._tuned = True
+            from . import tune
+            if tune.enabled():
+                tune.apply_cached(self)
+        return self
 
     def close(self):
         if self.h:
-            lib().dcrf_destroy(self.h)
+            if self.owner:
+                lib().dcrf_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -248,11 +278,13 @@ class Model:
         return np.empty((self.N, self.L), dtype=np.float32)
 
     def inference(self, n):
+        self.tuned()
         Q = self._buf()
         check(lib().dcrf_inference(self.h, n, fptr(Q)))
         return Q.T
 
     def start(self):
+        self.tuned()
         Q = self._buf()
         check(lib().dcrf_start_inference(self.h, fptr(Q)))
         return Q.T
@@ -276,6 +308,7 @@ class Model:
         return out.T
 
     def inference_device(self, n, Q_ptr, U_ptr=None, stream=None):
+        self.tuned()
         check(lib().dcrf_inference_device(self.h, n, U_ptr, Q_ptr, stream))
 
     def inference_device_noalloc(self, n, Q_ptr, U_ptr=None, stream=None):
@@ -284,6 +317,7 @@ class Model:
         check(lib().dcrf_inference_device_noalloc(self.h, n, U_ptr, Q_ptr, stream))
 
     def capture(self, n, Q_ptr, U_ptr=None):
+        self.tuned()
         check(lib().dcrf_capture_inference(self.h, n, U_ptr, Q_ptr))
 
     def launch_graph(self, stream=None):
@@ -338,6 +372,7 @@ class Model:
         param = class_weight_pow) or IOU: (objective, unary_grad (L, N) or None when not
         asked for, compat_grad packed, kernel-weight grad [T], kernel-parameter grad or
         None)."""
+        self.tuned()
         g = np.ascontiguousarray(gt, dtype=np.int16)
         if g.shape != (self.N,):
             raise ValueError("gt must have one label per variable (%d), got %s" % (self.N, g.shape))
@@ -382,6 +417,19 @@ class Model:
         check(lib().dcrf_unary_param_gradient(self.h, fptr(g)))
         return g[:n]
 
+    def set_blur_lanes(self, labels_per_lane, below_vertices=0):
+        """Blur passes of fewer than below_vertices vertices (all when 0) with the given
+        labels per lane (0 = default)."""
+        check(lib().dcrf_set_blur_lanes(self.h, int(labels_per_lane), int(below_vertices)))
+
+    def set_splat_sort(self, run):
+        """Sort the splat's segments by length inside runs of `run` segments (0 = off)."""
+        check(lib().dcrf_set_splat_sort(self.h, int(run)))
+
+    def set_q_blocks(self, on):
+        """Keep Q between inference iterations in 8 x 8 pixel blocks (opt-in)."""
+        check(lib().dcrf_set_q_blocks(self.h, int(bool(on))))
+
     def set_blur_pairs(self, on):
         check(lib().dcrf_set_blur_pairs(self.h, 1 if on else 0))
 
@@ -400,6 +448,7 @@ class Model:
     def gradient(self, n, gt, robust=0.0, kernel=False):
         """DenseCRF::gradient with LogLikelihood: (objective, unary_grad (L, N),
         compat_grad packed, kernel-weight grad [T][, kernel-parameter grad])."""
+        self.tuned()
         g = np.ascontiguousarray(gt, dtype=np.int16)
         if g.shape != (self.N,):
             raise ValueError("gt must have one label per variable (%d), got %s" % (self.N, g.shape))
@@ -415,6 +464,7 @@ class Model:
         return out + (kp[:nk],) if kernel else out
 
     def forward_device(self, n, U_ptr, Q_ptr, compat_ptr=None, kw_ptr=None, stream=None):
+        self.tuned()
         check(lib().dcrf_forward_device(self.h, n, U_ptr, compat_ptr, kw_ptr, Q_ptr, stream))
 
     def backward_device(self, dQ_ptr, dU_ptr, dcompat_ptr=None, dkw_ptr=None, stream=None, dkernel_ptr=None):

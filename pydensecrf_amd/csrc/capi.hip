@@ -94,6 +94,16 @@ struct dcrf_model {
     size_t lds_cap = 0;                // LDS bytes one workgroup may allocate
     TiledSplat tsplat{};
     DevBuf<int2> g_entF, g_entT;  // splat entries (pixel, weight bits), forward / transposed
+    // opt-in: Q between inference iterations in 8 x 8 pixel blocks (2-D single images),
+    // read by the splat through entries holding block positions
+    bool qblocks = false, qblocks_ready = false;
+    // opt-in: blur passes of fewer than blur_small_v vertices with blur_r labels per lane
+    int blur_r = 0, blur_small_v = 0;
+    // splat segments sorted by length (longest first) inside runs of seg_sort segments of
+    // one (tile, term) group; 0 or 1 = layout order
+    int seg_sort = 0;
+    DevBuf<int2> g_entB;
+    DevBuf<float> Qb;
     bool fused = false;           // inference through k_meanfield_fused
     DevBuf<int2> g_slc;           // its per-pixel slice pairs of every term
     DevBuf<int2> g_slcT;          // the same with the transposed filter's slice weights (adjoint;
@@ -669,6 +679,7 @@ int blur_launches(const dcrf_model& m) {
 void ensure_layout(dcrf_model& m) {
     if (m.layout_valid) return;
     m.drop_graph();
+    m.qblocks_ready = false;
     hipStream_t s = m.stream;
     const int T = (int)m.terms.size();
     const int N = m.N(), L = m.L;
@@ -707,6 +718,7 @@ void ensure_layout(dcrf_model& m) {
     std::vector<int> gts(NT + 1), gte(NT + 1);
     std::vector<std::vector<int>> ebase(T, std::vector<int>(NT));
     int e = 0, e_max = 0, s_max = 0;
+    std::vector<int> groups;  // first segment of each (tile, term) group
     for (int t = 0; t < NT; t++) {
         gts[t] = (int)gch.size();
         gte[t] = e;
@@ -714,6 +726,7 @@ void ensure_layout(dcrf_model& m) {
             const TermLattice& lat = m.terms[k]->lat;
             const int eb = e - lat.tile_ptr[t];
             ebase[k][t] = eb;
+            groups.push_back((int)gch.size());
             for (int c = lat.chunk_tile_ptr[t]; c < lat.chunk_tile_ptr[t + 1]; c++) {
                 Chunk x = lat.chunks[c];
                 x.v += m.vbase[k];
@@ -729,6 +742,7 @@ void ensure_layout(dcrf_model& m) {
     }
     gts[NT] = (int)gch.size();
     gte[NT] = e;
+    groups.push_back((int)gch.size());
     if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
     require_adjoining(gch);
     // Vertices split into more than kCombineSplit segments are recombined in two levels:
@@ -765,6 +779,39 @@ void ensure_layout(dcrf_model& m) {
     const TileGeom tg = tile_geom(m);
     m.tsplat = TiledSplat{m.g_chunks.get(), m.g_tile_seg.get(), m.g_tile_e.get(), NT, tg, e_max, s_max};
     m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
+    // A wave of the global splat sums the segments of its lane groups side by side and
+    // issues as many 8-entry gathers as its longest segment needs: segments of similar
+    // length in one wave waste fewer.  Sorting inside short runs keeps the wave's rows
+    // close together.  Each segment keeps its entries (in order) and its output row, so
+    // the sums are the same bits; the entries are moved to follow the new order.
+    std::vector<int4> moves;  // (old first entry, new first entry, count, 0)
+    if (m.seg_sort > 1 && !m.tiled) {
+        std::vector<Chunk> sorted;
+        sorted.reserve(gch.size());
+        std::vector<int> idx;
+        for (size_t gi = 0; gi + 1 < groups.size(); gi++)
+            for (int r0 = groups[gi]; r0 < groups[gi + 1]; r0 += m.seg_sort) {
+                const int r1 = std::min(r0 + m.seg_sort, groups[gi + 1]);
+                idx.resize(r1 - r0);
+                std::iota(idx.begin(), idx.end(), r0);
+                std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+                    return gch[a].end - gch[a].beg > gch[b].end - gch[b].beg;
+                });
+                int pos = gch[r0].beg;
+                for (int i : idx) {
+                    Chunk x = gch[i];
+                    const int n = x.end - x.beg;
+                    if (x.beg != pos) moves.push_back(make_int4(x.beg, pos, n, 0));
+                    x.beg = pos;
+                    x.end = pos + n;
+                    pos += n;
+                    sorted.push_back(x);
+                }
+            }
+        gch.swap(sorted);
+        require_adjoining(gch);
+        upload(m.g_chunks, gch, s);
+    }
 
     // vertex rows packed (stride L): rows padded to start cache lines (32 floats at
     // L = 21, for Q between iterations too) measured 8 % slower per step at 1024x768x21
@@ -810,6 +857,19 @@ void ensure_layout(dcrf_model& m) {
                                  m.vbase[k], lay->vtx.get(), lay->swT.get(), s);
         }
         m.tl[k] = std::move(lay);
+    }
+    if (!moves.empty()) {
+        DevBuf<int4> mv;
+        upload(mv, moves, s);
+        DevBuf<int2> tmp;
+        tmp.reserve(E);
+        for (DevBuf<int2>* ent : {&m.g_entF, &m.g_entT}) {
+            if (!ent->get()) continue;
+            if (ent == &m.g_entT && m.all_symmetric) continue;
+            DCRF_HIP_CHECK(hipMemcpyAsync(tmp.get(), ent->get(), sizeof(int2) * E, hipMemcpyDeviceToDevice, s));
+            launch_move_entries(tmp.get(), mv.get(), (int)moves.size(), ent->get(), s);
+        }
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
     }
     // interleaved slice pairs for the fused mean-field kernel
     m.runs = false;
@@ -912,9 +972,10 @@ double splat_bytes(const dcrf_model& m) {
 double meanfield_bytes(const dcrf_model& m) { return splat_bytes(m) + m.N() * (double)m.L * 4.0; }
 
 // in: the [N] input rows with row stride ldi (floats).
-void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStream_t s) {
+void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStream_t s,
+                const int2* ent_in = nullptr) {
     const int L = m.L;
-    const int2* ent = (transpose && !m.all_symmetric) ? m.g_entT.get() : m.g_entF.get();
+    const int2* ent = ent_in ? ent_in : ((transpose && !m.all_symmetric) ? m.g_entT.get() : m.g_entF.get());
     {
         Timed t(m, s, KC_SPLAT, splat_bytes(m));
         if (m.tiled)
@@ -948,7 +1009,8 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
                 if (on) bytes += (double)lat.V * steps * (2.0 * L * 4 + 8);
             }
             Timed t(m, s, KC_BLUR, bytes);
-            launch_blur_pass(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s);
+            const int pr = m.blur_r > 0 && (m.blur_small_v <= 0 || m.prefixP[p] < m.blur_small_v) ? m.blur_r : 0;
+            launch_blur_pass(m.vals[p & 1].get(), m.vals[(p + 1) & 1].get(), 0, m.prefixP[p], m.ldv, L, pt, s, pr);
             t.done();
         }
         return;
@@ -969,6 +1031,14 @@ void prepare(dcrf_model& m) {
     const size_t n = (size_t)m.N() * m.L;
     m.Q.reserve(n);
     if (m.fused && m.ldv != m.L) m.Qp.reserve((size_t)m.N() * m.ldv);  // before any graph capture
+    if (m.qblocks && !m.qblocks_ready && m.fused && !m.runs && !m.tiled && m.ldv == m.L && m.geo.is2d &&
+        m.geo.n_inst == 1) {
+        m.g_entB.reserve(m.E_total);
+        launch_entry_block(m.g_entF.get(), (size_t)m.E_total, m.geo.W[0], m.g_entB.get(), m.stream);
+        m.Qb.reserve(q_block_rows(m.geo.W[0], m.geo.H[0]) * m.L);
+        DCRF_HIP_CHECK(hipStreamSynchronize(m.stream));
+        m.qblocks_ready = true;
+    }
 }
 
 // With the fused kernel (and the global splat), the iterations between the first and
@@ -979,7 +1049,18 @@ void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStrea
     launch_softmax_neg(U, Q, N, L, s);
     const SliceTable st = slice_table(m, false, m.kw_set ? m.kw.get() : nullptr);
     const bool pad = allow_pad && m.fused && !m.runs && !m.tiled && n_iter > 1 && m.ldv != L && m.Qp.get();
+    // Q in 8 x 8 blocks between iterations: the first splat reads the caller's Q, the
+    // later ones m.Qb through the block-position entries; the last update writes Q
+    const bool blk = !pad && m.qblocks_ready && n_iter > 1 && m.fused && !m.runs && !m.tiled;
     for (int it = 0; it < n_iter; it++) {
+        if (blk) {
+            run_filter(m, it > 0 ? m.Qb.get() : Q, L, false, s, it > 0 ? m.g_entB.get() : nullptr);
+            Timed t(m, s, KC_MEANFIELD, meanfield_bytes(m));
+            const bool last = it + 1 == n_iter;
+            launch_meanfield_fused(U, last ? Q : m.Qb.get(), L, N, L, st, m.g_slc.get(), s, last ? 0 : m.geo.W[0]);
+            t.done();
+            continue;
+        }
         const bool in_p = pad && it > 0, out_p = pad && it + 1 < n_iter;
         run_filter(m, in_p ? m.Qp.get() : Q, in_p ? m.ldv : L, false, s);
         Timed t(m, s, KC_MEANFIELD, meanfield_bytes(m));
@@ -2080,6 +2161,41 @@ int dcrf_set_meanfield_kernel(dcrf_model* m, int mode, int run_rows) {
             m->mf_mode = mode;
             m->run_rows = rr;
             m->layout_valid = false;
+        }
+    });
+}
+
+int dcrf_set_blur_lanes(dcrf_model* m, int labels_per_lane, int below_vertices) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(labels_per_lane >= 0 && labels_per_lane <= 4, "labels per lane must be 0 (default) or 1 to 4");
+        require(labels_per_lane == 0 || m->L % labels_per_lane == 0, "labels per lane must divide the label count");
+        require(labels_per_lane == 0 || shape_for_r(m->L, labels_per_lane).wave,
+                "too few labels per lane for a lane group within one wave");
+        m->blur_r = labels_per_lane;
+        m->blur_small_v = below_vertices;
+        m->drop_graph();
+    });
+}
+
+int dcrf_set_splat_sort(dcrf_model* m, int run) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(run >= 0 && run <= 4096, "sort run must be in [0, 4096]");
+        if (m->seg_sort != run) {
+            m->seg_sort = run;
+            m->layout_valid = false;
+        }
+    });
+}
+
+int dcrf_set_q_blocks(dcrf_model* m, int on) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        if (m->qblocks != (on != 0)) {
+            m->qblocks = on != 0;
+            m->qblocks_ready = false;
+            m->drop_graph();
         }
     });
 }

@@ -266,6 +266,7 @@ struct LaunchShape {
 };
 int choose_labels_per_lane(int L);
 LaunchShape shape_for(int L);
+LaunchShape shape_for_r(int L, int R);  // the shape for R labels per lane
 
 // Splat: vals[out] = sum over a segment's entries i of w_i * in[p_i], in entry order;
 // ent[i] = (p_i, bits of w_i).  Global form: p_i is the pixel (row of `in`).
@@ -294,7 +295,7 @@ void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, c
                  int reverse, hipStream_t s);
 // One multi-step pass over the vertices [v0, Vp): bit-identical to its blur steps.
 void launch_blur_pass(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PassTable& pt,
-                      hipStream_t s);
+                      hipStream_t s, int R = 0);
 // The table of a K-step pass (axes[0] first) of one term from its global neighbour
 // rows nbr [(d+1)][V].
 void launch_pass_table(const int2* nbr, int V, int vbase, int zero_row, int K, const int* axes, int* tab,
@@ -338,6 +339,8 @@ void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTa
 void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
                             const int2* slc, hipStream_t s, int bw = 0);
 void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s);
+// dst[m.y + j] = src[m.x + j], j < m.z, for the n moves m = (from, to, count, -)
+void launch_move_entries(const int2* src, const int4* moves, int n, int2* dst, hipStream_t s);
 // rows of a Q held in 8 x 8 pixel blocks for an image of W x H pixels
 inline size_t q_block_rows(int W, int H) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64; }
 void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,

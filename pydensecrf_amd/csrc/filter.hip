@@ -35,8 +35,9 @@ int choose_labels_per_lane(int L) {
     return 1;
 }
 
-LaunchShape shape_for(int L) {
-    const int R = choose_labels_per_lane(L);
+LaunchShape shape_for(int L) { return shape_for_r(L, choose_labels_per_lane(L)); }
+
+LaunchShape shape_for_r(int L, int R) {
     const int S = (L + R - 1) / R;
     if (S <= 64) return LaunchShape{kWaveBlock, 64 / S, true, (kWaveBlock / 64) * (64 / S), R, S};
     const int block = S <= 256 ? 256 : (S <= 512 ? 512 : 1024);
@@ -1076,9 +1077,11 @@ void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, c
 }
 
 void launch_blur_pass(const float* src, float* dst, int v0, int Vp, int ldv, int L, const PassTable& pt,
-                      hipStream_t s) {
+                      hipStream_t s, int R) {
     if (Vp <= v0) return;
-    const LaunchShape sh = shape_for(L);
+    // R: labels per lane (0 = the default for L); the blur is elementwise per label, so
+    // any lane split gives the same bits
+    const LaunchShape sh = R > 0 && L % R == 0 ? shape_for_r(L, R) : shape_for(L);
     DCRF_DISPATCH(sh, k_blur_pass, blocks_for(Vp - v0, sh.items), 0, s, src, dst, v0, Vp, ldv, L, sh.S, sh.per, pt);
 }
 
@@ -1227,6 +1230,23 @@ __global__ void k_entry_block(const int2* __restrict__ ent, size_t E, int W, int
     }
 }
 }  // namespace
+
+namespace {
+// dst[m.y + j] = src[m.x + j] for j < m.z, for every move m (one thread per move).
+__global__ void k_move_entries(const int2* __restrict__ src, const int4* __restrict__ moves, int n,
+                               int2* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 mv = moves[i];
+    for (int j = 0; j < mv.z; j++) dst[mv.y + j] = src[mv.x + j];
+}
+}  // namespace
+
+void launch_move_entries(const int2* src, const int4* moves, int n, int2* dst, hipStream_t s) {
+    if (n <= 0) return;
+    k_move_entries<<<blocks_for(n), kThreads, 0, s>>>(src, moves, n, dst);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
 
 void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s) {
     if (!E) return;

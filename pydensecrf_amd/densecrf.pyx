@@ -131,18 +131,38 @@ cdef class LogisticUnary(Unary):
         self._u = (np.array(L, dtype=np.float32, copy=True), np.array(f, dtype=np.float32, copy=True))
 
 
+def _borrowed(size_t address):
+    from pydensecrf_amd import capi
+    return capi.Model.borrow(address)
+
+
 cdef class DenseCRF:
     cdef dcrf_model* _this
     cdef int _nlabel
     cdef int _nvar
+    cdef bint _tuned
 
     def __cinit__(self, int nvar, int nlabels, *_, **__):
         # Subclasses create their own model (pydensecrf/densecrf.pyx:51-62).
         self._this = NULL
+        self._tuned = False
         if type(self) is DenseCRF:
             _check(dcrf_create(nvar, nlabels, &self._this))
+            self._build_knobs()
         self._nvar = nvar
         self._nlabel = nlabels
+
+    cdef _build_knobs(self):
+        # cached build knobs of this size (pydensecrf_amd.tune), before any term
+        from pydensecrf_amd import tune
+        if tune.enabled():
+            tune.apply_build_knobs(_borrowed(<size_t>self._this))
+
+    cdef _tune(self):
+        # cached scheduling knobs of this model's shape, once, before the first computation
+        if not self._tuned:
+            self._tuned = True
+            _borrowed(<size_t>self._this).tuned()
 
     def __dealloc__(self):
         if self._this != NULL:
@@ -197,6 +217,7 @@ cdef class DenseCRF:
         _check(st)
 
     def inference(self, int niter):
+        self._tune()
         cdef MatrixXf Q = MatrixXf()
         Q.set_shape(self._nlabel, self._nvar)
         cdef int st
@@ -206,6 +227,7 @@ cdef class DenseCRF:
         return Q
 
     def startInference(self):
+        self._tune()
         cdef MatrixXf Q = MatrixXf()
         Q.set_shape(self._nlabel, self._nvar)
         cdef int st
@@ -249,6 +271,7 @@ cdef class DenseCRF2D(DenseCRF):
     def __cinit__(self, int w, int h, int nlabels, *_, **__):
         if type(self) is DenseCRF2D:
             _check(dcrf_create_2d(w, h, nlabels, &self._this))
+            self._build_knobs()
         self._w = w
         self._h = h
         self._nvar = w * h

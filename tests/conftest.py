@@ -9,6 +9,10 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
+    # tuned scheduling knobs come from a cache of the session's own, not the user's
+    if "PYDENSECRF_AMD_TUNE_CACHE" not in os.environ:
+        import tempfile
+        os.environ["PYDENSECRF_AMD_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="dcrf_tune_"), "tune.json")
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); calls the HIP kernels through the C ABI")
 
 

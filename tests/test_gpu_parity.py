@@ -306,3 +306,122 @@ def test_blur_pairs_bitwise(dims):
     assert ga[0] == gb[0]
     for x, y in zip(ga[1:], gb[1:]):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("W,H,L,n", [(203, 149, 21, 5), (64, 48, 4, 3), (9, 7, 5, 4), (1024, 768, 21, 10)])
+def test_q_blocks_bitwise(W, H, L, n):
+    """Q held in 8 x 8 pixel blocks between iterations (dcrf_set_q_blocks, image sides
+    not multiples of 8 included): the same bits as the default layout, through the host
+    path, the device path and a captured graph."""
+    import torch
+    from pydensecrf_amd import capi
+    img, U, _ = image_case(W, H, L, seed=W + H)
+    out = []
+    for on in (False, True):
+        g = capi.Model.create_2d(W, H, L)
+        g.set_unary(U)
+        g.add_gaussian(3.0, 3.0, 3.0)
+        g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+        g.set_q_blocks(on)
+        Qh = np.ascontiguousarray(g.inference(n))
+        Qd = torch.full((W * H * L,), -1.0, device="cuda")
+        g.inference_device(n, Qd.data_ptr())
+        Qg = torch.full_like(Qd, -1.0)
+        g.capture(n, Qg.data_ptr())
+        g.launch_graph()
+        torch.cuda.synchronize()
+        out.append((Qh, Qd.cpu().numpy(), Qg.cpu().numpy()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a).view(np.int32).ravel(), np.asarray(b).view(np.int32).ravel())
+    assert np.array_equal(out[1][0].T.ravel().view(np.int32), out[1][1].view(np.int32))
+
+
+def test_autotuned_knobs_keep_results(tmp_path, monkeypatch):
+    """The widened tuner: the bit-identical knobs (blur steps per pass, mean-field kernel,
+    Q in blocks) tuned on a model leave Q's bits as they were; the cached winners are
+    applied by a new model of the same shape before its first inference, and the
+    build knobs (splat tile side) by a new model of the same size at creation."""
+    from pydensecrf_amd import capi, tune
+    monkeypatch.setenv("PYDENSECRF_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setenv("PYDENSECRF_AMD_TUNE", "1")
+    W, H, L = 96, 72, 21
+    img, U, _ = image_case(W, H, L, seed=78)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    ref = np.ascontiguousarray(g.inference(5))
+    knobs = ("blur_pairs", "meanfield", "q_blocks")
+    best, times = tune.tune_model(g, iters=2, reps=2, knobs=knobs)
+    assert set(best) == set(knobs) and all(set(times[k]) == set(tune.MODEL_KNOBS[k]) for k in knobs)
+    assert np.array_equal(np.ascontiguousarray(g.inference(5)).view(np.int32), ref.view(np.int32))
+    again, t2 = tune.tune_model(g, knobs=knobs)
+    assert again == best and t2 is None  # cached
+    g2, _ = build_both(W, H, L, U, _std_terms(img))
+    Q2 = np.ascontiguousarray(g2.inference(5))  # applies the cached knobs first
+    assert g2.splat_layout()["meanfield"] == (2 if best["meanfield"] == 3 else 1)
+    assert np.array_equal(Q2.view(np.int32), ref.view(np.int32))
+
+    def build(side):
+        m = capi.Model.create_2d(W, H, L)
+        m.set_splat_tile(side)
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0)
+        m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+        return m
+    side, ts = tune.tune_splat_tile(build, candidates=(64, 128), iters=2, reps=2)
+    assert set(ts) == {64, 128}
+    g3, _ = build_both(W, H, L, U, _std_terms(img))
+    assert g3.splat_layout()["tile_side"] == side
+    assert np.abs(g3.inference(5) - o.inference(5)).max() <= Q_TOL
+
+
+@pytest.mark.parametrize("R,below", [(1, 0), (1, 50000), (3, 0)])
+def test_blur_lanes_bitwise(R, below):
+    """Blur passes with another lane split (dcrf_set_blur_lanes): the blur is elementwise
+    per label, so inference, the gradient (transposed filter) and a captured graph give
+    the same bits as the default split."""
+    import torch
+    from pydensecrf_amd import capi
+    W, H, L = 130, 97, 21
+    img, U, gt = image_case(W, H, L, seed=123)
+    out = []
+    for knob in (None, (R, below)):
+        g = capi.Model.create_2d(W, H, L)
+        g.set_unary(U)
+        g.add_gaussian(3.0, 3.0, 3.0)
+        g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+        if knob:
+            g.set_blur_lanes(*knob)
+        Q = np.ascontiguousarray(g.inference(5))
+        obj, ug, cg, kg = g.gradient(3, gt)
+        Qg = torch.full((W * H * L,), -1.0, device="cuda")
+        g.capture(5, Qg.data_ptr())
+        g.launch_graph()
+        torch.cuda.synchronize()
+        out.append((Q, ug, cg, Qg.cpu().numpy(), obj))
+    for a, b in zip(out[0][:4], out[1][:4]):
+        assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))
+    assert out[0][4] == out[1][4]
+    with pytest.raises(ValueError):
+        g.set_blur_lanes(2)  # 2 does not divide 21
+
+
+@pytest.mark.parametrize("run", [2, 36, 4096])
+def test_splat_sort_bitwise(run):
+    """Splat segments sorted by length inside runs (dcrf_set_splat_sort): every segment
+    keeps its entries and output row, so inference and the gradient (the transposed
+    entries move too, for a BEFORE-normalised term) give the same bits."""
+    from pydensecrf_amd import capi
+    W, H, L = 150, 110, 21
+    img, U, gt = image_case(W, H, L, seed=321)
+    out = []
+    for r in (0, run):
+        g = capi.Model.create_2d(W, H, L)
+        g.set_unary(U)
+        g.add_gaussian(3.0, 3.0, 3.0)
+        g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0, ntype=BEFORE)
+        g.set_splat_sort(r)
+        Q = np.ascontiguousarray(g.inference(5))
+        obj, ug, cg, kg = g.gradient(3, gt)
+        out.append((Q, ug, cg, obj))
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))
+    assert out[0][3] == out[1][3]
