@@ -4,13 +4,17 @@
 //   b_n   = sumAndNormalize(Q_n .* dL/dQ_n, Q_n)                  (= dL/dE_n, :257-258)
 //   for t = n-1 .. 0:
 //     G_k  += b_{t+1} (K~_k Q_t)^T         compatibility / kernel-weight statistics
-//                                          (labelcompatibility.cpp:57-61, 76-78, 101-108)
+//                                          (labelcompatibility.cpp:57-61, 76-78, 101-108);
+//                                          Potts / Diagonal take only its diagonal, summed
+//                                          as (K~_k^T b_{t+1}) .* Q_t inside the adjoint
+//                                          kernel (the transposed message is at hand there)
 //     g_t   = sum_k w_k K~_k^T C_k^T b_{t+1}   transposed filter (reverse blur order,
 //                                          pairwise.cpp:179-183, permutohedral.cpp:556)
 //     b_t   = sumAndNormalize(Q_t .* g_t, Q_t)                     (:290)
 //   dL/dU = sum_t b_t                                              (:261-262, 293-294)
-// K~_k Q_t comes from the forward pass's message cache, so the backward runs one
-// filter per term per iteration (the reference re-filters, pairwise.cpp:193).
+// Matrix terms take K~_k Q_t from the forward pass's message cache; either way the
+// backward runs one filter per term per iteration (the reference also re-filters Q_t
+// for the statistic, pairwise.cpp:193).
 // Reductions run in a fixed order (per-block float partials, then a double sum over
 // blocks): results are deterministic.
 #include "dcrf_internal.h"
@@ -49,17 +53,26 @@ __global__ void __launch_bounds__(1024) k_sn_first(const float* __restrict__ Q, 
     }
 }
 
+// Terms with a statistic row (SliceTerm::gstat) also leave the products of their raw
+// transposed message and Q in the statistic slots (after the compat region), summed per
+// block at the end: sum_p (K~^T b)[p] .* Q[p] = sum_p b[p] .* (K~ Q)[p], the diagonal of
+// the compatibility gradient's statistic, with no message cache.
 template <bool WAVE, int R, bool CWL>
 __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, float* __restrict__ b,
                                                   float* __restrict__ dU, int N, int L, int S, int per,
-                                                  SliceTable st) {
+                                                  SliceTable st, int cwf) {
     extern __shared__ float sh[];
     float* shc = sh + blockDim.x;
+    float* sst = shc + cwf;
     if (CWL) stage_compat(st, L, shc);
     LaneGroup<WAVE, R> G(L, S, per, sh);
     const int p = G.item();
     const bool active = G.on && p < N;
     const int k0 = G.k0;
+    float q[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) q[r] = 0.0f;
+    if (active) load_piece<R>(Q + (size_t)p * L, k0, L, q);
     float gsum[R];
 #pragma unroll
     for (int r = 0; r < R; r++) gsum[r] = 0.0f;
@@ -74,6 +87,12 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
 #pragma unroll
                 for (int r = 0; r < R; r++) m[r] = 0.0f;
             }
+            if (T.gslot >= 0) {
+                float v[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) v[r] = active ? m[r] * q[r] : 0.0f;
+                stat_put<R>(sst, T.gslot, v);
+            }
             if (T.kw) {
                 const float kw = T.kw[0];
 #pragma unroll
@@ -84,10 +103,7 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
             for (int r = 0; r < R; r++) gsum[r] = gsum[r] + c[r];
         }
     }
-    float q[R], bn[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) q[r] = 0.0f;
-    if (active) load_piece<R>(Q + (size_t)p * L, k0, L, q);
+    float bn[R];
     sum_normalize<WAVE, R>(q, gsum, bn, G);
     if (active) {
         float du[R];
@@ -97,52 +113,7 @@ __global__ void __launch_bounds__(1024) k_adjoint(const float* __restrict__ Q, f
         store_piece<R>(b + (size_t)p * L, k0, L, bn);
         store_piece<R>(dU + (size_t)p * L, k0, L, du);
     }
-}
-
-// partial[block][label] = sum over the block's pixels of b .* M (groups stride over the
-// pixels; the block's groups are then added in group order per label).  The grid covers
-// `iters` iterations of nb blocks each: block i * nb + j takes iteration i's operands
-// b + i * bstride and M + i * mstride.
-template <bool WAVE, int R>
-__global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b, const float* __restrict__ M, int N,
-                                                    int L, int S, int per, int nb, size_t bstride, size_t mstride,
-                                                    float* __restrict__ partial) {
-    extern __shared__ float sh[];  // block * R floats
-    const int it = blockIdx.x / nb, blk = blockIdx.x - it * nb;
-    b += (size_t)it * bstride;
-    M += (size_t)it * mstride;
-    LaneGroup<WAVE, R> G(L, S, per, sh);
-    const int items = LaneGroup<WAVE, R>::items_per_block(per);
-    float acc[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) acc[r] = 0.0f;
-    if (G.on) {
-        for (int p = G.item_in(blk); p < N; p += nb * items) {
-            float x[R], y[R];
-            load_piece<R>(b + (size_t)p * L, G.k0, L, x);
-            load_piece<R>(M + (size_t)p * L, G.k0, L, y);
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const float prod = x[r] * y[r];
-                acc[r] = acc[r] + prod;
-            }
-        }
-    }
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int r = 0; r < R; r++) sh[t * R + r] = G.on ? acc[r] : 0.0f;
-    __syncthreads();
-    // labels strided over the block's threads (a block-mode group may hold more labels
-    // than the block has threads)
-    for (int lab = t; lab < L; lab += blockDim.x) {
-        const int ln = lab / R, rr = lab - ln * R;  // lane and slot holding label lab
-        float s = 0.0f;
-        for (int gi = 0; gi < items; gi++) {
-            const int lane0 = WAVE ? (gi / per) * 64 + (gi % per) * S : gi * S;
-            s = s + sh[(lane0 + ln) * R + rr];
-        }
-        partial[(size_t)blockIdx.x * L + lab] = s;
-    }
+    if (st.nstat) stat_finish<WAVE, R>(st, G, sst);  // uniform condition
 }
 
 // The L x L statistic G[a][c] = sum_t sum_p b_t[p][a] M_t[p][c] is a tall-skinny GEMM
@@ -151,7 +122,7 @@ __global__ void __launch_bounds__(1024) k_grad_diag(const float* __restrict__ b,
 // instruction, kMfmaBatch pixel pairs of operands in flight; lane l supplies
 // A[l % 32][l / 32] = b[p + l/32][a] and B[l / 32][l % 32] = M[p + l/32][c] (labels past
 // L read as 0).  Iteration t's operands are b + t * bstride and M + t * mstride.  The
-// tile goes to partial[t * ranges + range][a * L + c]; k_reduce_cols adds them in double.
+// tile goes to partial[t * ranges + range][a * L + c]; launch_sum_rows adds them in double.
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int kMfmaBatch = 8;
 
@@ -198,21 +169,63 @@ __global__ void __launch_bounds__(256) k_grad_mfma(const float* __restrict__ b, 
     }
 }
 
-// acc[off + j] += sum over k < nblocks of partial[k * stride + j], in double: one block
-// per column j, a fixed split over the threads, then a fixed tree (deterministic).
-__global__ void k_reduce_cols(const float* __restrict__ partial, int nblocks, int stride, double* __restrict__ acc,
-                              int off) {
+// acc[j] += sum over r < rows of part[r * cols + j], in double and a fixed order.
+// Stage 1: block i takes rows [i * rpb, (i + 1) * rpb); the columns go in passes of
+// cpb = min(cols, 256), tpc = 256 / cpb threads per column, thread t adding column
+// c0 + t % cpb of every tpc-th row from row t / cpb (consecutive threads read
+// consecutive floats; four rows in flight, added in row order); the column's threads
+// are then added in thread order into mid[i * cols + j].  Stage 2: one block per
+// column adds mid over the blocks (strided over its threads, then a fixed tree).
+constexpr int kRowBlocks = 1024;
+constexpr int kRowsMin = 16;  // rows per first-stage block, at least
+
+__global__ void __launch_bounds__(256) k_rows_stage1(const float* __restrict__ part, long long rows, int cols,
+                                                      long long rpb, double* __restrict__ mid) {
     __shared__ double red[256];
-    const int j = blockIdx.x, t = threadIdx.x;
-    double s = 0.0;
-    for (int k = t; k < nblocks; k += blockDim.x) s += (double)partial[(size_t)k * stride + j];
-    red[t] = s;
+    const int t = threadIdx.x;
+    const int cpb = cols < 256 ? cols : 256, tpc = 256 / cpb;
+    const long long r0 = (long long)blockIdx.x * rpb, r1 = min(rows, r0 + rpb);
+    const int sub = t / cpb;
+    for (int c0 = 0; c0 < cols; c0 += cpb) {
+        const int c = c0 + t % cpb;
+        double acc = 0.0;
+        if (sub < tpc && c < cols) {
+            const float* src = part + c;
+            long long r = r0 + sub;
+            for (; r + 3 * tpc < r1; r += 4 * tpc) {
+                const float a0 = src[r * cols], a1 = src[(r + tpc) * cols];
+                const float a2 = src[(r + 2 * tpc) * cols], a3 = src[(r + 3 * tpc) * cols];
+                acc += (double)a0;
+                acc += (double)a1;
+                acc += (double)a2;
+                acc += (double)a3;
+            }
+            for (; r < r1; r += tpc) acc += (double)src[r * cols];
+        }
+        red[t] = acc;
+        __syncthreads();
+        if (t < cpb && c0 + t < cols) {
+            double v = 0.0;
+            for (int i = 0; i < tpc; i++) v += red[i * cpb + t];
+            mid[(size_t)blockIdx.x * cols + c0 + t] = v;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rows_stage2(const double* __restrict__ mid, int nb, int cols,
+                                                      double* __restrict__ acc) {
+    __shared__ double red[256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    double v = 0.0;
+    for (int i = t; i < nb; i += blockDim.x) v += mid[(size_t)i * cols + c];
+    red[t] = v;
     __syncthreads();
     for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
         if (t < h) red[t] += red[t + h];
         __syncthreads();
     }
-    if (t == 0) acc[off + j] += red[0];
+    if (t == 0) acc[c] += red[0];
 }
 
 __global__ void k_fill_double(double* p, size_t n) {
@@ -221,36 +234,41 @@ __global__ void k_fill_double(double* p, size_t n) {
 }
 
 // Gradients of one term from its statistic G (diagonal [L] for Potts/Diagonal, full
-// [L*L] for Matrix): raw compatibility parameters and the term weight.
-__global__ void k_compat_finish(const double* __restrict__ G, int ckind, int L, const float* __restrict__ cw,
-                                const float* __restrict__ kw, float* __restrict__ dcompat,
-                                float* __restrict__ dkw) {
+// [L*L] for Matrix): raw compatibility parameters and the term weight.  One block of
+// 256 threads; the term-weight sum is strided over the threads and added by a fixed
+// tree (deterministic).
+__global__ void __launch_bounds__(256) k_compat_finish(const double* __restrict__ G, int ckind, int L,
+                                                        const float* __restrict__ cw, const float* __restrict__ kw,
+                                                        float* __restrict__ dcompat, float* __restrict__ dkw) {
+    __shared__ double red[256];
     const int t = threadIdx.x;
     const double w = kw ? (double)kw[0] : 1.0;
-    if (ckind == DCRF_COMPAT_POTTS) {
-        if (t == 0) {
-            double s = 0.0;
-            for (int a = 0; a < L; a++) s += G[a];
-            if (dcompat) dcompat[0] = (float)(-w * s);
-            if (dkw) dkw[0] = (float)(-(double)cw[0] * s);
-        }
-    } else if (ckind == DCRF_COMPAT_DIAGONAL) {
-        for (int a = t; a < L; a += blockDim.x)
-            if (dcompat) dcompat[a] = (float)(w * G[a]);
-        if (t == 0 && dkw) {
-            double s = 0.0;
-            for (int a = 0; a < L; a++) s += (double)cw[a] * G[a];
-            dkw[0] = (float)s;
-        }
-    } else {
-        for (int j = t; j < L * L; j += blockDim.x) {
+    const int n = ckind == DCRF_COMPAT_MATRIX ? L * L : L;
+    double s = 0.0;
+    for (int j = t; j < n; j += blockDim.x) {
+        if (ckind == DCRF_COMPAT_POTTS) {
+            s += G[j];
+        } else if (ckind == DCRF_COMPAT_DIAGONAL) {
+            if (dcompat) dcompat[j] = (float)(w * G[j]);
+            s += (double)cw[j] * G[j];
+        } else {
             const int a = j / L, c = j - a * L;
             if (dcompat) dcompat[j] = (float)(w * 0.5 * (G[a * L + c] + G[c * L + a]));
+            s += (double)cw[j] * G[j];
         }
-        if (t == 0 && dkw) {
-            double s = 0.0;
-            for (int j = 0; j < L * L; j++) s += (double)cw[j] * G[j];
-            dkw[0] = (float)s;
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (ckind == DCRF_COMPAT_POTTS) {
+            if (dcompat) dcompat[0] = (float)(-w * red[0]);
+            if (dkw) dkw[0] = (float)(-(double)cw[0] * red[0]);
+        } else if (dkw) {
+            dkw[0] = (float)red[0];
         }
     }
 }
@@ -325,14 +343,14 @@ void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const Sli
     SliceTable st = st_in;
     const int cwf = assign_compat_lds(st, L);
     const LaunchShape sh = shape_for(L);
-    DCRF_DISPATCH_CWL(sh, cwf, k_adjoint, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, Q, b, dU,
-                      N, L, sh.S, sh.per, st);
+    const size_t lds =
+        sizeof(float) * ((size_t)sh.block + cwf + stat_lds_floats(sh.wave, st.nstat, L, sh.block, sh.R));
+    DCRF_DISPATCH_CWL(sh, cwf, k_adjoint, blocks_for(N, sh.items), lds, s, Q, b, dU, N, L, sh.S, sh.per, st, cwf);
 }
 
-int grad_partial_blocks(int N, int L) {
+int adjoint_blocks(int N, int L) {
     const LaunchShape sh = shape_for(L);
-    const int need = (N + sh.items - 1) / sh.items;
-    return need < 1024 ? need : 1024;
+    return (int)blocks_for(N, sh.items);
 }
 
 namespace {
@@ -351,29 +369,29 @@ void stat_ranges(int N, int L, int iters, int* ppw_out, long long* ranges_out) {
 }
 }  // namespace
 
-size_t grad_partial_floats(int N, int L, int iters, bool full) {
-    if (!full) return (size_t)iters * grad_partial_blocks(N, L) * L;
+size_t grad_partial_floats(int N, int L, int iters) {
     int ppw;
     long long ranges;
     stat_ranges(N, L, iters, &ppw, &ranges);
     return (size_t)iters * ranges * L * L;
 }
 
+size_t rows_mid_doubles(int cols) { return (size_t)kRowBlocks * cols; }
+
+void launch_sum_rows(const float* part, long long rows, int cols, double* mid, double* acc, hipStream_t s) {
+    if (rows < 1 || cols < 1) return;
+    const long long nb = std::max<long long>(1, std::min<long long>(kRowBlocks, rows / kRowsMin));
+    const long long rpb = (rows + nb - 1) / nb;
+    const int nbu = (int)((rows + rpb - 1) / rpb);  // blocks with rows
+    k_rows_stage1<<<nbu, 256, 0, s>>>(part, rows, cols, rpb, mid);
+    DCRF_HIP_CHECK(hipGetLastError());
+    k_rows_stage2<<<cols, 256, 0, s>>>(mid, nbu, cols, acc);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
 void launch_grad_stat(const float* b, size_t bstride, const float* M, size_t mstride, int iters, int N, int L,
-                      bool full, float* partial, double* acc, hipStream_t s) {
+                      float* partial, double* mid, double* acc, hipStream_t s) {
     if (iters < 1) return;
-    if (!full) {
-        const int nb = grad_partial_blocks(N, L);
-        const LaunchShape sh = shape_for(L);
-        DCRF_DISPATCH(sh, k_grad_diag, (unsigned)iters * nb, sizeof(float) * sh.block * sh.R, s, b, M, N, L, sh.S,
-                      sh.per, nb, bstride, mstride, partial);
-        const long long nblk = (long long)iters * nb;
-        const int threads = nblk >= 256 ? 256 : (nblk >= 64 ? 64 : 32);
-        k_reduce_cols<<<L, threads, 0, s>>>(partial, (int)nblk, L, acc, 0);
-        DCRF_HIP_CHECK(hipGetLastError());
-        return;
-    }
-    const long long LL = (long long)L * L;
     int ppw;
     long long ranges;
     stat_ranges(N, L, iters, &ppw, &ranges);
@@ -384,10 +402,7 @@ void launch_grad_stat(const float* b, size_t bstride, const float* M, size_t mst
     k_grad_mfma<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(b, M, N, L, tiles1, ppw, (int)ranges, bstride, mstride,
                                                             (int)waves, partial);
     DCRF_HIP_CHECK(hipGetLastError());
-    const long long nblk = (long long)iters * ranges;
-    const int threads = nblk >= 256 ? 256 : (nblk >= 64 ? 64 : 32);
-    k_reduce_cols<<<(unsigned)LL, threads, 0, s>>>(partial, (int)nblk, (int)LL, acc, 0);
-    DCRF_HIP_CHECK(hipGetLastError());
+    launch_sum_rows(partial, (long long)iters * ranges, L * L, mid, acc, s);
 }
 
 void launch_fill_double(double* p, size_t n, hipStream_t s) {
@@ -398,7 +413,7 @@ void launch_fill_double(double* p, size_t n, hipStream_t s) {
 
 void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
                           float* dkw, hipStream_t s) {
-    k_compat_finish<<<1, kThreads, 0, s>>>(G, ckind, L, cw, kw, dcompat, dkw);
+    k_compat_finish<<<1, 256, 0, s>>>(G, ckind, L, cw, kw, dcompat, dkw);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

@@ -126,6 +126,7 @@ struct dcrf_model {
     int train_n = -1;
     const float* train_kw = nullptr;
     DevBuf<float> qcache, mcache, bbuf, partial;
+    DevBuf<double> rmid;  // first-stage sums of launch_sum_rows
     DevBuf<double> gacc, gacc2;
     // objective on the device: ground truth, Hamming's class weights, partial sums
     DevBuf<int16_t> gt_dev;
@@ -1078,7 +1079,13 @@ void run_inference(dcrf_model& m, int n_iter, const float* U, float* Q, hipStrea
     }
 }
 
-// Forward pass that keeps Q_0..Q_n and the raw messages for the adjoint.
+int matrix_terms(const dcrf_model& m) {
+    int nm = 0;
+    for (const auto& t : m.terms) nm += t->ckind == DCRF_COMPAT_MATRIX;
+    return nm;
+}
+
+// Forward pass that keeps Q_0..Q_n and the Matrix terms' raw messages for the adjoint.
 void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compat_raw, const float* kw,
                    hipStream_t s) {
     const int N = m.N(), L = m.L, T = (int)m.terms.size();
@@ -1091,13 +1098,19 @@ void forward_train(dcrf_model& m, int n_iter, const float* U, const float* compa
         }
     }
     m.qcache.reserve((size_t)(n_iter + 1) * NL);
-    m.mcache.reserve(std::max<size_t>((size_t)n_iter * T * NL, 1));
+    // raw messages of the Matrix terms only (their statistic is a full L x L product
+    // taken after the adjoint loop); the others' statistic comes from the adjoint kernels
+    const int nm = matrix_terms(m);
+    m.mcache.reserve(std::max<size_t>((size_t)n_iter * nm * NL, 1));
     launch_softmax_neg(U, m.qcache.get(), N, L, s);
     for (int it = 0; it < n_iter; it++) {
         const float* Qt = m.qcache.get() + (size_t)it * NL;
         float* Qn = m.qcache.get() + (size_t)(it + 1) * NL;
         run_filter(m, Qt, L, false, s);
-        const SliceTable st = slice_table(m, false, kw, m.mcache.get() + (size_t)it * T * NL);
+        SliceTable st = slice_table(m, false, kw);
+        for (int k = 0, j = 0; k < T; k++)
+            if (m.terms[k]->ckind == DCRF_COMPAT_MATRIX)
+                st.t[k].mout = m.mcache.get() + ((size_t)it * nm + j++) * NL;
         if (m.fused)  // same arithmetic as k_meanfield, gathers of all entries up front
             launch_meanfield_fused(U, Qn, L, N, L, st, m.g_slc.get(), s);
         else
@@ -1136,19 +1149,26 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     if (m.train_n < 0) throw Error{DCRF_ERR_STATE, "backward needs a preceding training forward"};
     const int N = m.N(), L = m.L, T = (int)m.terms.size(), n = m.train_n;
     const size_t NL = (size_t)N * L;
-    // b_t, the adjoint entering reverse iteration t - 1, in slot t (t = n .. 0): every
-    // iteration's b stays for the compatibility statistics, taken once at the end
-    m.bbuf.reserve((size_t)(n + 1) * NL);
-    auto bslot = [&](int t) { return m.bbuf.get() + (size_t)t * NL; };
+    // b_t, the adjoint entering reverse iteration t - 1, in slot t (t = n .. 0).  The
+    // Matrix terms' statistic reads every b_t after the loop, so with such a term every
+    // slot stays; otherwise two slots alternate.
+    const int nm = matrix_terms(m);
+    m.bbuf.reserve((size_t)(nm ? n + 1 : std::min(n + 1, 2)) * NL);
+    auto bslot = [&](int t) { return m.bbuf.get() + (size_t)(nm ? t : (t & 1)) * NL; };
     const float* Qc = m.qcache.get();
     launch_sn_first(Qc + (size_t)n * NL, dQ, bslot(n), dU, N, L, s);
     std::vector<size_t> goff(T + 1, 0);
     for (int k = 0; k < T; k++) goff[k + 1] = goff[k] + stat_len(m.terms[k]->ckind, L);
     m.gacc.reserve(std::max<size_t>(goff[T], 1));
     launch_fill_double(m.gacc.get(), goff[T], s);
-    size_t pf = 1;
-    for (auto& t : m.terms) pf = std::max(pf, grad_partial_floats(N, L, n, t->ckind == DCRF_COMPAT_MATRIX));
+    // the other terms' statistic rows: term j (in term order among them), iteration it
+    // at partial + (j * n + it) * nblk * L, one row of L sums per adjoint block
+    const int nd = T - nm;
+    const size_t nblk = (size_t)adjoint_blocks(N, L);
+    size_t pf = std::max<size_t>((size_t)nd * n * nblk * L, 1);
+    if (nm) pf = std::max(pf, grad_partial_floats(N, L, n));
     m.partial.reserve(pf);
+    m.rmid.reserve(rows_mid_doubles(nm ? L * L : L));
     std::vector<int> koff(T + 1, 0);
     for (int k = 0; k < T; k++) koff[k + 1] = koff[k] + m.terms[k]->num_kparams();
     kernel = kernel && koff[T] > 0;
@@ -1172,17 +1192,34 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
                                      m.kacc.get() + koff[k], s);
         }
         run_filter(m, b, L, true, s);
+        SliceTable st = slice_table(m, true, m.train_kw);
+        for (int k = 0, j = 0; k < T; k++)
+            if (m.terms[k]->ckind != DCRF_COMPAT_MATRIX) {
+                st.t[k].gstat = m.partial.get() + ((size_t)j * n + it) * nblk * L;
+                st.t[k].gslot = j++;
+                st.nstat = j;
+            }
         if (m.fused)
-            launch_adjoint_fused(Qc + (size_t)it * NL, bslot(it), dU, N, L, slice_table(m, true, m.train_kw),
-                                 adjoint_pairs(m, s), s);
+            launch_adjoint_fused(Qc + (size_t)it * NL, bslot(it), dU, N, L, st, adjoint_pairs(m, s), s);
         else
-            launch_adjoint(Qc + (size_t)it * NL, bslot(it), dU, N, L, slice_table(m, true, m.train_kw), s);
+            launch_adjoint(Qc + (size_t)it * NL, bslot(it), dU, N, L, st, s);
     }
-    // G_k = sum_t b_{t+1} (K~_k Q_t)^T over all iterations: b_{t+1} in slot t + 1, the
-    // messages of iteration t at mcache[t][k]
-    for (int k = 0; k < T; k++)
-        launch_grad_stat(bslot(1), NL, m.mcache.get() + (size_t)k * NL, (size_t)T * NL, n, N, L,
-                         m.terms[k]->ckind == DCRF_COMPAT_MATRIX, m.partial.get(), m.gacc.get() + goff[k], s);
+    // G_k = sum_t b_{t+1} (K~_k Q_t)^T over all iterations: the statistic rows of the
+    // Potts / Diagonal terms first (they share the partial buffer with the Matrix
+    // terms' products), then for Matrix terms b_{t+1} in slot t + 1 against the messages
+    // of iteration t
+    for (int k = 0, j = 0; k < T; k++)
+        if (m.terms[k]->ckind != DCRF_COMPAT_MATRIX) {
+            launch_sum_rows(m.partial.get() + (size_t)j * n * nblk * L, (long long)n * nblk, L, m.rmid.get(),
+                            m.gacc.get() + goff[k], s);
+            j++;
+        }
+    for (int k = 0, j = 0; k < T; k++)
+        if (m.terms[k]->ckind == DCRF_COMPAT_MATRIX) {
+            launch_grad_stat(bslot(1), NL, m.mcache.get() + (size_t)j * NL, (size_t)nm * NL, n, N, L,
+                             m.partial.get(), m.rmid.get(), m.gacc.get() + goff[k], s);
+            j++;
+        }
     if (dcompat || dkw) {
         size_t coff = 0;
         for (int k = 0; k < T; k++) {

@@ -204,6 +204,66 @@ struct LaneGroup {
     }
 };
 
+// The adjoint kernels' per-block compatibility statistic (SliceTerm::gstat): the sum of
+// each label's products over the block's groups, in a fixed order.  stat_put leaves the
+// thread's R products (0 in lanes without a pixel) in slot `slot` of the region sst
+// (blockDim.x * R floats per slot).  stat_finish is called by every thread of the block
+// after all slots are put, at the kernel's end:
+//   WAVE: after a barrier, one thread per (slot, wave, label) adds the wave's groups in
+//   group order into the wave's row of L floats (after the slots), then, after a second
+//   barrier, the block's waves are added in wave order;
+//   block mode: after a barrier, the block's groups are added in group order.
+// It writes the sums to the term's row of this block.
+template <int R>
+__device__ __forceinline__ void stat_put(float* sst, int slot, const float (&v)[R]) {
+    float* d = sst + ((size_t)slot * blockDim.x + threadIdx.x) * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r] = v[r];
+}
+
+// NT: the terms to look the slots up in (st.t[0 .. NT-1]).
+template <bool WAVE, int R, int NT = DCRF_MAX_TERMS>
+__device__ __forceinline__ void stat_finish(const SliceTable& st, const LaneGroup<WAVE, R>& G, float* sst) {
+    const int L = G.L;
+    float* wrow = sst + (size_t)st.nstat * blockDim.x * R;  // WAVE: [slot][wave][L]
+    __syncthreads();
+    if (WAVE) {  // one thread per (slot, wave, label): the wave's groups in group order
+        constexpr int W = kWaveBlock / 64;
+        for (int j = threadIdx.x; j < st.nstat * W * L; j += blockDim.x) {
+            const int slot = j / (W * L), rem = j - slot * (W * L);
+            const int w = rem / L, lab = rem - w * L;
+            const int ln = lab / R, rr = lab - ln * R;
+            const float* src = sst + (size_t)slot * blockDim.x * R + (size_t)(w * 64 + ln) * R + rr;
+            float s = 0.0f;
+            for (int gi = 0; gi < G.per; gi++) s = s + src[(size_t)gi * G.S * R];
+            wrow[j] = s;
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < st.nstat * L; j += blockDim.x) {
+        const int slot = j / L, lab = j - slot * L;
+        float s = 0.0f;
+        if (WAVE) {
+            const float* src = wrow + (size_t)slot * (kWaveBlock / 64) * L + lab;
+            for (int w = 0; w < kWaveBlock / 64; w++) s = s + src[(size_t)w * L];
+        } else {
+            const int ln = lab / R, rr = lab - ln * R;  // lane and register holding label lab
+            const float* src = sst + (size_t)slot * blockDim.x * R;
+            for (int gi = 0; gi < G.per; gi++) s = s + src[(gi * G.S + ln) * R + rr];
+        }
+        float* out = nullptr;
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+            if (k < st.nterms && st.t[k].gslot == slot) out = st.t[k].gstat;
+        out[(size_t)blockIdx.x * L + lab] = s;
+    }
+}
+
+// Floats of the statistic region for nstat terms.
+__host__ __device__ inline size_t stat_lds_floats(bool wave, int nstat, int L, int block, int R) {
+    return (size_t)nstat * block * R + (wave ? (size_t)nstat * (kWaveBlock / 64) * L : 0);
+}
+
 // Slice of term T for pixel p: m[r] = sum_j sw_j * vals[vtx_j][k0 + r] in j order.  The
 // (vertex, weight) pairs are loaded first, then all vertex rows, so the d+1 gathers are
 // in flight together.

@@ -244,10 +244,16 @@ struct SliceTerm {
     int cw_lds = -1;    // Matrix: offset (floats) of the LDS copy of cw in the kernel's compat
                         // region, or -1 to read cw from global memory (set by the launchers)
     int ldw = 0;        // row stride (floats) of that LDS copy: L, or L rounded up to 4
+    // Adjoint kernels: the Potts / Diagonal statistic sum_p (K~^T b)[p] .* Q[p] of this
+    // term, one row of L float sums per block at gstat[block * L] (null: none), summed
+    // through LDS slot gslot of the kernel's statistic region.
+    float* gstat = nullptr;
+    int gslot = -1;
 };
 struct SliceTable {
     int nterms;
     SliceTerm t[DCRF_MAX_TERMS];
+    int nstat = 0;  // terms with a statistic (gslot 0 .. nstat-1)
 };
 // Give each Matrix term an LDS copy of its L x L compatibility (cw_lds) while they fit
 // kCompatLdsFloats in total; returns the floats of the compat LDS region.  pad: rows of
@@ -361,19 +367,25 @@ void launch_transpose(const float* in, float* out, int rows, int cols, hipStream
 // b = sumAndNormalize(Q .* dQ, Q); dU = b.
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s);
 // b <- sumAndNormalize(Q .* sum_k w_k C_k^T slice^T_k, Q) (b is written, not read); dU += b.
+// Terms with st.t[k].gstat also get their statistic rows (see SliceTerm).
 void launch_adjoint(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, hipStream_t s);
 // The same over interleaved slice pairs (those of launch_meanfield_fused, with the
 // transposed filter's slice weights); models that fused_meanfield_ok accepts.
 void launch_adjoint_fused(const float* Q, float* b, float* dU, int N, int L, const SliceTable& st, const int2* slc,
                           hipStream_t s);
-// acc += sum_{t < iters} sum_p b_t[p] M_t[p]^T (full L*L when `full`, else its diagonal
-// [L]), with b_t = b + t * bstride and M_t = M + t * mstride; one launch and one
-// reduction for all iterations.  partial holds grad_partial_floats(N, L, iters, full)
-// floats.
-int grad_partial_blocks(int N, int L);
-size_t grad_partial_floats(int N, int L, int iters, bool full);
+// Blocks of one adjoint launch over N pixels (the rows of its statistic output).
+int adjoint_blocks(int N, int L);
+// acc += sum_{t < iters} sum_p b_t[p] M_t[p]^T (L x L), with b_t = b + t * bstride and
+// M_t = M + t * mstride, on the matrix cores; one launch and one reduction for all
+// iterations.  partial holds grad_partial_floats(N, L, iters) floats, mid
+// rows_mid_doubles(L * L) doubles.
+size_t grad_partial_floats(int N, int L, int iters);
 void launch_grad_stat(const float* b, size_t bstride, const float* M, size_t mstride, int iters, int N, int L,
-                      bool full, float* partial, double* acc, hipStream_t s);
+                      float* partial, double* mid, double* acc, hipStream_t s);
+// acc[j] += sum_{r < rows} part[r * cols + j] for j < cols, in double, in a fixed order
+// (two kernels; mid holds rows_mid_doubles(cols) doubles).
+size_t rows_mid_doubles(int cols);
+void launch_sum_rows(const float* part, long long rows, int cols, double* mid, double* acc, hipStream_t s);
 void launch_fill_double(double* p, size_t n, hipStream_t s);
 void launch_compat_finish(const double* G, int ckind, int L, const float* cw, const float* kw, float* dcompat,
                           float* dkw, hipStream_t s);

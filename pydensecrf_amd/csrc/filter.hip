@@ -647,9 +647,9 @@ k_meanfield_fused(const float* __restrict__ U, float* __restrict__ Q,
 }
 
 // The adjoint step of k_adjoint (adjoint.hip) over the same interleaved slice pairs:
-// g = sum_k C_k(w_k slice_k(filtered b)), b = sumAndNormalize(Q .* g, Q), dU += b.  The
-// pairs carry the transposed filter's slice weights.  Same arithmetic, in the same
-// order, as k_adjoint.
+// g = sum_k C_k(w_k slice_k(filtered b)), b = sumAndNormalize(Q .* g, Q), dU += b, and
+// the terms' statistic rows (SliceTerm::gstat).  The pairs carry the transposed filter's
+// slice weights.  Same arithmetic, in the same order, as k_adjoint.
 template <int R, int D0, int D1, bool CWL>
 __global__ void __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __restrict__ dU, int N, int L, int S,
@@ -680,6 +680,16 @@ k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __res
     float raw[NT][R], mk[NT][R], c[NT][R];  // (the adjoint stores no raw messages)
 #pragma unroll
     for (int k = 0; k < NT; k++) term_message<R, D0, D1>(st.t[k], k, e, x, raw[k], mk[k]);
+    // statistic region after the message rows (present with a compat region)
+    float* sst = shc + cwf + (cwf ? (blockDim.x >> 6) * NT * per * ((L + 3) & ~3) : 0);
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+        if (st.t[k].gslot >= 0) {
+            float v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = active ? raw[k][r] * q[r] : 0.0f;
+            stat_put<R>(sst, st.t[k].gslot, v);
+        }
     float* msg = shc + cwf + (size_t)(threadIdx.x >> 6) * NT * per * ((L + 3) & ~3);
     compat_terms<R, NT, CWL>(G, st, mk, c, shc, msg);
     float gsum[R];
@@ -699,6 +709,7 @@ k_adjoint_fused(const float* __restrict__ Q, float* __restrict__ b, float* __res
         store_piece<R>(b + (size_t)p * L, k0, L, bn);
         store_piece<R>(dU + (size_t)p * L, k0, L, du);
     }
+    if (st.nstat) stat_finish<true, R, NT>(st, G, sst);  // uniform condition
 }
 
 // The same update over pixel runs (2-D rectangles whose rows hold whole waves of
@@ -1109,11 +1120,13 @@ void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int
 }
 
 // dynamic LDS of the fused kernels: the wave's slice pairs, the compat region (cwf
-// floats) and, with a compat region, the message rows (NT per group, (L + 3) & ~3 floats)
-inline size_t fused_lds_bytes(const LaunchShape& sh, int ds, int nt, int L, int cwf) {
+// floats), with a compat region the message rows (NT per group, (L + 3) & ~3 floats),
+// and the adjoint's statistic region (stat_lds_floats)
+inline size_t fused_lds_bytes(const LaunchShape& sh, int ds, int nt, int L, int cwf, int nstat = 0) {
     const size_t waves = kWaveBlock / 64;
     return waves * sh.per * ds * sizeof(int2) + (size_t)cwf * sizeof(float) +
-           (cwf ? waves * nt * sh.per * (size_t)((L + 3) & ~3) * sizeof(float) : 0);
+           (cwf ? waves * nt * sh.per * (size_t)((L + 3) & ~3) * sizeof(float) : 0) +
+           stat_lds_floats(true, nstat, L, kWaveBlock, sh.R) * sizeof(float);
 }
 
 #define DCRF_FUSED(R_, D0_, D1_)                                                                          \
@@ -1136,8 +1149,8 @@ void launch_fused_r(const LaunchShape& sh, const float* U, float* Q, int ldq, in
 
 #define DCRF_ADJ_FUSED(R_, D0_, D1_)                                                                      \
     (cwf ? k_adjoint_fused<R_, D0_, D1_, true> : k_adjoint_fused<R_, D0_, D1_, false>)<<<                   \
-        blocks_for(N, sh.items), kWaveBlock, fused_lds_bytes(sh, D0_ + D1_, D1_ ? 2 : 1, L, cwf), s>>>(     \
-        Q, b, dU, N, L, sh.S, sh.per, st, slc, cwf)
+        blocks_for(N, sh.items), kWaveBlock, fused_lds_bytes(sh, D0_ + D1_, D1_ ? 2 : 1, L, cwf, st.nstat),  \
+        s>>>(Q, b, dU, N, L, sh.S, sh.per, st, slc, cwf)
 
 template <int R>
 void launch_adj_fused_r(const LaunchShape& sh, const float* Q, float* b, float* dU, int N, int L,

@@ -240,6 +240,45 @@ def test_training_forward_fused_equals_generic(L, terms, nt):
         assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
+@pytest.mark.parametrize("L,nt,ck", [(21, capi.NORMALIZE_SYMMETRIC, "potts"), (21, capi.NORMALIZE_BEFORE, "diag"),
+                                      (4, capi.NORMALIZE_AFTER, "mixed"), (1, capi.NORMALIZE_SYMMETRIC, "diag")])
+def test_adjoint_statistic_fused_equals_generic(L, nt, ck):
+    """Potts and Diagonal terms take their compatibility statistic inside the adjoint
+    kernels, as sum_p (K~^T b)[p] .* Q[p], per block: the fused and the generic adjoint
+    write the same block rows in the same order, so the gradients are bit-identical
+    (with a Matrix term beside, "mixed", whose statistic comes from the message cache)."""
+    W, H, n = 37, 23, 3
+    img, U, gt = image_case(W, H, L, seed=200 + L)
+    rng = np.random.default_rng(L + 7)
+    if ck == "potts":
+        c1, c2 = 3.0, 10.0
+    elif ck == "diag":
+        c1 = rng.uniform(1, 4, L).astype(np.float32)
+        c2 = rng.uniform(2, 8, L).astype(np.float32)
+    else:
+        c1 = rng.uniform(1, 4, L).astype(np.float32)
+        c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    outs = []
+    for mode in (1, 0):
+        g = capi.Model.create_2d(W, H, L)
+        g.set_unary(U)
+        g.add_gaussian(3.0, 3.0, c1, ntype=nt)
+        g.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c2, ntype=nt)
+        g.set_meanfield_kernel(mode)
+        outs.append(g.gradient(n, gt))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    # against the oracle's reference-order statistic b . (K~ Q)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    o.set_unary(U)
+    o.add_gaussian(3.0, 3.0, c1, ntype=nt)
+    o.add_bilateral(20.0, 20.0, 13.0, 13.0, 13.0, img, c2, ntype=nt)
+    r, ugo, cgo = o.gradient(n, gt)
+    assert abs(outs[0][0] - r) <= 1e-5 * abs(r)
+    assert _rel_close(outs[0][1], ugo, 1e-4)
+    assert _rel_close(outs[0][2], cgo, 1e-4)
+
+
 # ---------------------------------------------------------------- kernel parameters
 # DenseCRF::gradient's kernel_grad (pairwise.cpp:152-163, 202-207, permutohedral.cpp:
 # 612-695) and kernelParameters / setKernelParameters (densecrf.cpp:334-360).  The GPU

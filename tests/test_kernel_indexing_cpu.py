@@ -13,7 +13,12 @@ use (the one-label fixtures included: one lane per pixel, 64 pixels per wave):
   inside its buffer, for idle lanes and whole idle waves too (they read clamped,
   in-bounds addresses and write nothing);
 * every LDS access stays inside the wave's slice of the dynamic LDS the launcher asks
-  for (fused_lds_bytes), the Matrix message rows included.
+  for (fused_lds_bytes), the Matrix message rows and the adjoint's statistic slots
+  included.
+
+It also restates launch_sum_rows's first kernel (adjoint.hip), which adds the adjoint's
+per-block statistic rows and the matrix-core statistic's tiles: every entry is read
+exactly once and every write stays inside its buffer.
 """
 import numpy as np
 import pytest
@@ -34,9 +39,10 @@ def xcd_block(b, nb):
     return x * q + np.minimum(x, r) + i
 
 
-def emulate(N, L, DS, NT, matrix_terms=0, ldq=None):
+def emulate(N, L, DS, NT, matrix_terms=0, ldq=None, nstat=0):
     """Check every index of one fused launch over N pixels, L labels, DS slice pairs per
-    pixel (NT terms, matrix_terms of them Matrix with an LDS weight copy)."""
+    pixel (NT terms, matrix_terms of them Matrix with an LDS weight copy; the adjoint
+    with nstat terms writing statistic rows)."""
     R = labels_per_lane(L)
     S = (L + R - 1) // R
     assert S <= 64, "the fused kernels run in wave mode only"
@@ -85,16 +91,42 @@ def emulate(N, L, DS, NT, matrix_terms=0, ldq=None):
     # each active group reads its own DS pairs from the wave's LDS slice
     own = np.broadcast_to(g[None, :] * DS, active.shape)[active]
     assert own.max(initial=0) + DS - 1 < per * DS
-    # LDS: pairs of all waves, the Matrix weight copies, the message rows
+    # LDS: pairs of all waves, the Matrix weight copies, the message rows, the
+    # adjoint's statistic slots
     ldm = (L + 3) & ~3
     cwf = matrix_terms * L * ldm
-    lds_bytes = (WAVE_BLOCK // 64) * per * DS * 8 + cwf * 4 + (
-        (WAVE_BLOCK // 64) * NT * per * ldm * 4 if cwf else 0)
+    waves = WAVE_BLOCK // 64
+    stat_floats = nstat * WAVE_BLOCK * R + nstat * waves * L  # stat_lds_floats (wave mode)
+    lds_bytes = waves * per * DS * 8 + cwf * 4 + (waves * NT * per * ldm * 4 if cwf else 0) + stat_floats * 4
     assert lds_bytes <= 160 * 1024, "more LDS than a workgroup has"
+    msg0 = waves * per * DS * 2 + cwf  # floats before the message rows
     if cwf:
-        msg0 = (WAVE_BLOCK // 64) * per * DS * 2 + cwf  # floats before the message rows
         rows = [msg0 + wave[on] * NT * per * ldm + (k * per + g[on]) * ldm + k0[on] + R - 1 for k in range(NT)]
         assert max(r.max() for r in rows) < lds_bytes // 4
+    if nstat:
+        sst = msg0 + (waves * NT * per * ldm if cwf else 0)
+        assert sst + stat_floats == lds_bytes // 4
+        # stat_put: thread t's R products of slot k at (k * 256 + t) * R
+        put = (np.arange(nstat)[:, None, None] * WAVE_BLOCK + t[None, :, None]) * R + np.arange(R)
+        assert put.max() < nstat * WAVE_BLOCK * R
+        # stat_finish, wave sums: thread j = (slot, w, lab) reads lane w*64 + gi*S + ln,
+        # register rr, for gi < per; that lane (group gi) holds label ln*R + rr = lab
+        j = np.arange(nstat * waves * L)
+        slot, rem = j // (waves * L), j % (waves * L)
+        w, lab = rem // L, rem % L
+        ln, rr = lab // R, lab % R
+        gi = np.arange(per)
+        lane_read = w[:, None] * 64 + gi[None, :] * S + ln[:, None]
+        src = slot[:, None] * WAVE_BLOCK * R + lane_read * R + rr[:, None]
+        assert src.min() >= 0 and src.max() < nstat * WAVE_BLOCK * R
+        assert np.all((lane_read % 64) // S == gi[None, :]) and np.all((lane_read % 64) % S * R + rr[:, None] == lab[:, None])
+        # the wave rows (written at j, after the slots) and the final sums' reads
+        assert j.max() < nstat * waves * L
+        fin = np.arange(nstat * L)
+        rd = (fin // L)[:, None] * waves * L + np.arange(waves)[None, :] * L + (fin % L)[:, None]
+        assert rd.max() < nstat * waves * L
+        # statistic rows: block b writes gstat[b * L + lab], one row per block of the launch
+        assert (grid - 1) * L + L - 1 < grid * L
     return grid, per
 
 
@@ -115,6 +147,14 @@ def test_fused_kernel_indices_matrix_rows(N, L):
     emulate(N, L, 9, 2, matrix_terms=2)
 
 
+@pytest.mark.parametrize("N,L", [(17 * 9, 1), (13 * 8, 4), (24 * 20, 21), (1024 * 768, 21), (700, 64)])
+@pytest.mark.parametrize("matrix_terms,nstat", [(0, 2), (1, 1), (0, 1)])
+def test_adjoint_statistic_slots(N, L, matrix_terms, nstat):
+    """The fused adjoint's statistic slots (after the message rows) and the rows it
+    writes per block."""
+    emulate(N, L, 9, 2, matrix_terms=matrix_terms, nstat=nstat)
+
+
 def test_padded_q_stride():
     """The inference's intermediate Q with a padded row stride (ldq > L)."""
     emulate(1024 * 768, 21, 9, 2, ldq=24)
@@ -124,3 +164,38 @@ def test_xcd_block_bijection_all_grids():
     for nb in range(1, 3000):
         b = np.arange(nb)
         assert np.array_equal(np.sort(xcd_block(b, nb)), b), nb
+
+
+def sum_rows_reads(rows, cols, row_blocks=256):
+    """(row, column) pairs read by launch_sum_rows's first kernel (k_rows_stage1, adjoint.hip),
+    and the mid entries it writes; asserts every read and write is inside its buffer."""
+    nb = min(row_blocks, rows)
+    rpb = (rows + nb - 1) // nb
+    nbu = (rows + rpb - 1) // rpb
+    cpb = cols if cols < 256 else 256
+    tpc = 256 // cpb
+    t = np.arange(256)
+    sub = t // cpb
+    reads = np.zeros((rows, cols), np.int64)
+    mids = set()
+    for blk in range(nbu):
+        r0, r1 = blk * rpb, min(rows, blk * rpb + rpb)
+        for c0 in range(0, cols, cpb):
+            c = c0 + t % cpb
+            ok = (sub < tpc) & (c < cols)
+            for tt in t[ok]:
+                rr = np.arange(r0 + sub[tt], r1, tpc)
+                assert rr.size == 0 or (rr.max() * cols + c[tt] < rows * cols)
+                reads[rr, c[tt]] += 1
+            for tt in range(min(cpb, cols - c0)):
+                mids.add(blk * cols + c0 + tt)
+    assert max(mids) < row_blocks * cols  # inside rows_mid_doubles(cols)
+    return reads, mids, nbu
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (7, 3), (2000, 21), (4095, 441), (3, 300), (1000, 256),
+                                       (999, 257), (30000, 42)])
+def test_sum_rows_reads_each_entry_once(rows, cols):
+    reads, mids, nbu = sum_rows_reads(rows, cols)
+    assert np.all(reads == 1)  # every entry once: no row or column dropped or doubled
+    assert mids == set(range(nbu * cols))  # the second kernel reads exactly these
