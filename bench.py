@@ -401,6 +401,8 @@ def main():
                     help="blur passes below BELOW_V vertices with R labels per lane (dcrf_set_blur_lanes)")
     ap.add_argument("--splat-sort", type=int, default=None, metavar="RUN",
                     help="sort the splat segments by length inside runs of RUN segments (dcrf_set_splat_sort)")
+    ap.add_argument("--blur-steps", type=int, default=None, choices=[2, 3],
+                    help="most blur steps per multi-step pass (dcrf_set_blur_steps; default 2)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy three times (a known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -444,6 +446,8 @@ def main():
         m.set_blur_lanes(*args.blur_lanes)
     if args.splat_sort is not None:
         m.set_splat_sort(args.splat_sort)
+    if args.blur_steps is not None:
+        m.set_blur_steps(args.blur_steps)
     t0 = time.perf_counter()
     m.add_gaussian(3.0, 3.0, 3.0)
     t1 = time.perf_counter()
@@ -533,6 +537,7 @@ def main():
                    "launch": "stream-ordered kernels on the torch stream",
                    "q_blocks": bool(args.q_blocks),
                    "blur_lanes": args.blur_lanes,
+                   "blur_steps": args.blur_steps if args.blur_steps is not None else 2,
                    "splat_sort": args.splat_sort,
                    "vertices": {"gaussian": Vg, "bilateral": Vb}},
         "roofline": {

@@ -276,6 +276,11 @@ int dcrf_set_meanfield_kernel(dcrf_model *m, int mode, int run_rows);
  * on (default) blurs two lattice axes per pass through a table of the 9 rows each
  * vertex's two steps read, off runs one pass per axis. */
 int dcrf_set_blur_pairs(dcrf_model *m, int on);
+/* No counterpart in the reference (a scheduling choice; results are bit-identical):
+ * the most blur steps per multi-step pass, 2 (default) or 3 (27 rows per vertex), each
+ * pass one launch over all terms that still have passes.  Applies while
+ * dcrf_set_blur_pairs is on. */
+int dcrf_set_blur_steps(dcrf_model *m, int steps);
 /* Inference only, 2-D single-image models with the fused mean-field kernel: keep Q
  * between iterations in 8 x 8 pixel blocks (the splat then gathers the rows of a
  * vertex's contributors from a few contiguous blocks instead of several image rows).

@@ -73,6 +73,7 @@ SIGNATURES = {
     "dcrf_set_unary_params": (_c, [_P, _F]),
     "dcrf_unary_param_gradient": (_c, [_P, _F]),
     "dcrf_set_blur_pairs": (_c, [_P, _c]),
+    "dcrf_set_blur_steps": (_c, [_P, _c]),
     "dcrf_set_q_blocks": (_c, [_P, _c]),
     "dcrf_set_splat_sort": (_c, [_P, _c]),
     "dcrf_set_blur_lanes": (_c, [_P, _c, _c]),
@@ -432,6 +433,10 @@ class Model:
 
     def set_blur_pairs(self, on):
         check(lib().dcrf_set_blur_pairs(self.h, 1 if on else 0))
+
+    def set_blur_steps(self, steps):
+        """Most blur steps per multi-step pass, 2 (default) or 3 (the same bits for both)."""
+        check(lib().dcrf_set_blur_steps(self.h, int(steps)))
 
     def kernel_params(self):
         n = lib().dcrf_num_kernel_params(self.h)

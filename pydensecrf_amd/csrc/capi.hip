@@ -111,6 +111,7 @@ struct dcrf_model {
     bool runs = false;            // ... or through k_meanfield_runs (rows staged per pixel run)
     int mf_mode = 0;              // dcrf_set_meanfield_kernel: 0 auto (fused), 1 generic, 2 fused, 3 runs
     bool pairs = true;            // blur up to three steps per pass (dcrf_set_blur_pairs)
+    int blur_steps = 2;           // most steps per multi-step pass, 2 or 3 (dcrf_set_blur_steps)
     int n_passes = 0;             // launches per filter with multi-step passes
     std::vector<int> prefixP;     // vertices of the terms with more than p passes
     int run_rows = 16;            // pixel rows of a run rectangle
@@ -411,22 +412,34 @@ void compute_norm(dcrf_model& m, Term& t) {
 // Multi-step blur schedule: a lattice runs its d+1 steps in ceil((d+1)/K) passes of at
 // most K steps, spread as evenly as possible (longer passes first); the number of passes
 // falls with d, so in the global filter pass p covers a prefix of the vertex space.
-// K = 2: passes of three steps (27 rows per vertex) measured slower at 1024x768x21 (49
-// us against 28 us of blur per iteration), as the 27 gathers per vertex cost more than
-// the launch saved.
+// K = 2 by default: passes of three steps (27 rows per vertex) for every lattice
+// measured slower at 1024x768x21 (49 us against 28 us of blur per iteration), as the
+// 27 gathers per vertex of the large Gaussian lattice cost more than the launch saved.
 constexpr int kMaxPassSteps = 2;
-std::vector<int> pass_schedule(int steps) {
-    const int n = (steps + kMaxPassSteps - 1) / kMaxPassSteps;
+std::vector<int> pass_schedule(int steps, int K = kMaxPassSteps) {
+    const int n = (steps + K - 1) / K;
     std::vector<int> out(n, steps / n);
     for (int i = 0; i < steps % n; i++) out[i]++;
     return out;
 }
 
+// Most steps per pass of every term (dcrf_set_blur_steps: 2, the default, or 3).  Each
+// pass is one launch over the terms that still have passes, and a short launch costs
+// ~5 us whatever its work (a pass over the 36 k vertices of the 1024x768 bilateral
+// lattice alone: 5.1-5.5 us), but three-step passes for that lattice only (3 + 3 beside
+// the Gaussian lattice's 2 + 1: two launches instead of three) measured 2 x 25.8 us of
+// blur per iteration against 3 x 12.9 us: its 27-row trees, folded one 9-row subtree
+// after another, make each launch's tail three memory round trips long.
+std::vector<int> pass_step_limits(const dcrf_model& m) {
+    return std::vector<int>(m.terms.size(), m.blur_steps);
+}
+
 // Pass tables of one lattice (forward axes 0..d, reverse d..0) over its neighbour rows
-// nbr [(d+1)][V] (global rows from vbase, zero row zero_row).
+// nbr [(d+1)][V] (global rows from vbase, zero row zero_row), passes of at most K steps.
 void build_pass_tables(const int2* nbr, int V, int d, int vbase, int zero_row, std::vector<int>& steps,
-                       std::vector<int>& off, DevBuf<int>& fwd, DevBuf<int>& rev, hipStream_t s) {
-    steps = pass_schedule(d + 1);
+                       std::vector<int>& off, DevBuf<int>& fwd, DevBuf<int>& rev, hipStream_t s,
+                       int K = kMaxPassSteps) {
+    steps = pass_schedule(d + 1, K);
     off.assign(1, 0);
     for (int q : steps) off.push_back(off.back() + V * (q == 1 ? 3 : q == 2 ? 9 : 27));
     fwd.reserve(off.back());
@@ -532,6 +545,7 @@ float* raw_blur(const Term& t, const float* src, float* const buf[2], int Lc, bo
         PassTable pt{};
         pt.nterms = 1;
         pt.t[0] = PassTerm{0, V, (reverse ? P.passT.get() : P.passF.get()) + P.pass_off[p], P.pass_steps[p]};
+        pt.zero_row = V;  // zero in every buffer of the plan
         launch_blur_pass(in, buf[p & 1], 0, V, Lc, Lc, pt, s);
         in = buf[p & 1];
     }
@@ -671,9 +685,10 @@ int result_parity(const dcrf_model& m, int k) {
     return (m.pairs ? (int)m.tl[k]->pass_steps.size() : m.terms[k]->lat.d + 1) & 1;
 }
 
+// Launches per filter with multi-step passes (after the terms' pass tables exist).
 int blur_launches(const dcrf_model& m) {
     int P = 0;
-    for (const auto& t : m.terms) P = std::max(P, (t->lat.d + kMaxPassSteps) / kMaxPassSteps);
+    for (const auto& tl : m.tl) P = std::max(P, (int)tl->pass_steps.size());
     return P;
 }
 
@@ -686,10 +701,17 @@ void ensure_layout(dcrf_model& m) {
     const int N = m.N(), L = m.L;
     for (auto& t : m.terms)
         if (t->lat.chunk_len != m.chunk()) build_chunks(t->lat, m.chunk());
+    // terms in layout order: more blur passes first (then higher d), so that the terms
+    // of pass p are a prefix of the vertex space
+    const std::vector<int> step_limit = pass_step_limits(m);
+    std::vector<int> npass(T);
+    for (int k = 0; k < T; k++) npass[k] = (int)pass_schedule(m.terms[k]->lat.d + 1, step_limit[k]).size();
     m.order.resize(T);
     std::iota(m.order.begin(), m.order.end(), 0);
-    std::stable_sort(m.order.begin(), m.order.end(),
-                     [&](int a, int b) { return m.terms[a]->lat.d > m.terms[b]->lat.d; });
+    std::stable_sort(m.order.begin(), m.order.end(), [&](int a, int b) {
+        if (npass[a] != npass[b]) return npass[a] > npass[b];
+        return m.terms[a]->lat.d > m.terms[b]->lat.d;
+    });
     m.vbase.assign(T, 0);
     std::vector<int> pbase(T, 0);
     int V = 0, P = 0, E = 0;
@@ -846,7 +868,7 @@ void ensure_layout(dcrf_model& m) {
         launch_remap_nbr(lat.nbr.get(), lay->nbr.get(), d1 * lat.V, m.vbase[k], V, s);
         if (m.pairs)  // forward passes walk axes 0..d, the transpose's d..0 (permutohedral.cpp:556)
             build_pass_tables(lay->nbr.get(), lat.V, lat.d, m.vbase[k], V, lay->pass_steps, lay->pass_off,
-                              lay->passF, lay->passT, s);
+                              lay->passF, lay->passT, s, step_limit[k]);
         lay->vtx.reserve(Ek);
         lay->swF.reserve(Ek);
         const float alpha = slice_alpha(lat.d);
@@ -997,6 +1019,7 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
         for (int p = 0; p < m.n_passes; p++) {
             PassTable pt{};
             pt.nterms = (int)m.order.size();
+            pt.zero_row = m.V_total;  // zero in both vertex buffers
             double bytes = 0.0;
             for (int i = 0; i < pt.nterms; i++) {
                 const int k = m.order[i];
@@ -2233,6 +2256,18 @@ int dcrf_set_q_blocks(dcrf_model* m, int on) {
             m->qblocks = on != 0;
             m->qblocks_ready = false;
             m->drop_graph();
+        }
+    });
+}
+
+int dcrf_set_blur_steps(dcrf_model* m, int steps) {
+    return guard([&] {
+        require(m != nullptr, "null model");
+        require(steps == 2 || steps == 3, "blur steps per pass: 2 or 3");
+        if (m->blur_steps != steps) {
+            m->blur_steps = steps;
+            m->layout_valid = false;
+            m->train_n = -1;
         }
     });
 }

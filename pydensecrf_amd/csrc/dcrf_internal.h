@@ -228,6 +228,7 @@ struct PassTerm {
 struct PassTable {
     int nterms;
     PassTerm t[DCRF_MAX_TERMS];
+    int zero_row = -1;  // the tables' row for absent vertices (all zeros), or -1
 };
 
 // One term as seen by the per-pixel kernels (slice + compatibility).

@@ -322,16 +322,27 @@ __device__ __forceinline__ float blur_fold(float o, float a, float b) {
     return o + h;
 }
 
+// zr: the zero row (absent neighbours point to it; it holds zeros in every vertex
+// buffer), whose loads are skipped: a group that has it takes 0.0f instead, the value
+// the load would give, and its lanes issue no memory access.  Half of the rows of a
+// sparse (bilateral) lattice's pass are absent neighbours.
 template <int K, int R>
 __device__ __forceinline__ void blur_pass_vertex(const float* __restrict__ src, float* __restrict__ dst,
-                                                 const int* __restrict__ e, int v, int ldv, int k0, int L) {
+                                                 const int* __restrict__ e, int v, int ldv, int k0, int L, int zr) {
     constexpr int NR = K == 1 ? 3 : (K == 2 ? 9 : 27);
     int idx[NR];
 #pragma unroll
     for (int i = 0; i < NR; i++) idx[i] = e[i];
     float x[NR][R];
 #pragma unroll
-    for (int i = 0; i < NR; i++) load_piece<R>(src + (size_t)idx[i] * ldv, k0, L, x[i]);
+    for (int i = 0; i < NR; i++) {
+        if (idx[i] != zr) {
+            load_piece<R>(src + (size_t)idx[i] * ldv, k0, L, x[i]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) x[i][r] = 0.0f;
+        }
+    }
     float out[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -348,6 +359,48 @@ __device__ __forceinline__ void blur_pass_vertex(const float* __restrict__ src, 
         }
         out[r] = y[0];
     }
+    store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
+}
+
+// A pass of three steps: the 27-row tree as its three 9-row subtrees (top-level child c
+// at rows 9c .. 9c + 8), one after the other, so that at most 9 rows are held at once
+// (a pass of three steps over all 27 rows would need ~80 VGPRs at three labels per
+// lane and lower the occupancy of every pass of the launch).  Same folds, in the same
+// order, as blur_pass_vertex<3>.
+template <int R>
+__device__ __forceinline__ void blur_pass_vertex3(const float* __restrict__ src, float* __restrict__ dst,
+                                                  const int* __restrict__ e, int v, int ldv, int k0, int L, int zr) {
+    float s[3][R];
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        int idx[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) idx[i] = e[9 * c + i];
+        float x[9][R];
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            if (idx[i] != zr) {
+                load_piece<R>(src + (size_t)idx[i] * ldv, k0, L, x[i]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) x[i][r] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float a = blur_fold(x[0][r], x[1][r], x[2][r]);
+            const float b = blur_fold(x[3][r], x[4][r], x[5][r]);
+            const float d = blur_fold(x[6][r], x[7][r], x[8][r]);
+            const float t = blur_fold(a, b, d);
+            // (a runtime index into s would put it in scratch memory)
+            if (c == 0) s[0][r] = t;
+            else if (c == 1) s[1][r] = t;
+            else s[2][r] = t;
+        }
+    }
+    float out[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) out[r] = blur_fold(s[0][r], s[1][r], s[2][r]);
     store_piece<R>(dst + (size_t)v * ldv, k0, L, out);
 }
 
@@ -368,13 +421,13 @@ __global__ void __launch_bounds__(1024) k_blur_pass(const float* __restrict__ sr
             steps = pt.t[k].steps;
         }
     }
-    // passes of three steps (27 rows) measured slower than two passes of up to two:
-    // the pass schedule never asks for them and the kernel does not instantiate them
     const size_t lv = (size_t)(v - vb);
     if (steps == 2)
-        blur_pass_vertex<2, R>(src, dst, tab + lv * 9, v, ldv, k0, L);
+        blur_pass_vertex<2, R>(src, dst, tab + lv * 9, v, ldv, k0, L, pt.zero_row);
+    else if (steps == 3)
+        blur_pass_vertex3<R>(src, dst, tab + lv * 27, v, ldv, k0, L, pt.zero_row);
     else
-        blur_pass_vertex<1, R>(src, dst, tab + lv * 3, v, ldv, k0, L);
+        blur_pass_vertex<1, R>(src, dst, tab + lv * 3, v, ldv, k0, L, pt.zero_row);
 }
 
 // Pass table of one term: entry (i_K, .., i_1) of vertex v at v*3^K + sum i_j 3^(j-1),

@@ -7,6 +7,7 @@ at their current best):
                 recombined (dcrf_set_splat_chunk).  Small chunks balance the heavy
                 vertices of bilateral lattices; large ones avoid recombination work.
   blur_pairs    two lattice axes per blur pass (the default) or one (dcrf_set_blur_pairs).
+  blur_steps    most steps per multi-step pass, 2 or 3 (dcrf_set_blur_steps).
   meanfield     the mean-field kernel: 0 automatic (fused where available), 3 fused over
                 LDS-staged pixel runs (dcrf_set_meanfield_kernel).
   q_blocks      Q between inference iterations in 8 x 8 pixel blocks (dcrf_set_q_blocks).
@@ -43,13 +44,14 @@ TILES = (128, 64, 256)
 MODEL_KNOBS = {
     "splat_chunk": CHUNKS,
     "blur_pairs": (True, False),
+    "blur_steps": (2, 3),
     "meanfield": (0, 3),
     "q_blocks": (False, True),
     "splat_sort": (0, 36, 72),
     "blur_lanes": ((0, 0), (1, 50000), (1, 0)),
 }
-DEFAULTS = {"splat_chunk": 32, "blur_pairs": True, "meanfield": 0, "q_blocks": False, "splat_sort": 0,
-            "blur_lanes": (0, 0)}
+DEFAULTS = {"splat_chunk": 32, "blur_pairs": True, "blur_steps": 2, "meanfield": 0, "q_blocks": False,
+            "splat_sort": 0, "blur_lanes": (0, 0)}
 
 
 def enabled():
@@ -96,6 +98,8 @@ def _set(model, knob, value):
         model.set_splat_chunk(int(value))
     elif knob == "blur_pairs":
         model.set_blur_pairs(bool(value))
+    elif knob == "blur_steps":
+        model.set_blur_steps(int(value))
     elif knob == "meanfield":
         model.set_meanfield_kernel(int(value))
     elif knob == "q_blocks":

@@ -276,36 +276,40 @@ def test_autotuned_chunk_keeps_results(tmp_path, monkeypatch):
     assert again == best and t2 is None  # cached
 
 
-@pytest.mark.parametrize("dims", [(2, 5), (1, 3), (4, 6), (7,)])
-def test_blur_pairs_bitwise(dims):
-    """Two blur steps per pass (the default) against one pass per step: the same bits for
-    inference, the transposed and forward messages, and the gradient (which runs the
-    transposed filter), for feature dimensions with an odd and an even number of steps,
-    mixed in one model."""
+@pytest.mark.parametrize("dims,scales", [((2, 5), (1.5, 1.5)), ((1, 3), (1.5, 1.5)), ((4, 6), (1.5, 1.5)),
+                                         ((7,), (1.5,)), ((2, 5), (12.0, 0.3)), ((2, 3, 5), (12.0, 1.0, 0.3))])
+def test_blur_pairs_bitwise(dims, scales):
+    """Multi-step blur passes (at most two steps, the default; at most three) against
+    one pass per step: the same bits for inference, the transposed and forward messages,
+    and the gradient (which runs the transposed filter), for feature dimensions with an
+    odd and an even number of steps, mixed in one model, and lattices of very different
+    vertex counts (scales 12 and 0.3)."""
     from pydensecrf_amd import capi
     rng = np.random.default_rng(sum(dims))
     N, L = 3000, 5
     U = rng.standard_normal((L, N)).astype(np.float32)
     gt = rng.integers(0, L, N).astype(np.int16)
-    feats = [(rng.standard_normal((d, N)) * 1.5).astype(np.float32) for d in dims]
+    feats = [(rng.standard_normal((d, N)) * sc).astype(np.float32) for d, sc in zip(dims, scales)]
     out = []
-    for pairs in (True, False):
+    for pairs, steps in ((False, 2), (True, 2), (True, 3)):
         g = capi.Model.create(N, L)
         g.set_unary(U)
         for i, f in enumerate(feats):
             g.add_pairwise(f, 2.0 + i, ntype=[SYM, AFTER, BEFORE, NONE][i % 4])
         g.set_blur_pairs(pairs)
+        g.set_blur_steps(steps)
         Q = g.inference(4)
         msgs = [g.apply(k, Q, tr) for k in range(len(feats)) for tr in (False, True)]
         grads = g.gradient(3, gt)
         out.append((Q, msgs, grads))
-    (Qa, ma, ga), (Qb, mb, gb) = out
-    np.testing.assert_array_equal(Qa, Qb)
-    for x, y in zip(ma, mb):
-        np.testing.assert_array_equal(x, y)
-    assert ga[0] == gb[0]
-    for x, y in zip(ga[1:], gb[1:]):
-        np.testing.assert_array_equal(x, y)
+    Qa, ma, ga = out[0]
+    for Qb, mb, gb in out[1:]:
+        np.testing.assert_array_equal(Qa, Qb)
+        for x, y in zip(ma, mb):
+            np.testing.assert_array_equal(x, y)
+        assert ga[0] == gb[0]
+        for x, y in zip(ga[1:], gb[1:]):
+            np.testing.assert_array_equal(x, y)
 
 
 @pytest.mark.parametrize("W,H,L,n", [(203, 149, 21, 5), (64, 48, 4, 3), (9, 7, 5, 4), (1024, 768, 21, 10)])
@@ -348,7 +352,7 @@ def test_autotuned_knobs_keep_results(tmp_path, monkeypatch):
     img, U, _ = image_case(W, H, L, seed=78)
     g, o = build_both(W, H, L, U, _std_terms(img))
     ref = np.ascontiguousarray(g.inference(5))
-    knobs = ("blur_pairs", "meanfield", "q_blocks")
+    knobs = ("blur_pairs", "blur_steps", "meanfield", "q_blocks")
     best, times = tune.tune_model(g, iters=2, reps=2, knobs=knobs)
     assert set(best) == set(knobs) and all(set(times[k]) == set(tune.MODEL_KNOBS[k]) for k in knobs)
     assert np.array_equal(np.ascontiguousarray(g.inference(5)).view(np.int32), ref.view(np.int32))
