@@ -89,6 +89,7 @@ struct dcrf_model {
     DevBuf<float> vals[2];
     DevBuf<Chunk> g_chunks;
     DevBuf<Combine> g_comb, g_comb2;  // combine pieces, then the second level of split vertices
+    bool comb_waves = false;          // g_comb is a one-launch list (launch_combine_waves)
     DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
     bool tiled = false;                // splat through LDS tiles (else the global form)
     size_t lds_cap = 0;                // LDS bytes one workgroup may allocate
@@ -768,17 +769,31 @@ void ensure_layout(dcrf_model& m) {
     groups.push_back((int)gch.size());
     if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
     require_adjoining(gch);
-    // Vertices split into more than kCombineSplit segments are recombined in two levels:
-    // pieces of about kCombineSplit partial rows into second-level rows (appended after
-    // the partial rows), then those into the vertex row, so that no lane group sums a
-    // long serial chain (the heaviest bilateral vertices at 1024x768 have ~245 rows).
-    std::vector<Combine> gcb2;
+    // Vertices split into more than kCombineSplit segments must not leave one lane group
+    // a long serial chain (the heaviest bilateral vertices at 1024x768 have ~245 partial
+    // rows).  Wave-mode shapes recombine in one launch (launch_combine_waves): such a
+    // vertex fills a whole wave, its rows cut into up to `per` pieces summed side by side
+    // and then added in piece order.  Block-mode shapes recombine in two levels: pieces
+    // of about kCombineSplit rows into second-level rows (appended after the partial
+    // rows), then those into the vertex row (a second launch).
+    std::vector<Combine> gcb2, heavy;
+    const LaunchShape csh = shape_for(L);
+    m.comb_waves = csh.wave;
     for (int k : m.order)
         for (Combine c : m.terms[k]->lat.combines) {
             c.v += m.vbase[k];
             c.first += pbase[k];
             if (c.count <= kCombineSplit) {
                 gcb.push_back(c);
+                continue;
+            }
+            if (m.comb_waves) {
+                const int pieces = std::min(csh.per, (c.count + kCombineSplit - 1) / kCombineSplit);
+                for (int j = 0, off = 0; j < csh.per; j++) {
+                    const int n = j < pieces ? (c.count - off) / (pieces - j) : 0;
+                    heavy.push_back(Combine{c.v, c.first + off, n, pieces});
+                    off += n;
+                }
                 continue;
             }
             const int pieces = (c.count + kCombineSplit - 1) / kCombineSplit;
@@ -791,6 +806,10 @@ void ensure_layout(dcrf_model& m) {
             }
             gcb2.push_back(Combine{c.v, row2, pieces, 0});
         }
+    if (!heavy.empty()) {  // light items padded to whole waves, then one wave per heavy vertex
+        while (gcb.size() % csh.per) gcb.push_back(Combine{0, 0, 0, 0});
+        gcb.insert(gcb.end(), heavy.begin(), heavy.end());
+    }
     m.C_total = (int)gch.size();
     m.CB_total = (int)gcb.size();
     m.CB2_total = (int)gcb2.size();
@@ -1009,7 +1028,10 @@ void run_filter(dcrf_model& m, const float* in, int ldi, bool transpose, hipStre
     }
     if (m.CB_total) {
         Timed t(m, s, KC_COMBINE, 0.0);
-        launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), m.ldv, L, s);
+        if (m.comb_waves)
+            launch_combine_waves(m.g_comb.get(), m.CB_total, m.vals[0].get(), m.ldv, L, s);
+        else
+            launch_combine(m.g_comb.get(), m.CB_total, m.vals[0].get(), m.ldv, L, s);
         if (m.CB2_total) launch_combine(m.g_comb2.get(), m.CB2_total, m.vals[0].get(), m.ldv, L, s);
         t.done();
     }

@@ -87,7 +87,7 @@ struct Combine {
     int v;      // vertex
     int first;  // first partial row
     int count;  // number of partial rows
-    int pad;
+    int pad;    // one-launch lists: 0 for a whole vertex, else its number of pieces
 };
 
 struct TermLattice {
@@ -297,6 +297,9 @@ size_t tiled_splat_lds(int L, int side, int e_max, int s_max, size_t lds_cap);
 void launch_splat_tiled(const TiledSplat& ts, const int2* ent, const float* in, float* vals, int ldv, int L,
                         size_t lds_cap, hipStream_t s);
 void launch_combine(const Combine* c, int n, float* vals, int ldv, int L, hipStream_t s);
+// The one-launch list (wave-mode shapes): light vertices one item each, every heavy
+// vertex a whole wave of pieces (Combine::pad).
+void launch_combine_waves(const Combine* c, int n, float* vals, int ldv, int L, hipStream_t s);
 // One blur step over the vertices [v0, Vp) (a range of the terms with d >= step).
 void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, const BlurTable& bt, int step,
                  int reverse, hipStream_t s);

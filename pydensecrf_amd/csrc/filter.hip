@@ -274,6 +274,63 @@ __global__ void __launch_bounds__(1024) k_combine(const Combine* __restrict__ cb
     store_piece<R>(vals + (size_t)rc.v * ldv, k0, L, acc);
 }
 
+// One-launch recombination (wave mode).  Items are lane-group sums of cb[c].count
+// partial rows from cb[c].first.  An item with pad == 0 is a whole vertex and stores
+// its sum as row cb[c].v (count 0: a filler, stores nothing).  A vertex with more rows
+// fills one whole wave: its items (pad = the number of pieces, the fillers after them
+// count 0) sum one piece each; the pieces go to the wave's LDS rows and group 0 adds them
+// in piece order and stores the vertex row.  A wave holds either light items or one
+// heavy vertex, so the branch is uniform per wave.
+template <int R>
+__global__ void __launch_bounds__(kWaveBlock) k_combine_waves(const Combine* __restrict__ cb, int n,
+                                                              float* __restrict__ vals, int ldv, int L, int S,
+                                                              int per) {
+    extern __shared__ float rows_lds[];  // per wave: per rows of L floats
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int c = G.item();
+    const int cw = c - G.g;  // the wave's first item
+    if (cw >= n) return;     // whole waves only: no item of this wave exists
+    const bool live = G.on && c < n;
+    const Combine rc = live ? cb[c] : Combine{0, 0, 0, 0};
+    const int k0 = G.k0;
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
+    for (int i = 0; i < rc.count; i += kBatch) {  // loads clamped to the last row
+        float xx[kBatch][R];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            load_piece<R>(vals + (size_t)(rc.first + min(i + u, rc.count - 1)) * ldv, k0, L, xx[u]);
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (i + u < rc.count)
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[r] = acc[r] + xx[u][r];
+    }
+    const Combine head = cb[cw];  // uniform over the wave
+    if (head.pad == 0) {
+        if (rc.count > 0) store_piece<R>(vals + (size_t)rc.v * ldv, k0, L, acc);
+        return;
+    }
+    float* wrows = rows_lds + (size_t)(threadIdx.x >> 6) * per * L;
+    if (G.on && G.g < head.pad) store_piece<R>(wrows + (size_t)G.g * L, k0, L, acc);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (G.on && G.g == 0) {
+        float sum[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) sum[r] = 0.0f;
+        for (int j = 0; j < head.pad; j++) {
+            float x[R];
+            load_piece<R>(wrows + (size_t)j * L, k0, L, x);
+#pragma unroll
+            for (int r = 0; r < R; r++) sum[r] = sum[r] + x[r];
+        }
+        store_piece<R>(vals + (size_t)head.v * ldv, k0, L, sum);
+    }
+}
+
 template <bool WAVE, int R>
 __global__ void __launch_bounds__(1024) k_blur(const float* __restrict__ src, float* __restrict__ dst, int v0,
                                                int Vp, int ldv, int L, int S, int per, BlurTable bt, int step,
@@ -1130,6 +1187,21 @@ void launch_combine(const Combine* c, int n, float* vals, int ldv, int L, hipStr
     if (!n) return;
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_combine, blocks_for(n, sh.items), 0, s, c, n, vals, ldv, L, sh.S, sh.per);
+}
+
+void launch_combine_waves(const Combine* c, int n, float* vals, int ldv, int L, hipStream_t s) {
+    if (!n) return;
+    const LaunchShape sh = shape_for(L);
+    if (!sh.wave) throw Error{DCRF_ERR_STATE, "one-launch combine: wave-mode shapes only"};
+    const size_t lds = sizeof(float) * (kWaveBlock / 64) * sh.per * L;
+    const unsigned grid = blocks_for(n, sh.items);
+    switch (sh.R) {
+        case 1: k_combine_waves<1><<<grid, kWaveBlock, lds, s>>>(c, n, vals, ldv, L, sh.S, sh.per); break;
+        case 2: k_combine_waves<2><<<grid, kWaveBlock, lds, s>>>(c, n, vals, ldv, L, sh.S, sh.per); break;
+        case 3: k_combine_waves<3><<<grid, kWaveBlock, lds, s>>>(c, n, vals, ldv, L, sh.S, sh.per); break;
+        default: k_combine_waves<4><<<grid, kWaveBlock, lds, s>>>(c, n, vals, ldv, L, sh.S, sh.per); break;
+    }
+    DCRF_HIP_CHECK(hipGetLastError());
 }
 
 void launch_blur(const float* src, float* dst, int v0, int Vp, int ldv, int L, const BlurTable& bt, int step,

@@ -217,6 +217,22 @@ def test_splat_chunk_invariance(chunk):
     assert np.abs(g.inference(5) - o.inference(5)).max() <= Q_TOL
 
 
+@pytest.mark.parametrize("L", [21, 4, 1, 64, 300])
+def test_heavy_vertex_recombination(L):
+    """Splat segments of one contributor each (chunk 1): every vertex with more than 16
+    contributors is recombined from more than 16 partial rows, in one launch with a
+    whole wave per such vertex (wave-mode shapes: L = 21, 4, 1, 64) or in two levels
+    (L = 300, groups wider than a wave).  Q and the transposed message against the
+    oracle."""
+    W, H = 64, 48
+    img, U, _ = image_case(W, H, L, seed=L + 300)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    g.set_splat_chunk(1)
+    assert np.abs(g.inference(4) - o.inference(4)).max() <= Q_TOL
+    Q = o.start()
+    np.testing.assert_allclose(g.apply(1, Q, True), o.apply(1, Q, True), rtol=1e-5, atol=1e-5)
+
+
 @pytest.mark.parametrize("L", [4, 7, 19, 64, 65, 130, 300])
 @pytest.mark.parametrize("ck", ["potts", "matrix"])
 def test_label_count_shapes(L, ck):

@@ -199,3 +199,61 @@ def test_sum_rows_reads_each_entry_once(rows, cols):
     reads, mids, nbu = sum_rows_reads(rows, cols)
     assert np.all(reads == 1)  # every entry once: no row or column dropped or doubled
     assert mids == set(range(nbu * cols))  # the second kernel reads exactly these
+
+
+def one_launch_combine_list(counts, per, split=16):
+    """The layout's one-launch recombination list (capi.hip ensure_layout, wave-mode
+    shapes): (v, first, count, pad) items, light vertices first, padded to whole waves,
+    then one wave per vertex with more than `split` partial rows."""
+    light, heavy, first = [], [], 0
+    for v, c in enumerate(counts):
+        if c <= split:
+            light.append((v, first, c, 0))
+        else:
+            pieces = min(per, (c + split - 1) // split)
+            off = 0
+            for j in range(per):
+                n = (c - off) // (pieces - j) if j < pieces else 0
+                heavy.append((v, first + off, n, pieces))
+                off += n
+        first += c
+    if heavy:
+        while len(light) % per:
+            light.append((0, 0, 0, 0))
+    return light + heavy, first
+
+
+@pytest.mark.parametrize("L", [21, 4, 1, 64])
+def test_one_launch_combine_waves(L):
+    """k_combine_waves (filter.hip) reads cb[wave's first item] as the wave's head and
+    lets a heavy vertex's pieces meet in the wave's LDS rows: every heavy vertex must
+    fill exactly one whole wave (pieces <= groups per wave), light waves hold no heavy
+    item, each vertex's pieces cover its partial rows exactly once, and no piece is
+    longer than ceil(count / pieces) rows."""
+    R = labels_per_lane(L)
+    S = (L + R - 1) // R
+    per = 64 // S
+    rng = np.random.default_rng(L)
+    counts = np.concatenate([rng.integers(2, 17, 500), [17, 31, 160, 245, 1000, 7844 // 32],
+                             rng.integers(17, 300, 40)])
+    rng.shuffle(counts)
+    items, total = one_launch_combine_list(counts.tolist(), per)
+    assert len(items) % per == 0 or all(it[3] == 0 for it in items)
+    covered = np.zeros(total, np.int64)
+    for w0 in range(0, len(items), per):
+        wave = items[w0:w0 + per]
+        pads = {it[3] for it in wave}
+        if wave[0][3] == 0:
+            assert pads == {0}, "a heavy piece in a light wave"
+        else:
+            assert len(wave) == per and len({it[0] for it in wave}) == 1 and pads == {wave[0][3]}
+            pieces = wave[0][3]
+            assert 2 <= pieces <= per
+            assert all(it[2] == 0 for it in wave[pieces:])  # fillers after the pieces
+            c = sum(it[2] for it in wave)
+            assert max(it[2] for it in wave) <= -(-c // pieces)
+        for v, f, n, _ in wave:
+            covered[f:f + n] += 1
+    assert np.all(covered == 1)
+    # LDS: per rows of L floats per wave, 4 waves per block
+    assert 4 * per * L * 4 <= 160 * 1024
