@@ -148,20 +148,28 @@ def kernel_rebuild(reps):
     Qd = torch.empty((W * H, L), dtype=torch.float32, device="cuda")
     kp = m.kernel_params()
 
+    split = [0.0, 0.0]
+
     def change(f):
         # new lattices, then the layout (built by the next call on the model: here a
         # 0-iteration inference, Q = softmax(-U))
+        t0 = time.perf_counter()
         m.set_kernel_params(kp * np.float32(f))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         m.inference_device(0, Qd.data_ptr())
         torch.cuda.synchronize()
+        split[0] += t1 - t0
+        split[1] += time.perf_counter() - t1
 
     change(1.001)
+    split[:] = [0.0, 0.0]
     t0 = time.perf_counter()
     for r in range(reps):
         change(1.0 + 0.01 * (r + 1))
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    print("kernel-rebuild 512x512x21, d = 2 + 5: %.3f ms per parameter change (lattices and layout)" % ms,
-          flush=True)
+    print("kernel-rebuild 512x512x21, d = 2 + 5: %.3f ms per parameter change (lattices %.3f ms, layout %.3f ms)"
+          % (ms, split[0] * 1e3 / reps, split[1] * 1e3 / reps), flush=True)
 
 
 def main():

@@ -407,6 +407,40 @@ def test_kernel_params_rebuild_and_reset():
     np.testing.assert_array_equal(g.inference(n), Q0)
 
 
+@pytest.mark.parametrize("bad_term", [0, 1])
+def test_kernel_params_parallel_rebuild(bad_term):
+    """Two terms with DIAG kernel parameters: inference follows the oracle at the new
+    parameters.  A parameter vector that overflows the lattice keys of either term (the
+    first built, or the second after the first succeeded) raises the overflow error and
+    leaves the model as it was: every term's new lattice is built before any is
+    committed."""
+    W, H, L, n = 48, 40, 5, 4
+    img, U, _ = image_case(W, H, L, seed=19)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0)
+        m.add_bilateral(30.0, 30.0, 13.0, 13.0, 13.0, img, 10.0)
+    p0 = g.kernel_params()
+    assert p0.size == 2 + 5
+    rng = np.random.default_rng(5)
+    p = (p0 * rng.uniform(0.8, 1.25, p0.size)).astype(np.float32)
+    for m in (g, o):
+        m.set_kernel_params(p)
+    Q1 = g.inference(n)
+    np.testing.assert_allclose(Q1, o.inference(n), atol=2e-5)
+    bad = p.copy()
+    if bad_term == 0:
+        bad[:2] = 1e4
+    else:
+        bad[2:] = 1e4
+    with pytest.raises(ValueError, match="int16"):
+        g.set_kernel_params(bad)
+    np.testing.assert_array_equal(g.kernel_params(), p)
+    np.testing.assert_array_equal(g.inference(n), Q1)
+
+
 def test_autograd_kernel_params():
     """CRFLayer(learn_kernel=True): the kernel-parameter gradient of the cross-entropy is
     minus dcrf_gradient's kernel_grad (of the mean log-likelihood), and a forward after
