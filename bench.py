@@ -399,8 +399,6 @@ def main():
                     help="keep Q between iterations in 8x8 pixel blocks (dcrf_set_q_blocks; opt-in layout)")
     ap.add_argument("--blur-lanes", type=int, nargs=2, metavar=("R", "BELOW_V"), default=None,
                     help="blur passes below BELOW_V vertices with R labels per lane (dcrf_set_blur_lanes)")
-    ap.add_argument("--splat-sort", type=int, default=None, metavar="RUN",
-                    help="sort the splat segments by length inside runs of RUN segments (dcrf_set_splat_sort)")
     ap.add_argument("--blur-steps", type=int, default=None, choices=[2, 3],
                     help="most blur steps per multi-step pass (dcrf_set_blur_steps; default 2)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -444,8 +442,6 @@ def main():
         m.set_q_blocks(True)
     if args.blur_lanes:
         m.set_blur_lanes(*args.blur_lanes)
-    if args.splat_sort is not None:
-        m.set_splat_sort(args.splat_sort)
     if args.blur_steps is not None:
         m.set_blur_steps(args.blur_steps)
     t0 = time.perf_counter()
@@ -538,7 +534,6 @@ def main():
                    "q_blocks": bool(args.q_blocks),
                    "blur_lanes": args.blur_lanes,
                    "blur_steps": args.blur_steps if args.blur_steps is not None else 2,
-                   "splat_sort": args.splat_sort,
                    "vertices": {"gaussian": Vg, "bilateral": Vb}},
         "roofline": {
             "bound": "hbm",

@@ -286,11 +286,6 @@ int dcrf_set_blur_steps(dcrf_model *m, int steps);
  * vertex's contributors from a few contiguous blocks instead of several image rows).
  * Same results, bit for bit; 0 (off) by default.  No reference counterpart. */
 int dcrf_set_q_blocks(dcrf_model *m, int on);
-/* Order the splat's segments by length (longest first) inside runs of `run` segments of
- * one pixel tile and term (0 or 1: layout order), so that the lane groups of a wave sum
- * segments of similar length.  Each segment keeps its entries and output row: the same
- * bits for any setting.  No reference counterpart. */
-int dcrf_set_splat_sort(dcrf_model *m, int run);
 /* Blur passes over fewer than below_vertices vertices (all passes when <= 0) with
  * labels_per_lane labels per lane (0: the default for the label count), i.e. more lanes
  * per vertex for small, latency-bound passes.  The blur is elementwise per label: the

@@ -75,7 +75,6 @@ SIGNATURES = {
     "dcrf_set_blur_pairs": (_c, [_P, _c]),
     "dcrf_set_blur_steps": (_c, [_P, _c]),
     "dcrf_set_q_blocks": (_c, [_P, _c]),
-    "dcrf_set_splat_sort": (_c, [_P, _c]),
     "dcrf_set_blur_lanes": (_c, [_P, _c, _c]),
     "dcrf_get_kernel_params": (_c, [_P, _F]),
     "dcrf_set_kernel_params": (_c, [_P, _F]),
@@ -422,10 +421,6 @@ class Model:
         """Blur passes of fewer than below_vertices vertices (all when 0) with the given
         labels per lane (0 = default)."""
         check(lib().dcrf_set_blur_lanes(self.h, int(labels_per_lane), int(below_vertices)))
-
-    def set_splat_sort(self, run):
-        """Sort the splat's segments by length inside runs of `run` segments (0 = off)."""
-        check(lib().dcrf_set_splat_sort(self.h, int(run)))
 
     def set_q_blocks(self, on):
         """Keep Q between inference iterations in 8 x 8 pixel blocks (opt-in)."""

@@ -100,9 +100,6 @@ struct dcrf_model {
     bool qblocks = false, qblocks_ready = false;
     // opt-in: blur passes of fewer than blur_small_v vertices with blur_r labels per lane
     int blur_r = 0, blur_small_v = 0;
-    // splat segments sorted by length (longest first) inside runs of seg_sort segments of
-    // one (tile, term) group; 0 or 1 = layout order
-    int seg_sort = 0;
     DevBuf<int2> g_entB;
     DevBuf<float> Qb;
     bool fused = false;           // inference through k_meanfield_fused
@@ -769,43 +766,103 @@ void ensure_layout(dcrf_model& m) {
     groups.push_back((int)gch.size());
     if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
     require_adjoining(gch);
-    // Vertices split into more than kCombineSplit segments must not leave one lane group
-    // a long serial chain (the heaviest bilateral vertices at 1024x768 have ~245 partial
-    // rows).  Wave-mode shapes recombine in one launch (launch_combine_waves): such a
-    // vertex fills a whole wave, its rows cut into up to `per` pieces summed side by side
-    // and then added in piece order.  Block-mode shapes recombine in two levels: pieces
-    // of about kCombineSplit rows into second-level rows (appended after the partial
-    // rows), then those into the vertex row (a second launch).
-    std::vector<Combine> gcb2, heavy;
+    upload(m.g_tile_seg, gts, s);
+    upload(m.g_tile_e, gte, s);
+    const TileGeom tg = tile_geom(m);
+    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
+    // A vertex's segments that sit side by side in one wave of the global splat are
+    // added inside the wave (k_splat folds each such run into its first lane group): the
+    // later segments of a run get out = -1 and write nothing, and the vertex has one
+    // partial row per run, or its final row when it has one run.  At 1024x768x21 this
+    // cuts the partial rows from 188 k to 72 k and the vertices with more than 16 of them
+    // from 2,581 to 52.  Block-mode shapes and the tiled splat keep one row per segment.
     const LaunchShape csh = shape_for(L);
-    m.comb_waves = csh.wave;
-    for (int k : m.order)
-        for (Combine c : m.terms[k]->lat.combines) {
-            c.v += m.vbase[k];
-            c.first += pbase[k];
-            if (c.count <= kCombineSplit) {
-                gcb.push_back(c);
-                continue;
-            }
-            if (m.comb_waves) {
-                const int pieces = std::min(csh.per, (c.count + kCombineSplit - 1) / kCombineSplit);
-                for (int j = 0, off = 0; j < csh.per; j++) {
-                    const int n = j < pieces ? (c.count - off) / (pieces - j) : 0;
-                    heavy.push_back(Combine{c.v, c.first + off, n, pieces});
-                    off += n;
+    std::vector<Combine> combs;  // (vertex, first partial row, rows) of every split vertex
+    if (csh.wave && !m.tiled) {
+        const int per = csh.per;
+        // Every (tile, term) group of segments starts a wave (empty fillers, v = out = -1,
+        // pad the group before), and the empty segments of vertices that only padding
+        // lanes touch (listed with the first tile of the whole list) go to waves of their
+        // own in front: the runs then depend only on the group's own segments, so an
+        // instance of a batch gets the bits of the same image alone.
+        {
+            std::vector<Chunk> padded;
+            padded.reserve(gch.size() + (groups.size() + 2) * per);
+            const int pos0 = gch.empty() ? 0 : gch[0].beg;
+            for (const Chunk& x : gch)
+                if (x.beg == x.end) padded.push_back(Chunk{x.v, pos0, pos0, x.out});
+            while (padded.size() % per) padded.push_back(Chunk{-1, pos0, pos0, -1});
+            int pos = pos0;
+            for (size_t gi = 0; gi + 1 < groups.size(); gi++) {
+                for (int c = groups[gi]; c < groups[gi + 1]; c++) {
+                    if (gch[c].beg == gch[c].end) continue;
+                    padded.push_back(gch[c]);
+                    pos = gch[c].end;
                 }
-                continue;
+                while (padded.size() % per) padded.push_back(Chunk{-1, pos, pos, -1});
             }
-            const int pieces = (c.count + kCombineSplit - 1) / kCombineSplit;
-            const int row2 = m.rows0;
-            m.rows0 += pieces;
-            for (int j = 0, off = 0; j < pieces; j++) {
-                const int n = (c.count - off) / (pieces - j);
-                gcb.push_back(Combine{row2 + j, c.first + off, n, 0});
+            gch.swap(padded);
+            require_adjoining(gch);
+        }
+        auto head = [&](size_t c) { return gch[c].v >= 0 && (c % per == 0 || gch[c - 1].v != gch[c].v); };
+        std::vector<int> nruns(V + 1, 0), pfirst(V + 1, -1), used(V + 1, 0);
+        for (size_t c = 0; c < gch.size(); c++)
+            if (head(c)) nruns[gch[c].v]++;
+        int prow = V + 1;
+        for (size_t c = 0; c < gch.size(); c++) {
+            const int v = gch[c].v;
+            if (head(c) && nruns[v] > 1 && pfirst[v] < 0) {
+                pfirst[v] = prow;
+                combs.push_back(Combine{v, prow, nruns[v], 0});
+                prow += nruns[v];
+            }
+        }
+        for (size_t c = 0; c < gch.size(); c++) {
+            const int v = gch[c].v;
+            gch[c].out = !head(c) ? -1 : (nruns[v] > 1 ? pfirst[v] + used[v]++ : v);
+        }
+        m.rows0 = prow;
+    } else {
+        for (int k : m.order)
+            for (Combine c : m.terms[k]->lat.combines) {
+                c.v += m.vbase[k];
+                c.first += pbase[k];
+                combs.push_back(c);
+            }
+    }
+    // Vertices split into more than kCombineSplit partial rows must not leave one lane
+    // group a long serial chain (before the in-wave runs, the heaviest bilateral vertices
+    // at 1024x768 had ~245 partial rows).  Wave-mode shapes recombine in one launch
+    // (launch_combine_waves): such a vertex fills a whole wave, its rows cut into up to
+    // `per` pieces summed side by side and then added in piece order.  Block-mode shapes
+    // recombine in two levels: pieces of about kCombineSplit rows into second-level rows
+    // (appended after the partial rows), then those into the vertex row (a second launch).
+    std::vector<Combine> gcb2, heavy;
+    m.comb_waves = csh.wave;
+    for (const Combine& c : combs) {
+        if (c.count <= kCombineSplit) {
+            gcb.push_back(c);
+            continue;
+        }
+        if (m.comb_waves) {
+            const int pieces = std::min(csh.per, (c.count + kCombineSplit - 1) / kCombineSplit);
+            for (int j = 0, off = 0; j < csh.per; j++) {
+                const int n = j < pieces ? (c.count - off) / (pieces - j) : 0;
+                heavy.push_back(Combine{c.v, c.first + off, n, pieces});
                 off += n;
             }
-            gcb2.push_back(Combine{c.v, row2, pieces, 0});
+            continue;
         }
+        const int pieces = (c.count + kCombineSplit - 1) / kCombineSplit;
+        const int row2 = m.rows0;
+        m.rows0 += pieces;
+        for (int j = 0, off = 0; j < pieces; j++) {
+            const int n = (c.count - off) / (pieces - j);
+            gcb.push_back(Combine{row2 + j, c.first + off, n, 0});
+            off += n;
+        }
+        gcb2.push_back(Combine{c.v, row2, pieces, 0});
+    }
     if (!heavy.empty()) {  // light items padded to whole waves, then one wave per heavy vertex
         while (gcb.size() % csh.per) gcb.push_back(Combine{0, 0, 0, 0});
         gcb.insert(gcb.end(), heavy.begin(), heavy.end());
@@ -816,44 +873,7 @@ void ensure_layout(dcrf_model& m) {
     upload(m.g_chunks, gch, s);
     upload(m.g_comb, gcb, s);
     upload(m.g_comb2, gcb2, s);
-    upload(m.g_tile_seg, gts, s);
-    upload(m.g_tile_e, gte, s);
-    const TileGeom tg = tile_geom(m);
     m.tsplat = TiledSplat{m.g_chunks.get(), m.g_tile_seg.get(), m.g_tile_e.get(), NT, tg, e_max, s_max};
-    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
-    // A wave of the global splat sums the segments of its lane groups side by side and
-    // issues as many 8-entry gathers as its longest segment needs: segments of similar
-    // length in one wave waste fewer.  Sorting inside short runs keeps the wave's rows
-    // close together.  Each segment keeps its entries (in order) and its output row, so
-    // the sums are the same bits; the entries are moved to follow the new order.
-    std::vector<int4> moves;  // (old first entry, new first entry, count, 0)
-    if (m.seg_sort > 1 && !m.tiled) {
-        std::vector<Chunk> sorted;
-        sorted.reserve(gch.size());
-        std::vector<int> idx;
-        for (size_t gi = 0; gi + 1 < groups.size(); gi++)
-            for (int r0 = groups[gi]; r0 < groups[gi + 1]; r0 += m.seg_sort) {
-                const int r1 = std::min(r0 + m.seg_sort, groups[gi + 1]);
-                idx.resize(r1 - r0);
-                std::iota(idx.begin(), idx.end(), r0);
-                std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-                    return gch[a].end - gch[a].beg > gch[b].end - gch[b].beg;
-                });
-                int pos = gch[r0].beg;
-                for (int i : idx) {
-                    Chunk x = gch[i];
-                    const int n = x.end - x.beg;
-                    if (x.beg != pos) moves.push_back(make_int4(x.beg, pos, n, 0));
-                    x.beg = pos;
-                    x.end = pos + n;
-                    pos += n;
-                    sorted.push_back(x);
-                }
-            }
-        gch.swap(sorted);
-        require_adjoining(gch);
-        upload(m.g_chunks, gch, s);
-    }
 
     // vertex rows packed (stride L): rows padded to start cache lines (32 floats at
     // L = 21, for Q between iterations too) measured 8 % slower per step at 1024x768x21
@@ -899,19 +919,6 @@ void ensure_layout(dcrf_model& m) {
                                  m.vbase[k], lay->vtx.get(), lay->swT.get(), s);
         }
         m.tl[k] = std::move(lay);
-    }
-    if (!moves.empty()) {
-        DevBuf<int4> mv;
-        upload(mv, moves, s);
-        DevBuf<int2> tmp;
-        tmp.reserve(E);
-        for (DevBuf<int2>* ent : {&m.g_entF, &m.g_entT}) {
-            if (!ent->get()) continue;
-            if (ent == &m.g_entT && m.all_symmetric) continue;
-            DCRF_HIP_CHECK(hipMemcpyAsync(tmp.get(), ent->get(), sizeof(int2) * E, hipMemcpyDeviceToDevice, s));
-            launch_move_entries(tmp.get(), mv.get(), (int)moves.size(), ent->get(), s);
-        }
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
     }
     // interleaved slice pairs for the fused mean-field kernel
     m.runs = false;
@@ -2257,17 +2264,6 @@ int dcrf_set_blur_lanes(dcrf_model* m, int labels_per_lane, int below_vertices) 
         m->blur_r = labels_per_lane;
         m->blur_small_v = below_vertices;
         m->drop_graph();
-    });
-}
-
-int dcrf_set_splat_sort(dcrf_model* m, int run) {
-    return guard([&] {
-        require(m != nullptr, "null model");
-        require(run >= 0 && run <= 4096, "sort run must be in [0, 4096]");
-        if (m->seg_sort != run) {
-            m->seg_sort = run;
-            m->layout_valid = false;
-        }
     });
 }
 

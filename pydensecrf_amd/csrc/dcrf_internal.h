@@ -349,8 +349,6 @@ void launch_meanfield_runs(const float* U, float* Q, int N, int L, const SliceTa
 void launch_meanfield_fused(const float* U, float* Q, int ldq, int N, int L, const SliceTable& st,
                             const int2* slc, hipStream_t s, int bw = 0);
 void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s);
-// dst[m.y + j] = src[m.x + j], j < m.z, for the n moves m = (from, to, count, -)
-void launch_move_entries(const int2* src, const int4* moves, int n, int2* dst, hipStream_t s);
 // rows of a Q held in 8 x 8 pixel blocks for an image of W x H pixels
 inline size_t q_block_rows(int W, int H) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64; }
 void launch_interleave_slice(const int* vtx, const float* sw, int N, int d1, int ds, int off, int2* slc,

@@ -56,7 +56,9 @@ __global__ void k_fill(float* p, float v, size_t n) {
 }
 
 // Splat: each group sums one chunk (<= C contributors of a vertex, in entry order)
-// sequentially.  Entries are (pixel, weight bits) pairs.
+// sequentially.  Entries are (pixel, weight bits) pairs.  A chunk with out < 0 continues
+// the chunk before it in the same wave (same vertex): WAVE mode adds it to that chunk's
+// sum instead of storing it.
 //   WAVE mode: the chunks of a wave are consecutive and their entry ranges adjoin (the
 //   layout guarantees chunk c+1 begins where chunk c ends), so the wave loads its whole
 //   entry range cooperatively (64 consecutive pairs per load instruction) into an LDS
@@ -122,7 +124,23 @@ __global__ void __launch_bounds__(1024) k_splat(const Chunk* __restrict__ chunks
                 }
             }
         }
-        if (mine) store_piece<R>(vals + (size_t)ch.out * ldv, k0, L, acc);
+        // A run of the wave's chunks with one vertex (the layout gives the later ones
+        // out = -1): their sums are folded into the run's first group, right to left, in
+        // a fixed order.  Lists without such runs skip this.
+        const int nv = __shfl(ch.v, (lane + S) & 63);
+        const int nout = __shfl(ch.out, (lane + S) & 63);
+        const bool cont = mine && G.g + 1 < per && c + 1 < n && nout < 0 && nv == ch.v;
+        if (__any(cont)) {
+            for (int j = per - 2; j >= 0; j--) {  // wave-uniform
+                float y[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) y[r] = __shfl(acc[r], (lane + S) & 63);
+                if (cont && G.g == j)
+#pragma unroll
+                    for (int r = 0; r < R; r++) acc[r] = acc[r] + y[r];
+            }
+        }
+        if (mine && ch.out >= 0) store_piece<R>(vals + (size_t)ch.out * ldv, k0, L, acc);
         return;
     }
     if (!G.on || c >= n) return;
@@ -1368,23 +1386,6 @@ __global__ void k_entry_block(const int2* __restrict__ ent, size_t E, int W, int
     }
 }
 }  // namespace
-
-namespace {
-// dst[m.y + j] = src[m.x + j] for j < m.z, for every move m (one thread per move).
-__global__ void k_move_entries(const int2* __restrict__ src, const int4* __restrict__ moves, int n,
-                               int2* __restrict__ dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int4 mv = moves[i];
-    for (int j = 0; j < mv.z; j++) dst[mv.y + j] = src[mv.x + j];
-}
-}  // namespace
-
-void launch_move_entries(const int2* src, const int4* moves, int n, int2* dst, hipStream_t s) {
-    if (n <= 0) return;
-    k_move_entries<<<blocks_for(n), kThreads, 0, s>>>(src, moves, n, dst);
-    DCRF_HIP_CHECK(hipGetLastError());
-}
 
 void launch_entry_block(const int2* ent, size_t E, int W, int2* out, hipStream_t s) {
     if (!E) return;

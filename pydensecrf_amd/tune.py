@@ -11,8 +11,6 @@ at their current best):
   meanfield     the mean-field kernel: 0 automatic (fused where available), 3 fused over
                 LDS-staged pixel runs (dcrf_set_meanfield_kernel).
   q_blocks      Q between inference iterations in 8 x 8 pixel blocks (dcrf_set_q_blocks).
-  splat_sort    splat segments sorted by length inside runs of this many segments
-                (dcrf_set_splat_sort; 0 = layout order).
   blur_lanes    (labels per lane, below this many vertices) of the blur passes
                 (dcrf_set_blur_lanes; (0, 0) = the default lane split).
 Knobs of the build (they must be set before the first pairwise term; timed through a
@@ -47,11 +45,10 @@ MODEL_KNOBS = {
     "blur_steps": (2, 3),
     "meanfield": (0, 3),
     "q_blocks": (False, True),
-    "splat_sort": (0, 36, 72),
     "blur_lanes": ((0, 0), (1, 50000), (1, 0)),
 }
 DEFAULTS = {"splat_chunk": 32, "blur_pairs": True, "blur_steps": 2, "meanfield": 0, "q_blocks": False,
-            "splat_sort": 0, "blur_lanes": (0, 0)}
+            "blur_lanes": (0, 0)}
 
 
 def enabled():
@@ -104,8 +101,6 @@ def _set(model, knob, value):
         model.set_meanfield_kernel(int(value))
     elif knob == "q_blocks":
         model.set_q_blocks(bool(value))
-    elif knob == "splat_sort":
-        model.set_splat_sort(int(value))
     elif knob == "blur_lanes":
         model.set_blur_lanes(int(value[0]), int(value[1]))
     else:

@@ -422,26 +422,3 @@ def test_blur_lanes_bitwise(R, below):
     assert out[0][4] == out[1][4]
     with pytest.raises(ValueError):
         g.set_blur_lanes(2)  # 2 does not divide 21
-
-
-@pytest.mark.parametrize("run", [2, 36, 4096])
-def test_splat_sort_bitwise(run):
-    """Splat segments sorted by length inside runs (dcrf_set_splat_sort): every segment
-    keeps its entries and output row, so inference and the gradient (the transposed
-    entries move too, for a BEFORE-normalised term) give the same bits."""
-    from pydensecrf_amd import capi
-    W, H, L = 150, 110, 21
-    img, U, gt = image_case(W, H, L, seed=321)
-    out = []
-    for r in (0, run):
-        g = capi.Model.create_2d(W, H, L)
-        g.set_unary(U)
-        g.add_gaussian(3.0, 3.0, 3.0)
-        g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0, ntype=BEFORE)
-        g.set_splat_sort(r)
-        Q = np.ascontiguousarray(g.inference(5))
-        obj, ug, cg, kg = g.gradient(3, gt)
-        out.append((Q, ug, cg, obj))
-    for a, b in zip(out[0][:3], out[1][:3]):
-        assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))
-    assert out[0][3] == out[1][3]

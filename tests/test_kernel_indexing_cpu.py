@@ -257,3 +257,56 @@ def test_one_launch_combine_waves(L):
     assert np.all(covered == 1)
     # LDS: per rows of L floats per wave, 4 waves per block
     assert 4 * per * L * 4 <= 160 * 1024
+
+
+def splat_wave_runs(groups, per):
+    """The layout's in-wave runs (capi.hip ensure_layout, wave-mode global splat):
+    groups = [[(v, n_entries), ...] per (tile, term) group], empty segments (n = 0)
+    anywhere.  Returns the padded segment list [(v, out_marker)], with out_marker 'row'
+    (a run head: final or partial row) or -1 (continues the segment before it)."""
+    segs = [(v, n) for g in groups for v, n in g if n == 0]
+    while len(segs) % per:
+        segs.append((-1, 0))
+    for g in groups:
+        segs.extend((v, n) for v, n in g if n > 0)
+        while len(segs) % per:
+            segs.append((-1, 0))
+    out = []
+    for c, (v, _) in enumerate(segs):
+        head = v >= 0 and (c % per == 0 or segs[c - 1][0] != v)
+        out.append((v, "row" if head else -1))
+    return out
+
+
+@pytest.mark.parametrize("per", [9, 64, 3])
+def test_splat_wave_runs(per):
+    """k_splat (filter.hip) folds a group's sum into the group before it when the next
+    segment (lane + S) has out < 0 and the same vertex, in the same wave: a run never
+    crosses a wave, every wave holds segments of one (tile, term) group only (so the
+    runs, and the bits, of an image are the same inside a batch), and a vertex's runs
+    are its only outputs."""
+    rng = np.random.default_rng(per)
+    groups, inst = [], []
+    for gi in range(40):
+        g = []
+        for v in sorted(rng.choice(200, rng.integers(0, 12), replace=False)):
+            nseg = int(rng.integers(1, 25))
+            g.extend([(int(v) + 1000 * gi, 32)] * nseg)
+        if gi % 7 == 0:
+            g.insert(0, (999999 - gi, 0))  # an empty segment (padding-only vertex)
+        groups.append(g)
+    segs = splat_wave_runs(groups, per)
+    assert len(segs) % per == 0
+    # every wave: one group (vertex ids of a group lie in [1000 gi, 1000 gi + 200))
+    for w0 in range(0, len(segs), per):
+        vs = {v // 1000 for v, _ in segs[w0:w0 + per] if 0 <= v < 900000}
+        assert len(vs) <= 1
+    # runs: a continuation follows its vertex in the same wave
+    for c, (v, o) in enumerate(segs):
+        if o == -1 and v >= 0:
+            assert c % per != 0 and segs[c - 1][0] == v
+    # the same group gives the same runs wherever it sits (alone vs after other groups)
+    alone = splat_wave_runs([groups[5]], per)
+    first = [s for s in alone if s[0] >= 0]
+    inlist = [s for s in segs if 5000 <= s[0] < 5200]
+    assert first == inlist
