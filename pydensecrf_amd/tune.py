@@ -38,7 +38,7 @@ import time
 
 CHUNKS = (16, 32, 64, 128, 256)
 LOADS = (0.5, 0.35, 0.25)
-TILES = (128, 64, 256)
+TILES = (128, 160, 192, 64, 256)
 MODEL_KNOBS = {
     "splat_chunk": CHUNKS,
     "blur_pairs": (True, False),
