@@ -89,6 +89,10 @@ struct dcrf_model {
     DevBuf<float> vals[2];
     DevBuf<Chunk> g_chunks;
     DevBuf<Combine> g_comb, g_comb2;  // combine pieces, then the second level of split vertices
+    // host scratch of the layout's in-wave runs, kept between layout builds (a kernel-
+    // parameter change rebuilds the layout; fresh vectors cost their page faults each time)
+    std::vector<Chunk> gch_scratch;
+    std::vector<int> runs_scratch, first_scratch;
     bool comb_waves = false;          // g_comb is a one-launch list (launch_combine_waves)
     DevBuf<int> g_tile_seg, g_tile_e;  // first segment / entry of each tile
     bool tiled = false;                // splat through LDS tiles (else the global form)
@@ -734,12 +738,55 @@ void ensure_layout(dcrf_model& m) {
     // Segments and entries, tile by tile, the terms in layout order inside a tile: tile
     // t's entries of term k (sorted positions tile_ptr[t] ..) move to ebase[k][t] + j.
     const int NT = m.tiles.n_tiles();
-    std::vector<Chunk> gch;
+    const LaunchShape csh = shape_for(L);
+    // the LDS-tiled splat (small tiles only) if the largest tile's entries and segments fit
+    int e_max = 0, s_max = 0;
+    for (int t = 0; t < NT; t++) {
+        int et = 0, st = 0;
+        for (int k : m.order) {
+            const TermLattice& lat = m.terms[k]->lat;
+            et += lat.tile_ptr[t + 1] - lat.tile_ptr[t];
+            st += lat.chunk_tile_ptr[t + 1] - lat.chunk_tile_ptr[t];
+        }
+        e_max = std::max(e_max, et);
+        s_max = std::max(s_max, st);
+    }
+    const TileGeom tg = tile_geom(m);
+    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
+    // In-wave runs (wave-mode shapes, global splat): a vertex's segments that sit side by
+    // side in one wave are added inside the wave (k_splat folds each such run into its
+    // first lane group); the later segments of a run get out = -1 and write nothing, and
+    // the vertex gets one partial row per run, or its final row when it has one run.  At
+    // 1024x768x21 this cuts the partial rows from 188 k to 72 k and the vertices with more
+    // than 16 of them from 2,581 to 52.  The runs are part of the summation order, so
+    // they must not depend on where a segment sits in the whole list: every (tile, term)
+    // group starts a wave (empty fillers, v = out = -1, pad the group before), and the
+    // empty segments of vertices that only padding lanes touch go to waves of their own
+    // at the end.  An instance of a batch then gets the bits of the same image alone.
+    // Block-mode shapes and the tiled splat keep one row per segment.
+    const bool runs = csh.wave && !m.tiled;
+    const int per = csh.per;
+    std::vector<Chunk>& gch = m.gch_scratch;  // (capacity kept between layout builds)
+    gch.clear();
+    std::vector<int>& nruns = m.runs_scratch;
+    std::vector<int>& pfirst = m.first_scratch;
+    if (runs) {
+        nruns.assign(V + 1, 0);
+        pfirst.assign(V + 1, -1);
+    }
+    int slot = 0, prev = -1;  // position inside the current wave, the vertex before it
+    auto put = [&](const Chunk& x) {  // marks run heads (out = 1) and continuations (out = -1)
+        const bool h = x.v >= 0 && (slot == 0 || prev != x.v);
+        gch.push_back(Chunk{x.v, x.beg, x.end, h ? 1 : -1});
+        if (h) nruns[x.v]++;
+        prev = x.v;
+        if (++slot == per) slot = 0;
+    };
     std::vector<Combine> gcb;
     std::vector<int> gts(NT + 1), gte(NT + 1);
     std::vector<std::vector<int>> ebase(T, std::vector<int>(NT));
-    int e = 0, e_max = 0, s_max = 0;
-    std::vector<int> groups;  // first segment of each (tile, term) group
+    std::vector<int> empty_segs;  // vertices of the empty segments (padding lanes only)
+    int e = 0;
     for (int t = 0; t < NT; t++) {
         gts[t] = (int)gch.size();
         gte[t] = e;
@@ -747,79 +794,52 @@ void ensure_layout(dcrf_model& m) {
             const TermLattice& lat = m.terms[k]->lat;
             const int eb = e - lat.tile_ptr[t];
             ebase[k][t] = eb;
-            groups.push_back((int)gch.size());
             for (int c = lat.chunk_tile_ptr[t]; c < lat.chunk_tile_ptr[t + 1]; c++) {
                 Chunk x = lat.chunks[c];
                 x.v += m.vbase[k];
                 x.beg += eb;
                 x.end += eb;
+                if (runs) {
+                    if (x.beg == x.end)
+                        empty_segs.push_back(x.v);
+                    else
+                        put(x);
+                    continue;
+                }
                 x.out = x.out >= 0 ? x.out + m.vbase[k] : pbase[k] + (-x.out - 1);
                 gch.push_back(x);
             }
             e += lat.tile_ptr[t + 1] - lat.tile_ptr[t];
+            if (runs)
+                while (slot) put(Chunk{-1, e, e, 0});
         }
-        e_max = std::max(e_max, e - gte[t]);
-        s_max = std::max(s_max, (int)gch.size() - gts[t]);
+    }
+    if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
+    if (runs) {
+        for (int v : empty_segs) put(Chunk{v, e, e, 0});
+        while (slot) put(Chunk{-1, e, e, 0});
     }
     gts[NT] = (int)gch.size();
     gte[NT] = e;
-    groups.push_back((int)gch.size());
-    if (e != E) throw Error{DCRF_ERR_STATE, "splat layout: entry count mismatch"};
     require_adjoining(gch);
     upload(m.g_tile_seg, gts, s);
     upload(m.g_tile_e, gte, s);
-    const TileGeom tg = tile_geom(m);
-    m.tiled = tiled_splat_lds(L, tg.side, e_max, s_max, m.lds_cap) > 0;
-    // A vertex's segments that sit side by side in one wave of the global splat are
-    // added inside the wave (k_splat folds each such run into its first lane group): the
-    // later segments of a run get out = -1 and write nothing, and the vertex has one
-    // partial row per run, or its final row when it has one run.  At 1024x768x21 this
-    // cuts the partial rows from 188 k to 72 k and the vertices with more than 16 of them
-    // from 2,581 to 52.  Block-mode shapes and the tiled splat keep one row per segment.
-    const LaunchShape csh = shape_for(L);
     std::vector<Combine> combs;  // (vertex, first partial row, rows) of every split vertex
-    if (csh.wave && !m.tiled) {
-        const int per = csh.per;
-        // Every (tile, term) group of segments starts a wave (empty fillers, v = out = -1,
-        // pad the group before), and the empty segments of vertices that only padding
-        // lanes touch (listed with the first tile of the whole list) go to waves of their
-        // own in front: the runs then depend only on the group's own segments, so an
-        // instance of a batch gets the bits of the same image alone.
-        {
-            std::vector<Chunk> padded;
-            padded.reserve(gch.size() + (groups.size() + 2) * per);
-            const int pos0 = gch.empty() ? 0 : gch[0].beg;
-            for (const Chunk& x : gch)
-                if (x.beg == x.end) padded.push_back(Chunk{x.v, pos0, pos0, x.out});
-            while (padded.size() % per) padded.push_back(Chunk{-1, pos0, pos0, -1});
-            int pos = pos0;
-            for (size_t gi = 0; gi + 1 < groups.size(); gi++) {
-                for (int c = groups[gi]; c < groups[gi + 1]; c++) {
-                    if (gch[c].beg == gch[c].end) continue;
-                    padded.push_back(gch[c]);
-                    pos = gch[c].end;
-                }
-                while (padded.size() % per) padded.push_back(Chunk{-1, pos, pos, -1});
-            }
-            gch.swap(padded);
-            require_adjoining(gch);
-        }
-        auto head = [&](size_t c) { return gch[c].v >= 0 && (c % per == 0 || gch[c - 1].v != gch[c].v); };
-        std::vector<int> nruns(V + 1, 0), pfirst(V + 1, -1), used(V + 1, 0);
-        for (size_t c = 0; c < gch.size(); c++)
-            if (head(c)) nruns[gch[c].v]++;
+    if (runs) {  // outputs: partial rows contiguous per vertex, in the order of its first run
         int prow = V + 1;
-        for (size_t c = 0; c < gch.size(); c++) {
-            const int v = gch[c].v;
-            if (head(c) && nruns[v] > 1 && pfirst[v] < 0) {
+        for (Chunk& x : gch) {
+            if (x.out < 0) continue;  // a filler or a run's continuation
+            const int v = x.v;
+            if (nruns[v] == 1) {
+                x.out = v;
+                continue;
+            }
+            if (pfirst[v] < 0) {
                 pfirst[v] = prow;
                 combs.push_back(Combine{v, prow, nruns[v], 0});
                 prow += nruns[v];
             }
-        }
-        for (size_t c = 0; c < gch.size(); c++) {
-            const int v = gch[c].v;
-            gch[c].out = !head(c) ? -1 : (nruns[v] > 1 ? pfirst[v] + used[v]++ : v);
+            x.out = pfirst[v]++;  // pfirst[v] walks the vertex's partial rows
         }
         m.rows0 = prow;
     } else {

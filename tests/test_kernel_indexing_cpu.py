@@ -264,13 +264,14 @@ def splat_wave_runs(groups, per):
     groups = [[(v, n_entries), ...] per (tile, term) group], empty segments (n = 0)
     anywhere.  Returns the padded segment list [(v, out_marker)], with out_marker 'row'
     (a run head: final or partial row) or -1 (continues the segment before it)."""
-    segs = [(v, n) for g in groups for v, n in g if n == 0]
-    while len(segs) % per:
-        segs.append((-1, 0))
+    segs = []
     for g in groups:
         segs.extend((v, n) for v, n in g if n > 0)
         while len(segs) % per:
             segs.append((-1, 0))
+    segs.extend((v, n) for g in groups for v, n in g if n == 0)  # empty segments at the end
+    while len(segs) % per:
+        segs.append((-1, 0))
     out = []
     for c, (v, _) in enumerate(segs):
         head = v >= 0 and (c % per == 0 or segs[c - 1][0] != v)
