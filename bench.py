@@ -399,6 +399,8 @@ def main():
                     help="keep Q between iterations in 8x8 pixel blocks (dcrf_set_q_blocks; opt-in layout)")
     ap.add_argument("--blur-lanes", type=int, nargs=2, metavar=("R", "BELOW_V"), default=None,
                     help="blur passes below BELOW_V vertices with R labels per lane (dcrf_set_blur_lanes)")
+    ap.add_argument("--splat-chunk", type=int, default=None, metavar="C",
+                    help="contributors per splat segment (dcrf_set_splat_chunk; default 32)")
     ap.add_argument("--splat-tile", type=int, default=None, metavar="SIDE",
                     help="pixel tile side of the splat's segment order (dcrf_set_splat_tile; default 128)")
     ap.add_argument("--blur-steps", type=int, default=None, choices=[2, 3],
@@ -442,6 +444,8 @@ def main():
     m.set_unary(U)
     if args.splat_tile is not None:
         m.set_splat_tile(args.splat_tile)
+    if args.splat_chunk is not None:
+        m.set_splat_chunk(args.splat_chunk)
     if args.q_blocks:
         m.set_q_blocks(True)
     if args.blur_lanes:
@@ -537,6 +541,7 @@ def main():
                    "launch": "stream-ordered kernels on the torch stream",
                    "q_blocks": bool(args.q_blocks),
                    "splat_tile": args.splat_tile if args.splat_tile is not None else 128,
+                   "splat_chunk": args.splat_chunk if args.splat_chunk is not None else 32,
                    "blur_lanes": args.blur_lanes,
                    "blur_steps": args.blur_steps if args.blur_steps is not None else 2,
                    "vertices": {"gaussian": Vg, "bilateral": Vb}},
