@@ -3,7 +3,7 @@
     python -m pydensecrf_amd.workloads kernel-gradient [--steps S]
         DenseCRF2D 512x512, 21 labels, 5 iterations, Matrix compatibilities, learnable
         DIAG kernel parameters: S forward+backward passes through the torch.autograd
-        layer (the kernel-parameter gradient in every backward).
+        layer (the kernel-parameter gradient in every backward), after two warm-up passes.
     python -m pydensecrf_amd.workloads logistic-unary [--reps R] [--features F]
         1024x768 pixels, 21 labels, a logistic unary with F features (default 64):
         R evaluations of U = L f on the matrix cores, with the parameters changed
@@ -52,7 +52,8 @@ def kernel_gradient(steps):
         loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
         loss.backward()
 
-    step()
+    for _ in range(2):  # the second step is the first to accumulate into .grad (torch loads those kernels then)
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -89,7 +90,8 @@ def fwd_bwd(steps, size):
         loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
         loss.backward()
 
-    step()
+    for _ in range(2):  # the second step is the first to accumulate into .grad (torch loads those kernels then)
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
