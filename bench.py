@@ -324,14 +324,17 @@ def bench_fwd_bwd(torch, capi, syn, iters, reps, log):
     ms_fbk = timed(fwd_bwd)
     kp = m.kernel_params()
     Qd = torch.empty((W * H, L), dtype=torch.float32, device="cuda")
+    # one untimed change first (its buffers come from the pool once), then 5 timed ones
+    m.set_kernel_params(kp * np.float32(1.001))
+    m.inference_device(0, Qd.data_ptr())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for r in range(3):
+    for r in range(5):
         # the lattices, then the layout (built by the next call: a 0-iteration inference)
         m.set_kernel_params(kp * np.float32(1.0 + 0.01 * (r + 1)))
         m.inference_device(0, Qd.data_ptr())
     torch.cuda.synchronize()
-    ms_rebuild = (time.perf_counter() - t0) * 1e3 / 3
+    ms_rebuild = (time.perf_counter() - t0) * 1e3 / 5
     m.set_kernel_params(kp)
     log("fwd+bwd with kernel gradient %.3f ms; lattice rebuild %.2f ms" % (ms_fbk, ms_rebuild))
     # one energy evaluation of the learning loop (dcrf_gradient_objective, LogLikelihood,
