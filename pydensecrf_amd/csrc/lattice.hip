@@ -476,8 +476,11 @@ void build_lattice_d(const Geometry& g, const GeometryDev& gd, const TileMap& tm
                                                                    rep.get(), V, lat.nbr.get(), lat.keys.get());
     DCRF_HIP_CHECK(hipGetLastError());
 
-    // first vertex of every instance: the first entry of an instance is always new
-    {
+    // first vertex of every instance: the first entry of an instance is always new (one
+    // instance: vertex 0, no device round trip)
+    if (g.n_inst == 1) {
+        lat.inst_vbase.assign({0, V});
+    } else {
         std::vector<int> idx(g.n_inst);
         for (int i = 0; i < g.n_inst; i++) idx[i] = g.pad_base[i] * D1;
         DevBuf<int> didx, dout;
