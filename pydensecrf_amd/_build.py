@@ -34,10 +34,10 @@ HIPCC_FLAGS = [
 ]
 CY_MODULES = ["eigen", "densecrf"]
 # Sources whose kernels must not use scratch (per-lane private memory): the per-pixel
-# and per-vertex filter and adjoint kernels keep every array in registers or LDS.  The
-# compiler's resource-usage remarks are read at every build and the build fails on a
-# kernel with ScratchSize > 0.
-NO_SCRATCH_SOURCES = ["filter.hip", "adjoint.hip"]
+# and per-vertex filter, adjoint and kernel-gradient kernels keep every array in
+# registers or LDS.  The compiler's resource-usage remarks are read at every build and
+# the build fails on a kernel with ScratchSize > 0.
+NO_SCRATCH_SOURCES = ["filter.hip", "adjoint.hip", "kgrad.hip"]
 
 
 def _hipcc():

@@ -290,49 +290,6 @@ __global__ void k_set_compat(const float* __restrict__ raw, int ckind, int L, fl
 
 }  // namespace
 
-#define DCRF_DISPATCH(sh, kernel, grid, shm, s, ...)                                     \
-    do {                                                                                \
-        const unsigned g_ = (grid);                                                     \
-        const size_t m_ = (shm);                                                        \
-        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                     \
-            case 3: kernel<true, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 2: kernel<false, 1><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 5: kernel<true, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 4: kernel<false, 2><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 7: kernel<true, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 6: kernel<false, 3><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 9: kernel<true, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            default: kernel<false, 4><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
-        }                                                                               \
-        DCRF_HIP_CHECK(hipGetLastError());                                              \
-    } while (0)
-
-// As DCRF_DISPATCH for kernels with a third template argument CWL (compiled-in LDS
-// staging of Matrix compatibilities): CWL = (cwl != 0).
-#define DCRF_DISPATCH_T(sh, kernel, C, grid, shm, s, ...)                                   \
-    do {                                                                                   \
-        const unsigned g_ = (grid);                                                        \
-        const size_t m_ = (shm);                                                           \
-        switch ((sh).R * 2 + ((sh).wave ? 1 : 0)) {                                        \
-            case 3: kernel<true, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 2: kernel<false, 1, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 5: kernel<true, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 4: kernel<false, 2, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 7: kernel<true, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            case 6: kernel<false, 3, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;  \
-            case 9: kernel<true, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break;   \
-            default: kernel<false, 4, C><<<g_, (sh).block, m_, (s)>>>(__VA_ARGS__); break; \
-        }                                                                                  \
-        DCRF_HIP_CHECK(hipGetLastError());                                                 \
-    } while (0)
-#define DCRF_DISPATCH_CWL(sh, cwl, kernel, grid, shm, s, ...)                     \
-    do {                                                                         \
-        if (cwl)                                                                 \
-            DCRF_DISPATCH_T(sh, kernel, true, grid, shm, s, __VA_ARGS__);        \
-        else                                                                     \
-            DCRF_DISPATCH_T(sh, kernel, false, grid, shm, s, __VA_ARGS__);       \
-    } while (0)
-
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s) {
     const LaunchShape sh = shape_for(L);
     DCRF_DISPATCH(sh, k_sn_first, blocks_for(N, sh.items), sizeof(float) * sh.block, s, Q, dQ, b, dU, N, L, sh.S,

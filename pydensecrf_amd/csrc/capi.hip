@@ -135,12 +135,11 @@ struct dcrf_model {
     DevBuf<int16_t> gt_dev;
     DevBuf<float> obj_w;
     DevBuf<double> obj_partial;
-    // kernel-parameter gradient: 5 [N][L] work slabs, 2 [N][d] feature gradients,
-    // per-block partials and the accumulated gradient of all terms
+    // kernel-parameter gradient: 4 [N][L] work slabs (lbl, [a | c], x), 2 [N][d] feature
+    // gradients (one-thread-per-point form), per-block partials and the accumulated
+    // gradient of all terms
     DevBuf<float> kgbuf, kgdf;
     DevBuf<double> kpartial, kacc;
-    DevBuf<int> id_vtx;  // [N] 0..N-1 and [N] ones: Q_t read as a one-vertex slice
-    DevBuf<float> id_sw;
 
     // captured inference (dcrf_capture_inference)
     hipGraph_t graph = nullptr;
@@ -554,20 +553,14 @@ float* raw_blur(const Term& t, const float* src, float* const buf[2], int Lc, bo
     return buf[(np - 1) & 1];
 }
 
-// out = slice of the blurred rows `rows` (row stride ldv, the first L labels used).
-void raw_slice(dcrf_model& m, const Term& t, const float* rows, int ldv, float* out, hipStream_t s) {
-    SliceTable st{};
-    st.nterms = 1;
-    st.t[0] = SliceTerm{t.lat.vtx.get(), t.raw->sw.get(), rows, t.lat.d + 1, -1, nullptr, nullptr, nullptr, ldv};
-    launch_slice_term(out, m.N(), m.L, st, 0, s);
-}
-
 // acc += kernelGradient(b, Q_t) of one term (pairwise.cpp:202-207, 152-163, 88-114):
-// fg = -G(x, 1) + G(a, c) with a, c and x per the normalisation type.  The inputs a
-// and c are splatted together as [a | c] and blurred in both orders: the forward rows
-// give K a (G(a, c), direction 0) and K c, the reverse rows K^T a and K^T c (G(a, c),
-// direction 1); x needs one more splat and forward blur, and G(x, 1)'s direction 1 is
-// the plan's reverse-blurred ones.
+// fg = -G(x, 1) + G(a, c) with a, c and x per the normalisation type.  One pass forms
+// lbl = C (w Q_t) and the inputs a and c, which are splatted together as [a | c] and
+// blurred in both orders: the forward rows give K a (G(a, c), direction 0) and K c, the
+// reverse rows K^T a and K^T c (G(a, c), direction 1).  x (from the slices K^T a and K c
+// in one pass) needs one more splat and forward blur, and G(x, 1)'s direction 1 is the
+// plan's reverse-blurred ones.  The four slicing gradients and the parameter products
+// then run in one pass over the points where the lane-group kernel covers the shape.
 void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, const float* kw, double* acc,
                       hipStream_t s) {
     const int N = m.N(), L = m.L, d = t.lat.d, L2 = 2 * L;
@@ -575,26 +568,17 @@ void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, c
     RawPlan& P = raw_plan(m, t);
     float* lbl = m.kgbuf.get();
     float* pk = lbl + NL;     // [N][2L]
-    float* F1 = pk + 2 * NL;  // fa, then x
-    float* F2 = F1 + NL;      // fb
-    float* dfr = m.kgdf.get();
-    float* dfg = dfr + (size_t)N * d;
+    float* xb = pk + 2 * NL;  // x
     const float* nm = t.norm.get();
+    const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
+    const bool need_fa = has_x && t.ntype != DCRF_NORMALIZE_AFTER;  // x reads fa and lbl
+    const bool need_fb = has_x && t.ntype != DCRF_NORMALIZE_BEFORE;
     {
-        // lbl = C (w Q_t): the per-pixel compatibility kernel over a one-vertex "slice"
-        // that reads Q_t itself (vertex p = pixel p, weight 1)
-        if (!m.id_vtx.get() || m.id_vtx.n < (size_t)N) {
-            m.id_vtx.reserve(N);
-            m.id_sw.reserve(N);
-            launch_iota(m.id_vtx.get(), N, s);
-            launch_fill(m.id_sw.get(), 1.f, N, s);
-        }
         SliceTable st{};
         st.nterms = 1;
-        st.t[0] = SliceTerm{m.id_vtx.get(), m.id_sw.get(), Qt, 1, t.ckind, t.d_compat.get(), kw, nullptr, L};
-        launch_slice_term(lbl, N, L, st, 0, s);
+        st.t[0] = SliceTerm{nullptr, nullptr, nullptr, 1, t.ckind, t.d_compat.get(), kw, nullptr, L};
+        launch_kg_prepare(Qt, b, nm, N, L, st, t.ntype, need_fa ? lbl : nullptr, pk, s);
     }
-    launch_kg_pack(b, lbl, nm, N, L, t.ntype, pk, s);
     raw_splat(t, pk, L2, P.S2.get(), s);
     float* const fbuf[2] = {P.F2[0].get(), P.F2[1].get()};
     float* const rbuf[2] = {P.R2[0].get(), P.R2[1].get()};
@@ -602,22 +586,35 @@ void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, c
     const float* rev = raw_blur(t, P.S2.get(), rbuf, L2, true, s);
     const int* vtx = t.lat.vtx.get();
     const int* rank = t.lat.rank.get();
-    const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
+    const float* xr = nullptr;
     if (has_x) {
-        const bool need_fa = t.ntype != DCRF_NORMALIZE_AFTER, need_fb = t.ntype != DCRF_NORMALIZE_BEFORE;
-        if (need_fa) raw_slice(m, t, rev, L2, F1, s);      // fa = K^T a
-        if (need_fb) raw_slice(m, t, fwd + L, L2, F2, s);  // fb = K c
-        launch_kg_x(b, lbl, need_fa ? F1 : nullptr, need_fb ? F2 : nullptr, nm, N, L, t.ntype, F1, s);
-        raw_splat(t, F1, L, P.S1.get(), s);
+        SliceTable st{};
+        st.nterms = 2;
+        const float* sw = t.raw->sw.get();
+        st.t[0] = SliceTerm{vtx, sw, need_fa ? rev : nullptr, d + 1, -1, nullptr, nullptr, nullptr, L2};      // K^T a
+        st.t[1] = SliceTerm{vtx, sw, need_fb ? fwd + L : nullptr, d + 1, -1, nullptr, nullptr, nullptr, L2};  // K c
+        launch_kg_x(b, lbl, nm, N, L, st, t.ntype, xb, s);
+        raw_splat(t, xb, L, P.S1.get(), s);
         float* const xbuf[2] = {P.B1.get(), P.S1.get()};
-        const float* xr = raw_blur(t, P.S1.get(), xbuf, L, false, s);
+        xr = raw_blur(t, P.S1.get(), xbuf, L, false, s);
+    }
+    const bool full = t.ktype == DCRF_FULL_KERNEL;
+    if (kg_grad_fused(L, d)) {
+        launch_kg_grad(vtx, rank, fwd, rev, pk, xr, P.ones_rev.get(), xb, t.feat0.get(), m.geod.pix_base.get(),
+                       m.geod.pad_base.get(), m.geo.n_inst, N, L, d, full, m.kpartial.get(), acc, s);
+        return;
+    }
+    // more labels than a wave's lanes hold, or d > 6: one thread per point and direction
+    float* dfr = m.kgdf.get();
+    float* dfg = dfr + (size_t)N * d;
+    if (has_x) {
         launch_slice_grad(xr, L, 1, vtx, rank, nullptr, 0, N, L, d, 0, dfr, s);
-        launch_slice_grad(P.ones_rev.get(), 1, 0, vtx, rank, F1, L, N, L, d, 1, dfr, s);
+        launch_slice_grad(P.ones_rev.get(), 1, 0, vtx, rank, xb, L, N, L, d, 1, dfr, s);
     }
     launch_slice_grad(fwd, L2, 1, vtx, rank, pk + L, L2, N, L, d, 0, dfg, s);
     launch_slice_grad(rev + L, L2, 1, vtx, rank, pk, L2, N, L, d, 1, dfg, s);
     launch_kparam_reduce(has_x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
-                         m.geo.n_inst, N, d, t.ktype == DCRF_FULL_KERNEL, m.kpartial.get(), acc, s);
+                         m.geo.n_inst, N, d, full, m.kpartial.get(), acc, s);
 }
 
 int num_kernel_params(const dcrf_model& m) {
@@ -1245,10 +1242,13 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     for (int k = 0; k < T; k++) koff[k + 1] = koff[k] + m.terms[k]->num_kparams();
     kernel = kernel && koff[T] > 0;
     if (kernel) {
-        int dmax = 1;
-        for (auto& t : m.terms) dmax = std::max(dmax, t->lat.d);
-        m.kgbuf.reserve(5 * NL);
-        m.kgdf.reserve((size_t)2 * N * dmax);
+        int dmax = 1, dslow = 0;  // dslow: largest d of a term without the one-pass gradient
+        for (auto& t : m.terms) {
+            dmax = std::max(dmax, t->lat.d);
+            if (t->num_kparams() && !kg_grad_fused(L, t->lat.d)) dslow = std::max(dslow, t->lat.d);
+        }
+        m.kgbuf.reserve(4 * NL);
+        if (dslow) m.kgdf.reserve((size_t)2 * N * dslow);
         m.kpartial.reserve(kparam_partial_doubles(dmax));
         m.kacc.reserve(koff[T]);
         launch_fill_double(m.kacc.get(), koff[T], s);

@@ -394,13 +394,24 @@ void launch_compat_finish(const double* G, int ckind, int L, const float* cw, co
 void launch_set_compat(const float* raw, int ckind, int L, float* cw, hipStream_t s);
 
 // ---------------------------------------------------------------- kernel-parameter gradient
+// lbl (optional) = C (w Q_t) of the term in st.t[0] (its compatibility and weight), and
 // P [N][2L] = [a | c], a = b (times n for SYMMETRIC / AFTER), c = lbl (times n for
 // SYMMETRIC / BEFORE).
-void launch_kg_pack(const float* b, const float* lbl, const float* norm, int N, int L, int ntype, float* P,
+void launch_kg_prepare(const float* Qt, const float* b, const float* norm, int N, int L, const SliceTable& st,
+                       int ntype, float* lbl, float* P, hipStream_t s);
+// The normaliser-derivative input x of featureGradient for ntype, with fa sliced by
+// st.t[0] and fb by st.t[1] (a term with null vals: not needed).
+void launch_kg_x(const float* b, const float* lbl, const float* norm, int N, int L, const SliceTable& st, int ntype,
+                 float* x, hipStream_t s);
+// Whether launch_kg_grad has a kernel for L labels and feature dimension d.
+bool kg_grad_fused(int L, int d);
+// acc (d or d*d doubles) += the parameter gradient of fg = -G(x, 1) + G(a, c) (xr null:
+// fg = G(a, c)) against the padded feature table f, in one pass over the points plus
+// one ordered sum of the per-block partials (partial holds kparam_partial_doubles(d)).
+void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const float* rev, const float* P,
+                    const float* xr, const float* ones, const float* x, const float* f, const int* pix_base,
+                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial, double* acc,
                     hipStream_t s);
-// The normaliser-derivative input x of featureGradient for ntype (may overwrite fa).
-void launch_kg_x(const float* b, const float* lbl, const float* fa, const float* fb, const float* norm, int N, int L,
-                 int ntype, float* x, hipStream_t s);
 // One direction of the lattice gradient: df (+)= slicing gradient of the blurred rows
 // vals (row stride ldv, label stride ks) paired with c [N][L] (row stride ldc; null:
 // ones).
@@ -411,8 +422,6 @@ void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const
 size_t kparam_partial_doubles(int d);
 void launch_kparam_reduce(const float* r, const float* g, const float* f, const int* pix_base, const int* pad_base,
                           int n_inst, int N, int d, bool full, double* partial, double* acc, hipStream_t s);
-// out[i] = i.
-void launch_iota(int* out, int n, hipStream_t s);
 // out[i] = (float)acc[i].
 void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
 // Scaled features diag(P) f or P f (P column-major) over the padded table.

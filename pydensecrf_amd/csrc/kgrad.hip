@@ -36,49 +36,95 @@ namespace {
 constexpr int kThreads = 256;
 inline unsigned blocks_for(size_t n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
-// The two filter inputs of featureGradient (pairwise.cpp:88-114) side by side,
-// P[p] = [a | c] (2L floats): a = b n (SYMMETRIC, AFTER) or b, c = lbl n (SYMMETRIC,
-// BEFORE) or lbl.  One splat and the two blur orders of P give K a, K^T a, K c, K^T c.
-__global__ void k_kg_pack(const float* __restrict__ b, const float* __restrict__ lbl, const float* __restrict__ norm,
-                          int N, int L, int ntype, float* __restrict__ P) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * L) return;
-    const size_t p = i / L;
-    const int k = (int)(i - p * L);
-    const float n = norm[p];
+// lbl = C (w Q_t), the term's compatibility applied to its weighted Q_t
+// (PairwisePotential::kernelGradient, pairwise.cpp:202-207; st.t[0] carries the
+// compatibility and the weight), and the two filter inputs of featureGradient
+// (pairwise.cpp:88-114) side by side, P[p] = [a | c] (2L floats): a = b n (SYMMETRIC,
+// AFTER) or b, c = lbl n (SYMMETRIC, BEFORE) or lbl.  One splat and the two blur orders
+// of P give K a, K^T a, K c, K^T c.  lbl is stored only where the caller gives a buffer
+// (the normaliser input x reads it).
+template <bool WAVE, int R, bool CWL>
+__global__ void __launch_bounds__(1024) k_kg_prepare(const float* __restrict__ Qt, const float* __restrict__ b,
+                                                     const float* __restrict__ norm, int N, int L, int S, int per,
+                                                     SliceTable st, int ntype, float* __restrict__ lbl,
+                                                     float* __restrict__ P) {
+    extern __shared__ float sh[];
+    float* shc = sh + blockDim.x;
+    if (CWL) stage_compat(st, L, shc);
+    LaneGroup<WAVE, R> G(L, S, per, sh);
+    const int p = G.item();
+    const bool active = G.on && p < N;
+    const SliceTerm& T = st.t[0];
+    float m[R], bv[R], c[R];
+    float n = 0.0f;
+    if (active) {
+        load_piece<R>(Qt + (size_t)p * L, G.k0, L, m);
+        load_piece<R>(b + (size_t)p * L, G.k0, L, bv);
+        n = norm[p];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = bv[r] = 0.0f;
+    }
+    if (T.kw) {
+        const float kw = T.kw[0];
+#pragma unroll
+        for (int r = 0; r < R; r++) m[r] = m[r] * kw;
+    }
+    compat_piece<WAVE, R>(T, m, c, G, shc);  // every lane: Matrix reads its group
+    if (!active) return;
+    if (lbl) store_piece<R>(lbl + (size_t)p * L, G.k0, L, c);
     const bool na = ntype == DCRF_NORMALIZE_SYMMETRIC || ntype == DCRF_NORMALIZE_AFTER;
     const bool nc = ntype == DCRF_NORMALIZE_SYMMETRIC || ntype == DCRF_NORMALIZE_BEFORE;
-    float* row = P + p * 2 * L;
-    row[k] = na ? b[i] * n : b[i];
-    row[L + k] = nc ? lbl[i] * n : lbl[i];
+    float av[R], cv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        av[r] = na ? bv[r] * n : bv[r];
+        cv[r] = nc ? c[r] * n : c[r];
+    }
+    float* row = P + (size_t)p * 2 * L;
+    store_piece<R>(row, G.k0, L, av);
+    store_piece<R>(row + L, G.k0, L, cv);
 }
 
-// The first argument of the norm-derivative term, in place of fa (or into x):
+// The first argument x of featureGradient's norm-derivative term, with fa = K^T a
+// (st.t[0]: the reverse-blurred rows of P) and fb = K c (st.t[1]: the forward-blurred
+// rows, from label L on) sliced in the same pass; a slice whose rows are null is not
+// needed by the normalisation type:
 //   SYMMETRIC  x = (0.5 (b fb + fa lbl)) n^3      (pairwise.cpp:90-97)
 //   AFTER      x = (b fb) n^2                     (:98-105)
 //   BEFORE     x = (fa lbl) n^2                   (:106-113)
-__global__ void k_kg_x(const float* __restrict__ b, const float* __restrict__ lbl, const float* __restrict__ fa,
-                       const float* __restrict__ fb, const float* __restrict__ norm, int N, int L, int ntype,
-                       float* __restrict__ x) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * L) return;
-    const float n = norm[i / L];
+template <bool WAVE, int R>
+__global__ void __launch_bounds__(1024) k_kg_x(const float* __restrict__ b, const float* __restrict__ lbl,
+                                               const float* __restrict__ norm, int N, int L, int S, int per,
+                                               SliceTable st, int ntype, float* __restrict__ x) {
+    LaneGroup<WAVE, R> G(L, S, per, nullptr);  // no group-wide operation here
+    const int p = G.item();
+    if (!G.on || p >= N) return;
+    float fa[R] = {}, fb[R] = {}, bv[R] = {}, lv[R] = {}, v[R];
+    const bool sym = ntype == DCRF_NORMALIZE_SYMMETRIC, after = ntype == DCRF_NORMALIZE_AFTER;
+    if (st.t[0].vals) slice_piece<R>(st.t[0], p, G.k0, L, fa);
+    if (st.t[1].vals) slice_piece<R>(st.t[1], p, G.k0, L, fb);
+    if (sym || after) load_piece<R>(b + (size_t)p * L, G.k0, L, bv);
+    if (!after) load_piece<R>(lbl + (size_t)p * L, G.k0, L, lv);
+    const float n = norm[p];
     const float n2 = n * n;
-    float v;
-    if (ntype == DCRF_NORMALIZE_SYMMETRIC) {
-        const float p1 = b[i] * fb[i], p2 = fa[i] * lbl[i];
-        const float sum = p1 + p2;
-        const float h = 0.5f * sum;
-        const float n3 = n2 * n;
-        v = h * n3;
-    } else if (ntype == DCRF_NORMALIZE_AFTER) {
-        const float p = b[i] * fb[i];
-        v = p * n2;
-    } else {
-        const float p = fa[i] * lbl[i];
-        v = p * n2;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (sym) {
+            const float p1 = bv[r] * fb[r], p2 = fa[r] * lv[r];
+            const float sum = p1 + p2;
+            const float h = 0.5f * sum;
+            const float n3 = n2 * n;
+            v[r] = h * n3;
+        } else if (after) {
+            const float q = bv[r] * fb[r];
+            v[r] = q * n2;
+        } else {
+            const float q = fa[r] * lv[r];
+            v[r] = q * n2;
+        }
     }
-    x[i] = v;
+    store_piece<R>(x + (size_t)p * L, G.k0, L, v);
 }
 
 struct GradScale {
@@ -153,51 +199,37 @@ __global__ void __launch_bounds__(kThreads) k_slice_grad(const float* __restrict
     }
 }
 
-// The same with S lanes per point, R consecutive labels per lane (S <= 64: the lane
-// groups of one wave), for d <= 6: each lane loads its pieces of the d+1 vertex rows
-// once, forms the rotated differences by select, and the d gradients are summed over
-// the lanes of the group.  The label order of the sum differs from the reference's
-// sequential one (float tolerance).
+// The lane's pieces of the point's d+1 vertex rows for one direction of the slicing
+// gradient: au[q][r] = vals[v[q]][k0 + r] (row stride ldv), or with per_vertex the one
+// value per vertex for every label.
 template <int R, int D>
-__global__ void __launch_bounds__(kWaveBlock) k_slice_grad_lanes(const float* __restrict__ vals, int ldv, int ks,
-                                                                 const int* __restrict__ vtx,
-                                                                 const int* __restrict__ rank,
-                                                                 const float* __restrict__ c, int ldc, int N, int L,
-                                                                 int S, int per, GradScale gs, int accumulate,
-                                                                 float* __restrict__ df) {
-    constexpr int D1 = D + 1;
-    LaneGroup<true, R> G(L, S, per, nullptr);
-    const int p = G.item();
-    const bool active = G.on && p < N;
-    const int pp = active ? p : 0;
-    int v[D1], r0[D1];
+__device__ __forceinline__ void grad_rows(const float* __restrict__ vals, int ldv, bool per_vertex,
+                                          const int (&v)[D + 1], int k0, int L, float (&au)[D + 1][R]) {
 #pragma unroll
-    for (int q = 0; q < D1; q++) {
-        v[q] = vtx[(size_t)pp * D1 + q];
-        r0[q] = D - rank[(size_t)pp * D1 + q];
-    }
-    // au[q][r] = alpha * vals[v[q]][k0 + r]
-    float au[D1][R];
-#pragma unroll
-    for (int q = 0; q < D1; q++) {
-        if (ks) {
-            load_piece<R>(vals + (size_t)v[q] * ldv, G.k0, L, au[q]);
+    for (int q = 0; q < D + 1; q++) {
+        if (!per_vertex) {
+            load_piece<R>(vals + (size_t)v[q] * ldv, k0, L, au[q]);
         } else {
-            const float x = vals[(size_t)v[q] * ldv];
+            const float x = vals[v[q]];
 #pragma unroll
             for (int r = 0; r < R; r++) au[q][r] = x;
         }
-#pragma unroll
-        for (int r = 0; r < R; r++) au[q][r] = gs.alpha * au[q][r];
     }
-    float cv[R];
-    if (c) {
-        load_piece<R>(c + (size_t)pp * ldc, G.k0, L, cv);
-    } else {
+}
+
+// The lane's share of one direction of the slicing gradient (the loop body of
+// k_slice_grad) for its R consecutive labels, from the rows grad_rows loaded: the rotated
+// differences of alpha * au are formed by select, elevated and paired with cv; grad[j]
+// sums the lane's labels in order.  The lane group adds the shares afterwards.
+template <int R, int D>
+__device__ __forceinline__ void grad_share(const float (&au_in)[D + 1][R], const int (&r0)[D + 1],
+                                           const float (&cv)[R], const GradScale& gs, float* grad) {
+    constexpr int D1 = D + 1;
+    float au[D1][R];
 #pragma unroll
-        for (int r = 0; r < R; r++) cv[r] = 1.0f;
-    }
-    float grad[D];
+    for (int q = 0; q < D1; q++)
+#pragma unroll
+        for (int r = 0; r < R; r++) au[q][r] = gs.alpha * au_in[q][r];
 #pragma unroll
     for (int j = 0; j < D; j++) grad[j] = 0.0f;
 #pragma unroll
@@ -225,20 +257,141 @@ __global__ void __launch_bounds__(kWaveBlock) k_slice_grad_lanes(const float* __
             sm = sm + ra[j];
         }
     }
+}
+
+// The operands of k_kg_grad.
+struct KgGrad {
+    const float* fwd;   // forward-blurred rows of P (stride 2L): K a in labels 0..L-1
+    const float* rev;   // reverse-blurred rows of P (stride 2L): K^T c in labels L..2L-1
+    const float* P;     // [N][2L] = [a | c]
+    const float* xr;    // forward-blurred rows of x (stride L); null: no normaliser term
+    const float* ones;  // the reverse-blurred all-ones lattice, one value per vertex
+    const float* x;     // [N][L]
+    const float* f;     // the term's features in the padded point layout, d per point
+    const int* pix_base;
+    const int* pad_base;
+    int n_inst;
+    int full;  // FULL_KERNEL: d*d parameters, else d
+};
+
+// The whole per-point part of one term's kernel gradient in one pass, S lanes per point
+// and R labels per lane (S <= 64: lane groups inside one wave), d <= 6:
+//   g = G(a, c) = dir 0 (K a rows, paired with c) + dir 1 (K^T c rows, paired with a),
+//   r = G(x, 1) = dir 0 (K x rows, paired with ones) + dir 1 (K^T 1, paired with x),
+//   fg = -r + g (g alone without a normaliser term),
+// each direction summed over the group's lanes (a pairwise tree towards the group's
+// first lane) before the two are added; the 4d lane shares go through one tree together.
+// The parameter products (pairwise.cpp:157-161), DIAG f[q(p)][j] fg[j], FULL fg[r]
+// f[q(p)][c] at c*d + r (column-major, like the reference), are added in double by one
+// lane per parameter: per round the group leaders leave fg and the point's feature row
+// in LDS and lane q of the wave adds its product for the wave's points in point order,
+// over the block's `rounds`
+// consecutive blocks' worth of points; at the end the waves' sums are added in wave
+// order into partial[blockIdx.x][q].  The label order of each direction's sum differs
+// from the reference's sequential one (float tolerance).
+template <int R, int D>
+__global__ void __launch_bounds__(kWaveBlock) k_kg_grad(const int* __restrict__ vtx, const int* __restrict__ rank,
+                                                        KgGrad a, int N, int L, int S, int per, int rounds,
+                                                        GradScale gs, double* __restrict__ partial) {
+    constexpr int D1 = D + 1, W = kWaveBlock / 64;
+    __shared__ float fgl[W][64][D];  // per wave: fg of each group's point
+    __shared__ float ffl[W][64][D];  // and the point's feature row
+    __shared__ double wsum[W][D * D];
+    const int np = a.full ? D * D : D;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    LaneGroup<true, R> G(L, S, per, nullptr);
+    const int L2 = 2 * L;
+    const bool nx = a.xr != nullptr;
+    // lane q < np: parameter q = (row, col) of the products fg[row] * f[col]
+    const int prow = a.full ? lane % D : lane, pcol = a.full ? lane / D : lane;
+    double acc = 0.0;
+    const int b0 = xcd_block(blockIdx.x, gridDim.x) * rounds;
+    for (int i = 0; i < rounds; i++) {  // uniform over the block: every lane takes part in every shuffle
+        const int p = G.item_in(b0 + i);
+        const bool active = G.on && p < N;
+        const int pp = active ? p : 0;
+        const bool lead = active && G.l == 0;
+        int v[D1], r0[D1];
 #pragma unroll
-    for (int j = 0; j < D; j++) {
-        float x[R];
-        x[0] = grad[j];
-#pragma unroll
-        for (int r = 1; r < R; r++) x[r] = 0.0f;
-        grad[j] = G.all_sum(x);  // every lane takes part in the shuffles
-    }
-    if (active && G.l == 0) {
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            float* o = df + (size_t)p * D + j;
-            *o = accumulate ? *o + grad[j] : grad[j];
+        for (int q = 0; q < D1; q++) {
+            v[q] = vtx[(size_t)pp * D1 + q];
+            r0[q] = D - rank[(size_t)pp * D1 + q];
         }
+        // the leader's feature row of the point, for the products below
+        float fv[D];
+        {
+            int fq = 0;
+            if (lead) {
+                const int inst = a.n_inst > 1 ? last_le(a.pix_base, a.n_inst, p) : 0;
+                fq = a.pad_base[inst] + (p - a.pix_base[inst]);
+            }
+#pragma unroll
+            for (int j = 0; j < D; j++) fv[j] = lead ? a.f[(size_t)fq * D + j] : 0.0f;
+        }
+        // shares [dir 0 of G(a, c) | dir 1 | dir 0 of G(x, 1) | dir 1], d each; a
+        // direction's rows are loaded where its share is formed
+        float sh[4 * D];
+        {
+            float u[D1][R], cv[R];
+            load_piece<R>(a.P + (size_t)pp * L2 + L, G.k0, L, cv);
+            grad_rows<R, D>(a.fwd, L2, false, v, G.k0, L, u);
+            grad_share<R, D>(u, r0, cv, gs, sh);
+            load_piece<R>(a.P + (size_t)pp * L2, G.k0, L, cv);
+            grad_rows<R, D>(a.rev + L, L2, false, v, G.k0, L, u);
+            grad_share<R, D>(u, r0, cv, gs, sh + D);
+            if (nx) {
+#pragma unroll
+                for (int r = 0; r < R; r++) cv[r] = 1.0f;
+                grad_rows<R, D>(a.xr, L, false, v, G.k0, L, u);
+                grad_share<R, D>(u, r0, cv, gs, sh + 2 * D);
+                load_piece<R>(a.x + (size_t)pp * L, G.k0, L, cv);
+                grad_rows<R, D>(a.ones, 1, true, v, G.k0, L, u);
+                grad_share<R, D>(u, r0, cv, gs, sh + 3 * D);
+            } else {
+#pragma unroll
+                for (int j = 2 * D; j < 4 * D; j++) sh[j] = 0.0f;
+            }
+        }
+        // the group's sums in its first lane (the tree of LaneGroup::all_sum)
+        for (int k = 1; k < S; k <<= 1) {
+            float y[4 * D];
+#pragma unroll
+            for (int j = 0; j < 4 * D; j++) y[j] = __shfl(sh[j], (G.base + G.l + k) & 63);
+            if ((G.l & (2 * k - 1)) == 0 && G.l + k < S) {
+#pragma unroll
+                for (int j = 0; j < 4 * D; j++) sh[j] = sh[j] + y[j];
+            }
+        }
+        if (lead) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const float g = sh[j] + sh[D + j];
+                const float r = sh[2 * D + j] + sh[3 * D + j];
+                fgl[w][G.g][j] = nx ? -r + g : g;
+                ffl[w][G.g][j] = fv[j];
+            }
+        }
+        __syncthreads();
+        if (lane < np) {
+            const int ng = min(per, N - (G.item_in(b0 + i) - G.g));  // the wave's points in this round
+            for (int gi = 0; gi < ng; gi++) {
+                float fgv = fgl[w][gi][0], fcv = ffl[w][gi][0];
+#pragma unroll
+                for (int j = 1; j < D; j++) {
+                    fgv = prow == j ? fgl[w][gi][j] : fgv;
+                    fcv = pcol == j ? ffl[w][gi][j] : fcv;
+                }
+                acc += (double)fgv * (double)fcv;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane < np) wsum[w][lane] = acc;
+    __syncthreads();
+    for (int q = threadIdx.x; q < np; q += blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < W; k++) s = s + wsum[k][q];
+        partial[(size_t)blockIdx.x * np + q] = s;
     }
 }
 
@@ -300,11 +453,6 @@ __global__ void __launch_bounds__(kThreads) k_kparam_final(const double* __restr
     if (threadIdx.x == 0) acc[q] += red[0];
 }
 
-__global__ void k_iota(int* __restrict__ out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = i;
-}
-
 __global__ void k_kparam_store(const double* __restrict__ acc, int n, float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (float)acc[i];
@@ -334,60 +482,74 @@ __global__ void k_scale_features(const float* __restrict__ f, int npad, int d, i
 
 }  // namespace
 
-constexpr int kKparamBlocks = 512;
+constexpr int kKparamBlocks = 512;   // k_kparam_partial's grid
+constexpr int kKgGradBlocks = 2048;  // at most this many blocks of k_kg_grad
 
-size_t kparam_partial_doubles(int d) { return (size_t)kKparamBlocks * d * d; }
+size_t kparam_partial_doubles(int d) { return (size_t)std::max(kKparamBlocks, kKgGradBlocks) * d * d; }
 
-void launch_kg_pack(const float* b, const float* lbl, const float* norm, int N, int L, int ntype, float* P,
+// The reference's constants (permutohedral.cpp:620-628): double, stored as float.
+static GradScale grad_scale(int d) {
+    GradScale gs{};
+    const float inv_std_dev = (float)(std::sqrt(2.0 / 3.0) * (d + 1));
+    for (int i = 0; i < d; i++) gs.sf[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * inv_std_dev);
+    gs.alpha = 1.0f / (1 + std::pow(2.0f, (float)-d)) / (float)(d + 1);
+    return gs;
+}
+
+void launch_kg_prepare(const float* Qt, const float* b, const float* norm, int N, int L, const SliceTable& st_in,
+                       int ntype, float* lbl, float* P, hipStream_t s) {
+    if (!N) return;
+    SliceTable st = st_in;
+    const int cwf = assign_compat_lds(st, L);
+    const LaunchShape sh = shape_for(L);
+    DCRF_DISPATCH_CWL(sh, cwf, k_kg_prepare, blocks_for(N, sh.items), sizeof(float) * (sh.block + cwf), s, Qt, b,
+                      norm, N, L, sh.S, sh.per, st, ntype, lbl, P);
+}
+
+void launch_kg_x(const float* b, const float* lbl, const float* norm, int N, int L, const SliceTable& st, int ntype,
+                 float* x, hipStream_t s) {
+    if (!N) return;
+    const LaunchShape sh = shape_for(L);
+    DCRF_DISPATCH(sh, k_kg_x, blocks_for(N, sh.items), 0, s, b, lbl, norm, N, L, sh.S, sh.per, st, ntype, x);
+}
+
+bool kg_grad_fused(int L, int d) { return d >= 1 && d <= 6 && shape_for(L).wave; }
+
+void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const float* rev, const float* P,
+                    const float* xr, const float* ones, const float* x, const float* f, const int* pix_base,
+                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial, double* acc,
                     hipStream_t s) {
-    const size_t n = (size_t)N * L;
-    if (!n) return;
-    k_kg_pack<<<blocks_for(n), kThreads, 0, s>>>(b, lbl, norm, N, L, ntype, P);
+    const int np = full ? d * d : d;
+    if (!N || !np) return;
+    if (!kg_grad_fused(L, d)) throw Error{DCRF_ERR_STATE, "kernel gradient: no lane-group kernel for this shape"};
+    const LaunchShape sh = shape_for(L);
+    const int need = (int)blocks_for(N, sh.items);
+    const int rounds = (need + kKgGradBlocks - 1) / kKgGradBlocks;
+    const int nb = (need + rounds - 1) / rounds;  // nb * rounds >= need, nb <= kKgGradBlocks
+    const KgGrad a{fwd, rev, P, xr, ones, x, f, pix_base, pad_base, n_inst, full ? 1 : 0};
+    const GradScale gs = grad_scale(d);
+#define DCRF_KGG(RR, DD)                                                                                        \
+    case RR * 8 + DD:                                                                                           \
+        k_kg_grad<RR, DD><<<nb, kWaveBlock, 0, s>>>(vtx, rank, a, N, L, sh.S, sh.per, rounds, gs, partial); \
+        break;
+    switch (sh.R * 8 + d) {
+        DCRF_KGG(1, 1) DCRF_KGG(1, 2) DCRF_KGG(1, 3) DCRF_KGG(1, 4) DCRF_KGG(1, 5) DCRF_KGG(1, 6)
+        DCRF_KGG(2, 1) DCRF_KGG(2, 2) DCRF_KGG(2, 3) DCRF_KGG(2, 4) DCRF_KGG(2, 5) DCRF_KGG(2, 6)
+        DCRF_KGG(3, 1) DCRF_KGG(3, 2) DCRF_KGG(3, 3) DCRF_KGG(3, 4) DCRF_KGG(3, 5) DCRF_KGG(3, 6)
+        DCRF_KGG(4, 1) DCRF_KGG(4, 2) DCRF_KGG(4, 3) DCRF_KGG(4, 4) DCRF_KGG(4, 5) DCRF_KGG(4, 6)
+        default:
+            throw Error{DCRF_ERR_STATE, "kernel gradient: no lane-group kernel for this shape"};
+    }
+#undef DCRF_KGG
     DCRF_HIP_CHECK(hipGetLastError());
-}
-
-void launch_kg_x(const float* b, const float* lbl, const float* fa, const float* fb, const float* norm, int N, int L,
-                 int ntype, float* x, hipStream_t s) {
-    const size_t n = (size_t)N * L;
-    if (!n) return;
-    k_kg_x<<<blocks_for(n), kThreads, 0, s>>>(b, lbl, fa, fb, norm, N, L, ntype, x);
-    DCRF_HIP_CHECK(hipGetLastError());
-}
-
-void launch_iota(int* out, int n, hipStream_t s) {
-    if (n <= 0) return;
-    k_iota<<<blocks_for(n), kThreads, 0, s>>>(out, n);
+    k_kparam_final<<<np, kThreads, 0, s>>>(partial, nb, np, acc);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 
 void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const int* rank, const float* c, int ldc,
                        int N, int L, int d, int accumulate, float* df, hipStream_t s) {
     if (!N) return;
-    GradScale gs{};
-    // the reference's constants (permutohedral.cpp:620-628): double, stored as float
-    const float inv_std_dev = (float)(std::sqrt(2.0 / 3.0) * (d + 1));
-    for (int i = 0; i < d; i++) gs.sf[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * inv_std_dev);
-    gs.alpha = 1.0f / (1 + std::pow(2.0f, (float)-d)) / (float)(d + 1);
-    const LaunchShape sh = shape_for(L);
-    if (sh.wave && d <= 6) {
-        const unsigned g = blocks_for(N, sh.items);
-#define DCRF_SGL(RR, DD)                                                                                       \
-    case RR * 8 + DD:                                                                                          \
-        k_slice_grad_lanes<RR, DD><<<g, kWaveBlock, 0, s>>>(vals, ldv, ks, vtx, rank, c, ldc, N, L, sh.S, sh.per, \
-                                                           gs, accumulate, df);                                \
-        break;
-        switch (sh.R * 8 + d) {
-            DCRF_SGL(1, 1) DCRF_SGL(1, 2) DCRF_SGL(1, 3) DCRF_SGL(1, 4) DCRF_SGL(1, 5) DCRF_SGL(1, 6)
-            DCRF_SGL(2, 1) DCRF_SGL(2, 2) DCRF_SGL(2, 3) DCRF_SGL(2, 4) DCRF_SGL(2, 5) DCRF_SGL(2, 6)
-            DCRF_SGL(3, 1) DCRF_SGL(3, 2) DCRF_SGL(3, 3) DCRF_SGL(3, 4) DCRF_SGL(3, 5) DCRF_SGL(3, 6)
-            DCRF_SGL(4, 1) DCRF_SGL(4, 2) DCRF_SGL(4, 3) DCRF_SGL(4, 4) DCRF_SGL(4, 5) DCRF_SGL(4, 6)
-            default:
-                throw Error{DCRF_ERR_STATE, "slice gradient: no lane-group kernel for this shape"};
-        }
-#undef DCRF_SGL
-        DCRF_HIP_CHECK(hipGetLastError());
-        return;
-    }
+    const GradScale gs = grad_scale(d);
     const unsigned nb = blocks_for(N);
 #define DCRF_SG(DD)                                                                                              \
     case DD:                                                                                                     \

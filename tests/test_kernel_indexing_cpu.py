@@ -18,7 +18,10 @@ use (the one-label fixtures included: one lane per pixel, 64 pixels per wave):
 
 It also restates launch_sum_rows's first kernel (adjoint.hip), which adds the adjoint's
 per-block statistic rows and the matrix-core statistic's tiles: every entry is read
-exactly once and every write stays inside its buffer.
+exactly once and every write stays inside its buffer; and the one-pass kernel gradient
+k_kg_grad (kgrad.hip): its capped grid with several rounds per block takes every point
+exactly once, each parameter lane adds every point's product exactly once from the LDS
+row its group's leader wrote, and its partial rows fit the buffer the backward reserves.
 """
 import numpy as np
 import pytest
@@ -311,3 +314,65 @@ def test_splat_wave_runs(per):
     first = [s for s in alone if s[0] >= 0]
     inlist = [s for s in segs if 5000 <= s[0] < 5200]
     assert first == inlist
+
+
+KG_GRAD_BLOCKS = 2048  # kgrad.hip kKgGradBlocks
+
+
+def kg_grad_launch(N, L, d, full, kparam_blocks=512):
+    """k_kg_grad (kgrad.hip): the grid and rounds launch_kg_grad picks, the points each
+    lane group takes over its rounds (its leader leaves fg in the wave's LDS rows), and
+    the points lane q < np of each wave reads back for its parameter products, with the
+    feature-row and partial-row indices.  Returns, per point, how many groups took it
+    and how many times each parameter lane added its product."""
+    R = labels_per_lane(L)
+    S = (L + R - 1) // R
+    assert S <= 64
+    per = 64 // S
+    items = (WAVE_BLOCK // 64) * per
+    need = (N + items - 1) // items
+    rounds = (need + KG_GRAD_BLOCKS - 1) // KG_GRAD_BLOCKS
+    nb = (need + rounds - 1) // rounds
+    assert nb <= KG_GRAD_BLOCKS and nb * rounds >= need
+    np_ = d * d if full else d
+    assert np_ <= 64  # one lane per parameter
+    # partial rows: blockIdx.x * np + q inside kparam_partial_doubles(dmax) for d <= dmax
+    assert nb * np_ <= max(kparam_blocks, KG_GRAD_BLOCKS) * d * d
+    t = np.arange(WAVE_BLOCK)
+    w = t >> 6
+    lane = t & 63
+    g = lane // S
+    l = lane - g * S
+    on = g < per
+    g = np.where(on, g, 0)
+    l = np.where(on, l, np.where(l < S, l, 0))
+    assert np.all(g < 64)  # fgl[w][g] rows
+    taken = np.zeros(N, np.int64)
+    added = np.zeros((np_, N), np.int64)
+    for b in range(nb):
+        b0 = xcd_block(b, nb) * rounds
+        for i in range(rounds):
+            p = ((b0 + i) * (WAVE_BLOCK // 64) + w) * per + g
+            active = on & (p < N)
+            assert np.all(np.where(active, p, 0) < max(N, 1))  # vtx / rank / P / x rows
+            lead = active & (l == 0)
+            np.add.at(taken, p[lead], 1)
+            written = {(int(ww), int(gg)): int(pt) for ww, gg, pt in zip(w[lead], g[lead], p[lead])}
+            for q in range(np_):
+                for ww in range(WAVE_BLOCK // 64):
+                    tq = ww * 64 + q
+                    for gi in range(per):
+                        pg = int(p[tq]) - int(g[tq]) + gi
+                        if pg >= N:
+                            break
+                        assert written[(ww, gi)] == pg  # the row lane q reads holds pg's fg
+                        added[q, pg] += 1
+    return taken, added
+
+
+@pytest.mark.parametrize("N,L,d,full", [(262144, 21, 5, False), (262144, 21, 2, True), (3072, 21, 5, True),
+                                        (100, 1, 2, False), (4097, 4, 6, True), (100003, 21, 5, False),
+                                        (12, 64, 3, False), (61, 256, 1, True)])
+def test_kg_grad_points_once(N, L, d, full):
+    taken, added = kg_grad_launch(N, L, d, full)
+    assert np.all(taken == 1) and np.all(added == 1)
