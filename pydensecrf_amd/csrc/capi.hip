@@ -135,9 +135,12 @@ struct dcrf_model {
     DevBuf<int16_t> gt_dev;
     DevBuf<float> obj_w;
     DevBuf<double> obj_partial;
-    // kernel-parameter gradient: 4 [N][L] work slabs (lbl, [a | c], x), 2 [N][d] feature
-    // gradients (one-thread-per-point form), per-block partials and the accumulated
-    // gradient of all terms
+    // kernel-parameter gradient: the filter plans (built on first use for the lattices of
+    // generation lat_gen, which every lattice change advances), 4 [N][L] work slabs per
+    // term of a plan (lbl, [a | c], x), 2 [N][d] feature gradients (one-thread-per-point
+    // form), per-block partials and the accumulated gradient of all terms
+    std::vector<std::unique_ptr<KgPlan>> kgplans;
+    int lat_gen = 0;
     DevBuf<float> kgbuf, kgdf;
     DevBuf<double> kpartial, kacc;
 
@@ -460,161 +463,325 @@ void build_pass_tables(const int2* nbr, int V, int d, int vbase, int zero_row, s
 }
 
 // ---------------------------------------------------------------- kernel-parameter gradient
-// The term's own lattice filter without normaliser (the lattice_.compute calls of
-// DenseKernel::featureGradient, pairwise.cpp:88-114), built on first use: segments and
-// entries in the term's sorted order, term-local rows (zero row V, partials from V+1),
-// plus the all-ones input splatted and blurred in reverse order (one value per vertex).
-RawPlan& raw_plan(dcrf_model& m, Term& t) {
-    if (t.raw) return *t.raw;
-    const TermLattice& lat = t.lat;
-    const int N = m.N(), d1 = lat.d + 1, V = lat.V, E = N * d1;
+// The filter plan of the kernel-parameter gradient for the model terms `terms` (at most
+// DCRF_MAX_TERMS / 2): the layouts of KgPlan, the merged segment and entry lists, the
+// pass tables of every chain, and per term the slice weights and the all-ones input
+// splatted and blurred in reverse order (one value per vertex).
+std::unique_ptr<KgPlan> build_kg_plan(dcrf_model& m, std::vector<int> terms) {
     hipStream_t s = m.stream;
-    auto P = std::make_unique<RawPlan>();
-    std::vector<Chunk> hch = lat.chunks;
-    for (auto& c : hch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
-    require_adjoining(hch);
-    std::vector<Combine> hcb = lat.combines;
-    for (auto& c : hcb) c.first += V + 1;
-    upload(P->ch, hch, s);
-    upload(P->cb, hcb, s);
-    P->nch = (int)hch.size();
-    P->ncb = (int)hcb.size();
-    P->ent.reserve(E);
-    launch_entry_gather(lat.csr_e.get(), E, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
-                        P->ent.get(), s);
-    P->nbr.reserve((size_t)d1 * V);
-    launch_remap_nbr(lat.nbr.get(), P->nbr.get(), d1 * V, 0, V, s);
-    build_pass_tables(P->nbr.get(), V, lat.d, 0, V, P->pass_steps, P->pass_off, P->passF, P->passT, s);
-    DevBuf<int> gv;
-    gv.reserve(E);
-    P->sw.reserve(E);
-    launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(lat.d), N, d1, 0, gv.get(),
-                         P->sw.get(), s);
-    DevBuf<float> ones, v0, v1;
-    ones.reserve(N);
-    launch_fill(ones.get(), 1.f, N, s);
-    const int rows = V + 1 + lat.n_partials;
-    v0.reserve(rows);
-    v1.reserve(V + 1);
-    launch_fill(v0.get(), 0.f, rows, s);
-    launch_fill(v1.get(), 0.f, V + 1, s);
-    launch_splat(P->ch.get(), P->nch, P->ent.get(), ones.get(), 1, v0.get(), 1, 1, s);
-    launch_combine(P->cb.get(), P->ncb, v0.get(), 1, 1, s);
-    BlurTable bt{};
-    bt.nterms = 1;
-    bt.t[0] = BlurTerm{0, V, lat.d, P->nbr.get()};
-    float* bufs[2] = {v0.get(), v1.get()};
-    for (int step = 0; step <= lat.d; step++)
-        launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, 1, 1, bt, step, 1, s);
-    if (d1 & 1)
-        P->ones_rev = std::move(v1);
-    else
-        P->ones_rev = std::move(v0);
-    // the plan's own vertex rows: the zero rows V stay zero (the splat writes rows < V
-    // and the partial rows, the blur rows < V)
+    const int N = m.N();
     const size_t L = m.L;
+    auto P = std::make_unique<KgPlan>();
+    auto passes_of = [&](int k) { return (int)pass_schedule(m.terms[k]->lat.d + 1, kMaxPassSteps).size(); };
+    std::stable_sort(terms.begin(), terms.end(), [&](int a, int b) { return passes_of(a) > passes_of(b); });
+    int row = 0, xrow = 0, parts_total = 0;
+    size_t E = 0;
+    for (int k : terms) {
+        const TermLattice& lat = m.terms[k]->lat;
+        KgPart part;
+        part.term = k;
+        part.V = lat.V;
+        part.steps = pass_schedule(lat.d + 1, kMaxPassSteps);
+        part.passes = (int)part.steps.size();
+        part.F = row;
+        part.R = row + lat.V;
+        part.X = xrow;
+        row += 2 * lat.V;
+        xrow += lat.V;
+        parts_total += lat.n_partials;
+        E += (size_t)N * (lat.d + 1);
+        P->passes = std::max(P->passes, part.passes);
+        P->parts.push_back(std::move(part));
+    }
+    P->Z2 = row;
+    P->Z1 = xrow;
+    // merged segments: entries of the i-th term from e0, its partial rows from p0 after
+    // each layout's zero row
+    std::vector<Chunk> ch2, ch1;
+    std::vector<Combine> cb2, cb1;
+    P->ent.reserve(E);
+    size_t e0 = 0;
+    int p0 = 0;
+    for (size_t i = 0; i < P->parts.size(); i++) {
+        const KgPart& part = P->parts[i];
+        const TermLattice& lat = m.terms[part.term]->lat;
+        const int d1 = lat.d + 1;
+        launch_entry_gather(lat.csr_e.get(), N * d1, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
+                            P->ent.get() + e0, s, (int)i * N);
+        for (const Chunk& c : lat.chunks) {
+            Chunk a = c;
+            a.beg += (int)e0;
+            a.end += (int)e0;
+            a.v = part.F + c.v;
+            a.out = c.out >= 0 ? part.F + c.out : P->Z2 + 1 + p0 + (-c.out - 1);
+            ch2.push_back(a);
+            a.v = part.X + c.v;
+            a.out = c.out >= 0 ? part.X + c.out : P->Z1 + 1 + p0 + (-c.out - 1);
+            ch1.push_back(a);
+        }
+        for (const Combine& c : lat.combines) {
+            Combine a = c;
+            a.v = part.F + c.v;
+            a.first = P->Z2 + 1 + p0 + c.first;
+            cb2.push_back(a);
+            a.v = part.X + c.v;
+            a.first = P->Z1 + 1 + p0 + c.first;
+            cb1.push_back(a);
+        }
+        e0 += (size_t)N * d1;
+        p0 += lat.n_partials;
+    }
+    require_adjoining(ch2);
+    if (!ch2.empty() && (ch2.front().beg != 0 || (size_t)ch2.back().end != E))
+        throw Error{DCRF_ERR_STATE, "kernel gradient: segments do not cover the entries"};
+    upload(P->ch2, ch2, s);
+    upload(P->ch1, ch1, s);
+    upload(P->cb2, cb2, s);
+    upload(P->cb1, cb1, s);
+    P->nch = (int)ch2.size();
+    P->ncb = (int)cb2.size();
+    // pass tables: pass p of term i expands the axes of its p-th pass; the forward chain
+    // reads rows from F, the reverse chain from F in its first pass (the splat rows) and
+    // from R after, the x chain from X
+    size_t tab_total = 0;
+    for (const KgPart& part : P->parts)
+        for (int q : part.steps) tab_total += (size_t)3 * part.V * (q == 1 ? 3 : 9);
+    P->tab.reserve(std::max<size_t>(tab_total, 1));
+    P->tab_off.assign(P->passes, std::vector<int>(3 * P->parts.size(), -1));
+    P->vp2.assign(P->passes, 0);
+    P->vp1.assign(P->passes, 0);
+    size_t toff = 0;
+    for (size_t i = 0; i < P->parts.size(); i++) {
+        const KgPart& part = P->parts[i];
+        const TermLattice& lat = m.terms[part.term]->lat;
+        const int V = lat.V, d = lat.d, d1 = d + 1;
+        DevBuf<int2> nF, nR, nX;  // the neighbour rows in each chain's numbering
+        nF.reserve((size_t)d1 * V);
+        nR.reserve((size_t)d1 * V);
+        nX.reserve((size_t)d1 * V);
+        launch_remap_nbr(lat.nbr.get(), nF.get(), d1 * V, part.F, P->Z2, s);
+        launch_remap_nbr(lat.nbr.get(), nR.get(), d1 * V, part.R, P->Z2, s);
+        launch_remap_nbr(lat.nbr.get(), nX.get(), d1 * V, part.X, P->Z1, s);
+        int first = 0;
+        for (int p = 0; p < part.passes; p++) {
+            const int q = part.steps[p];
+            int fa[3], ta[3];
+            for (int j = 0; j < q; j++) {
+                fa[j] = first + j;
+                ta[j] = d - first - j;
+            }
+            const size_t n = (size_t)V * (q == 1 ? 3 : 9);
+            std::vector<int>& off = P->tab_off[p];
+            off[3 * i] = (int)toff;
+            launch_pass_table(nF.get(), V, part.F, P->Z2, q, fa, P->tab.get() + toff, s);
+            toff += n;
+            off[3 * i + 1] = (int)toff;
+            if (p == 0)
+                launch_pass_table(nF.get(), V, part.F, P->Z2, q, ta, P->tab.get() + toff, s);
+            else
+                launch_pass_table(nR.get(), V, part.R, P->Z2, q, ta, P->tab.get() + toff, s);
+            toff += n;
+            off[3 * i + 2] = (int)toff;
+            launch_pass_table(nX.get(), V, part.X, P->Z1, q, fa, P->tab.get() + toff, s);
+            toff += n;
+            P->vp2[p] = part.R + V;  // the parts are in decreasing pass count
+            P->vp1[p] = part.X + V;
+            first += q;
+        }
+        // slice weights and the reverse-blurred all-ones lattice (term-local rows)
+        const int En = N * d1;
+        DevBuf<int> gv;
+        gv.reserve(En);
+        P->parts[i].sw.reserve(En);
+        launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(d), N, d1, 0, gv.get(),
+                             P->parts[i].sw.get(), s);
+        std::vector<Chunk> lch = lat.chunks;
+        for (auto& c : lch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
+        std::vector<Combine> lcb = lat.combines;
+        for (auto& c : lcb) c.first += V + 1;
+        DevBuf<Chunk> dch;
+        DevBuf<Combine> dcb;
+        upload(dch, lch, s);
+        upload(dcb, lcb, s);
+        DevBuf<float> ones, v0, v1;
+        ones.reserve(N);
+        launch_fill(ones.get(), 1.f, N, s);
+        const int rows = V + 1 + lat.n_partials;
+        v0.reserve(rows);
+        v1.reserve(V + 1);
+        launch_fill(v0.get(), 0.f, rows, s);
+        launch_fill(v1.get(), 0.f, V + 1, s);
+        // the term's own entries: the plan's entries of part i, less the input offset
+        DevBuf<int2> lent;
+        lent.reserve(En);
+        launch_entry_gather(lat.csr_e.get(), En, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
+                            lent.get(), s);
+        launch_splat(dch.get(), (int)lch.size(), lent.get(), ones.get(), 1, v0.get(), 1, 1, s);
+        launch_combine(dcb.get(), (int)lcb.size(), v0.get(), 1, 1, s);
+        DevBuf<int2> nbr;
+        nbr.reserve((size_t)d1 * V);
+        launch_remap_nbr(lat.nbr.get(), nbr.get(), d1 * V, 0, V, s);
+        BlurTable bt{};
+        bt.nterms = 1;
+        bt.t[0] = BlurTerm{0, V, d, nbr.get()};
+        float* bufs[2] = {v0.get(), v1.get()};
+        for (int step = 0; step <= d; step++)
+            launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, 1, 1, bt, step, 1, s);
+        P->parts[i].ones_rev = (d1 & 1) ? std::move(v1) : std::move(v0);
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));  // before the temporaries return to the pool
+    }
+    // vertex rows: the zero rows stay zero (splats write vertex and partial rows, the
+    // blur passes rows below the zero row)
     auto zeroed = [&](DevBuf<float>& b, size_t n) {
         b.reserve(n);
         launch_fill(b.get(), 0.f, n, s);
     };
-    zeroed(P->S2, (size_t)rows * 2 * L);
-    for (int i = 0; i < 2; i++) {
-        zeroed(P->F2[i], (size_t)(V + 1) * 2 * L);
-        zeroed(P->R2[i], (size_t)(V + 1) * 2 * L);
-    }
-    zeroed(P->S1, (size_t)rows * L);
-    zeroed(P->B1, (size_t)(V + 1) * L);
+    zeroed(P->S2, (size_t)(P->Z2 + 1 + parts_total) * 2 * L);
+    zeroed(P->X2[0], (size_t)(P->Z2 + 1) * 2 * L);
+    zeroed(P->X2[1], (size_t)(P->Z2 + 1) * 2 * L);
+    zeroed(P->S1, (size_t)(P->Z1 + 1 + parts_total) * L);
+    zeroed(P->B1, (size_t)(P->Z1 + 1) * L);
+    P->gen = m.lat_gen;
     DCRF_HIP_CHECK(hipStreamSynchronize(s));
-    t.raw = std::move(P);
-    return *t.raw;
+    return P;
 }
 
-// Splat `in` ([N][Lc], row stride Lc) on the term's own lattice into `dst` (stride Lc).
-void raw_splat(const Term& t, const float* in, int Lc, float* dst, hipStream_t s) {
-    const RawPlan& P = *t.raw;
-    launch_splat(P.ch.get(), P.nch, P.ent.get(), in, Lc, dst, Lc, Lc, s);
-    launch_combine(P.cb.get(), P.ncb, dst, Lc, Lc, s);
+// The plans of the terms with kernel parameters, in groups of DCRF_MAX_TERMS / 2 (a
+// pass launch holds two chains per term), rebuilt when a lattice changed.
+void ensure_kg_plans(dcrf_model& m) {
+    if (!m.kgplans.empty() && m.kgplans.front()->gen == m.lat_gen) return;
+    m.kgplans.clear();
+    std::vector<int> group;
+    for (int k = 0; k < (int)m.terms.size(); k++) {
+        if (!m.terms[k]->num_kparams()) continue;
+        group.push_back(k);
+        if ((int)group.size() == DCRF_MAX_TERMS / 2) {
+            m.kgplans.push_back(build_kg_plan(m, group));
+            group.clear();
+        }
+    }
+    if (!group.empty()) m.kgplans.push_back(build_kg_plan(m, group));
 }
 
-// The d+1 blur steps of the term's lattice (reverse: axes d..0) from src through the
-// ping-pong pair buf (src itself is only read); returns the buffer holding the result.
-float* raw_blur(const Term& t, const float* src, float* const buf[2], int Lc, bool reverse, hipStream_t s) {
-    const RawPlan& P = *t.raw;
-    const int V = t.lat.V, np = (int)P.pass_steps.size();
+// The blur passes of one layout (x: the x chains, else the forward and reverse chains
+// of the paired inputs) from the splat rows src through the ping-pong pair buf: pass p
+// runs the chains of the parts with more than p passes over rows [0, vp[p]).  Part i's
+// result is in buf[(passes - 1) & 1].
+void kg_blur(const KgPlan& P, bool x, const float* src, float* const buf[2], int Lc, hipStream_t s) {
     const float* in = src;
-    for (int p = 0; p < np; p++) {
+    for (int p = 0; p < P.passes; p++) {
         PassTable pt{};
-        pt.nterms = 1;
-        pt.t[0] = PassTerm{0, V, (reverse ? P.passT.get() : P.passF.get()) + P.pass_off[p], P.pass_steps[p]};
-        pt.zero_row = V;  // zero in every buffer of the plan
-        launch_blur_pass(in, buf[p & 1], 0, V, Lc, Lc, pt, s);
+        pt.nterms = 0;
+        for (size_t i = 0; i < P.parts.size() && P.parts[i].passes > p; i++) {
+            const KgPart& part = P.parts[i];
+            const int q = part.steps[p];
+            const int* tab = P.tab.get();
+            const std::vector<int>& off = P.tab_off[p];
+            if (x) {
+                pt.t[pt.nterms++] = PassTerm{part.X, part.V, tab + off[3 * i + 2], q};
+            } else {
+                pt.t[pt.nterms++] = PassTerm{part.F, part.V, tab + off[3 * i], q};
+                pt.t[pt.nterms++] = PassTerm{part.R, part.V, tab + off[3 * i + 1], q};
+            }
+        }
+        pt.zero_row = x ? P.Z1 : P.Z2;
+        launch_blur_pass(in, buf[p & 1], 0, x ? P.vp1[p] : P.vp2[p], Lc, Lc, pt, s);
         in = buf[p & 1];
     }
-    return buf[(np - 1) & 1];
 }
 
-// acc += kernelGradient(b, Q_t) of one term (pairwise.cpp:202-207, 152-163, 88-114):
-// fg = -G(x, 1) + G(a, c) with a, c and x per the normalisation type.  One pass forms
-// lbl = C (w Q_t) and the inputs a and c, which are splatted together as [a | c] and
-// blurred in both orders: the forward rows give K a (G(a, c), direction 0) and K c, the
-// reverse rows K^T a and K^T c (G(a, c), direction 1).  x (from the slices K^T a and K c
-// in one pass) needs one more splat and forward blur, and G(x, 1)'s direction 1 is the
-// plan's reverse-blurred ones.  The four slicing gradients and the parameter products
-// then run in one pass over the points where the lane-group kernel covers the shape.
-void kernel_grad_term(dcrf_model& m, Term& t, const float* b, const float* Qt, const float* kw, double* acc,
-                      hipStream_t s) {
-    const int N = m.N(), L = m.L, d = t.lat.d, L2 = 2 * L;
+// acc_k += kernelGradient(b, Q_t) for every term k of plan P (pairwise.cpp:202-207,
+// 152-163, 88-114): fg = -G(x, 1) + G(a, c) with a, c and x per the normalisation type.
+// Per term one pass forms lbl = C (w Q_t) and the inputs a and c; the [a | c] of all the
+// plan's terms are splatted together and blurred in both orders: the forward rows give
+// K a (G(a, c), direction 0) and K c, the reverse rows K^T a and K^T c (G(a, c),
+// direction 1).  x (from the slices K^T a and K c in one pass per term) needs one more
+// splat and forward blur, and G(x, 1)'s direction 1 is the term's reverse-blurred ones.
+// The four slicing gradients and the parameter products then run in one pass per term
+// (its partial rows at partial + k * pstride, summed after the last iteration) where the
+// lane-group kernel covers the shape, else per direction with a reduction into acc.
+void kernel_grad_group(dcrf_model& m, const KgPlan& P, const float* b, const float* Qt, const float* kw,
+                       const std::vector<int>& koff, size_t pstride, hipStream_t s) {
+    const int N = m.N(), L = m.L, L2 = 2 * L;
     const size_t NL = (size_t)N * L;
-    RawPlan& P = raw_plan(m, t);
-    float* lbl = m.kgbuf.get();
-    float* pk = lbl + NL;     // [N][2L]
-    float* xb = pk + 2 * NL;  // x
-    const float* nm = t.norm.get();
-    const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
-    const bool need_fa = has_x && t.ntype != DCRF_NORMALIZE_AFTER;  // x reads fa and lbl
-    const bool need_fb = has_x && t.ntype != DCRF_NORMALIZE_BEFORE;
-    {
+    const int T = (int)P.parts.size();
+    float* lbl = m.kgbuf.get();                // [T][N][L]
+    float* pk = lbl + (size_t)T * NL;          // [T][N][2L]
+    float* xb = pk + (size_t)2 * T * NL;       // [T][N][L]
+    bool any_x = false;
+    for (int i = 0; i < T; i++) {
+        const int k = P.parts[i].term;
+        const Term& t = *m.terms[k];
+        const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
+        const bool need_fa = has_x && t.ntype != DCRF_NORMALIZE_AFTER;  // x reads fa and lbl
+        any_x = any_x || has_x;
         SliceTable st{};
         st.nterms = 1;
-        st.t[0] = SliceTerm{nullptr, nullptr, nullptr, 1, t.ckind, t.d_compat.get(), kw, nullptr, L};
-        launch_kg_prepare(Qt, b, nm, N, L, st, t.ntype, need_fa ? lbl : nullptr, pk, s);
+        st.t[0] = SliceTerm{nullptr, nullptr, nullptr, 1, t.ckind, t.d_compat.get(), kw ? kw + k : nullptr, nullptr, L};
+        launch_kg_prepare(Qt, b, t.norm.get(), N, L, st, t.ntype, need_fa ? lbl + i * NL : nullptr,
+                          pk + (size_t)i * 2 * NL, s);
     }
-    raw_splat(t, pk, L2, P.S2.get(), s);
-    float* const fbuf[2] = {P.F2[0].get(), P.F2[1].get()};
-    float* const rbuf[2] = {P.R2[0].get(), P.R2[1].get()};
-    const float* fwd = raw_blur(t, P.S2.get(), fbuf, L2, false, s);
-    const float* rev = raw_blur(t, P.S2.get(), rbuf, L2, true, s);
-    const int* vtx = t.lat.vtx.get();
-    const int* rank = t.lat.rank.get();
-    const float* xr = nullptr;
-    if (has_x) {
-        SliceTable st{};
-        st.nterms = 2;
-        const float* sw = t.raw->sw.get();
-        st.t[0] = SliceTerm{vtx, sw, need_fa ? rev : nullptr, d + 1, -1, nullptr, nullptr, nullptr, L2};      // K^T a
-        st.t[1] = SliceTerm{vtx, sw, need_fb ? fwd + L : nullptr, d + 1, -1, nullptr, nullptr, nullptr, L2};  // K c
-        launch_kg_x(b, lbl, nm, N, L, st, t.ntype, xb, s);
-        raw_splat(t, xb, L, P.S1.get(), s);
-        float* const xbuf[2] = {P.B1.get(), P.S1.get()};
-        xr = raw_blur(t, P.S1.get(), xbuf, L, false, s);
+    launch_splat(P.ch2.get(), P.nch, P.ent.get(), pk, L2, P.S2.get(), L2, L2, s);
+    launch_combine(P.cb2.get(), P.ncb, P.S2.get(), L2, L2, s);
+    float* const pbuf[2] = {P.X2[0].get(), P.X2[1].get()};
+    kg_blur(P, false, P.S2.get(), pbuf, L2, s);
+    float* const xbuf[2] = {P.B1.get(), P.S1.get()};  // pass 0 reads the splat rows S1
+    auto fwd_of = [&](int i) { return pbuf[(P.parts[i].passes - 1) & 1] + (size_t)P.parts[i].F * L2; };
+    auto rev_of = [&](int i) { return pbuf[(P.parts[i].passes - 1) & 1] + (size_t)P.parts[i].R * L2; };
+    if (any_x) {
+        for (int i = 0; i < T; i++) {
+            const Term& t = *m.terms[P.parts[i].term];
+            float* x = xb + (size_t)i * NL;
+            if (t.ntype == DCRF_NO_NORMALIZATION) {  // splatted with the others, never read
+                launch_fill(x, 0.f, NL, s);
+                continue;
+            }
+            const bool need_fa = t.ntype != DCRF_NORMALIZE_AFTER, need_fb = t.ntype != DCRF_NORMALIZE_BEFORE;
+            SliceTable st{};
+            st.nterms = 2;
+            const int* vtx = t.lat.vtx.get();
+            const float* sw = P.parts[i].sw.get();
+            const int d1 = t.lat.d + 1;
+            st.t[0] = SliceTerm{vtx, sw, need_fa ? rev_of(i) : nullptr, d1, -1, nullptr, nullptr, nullptr, L2};  // K^T a
+            st.t[1] = SliceTerm{vtx, sw, need_fb ? fwd_of(i) + L : nullptr, d1, -1, nullptr, nullptr, nullptr, L2};  // K c
+            launch_kg_x(b, lbl + (size_t)i * NL, t.norm.get(), N, L, st, t.ntype, x, s);
+        }
+        launch_splat(P.ch1.get(), P.nch, P.ent.get(), xb, L, P.S1.get(), L, L, s);
+        launch_combine(P.cb1.get(), P.ncb, P.S1.get(), L, L, s);
+        kg_blur(P, true, P.S1.get(), xbuf, L, s);
     }
-    const bool full = t.ktype == DCRF_FULL_KERNEL;
-    if (kg_grad_fused(L, d)) {
-        launch_kg_grad(vtx, rank, fwd, rev, pk, xr, P.ones_rev.get(), xb, t.feat0.get(), m.geod.pix_base.get(),
-                       m.geod.pad_base.get(), m.geo.n_inst, N, L, d, full, m.kpartial.get(), acc, s);
-        return;
+    for (int i = 0; i < T; i++) {
+        const KgPart& part = P.parts[i];
+        const int k = part.term;
+        const Term& t = *m.terms[k];
+        const int d = t.lat.d;
+        const bool has_x = t.ntype != DCRF_NO_NORMALIZATION;
+        const float* fwd = fwd_of(i);
+        const float* rev = rev_of(i);
+        const float* xr = has_x ? xbuf[(part.passes - 1) & 1] + (size_t)part.X * L : nullptr;
+        const float* pki = pk + (size_t)i * 2 * NL;
+        const float* x = xb + (size_t)i * NL;
+        const int* vtx = t.lat.vtx.get();
+        const int* rank = t.lat.rank.get();
+        const bool full = t.ktype == DCRF_FULL_KERNEL;
+        double* partial = m.kpartial.get() + (size_t)k * pstride;
+        if (kg_grad_fused(L, d)) {
+            launch_kg_grad(vtx, rank, fwd, rev, pki, xr, part.ones_rev.get(), x, t.feat0.get(), m.geod.pix_base.get(),
+                           m.geod.pad_base.get(), m.geo.n_inst, N, L, d, full, partial, s);
+            continue;
+        }
+        // more labels than a wave's lanes hold, or d > 6: one thread per point and direction
+        float* dfr = m.kgdf.get();
+        float* dfg = dfr + (size_t)N * d;
+        if (has_x) {
+            launch_slice_grad(xr, L, 1, vtx, rank, nullptr, 0, N, L, d, 0, dfr, s);
+            launch_slice_grad(part.ones_rev.get(), 1, 0, vtx, rank, x, L, N, L, d, 1, dfr, s);
+        }
+        launch_slice_grad(fwd, L2, 1, vtx, rank, pki + L, L2, N, L, d, 0, dfg, s);
+        launch_slice_grad(rev + L, L2, 1, vtx, rank, pki, L2, N, L, d, 1, dfg, s);
+        launch_kparam_reduce(has_x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
+                             m.geo.n_inst, N, d, full, partial, m.kacc.get() + koff[k], s);
     }
-    // more labels than a wave's lanes hold, or d > 6: one thread per point and direction
-    float* dfr = m.kgdf.get();
-    float* dfg = dfr + (size_t)N * d;
-    if (has_x) {
-        launch_slice_grad(xr, L, 1, vtx, rank, nullptr, 0, N, L, d, 0, dfr, s);
-        launch_slice_grad(P.ones_rev.get(), 1, 0, vtx, rank, xb, L, N, L, d, 1, dfr, s);
-    }
-    launch_slice_grad(fwd, L2, 1, vtx, rank, pk + L, L2, N, L, d, 0, dfg, s);
-    launch_slice_grad(rev + L, L2, 1, vtx, rank, pk, L2, N, L, d, 1, dfg, s);
-    launch_kparam_reduce(has_x ? dfr : nullptr, dfg, t.feat0.get(), m.geod.pix_base.get(), m.geod.pad_base.get(),
-                         m.geo.n_inst, N, d, full, m.kpartial.get(), acc, s);
 }
 
 int num_kernel_params(const dcrf_model& m) {
@@ -672,6 +839,7 @@ void add_term(dcrf_model& m, int d, int ckind, const float* compat, int ktype, i
     for (int j = 0; j < d; j++) t->kparams[ktype == DCRF_FULL_KERNEL ? j * d + j : j] = 1.0f;
     m.terms.push_back(std::move(t));
     m.layout_valid = false;
+    m.lat_gen++;
     m.kw_set = false;  // term weights are per term: set them again after adding terms
     m.train_n = -1;
 }
@@ -1241,27 +1409,34 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
     std::vector<int> koff(T + 1, 0);
     for (int k = 0; k < T; k++) koff[k + 1] = koff[k] + m.terms[k]->num_kparams();
     kernel = kernel && koff[T] > 0;
+    size_t pstride = 0;  // per term: its partial rows in kpartial
     if (kernel) {
+        ensure_kg_plans(m);
         int dmax = 1, dslow = 0;  // dslow: largest d of a term without the one-pass gradient
         for (auto& t : m.terms) {
             dmax = std::max(dmax, t->lat.d);
             if (t->num_kparams() && !kg_grad_fused(L, t->lat.d)) dslow = std::max(dslow, t->lat.d);
         }
-        m.kgbuf.reserve(4 * NL);
+        size_t tmax = 1;
+        for (const auto& P : m.kgplans) tmax = std::max(tmax, P->parts.size());
+        m.kgbuf.reserve(4 * tmax * NL);
         if (dslow) m.kgdf.reserve((size_t)2 * N * dslow);
-        m.kpartial.reserve(kparam_partial_doubles(dmax));
+        pstride = kparam_partial_doubles(dmax);
+        m.kpartial.reserve(pstride * T);
         m.kacc.reserve(koff[T]);
         launch_fill_double(m.kacc.get(), koff[T], s);
+        for (int k = 0; k < T; k++)  // the one-pass kernel adds every iteration to its rows
+            if (m.terms[k]->num_kparams() && kg_grad_fused(L, m.terms[k]->lat.d))
+                launch_fill_double(m.kpartial.get() + (size_t)k * pstride,
+                                   (size_t)kg_grad_blocks(N, L) * m.terms[k]->num_kparams(), s);
     }
     for (int it = n - 1; it >= 0; it--) {
         float* b = bslot(it + 1);
         if (kernel) {
             // DenseCRF::gradient takes the kernel gradient with b before its update
             // (densecrf.cpp:280-284)
-            for (int k = 0; k < T; k++)
-                if (m.terms[k]->num_kparams())
-                    kernel_grad_term(m, *m.terms[k], b, Qc + (size_t)it * NL, m.train_kw ? m.train_kw + k : nullptr,
-                                     m.kacc.get() + koff[k], s);
+            for (const auto& P : m.kgplans)
+                kernel_grad_group(m, *P, b, Qc + (size_t)it * NL, m.train_kw, koff, pstride, s);
         }
         run_filter(m, b, L, true, s);
         SliceTable st = slice_table(m, true, m.train_kw);
@@ -1276,6 +1451,11 @@ void backward(dcrf_model& m, const float* dQ, float* dU, float* dcompat, float* 
         else
             launch_adjoint(Qc + (size_t)it * NL, bslot(it), dU, N, L, st, s);
     }
+    if (kernel)  // the one-pass kernel gradient's partial rows hold all iterations
+        for (int k = 0; k < T; k++)
+            if (m.terms[k]->num_kparams() && kg_grad_fused(L, m.terms[k]->lat.d))
+                launch_kparam_final(m.kpartial.get() + (size_t)k * pstride, kg_grad_blocks(N, L),
+                                    m.terms[k]->num_kparams(), m.kacc.get() + koff[k], s);
     // G_k = sum_t b_{t+1} (K~_k Q_t)^T over all iterations: the statistic rows of the
     // Potts / Diagonal terms first (they share the partial buffer with the Matrix
     // terms' products), then for Matrix terms b_{t+1} in slot t + 1 against the messages
@@ -2018,10 +2198,10 @@ int dcrf_set_kernel_params(dcrf_model* m, const float* params) {
             r->t->lat = std::move(r->tmp.lat);
             r->t->norm = std::move(r->tmp.norm);
             r->t->kparams = std::move(r->kp);
-            r->t->raw.reset();
         }
         if (!done.empty()) {
             m->layout_valid = false;
+            m->lat_gen++;
             m->train_n = -1;
             m->drop_graph();
         }

@@ -114,23 +114,36 @@ struct TermLattice {
     int chunk_len = 0;
 };
 
-// One term's lattice filter on its own (term-local vertex rows, zero row V, partial
-// rows from V+1), without normaliser: the filters of the kernel-parameter gradient.
-struct RawPlan {
-    DevBuf<Chunk> ch;
-    DevBuf<Combine> cb;
-    int nch = 0, ncb = 0;
-    DevBuf<int2> ent;        // (pixel, barycentric weight bits) in segment order
-    DevBuf<int2> nbr;        // [(d+1)][V] neighbour rows, zero row V for an absent one
+// The lattice filters of the kernel-parameter gradient (the lattice_.compute and
+// Permutohedral::gradient calls of DenseKernel::featureGradient, pairwise.cpp:88-114,
+// without normaliser) for up to DCRF_MAX_TERMS / 2 terms at once, in vertex rows of
+// their own.  The paired inputs [a | c] (2L floats per pixel) of all the plan's terms are
+// splatted by one launch and blurred forward and in reverse by shared launches: rows
+// [F, F + V) and [R, R + V) hold a term's forward and reverse chain, terms with more
+// blur passes first (each pass then covers a prefix of the rows), then the zero row Z2
+// and the splat's partial rows.  The normaliser inputs x use a second layout with the
+// forward rows only, [X, X + V), zero row Z1, partial rows.  The splat entries of the
+// plan's i-th term read input row i*N + p for pixel p (inputs stacked term-major).
+struct KgPart {
+    int term = 0;            // index into the model's terms
+    int V = 0;               // vertices of its lattice
+    int F = 0, R = 0, X = 0;  // first rows of the forward, reverse and x chains
+    int passes = 0;          // blur passes of the lattice
+    std::vector<int> steps;  // lattice steps of each pass
     DevBuf<float> sw;        // [N][d+1] slice weights bary / (1 + 2^-d)
     DevBuf<float> ones_rev;  // [V+1] splat of ones blurred in reverse order (row V zero)
-    // the plan's own vertex rows, zero rows V kept zero: the splat of the paired inputs
-    // [V+1+partials][2L], its forward and reverse blur chains [V+1][2L] x 2 each; the
-    // splat of one input [V+1+partials][L] and its blur partner [V+1][L]
-    DevBuf<float> S2, F2[2], R2[2], S1, B1;
-    // multi-step blur passes of the lattice: steps per pass, table offsets, tables
-    std::vector<int> pass_steps, pass_off;
-    DevBuf<int> passF, passT;
+};
+struct KgPlan {
+    std::vector<KgPart> parts;
+    int gen = -1;  // the model's lattice generation the plan was built for
+    int Z2 = 0, Z1 = 0, passes = 0, nch = 0, ncb = 0;
+    DevBuf<int2> ent;          // (stacked input row, barycentric weight bits) in segment order
+    DevBuf<Chunk> ch2, ch1;    // segments with output rows of the [a | c] and x layouts
+    DevBuf<Combine> cb2, cb1;  // vertices with several segments, in the two layouts
+    std::vector<int> vp2, vp1;  // per pass: one past the last row it writes
+    DevBuf<int> tab;            // the pass tables of all chains
+    std::vector<std::vector<int>> tab_off;  // per pass: forward, reverse, x table of each part
+    DevBuf<float> S2, X2[2], S1, B1;  // splat and ping-pong rows of the two layouts
 };
 
 struct Term {
@@ -141,7 +154,6 @@ struct Term {
     DevBuf<float> norm;  // [N]
     DevBuf<float> feat0;         // [npad][d] features as given (DenseKernel::f_)
     std::vector<float> kparams;  // DIAG [d] scales, FULL [d*d] column-major (parameters_)
-    std::unique_ptr<RawPlan> raw;
     int num_kparams() const {
         return ktype == DCRF_CONST_KERNEL ? 0 : (ktype == DCRF_DIAG_KERNEL ? lat.d : lat.d * lat.d);
     }
@@ -405,13 +417,18 @@ void launch_kg_x(const float* b, const float* lbl, const float* norm, int N, int
                  float* x, hipStream_t s);
 // Whether launch_kg_grad has a kernel for L labels and feature dimension d.
 bool kg_grad_fused(int L, int d);
-// acc (d or d*d doubles) += the parameter gradient of fg = -G(x, 1) + G(a, c) (xr null:
-// fg = G(a, c)) against the padded feature table f, in one pass over the points plus
-// one ordered sum of the per-block partials (partial holds kparam_partial_doubles(d)).
+// The parameter gradient of fg = -G(x, 1) + G(a, c) (xr null: fg = G(a, c)) against
+// the padded feature table f, in one pass over the points: each of the
+// kg_grad_blocks(N, L) blocks adds its sums to its row of np = d (or d*d) doubles in
+// partial, rows that the caller zeroes before the first reverse iteration; then
+// launch_kparam_final(partial, kg_grad_blocks(N, L), np, acc) adds them to acc.
+int kg_grad_blocks(int N, int L);
 void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const float* rev, const float* P,
                     const float* xr, const float* ones, const float* x, const float* f, const int* pix_base,
-                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial, double* acc,
+                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial,
                     hipStream_t s);
+// acc[q] += sum over rows b < rows of partial[b * np + q], for q < np (fixed order).
+void launch_kparam_final(const double* partial, int rows, int np, double* acc, hipStream_t s);
 // One direction of the lattice gradient: df (+)= slicing gradient of the blurred rows
 // vals (row stride ldv, label stride ks) paired with c [N][L] (row stride ldc; null:
 // ones).
@@ -454,11 +471,11 @@ void launch_gram(const float* A, int LA, const float* B, int LB, int N, float* p
 void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row, hipStream_t s);
 // Entry arrays of one term: sorted entry j (csr_e[j] = p*(d+1)+r) goes to position
 // base[t] + j, t the tile of j (tile_ptr: first j of each tile; base null: position j);
-// ent = (the pixel p, or its index inside its tile when `local` is given; bits of the
-// barycentric weight, times norm[p] when use_norm).
+// ent = (the pixel p, or its index inside its tile when `local` is given, plus pix_off;
+// bits of the barycentric weight, times norm[p] when use_norm).
 void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
                          const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int2* ent_out,
-                         hipStream_t s);
+                         hipStream_t s, int pix_off = 0);
 void launch_slice_weights(const int* vtx, const float* bary, const float* norm, int use_norm,
                           float alpha, int N, int d1, int vbase, int* gvtx, float* sw,
                           hipStream_t s);

@@ -1051,7 +1051,7 @@ __global__ void k_remap_nbr(const int* __restrict__ src, int2* __restrict__ dst,
 
 __global__ void k_entry_gather(const int* __restrict__ csr_e, int E, int d1, const float* __restrict__ bary,
                                const float* __restrict__ norm, int use_norm, const int* __restrict__ tile_ptr,
-                               const int* __restrict__ base, int n_tiles, TileGeom g, int local,
+                               const int* __restrict__ base, int n_tiles, TileGeom g, int local, int pix_off,
                                int2* __restrict__ ent) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= E) return;
@@ -1060,7 +1060,7 @@ __global__ void k_entry_gather(const int* __restrict__ csr_e, int E, int d1, con
     const int pos = base ? base[last_le(tile_ptr, n_tiles, j)] + j : j;
     const float b = bary[e];
     const float w = use_norm ? b * norm[p] : b;
-    ent[pos] = make_int2(local ? tile_local(g, p) : p, __float_as_int(w));
+    ent[pos] = make_int2((local ? tile_local(g, p) : p) + pix_off, __float_as_int(w));
 }
 
 __global__ void k_slice_weights(const int* __restrict__ vtx, const float* __restrict__ bary,
@@ -1426,11 +1426,11 @@ void launch_remap_nbr(const int* src, int2* dst, int n, int vbase, int zero_row,
 
 void launch_entry_gather(const int* csr_e, int E, int d1, const float* bary, const float* norm, int use_norm,
                          const int* tile_ptr, const int* base, int n_tiles, const TileGeom* local, int2* ent_out,
-                         hipStream_t s) {
+                         hipStream_t s, int pix_off) {
     if (!E) return;
     const TileGeom g = local ? *local : TileGeom{};
     k_entry_gather<<<blocks_for(E), kThreads, 0, s>>>(csr_e, E, d1, bary, norm, use_norm, tile_ptr, base, n_tiles, g,
-                                                      local ? 1 : 0, ent_out);
+                                                      local ? 1 : 0, pix_off, ent_out);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

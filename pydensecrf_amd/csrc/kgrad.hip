@@ -287,7 +287,8 @@ struct KgGrad {
 // in LDS and lane q of the wave adds its product for the wave's points in point order,
 // over the block's `rounds`
 // consecutive blocks' worth of points; at the end the waves' sums are added in wave
-// order into partial[blockIdx.x][q].  The label order of each direction's sum differs
+// order and the block adds them to partial[blockIdx.x][q] (so one row per block sums
+// all reverse iterations, in iteration order).  The label order of each direction's sum differs
 // from the reference's sequential one (float tolerance).
 template <int R, int D>
 __global__ void __launch_bounds__(kWaveBlock) k_kg_grad(const int* __restrict__ vtx, const int* __restrict__ rank,
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_kg_grad(const int* __restrict__ 
     for (int q = threadIdx.x; q < np; q += blockDim.x) {
         double s = 0.0;
         for (int k = 0; k < W; k++) s = s + wsum[k][q];
-        partial[(size_t)blockIdx.x * np + q] = s;
+        partial[(size_t)blockIdx.x * np + q] += s;  // rows zeroed before the first iteration
     }
 }
 
@@ -515,9 +516,15 @@ void launch_kg_x(const float* b, const float* lbl, const float* norm, int N, int
 
 bool kg_grad_fused(int L, int d) { return d >= 1 && d <= 6 && shape_for(L).wave; }
 
+int kg_grad_blocks(int N, int L) {
+    const int need = std::max(1, (int)blocks_for(N, shape_for(L).items));
+    const int rounds = (need + kKgGradBlocks - 1) / kKgGradBlocks;
+    return (need + rounds - 1) / rounds;  // nb * rounds >= need, nb <= kKgGradBlocks
+}
+
 void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const float* rev, const float* P,
                     const float* xr, const float* ones, const float* x, const float* f, const int* pix_base,
-                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial, double* acc,
+                    const int* pad_base, int n_inst, int N, int L, int d, bool full, double* partial,
                     hipStream_t s) {
     const int np = full ? d * d : d;
     if (!N || !np) return;
@@ -525,7 +532,7 @@ void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const flo
     const LaunchShape sh = shape_for(L);
     const int need = (int)blocks_for(N, sh.items);
     const int rounds = (need + kKgGradBlocks - 1) / kKgGradBlocks;
-    const int nb = (need + rounds - 1) / rounds;  // nb * rounds >= need, nb <= kKgGradBlocks
+    const int nb = kg_grad_blocks(N, L);
     const KgGrad a{fwd, rev, P, xr, ones, x, f, pix_base, pad_base, n_inst, full ? 1 : 0};
     const GradScale gs = grad_scale(d);
 #define DCRF_KGG(RR, DD)                                                                                        \
@@ -542,7 +549,11 @@ void launch_kg_grad(const int* vtx, const int* rank, const float* fwd, const flo
     }
 #undef DCRF_KGG
     DCRF_HIP_CHECK(hipGetLastError());
-    k_kparam_final<<<np, kThreads, 0, s>>>(partial, nb, np, acc);
+}
+
+void launch_kparam_final(const double* partial, int rows, int np, double* acc, hipStream_t s) {
+    if (rows <= 0 || np <= 0) return;
+    k_kparam_final<<<np, kThreads, 0, s>>>(partial, rows, np, acc);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

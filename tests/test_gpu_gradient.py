@@ -359,6 +359,46 @@ def test_kernel_gradient_vs_oracle_l21(kt, nt):
     assert err <= KG_TOL_L21
 
 
+@pytest.mark.parametrize("case", ["mixed_normalisation", "five_terms", "high_dim"])
+def test_kernel_gradient_term_mixes(case):
+    """The kernel-parameter gradient's shared filter plans: an unnormalised term beside
+    normalised ones (its x inputs are zeros splatted with the others and never read),
+    five terms with kernel parameters (two plans of at most four terms), and a 7-d
+    generic feature term (the thread-per-pixel slicing gradient) beside the lane-group
+    ones."""
+    W, H, L, n = 48, 40, 21, 3
+    img, U, gt = image_case(W, H, L, seed={"mixed_normalisation": 5, "five_terms": 6, "high_dim": 7}[case])
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    rng = np.random.default_rng(11)
+    f7 = rng.normal(size=(7, W * H)).astype(np.float32) * np.float32(0.3)
+    for m in (g, o):
+        m.set_unary(U)
+        if case == "mixed_normalisation":
+            m.add_gaussian(3.0, 3.0, 0.1, ktype=capi.DIAG_KERNEL, ntype=po.NO_NORMALIZATION)
+            m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.DIAG_KERNEL,
+                            ntype=po.NORMALIZE_SYMMETRIC)
+            m.add_gaussian(5.0, 4.0, 2.0, ktype=capi.FULL_KERNEL, ntype=po.NORMALIZE_AFTER)
+        elif case == "five_terms":
+            for k, sxy in enumerate((2.0, 3.0, 5.0)):
+                m.add_gaussian(sxy, sxy + 1.0, 1.0 + k, ktype=capi.DIAG_KERNEL)
+            m.add_bilateral(40.0, 30.0, 13.0, 13.0, 13.0, img, 5.0, ktype=capi.DIAG_KERNEL)
+            m.add_bilateral(20.0, 20.0, 20.0, 20.0, 20.0, img, 3.0, ktype=capi.FULL_KERNEL)
+        else:
+            m.add_gaussian(3.0, 3.0, 3.0, ktype=capi.DIAG_KERNEL)
+            m.add_pairwise(f7, 2.0, ktype=capi.DIAG_KERNEL)
+            m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.DIAG_KERNEL)
+    obj, ug, cg, _, kg = g.gradient(n, gt, kernel=True)
+    r, ugo, cgo, kgo = o.gradient(n, gt, kernel=True)
+    assert abs(obj - r) <= 1e-5 * abs(r)
+    assert _rel_close(ug, ugo, 1e-4)
+    assert _rel_close(cg, cgo, 1e-4)
+    assert kg.shape == kgo.shape
+    err = np.abs(kg - kgo).max() / np.abs(kgo).max()
+    print("kernel gradient %s: max error / max entry %.3g" % (case, err))
+    assert err <= KG_TOL_L21, (err, kg, kgo)
+
+
 @pytest.mark.parametrize("L,nt", [(300, 3), (130, 2)])
 def test_kernel_gradient_many_labels(L, nt):
     """The kernel-parameter gradient with more labels than a wave's lanes hold (the
