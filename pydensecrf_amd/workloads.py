@@ -4,6 +4,10 @@
         DenseCRF2D 512x512, 21 labels, 5 iterations, Matrix compatibilities, learnable
         DIAG kernel parameters: S forward+backward passes through the torch.autograd
         layer (the kernel-parameter gradient in every backward), after two warm-up passes.
+    python -m pydensecrf_amd.workloads kernel-learning [--steps S]
+        The same model and layer with the kernel parameters changed after every step (a
+        sign step of 0.1 %): each step rebuilds both lattices, the layout and the
+        kernel gradient's filter plan, then runs the forward+backward.
     python -m pydensecrf_amd.workloads logistic-unary [--reps R] [--features F]
         1024x768 pixels, 21 labels, a logistic unary with F features (default 64):
         R evaluations of U = L f on the matrix cores, with the parameters changed
@@ -61,6 +65,45 @@ def kernel_gradient(steps):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     print("kernel-gradient 512x512x21, 5 iterations: %.3f ms per forward+backward" % ms, flush=True)
+
+
+def kernel_learning(steps):
+    import torch
+
+    from . import capi, synthetic as syn
+    from .autograd import CRFLayer
+
+    cfg = syn.IMAGE_512x512
+    W, H, L = cfg["W"], cfg["H"], cfg["L"]
+    img, U, gt = syn.make_image(**cfg)
+    rng = np.random.default_rng(1003)
+    c1 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    c2 = (-np.eye(L) + rng.normal(0, 0.1, (L, L))).astype(np.float32)
+    m = capi.Model.create_2d(W, H, L)
+    m.add_gaussian(3.0, 3.0, c1)
+    m.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, c2)
+    layer = CRFLayer(m, 5, [c1, c2], kweights=[3.0, 10.0], learn_kernel=True)
+    Ut = torch.tensor(np.ascontiguousarray(U.T), device="cuda", requires_grad=True)
+    gtt = torch.tensor(gt.astype(np.int64), device="cuda")
+
+    def step():
+        Q = layer(Ut)  # new kernel parameters: both lattices and the layout are rebuilt
+        loss = -torch.log(Q.gather(1, gtt[:, None])).mean()
+        loss.backward()  # the kernel-gradient filter plan of the new lattices is built
+        with torch.no_grad():
+            layer.kparams.mul_(1.0 - 1e-3 * torch.sign(layer.kparams.grad))
+            layer.kparams.grad = None
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print("kernel-learning 512x512x21, 5 iterations: %.3f ms per step (parameter change, forward+backward)" % ms,
+          flush=True)
 
 
 def fwd_bwd(steps, size):
@@ -176,7 +219,8 @@ def kernel_rebuild(reps):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["kernel-gradient", "logistic-unary", "kernel-rebuild", "fwd-bwd"])
+    ap.add_argument("workload", choices=["kernel-gradient", "kernel-learning", "logistic-unary", "kernel-rebuild",
+                                         "fwd-bwd"])
     ap.add_argument("--size", type=int, default=512, choices=[512, 1024])
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--reps", type=int, default=5)
@@ -184,6 +228,8 @@ def main():
     a = ap.parse_args()
     if a.workload == "kernel-gradient":
         kernel_gradient(a.steps)
+    elif a.workload == "kernel-learning":
+        kernel_learning(a.steps)
     elif a.workload == "fwd-bwd":
         fwd_bwd(a.steps, a.size)
     elif a.workload == "kernel-rebuild":
