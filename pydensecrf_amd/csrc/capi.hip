@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -463,206 +464,6 @@ void build_pass_tables(const int2* nbr, int V, int d, int vbase, int zero_row, s
 }
 
 // ---------------------------------------------------------------- kernel-parameter gradient
-// The filter plan of the kernel-parameter gradient for the model terms `terms` (at most
-// DCRF_MAX_TERMS / 2): the layouts of KgPlan, the merged segment and entry lists, the
-// pass tables of every chain, and per term the slice weights and the all-ones input
-// splatted and blurred in reverse order (one value per vertex).
-std::unique_ptr<KgPlan> build_kg_plan(dcrf_model& m, std::vector<int> terms) {
-    hipStream_t s = m.stream;
-    const int N = m.N();
-    const size_t L = m.L;
-    auto P = std::make_unique<KgPlan>();
-    auto passes_of = [&](int k) { return (int)pass_schedule(m.terms[k]->lat.d + 1, kMaxPassSteps).size(); };
-    std::stable_sort(terms.begin(), terms.end(), [&](int a, int b) { return passes_of(a) > passes_of(b); });
-    int row = 0, xrow = 0, parts_total = 0;
-    size_t E = 0;
-    for (int k : terms) {
-        const TermLattice& lat = m.terms[k]->lat;
-        KgPart part;
-        part.term = k;
-        part.V = lat.V;
-        part.steps = pass_schedule(lat.d + 1, kMaxPassSteps);
-        part.passes = (int)part.steps.size();
-        part.F = row;
-        part.R = row + lat.V;
-        part.X = xrow;
-        row += 2 * lat.V;
-        xrow += lat.V;
-        parts_total += lat.n_partials;
-        E += (size_t)N * (lat.d + 1);
-        P->passes = std::max(P->passes, part.passes);
-        P->parts.push_back(std::move(part));
-    }
-    P->Z2 = row;
-    P->Z1 = xrow;
-    // merged segments: entries of the i-th term from e0, its partial rows from p0 after
-    // each layout's zero row
-    std::vector<Chunk> ch2, ch1;
-    std::vector<Combine> cb2, cb1;
-    P->ent.reserve(E);
-    size_t e0 = 0;
-    int p0 = 0;
-    for (size_t i = 0; i < P->parts.size(); i++) {
-        const KgPart& part = P->parts[i];
-        const TermLattice& lat = m.terms[part.term]->lat;
-        const int d1 = lat.d + 1;
-        launch_entry_gather(lat.csr_e.get(), N * d1, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
-                            P->ent.get() + e0, s, (int)i * N);
-        for (const Chunk& c : lat.chunks) {
-            Chunk a = c;
-            a.beg += (int)e0;
-            a.end += (int)e0;
-            a.v = part.F + c.v;
-            a.out = c.out >= 0 ? part.F + c.out : P->Z2 + 1 + p0 + (-c.out - 1);
-            ch2.push_back(a);
-            a.v = part.X + c.v;
-            a.out = c.out >= 0 ? part.X + c.out : P->Z1 + 1 + p0 + (-c.out - 1);
-            ch1.push_back(a);
-        }
-        for (const Combine& c : lat.combines) {
-            Combine a = c;
-            a.v = part.F + c.v;
-            a.first = P->Z2 + 1 + p0 + c.first;
-            cb2.push_back(a);
-            a.v = part.X + c.v;
-            a.first = P->Z1 + 1 + p0 + c.first;
-            cb1.push_back(a);
-        }
-        e0 += (size_t)N * d1;
-        p0 += lat.n_partials;
-    }
-    require_adjoining(ch2);
-    if (!ch2.empty() && (ch2.front().beg != 0 || (size_t)ch2.back().end != E))
-        throw Error{DCRF_ERR_STATE, "kernel gradient: segments do not cover the entries"};
-    upload(P->ch2, ch2, s);
-    upload(P->ch1, ch1, s);
-    upload(P->cb2, cb2, s);
-    upload(P->cb1, cb1, s);
-    P->nch = (int)ch2.size();
-    P->ncb = (int)cb2.size();
-    // pass tables: pass p of term i expands the axes of its p-th pass; the forward chain
-    // reads rows from F, the reverse chain from F in its first pass (the splat rows) and
-    // from R after, the x chain from X
-    size_t tab_total = 0;
-    for (const KgPart& part : P->parts)
-        for (int q : part.steps) tab_total += (size_t)3 * part.V * (q == 1 ? 3 : 9);
-    P->tab.reserve(std::max<size_t>(tab_total, 1));
-    P->tab_off.assign(P->passes, std::vector<int>(3 * P->parts.size(), -1));
-    P->vp2.assign(P->passes, 0);
-    P->vp1.assign(P->passes, 0);
-    size_t toff = 0;
-    for (size_t i = 0; i < P->parts.size(); i++) {
-        const KgPart& part = P->parts[i];
-        const TermLattice& lat = m.terms[part.term]->lat;
-        const int V = lat.V, d = lat.d, d1 = d + 1;
-        DevBuf<int2> nF, nR, nX;  // the neighbour rows in each chain's numbering
-        nF.reserve((size_t)d1 * V);
-        nR.reserve((size_t)d1 * V);
-        nX.reserve((size_t)d1 * V);
-        launch_remap_nbr(lat.nbr.get(), nF.get(), d1 * V, part.F, P->Z2, s);
-        launch_remap_nbr(lat.nbr.get(), nR.get(), d1 * V, part.R, P->Z2, s);
-        launch_remap_nbr(lat.nbr.get(), nX.get(), d1 * V, part.X, P->Z1, s);
-        int first = 0;
-        for (int p = 0; p < part.passes; p++) {
-            const int q = part.steps[p];
-            int fa[3], ta[3];
-            for (int j = 0; j < q; j++) {
-                fa[j] = first + j;
-                ta[j] = d - first - j;
-            }
-            const size_t n = (size_t)V * (q == 1 ? 3 : 9);
-            std::vector<int>& off = P->tab_off[p];
-            off[3 * i] = (int)toff;
-            launch_pass_table(nF.get(), V, part.F, P->Z2, q, fa, P->tab.get() + toff, s);
-            toff += n;
-            off[3 * i + 1] = (int)toff;
-            if (p == 0)
-                launch_pass_table(nF.get(), V, part.F, P->Z2, q, ta, P->tab.get() + toff, s);
-            else
-                launch_pass_table(nR.get(), V, part.R, P->Z2, q, ta, P->tab.get() + toff, s);
-            toff += n;
-            off[3 * i + 2] = (int)toff;
-            launch_pass_table(nX.get(), V, part.X, P->Z1, q, fa, P->tab.get() + toff, s);
-            toff += n;
-            P->vp2[p] = part.R + V;  // the parts are in decreasing pass count
-            P->vp1[p] = part.X + V;
-            first += q;
-        }
-        // slice weights and the reverse-blurred all-ones lattice (term-local rows)
-        const int En = N * d1;
-        DevBuf<int> gv;
-        gv.reserve(En);
-        P->parts[i].sw.reserve(En);
-        launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(d), N, d1, 0, gv.get(),
-                             P->parts[i].sw.get(), s);
-        std::vector<Chunk> lch = lat.chunks;
-        for (auto& c : lch) c.out = c.out >= 0 ? c.out : V + 1 + (-c.out - 1);
-        std::vector<Combine> lcb = lat.combines;
-        for (auto& c : lcb) c.first += V + 1;
-        DevBuf<Chunk> dch;
-        DevBuf<Combine> dcb;
-        upload(dch, lch, s);
-        upload(dcb, lcb, s);
-        DevBuf<float> ones, v0, v1;
-        ones.reserve(N);
-        launch_fill(ones.get(), 1.f, N, s);
-        const int rows = V + 1 + lat.n_partials;
-        v0.reserve(rows);
-        v1.reserve(V + 1);
-        launch_fill(v0.get(), 0.f, rows, s);
-        launch_fill(v1.get(), 0.f, V + 1, s);
-        // the term's own entries: the plan's entries of part i, less the input offset
-        DevBuf<int2> lent;
-        lent.reserve(En);
-        launch_entry_gather(lat.csr_e.get(), En, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
-                            lent.get(), s);
-        launch_splat(dch.get(), (int)lch.size(), lent.get(), ones.get(), 1, v0.get(), 1, 1, s);
-        launch_combine(dcb.get(), (int)lcb.size(), v0.get(), 1, 1, s);
-        DevBuf<int2> nbr;
-        nbr.reserve((size_t)d1 * V);
-        launch_remap_nbr(lat.nbr.get(), nbr.get(), d1 * V, 0, V, s);
-        BlurTable bt{};
-        bt.nterms = 1;
-        bt.t[0] = BlurTerm{0, V, d, nbr.get()};
-        float* bufs[2] = {v0.get(), v1.get()};
-        for (int step = 0; step <= d; step++)
-            launch_blur(bufs[step & 1], bufs[(step + 1) & 1], 0, V, 1, 1, bt, step, 1, s);
-        P->parts[i].ones_rev = (d1 & 1) ? std::move(v1) : std::move(v0);
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));  // before the temporaries return to the pool
-    }
-    // vertex rows: the zero rows stay zero (splats write vertex and partial rows, the
-    // blur passes rows below the zero row)
-    auto zeroed = [&](DevBuf<float>& b, size_t n) {
-        b.reserve(n);
-        launch_fill(b.get(), 0.f, n, s);
-    };
-    zeroed(P->S2, (size_t)(P->Z2 + 1 + parts_total) * 2 * L);
-    zeroed(P->X2[0], (size_t)(P->Z2 + 1) * 2 * L);
-    zeroed(P->X2[1], (size_t)(P->Z2 + 1) * 2 * L);
-    zeroed(P->S1, (size_t)(P->Z1 + 1 + parts_total) * L);
-    zeroed(P->B1, (size_t)(P->Z1 + 1) * L);
-    P->gen = m.lat_gen;
-    DCRF_HIP_CHECK(hipStreamSynchronize(s));
-    return P;
-}
-
-// The plans of the terms with kernel parameters, in groups of DCRF_MAX_TERMS / 2 (a
-// pass launch holds two chains per term), rebuilt when a lattice changed.
-void ensure_kg_plans(dcrf_model& m) {
-    if (!m.kgplans.empty() && m.kgplans.front()->gen == m.lat_gen) return;
-    m.kgplans.clear();
-    std::vector<int> group;
-    for (int k = 0; k < (int)m.terms.size(); k++) {
-        if (!m.terms[k]->num_kparams()) continue;
-        group.push_back(k);
-        if ((int)group.size() == DCRF_MAX_TERMS / 2) {
-            m.kgplans.push_back(build_kg_plan(m, group));
-            group.clear();
-        }
-    }
-    if (!group.empty()) m.kgplans.push_back(build_kg_plan(m, group));
-}
-
 // The blur passes of one layout (x: the x chains, else the forward and reverse chains
 // of the paired inputs) from the splat rows src through the ping-pong pair buf: pass p
 // runs the chains of the parts with more than p passes over rows [0, vp[p]).  Part i's
@@ -687,6 +488,198 @@ void kg_blur(const KgPlan& P, bool x, const float* src, float* const buf[2], int
         pt.zero_row = x ? P.Z1 : P.Z2;
         launch_blur_pass(in, buf[p & 1], 0, x ? P.vp1[p] : P.vp2[p], Lc, Lc, pt, s);
         in = buf[p & 1];
+    }
+}
+
+// Fill plan P for the model terms `terms` (at most DCRF_MAX_TERMS / 2; P's device
+// buffers are reused where they are large enough): the layouts of KgPlan, the merged
+// segment and entry lists (the x layout's lists derived on the device), the pass tables
+// of every chain, and per term the slice weights and the all-ones input splatted and
+// blurred in reverse order (one value per vertex, through the plan's own launches: the
+// same segments, sums and pass order as a filter of that term alone).
+void build_kg_plan(dcrf_model& m, std::vector<int> terms, KgPlan& P) {
+    hipStream_t s = m.stream;
+    const int N = m.N();
+    const size_t L = m.L;
+    auto steps_of = [&](int k) { return pass_schedule(m.terms[k]->lat.d + 1, kMaxPassSteps); };
+    std::stable_sort(terms.begin(), terms.end(),
+                     [&](int a, int b) { return steps_of(a).size() > steps_of(b).size(); });
+    std::vector<KgPart> old = std::move(P.parts);
+    P.parts.clear();
+    P.passes = 0;
+    int row = 0, xrow = 0, parts_total = 0;
+    size_t E = 0;
+    for (size_t i = 0; i < terms.size(); i++) {
+        const int k = terms[i];
+        const TermLattice& lat = m.terms[k]->lat;
+        KgPart part;
+        if (i < old.size()) part.sw = std::move(old[i].sw);
+        part.term = k;
+        part.V = lat.V;
+        part.steps = steps_of(k);
+        part.passes = (int)part.steps.size();
+        part.F = row;
+        part.R = row + lat.V;
+        part.X = xrow;
+        row += 2 * lat.V;
+        xrow += lat.V;
+        parts_total += lat.n_partials;
+        E += (size_t)N * (lat.d + 1);
+        P.passes = std::max(P.passes, part.passes);
+        P.parts.push_back(std::move(part));
+    }
+    P.Z2 = row;
+    P.Z1 = xrow;
+    // merged segments: entries of the i-th term from e0, its partial rows from p0 after
+    // the zero row
+    std::vector<Chunk> ch2;
+    std::vector<Combine> cb2;
+    P.ent.reserve(E);
+    size_t e0 = 0;
+    int p0 = 0;
+    for (size_t i = 0; i < P.parts.size(); i++) {
+        const KgPart& part = P.parts[i];
+        const TermLattice& lat = m.terms[part.term]->lat;
+        const int d1 = lat.d + 1;
+        launch_entry_gather(lat.csr_e.get(), N * d1, d1, lat.bary.get(), nullptr, 0, nullptr, nullptr, 0, nullptr,
+                            P.ent.get() + e0, s, (int)i * N);
+        for (const Chunk& c : lat.chunks)
+            ch2.push_back(Chunk{part.F + c.v, c.beg + (int)e0, c.end + (int)e0,
+                                c.out >= 0 ? part.F + c.out : P.Z2 + 1 + p0 + (-c.out - 1)});
+        for (const Combine& c : lat.combines)
+            cb2.push_back(Combine{part.F + c.v, P.Z2 + 1 + p0 + c.first, c.count, c.pad});
+        e0 += (size_t)N * d1;
+        p0 += lat.n_partials;
+    }
+    require_adjoining(ch2);
+    if (!ch2.empty() && (ch2.front().beg != 0 || (size_t)ch2.back().end != E))
+        throw Error{DCRF_ERR_STATE, "kernel gradient: segments do not cover the entries"};
+    upload(P.ch2, ch2, s);
+    upload(P.cb2, cb2, s);
+    P.nch = (int)ch2.size();
+    P.ncb = (int)cb2.size();
+    KgRowMap rm{};
+    rm.n = (int)P.parts.size();
+    rm.Z2 = P.Z2;
+    rm.Z1 = P.Z1;
+    for (size_t i = 0; i < P.parts.size(); i++) {
+        rm.F[i] = P.parts[i].F;
+        rm.X[i] = P.parts[i].X;
+    }
+    P.ch1.reserve(P.nch);
+    P.cb1.reserve(P.ncb);
+    launch_kg_relayout(P.ch2.get(), P.nch, P.cb2.get(), P.ncb, rm, P.ch1.get(), P.cb1.get(), s);
+    // pass tables: pass p of term i expands the axes of its p-th pass; the forward chain
+    // reads rows from F, the reverse chain from F in its first pass (the splat rows) and
+    // from R after, the x chain from X
+    size_t tab_total = 0;
+    for (const KgPart& part : P.parts)
+        for (int q : part.steps) tab_total += (size_t)3 * part.V * (q == 1 ? 3 : 9);
+    P.tab.reserve(std::max<size_t>(tab_total, 1));
+    P.tab_off.assign(P.passes, std::vector<int>(3 * P.parts.size(), -1));
+    P.vp2.assign(P.passes, 0);
+    P.vp1.assign(P.passes, 0);
+    size_t toff = 0;
+    for (size_t i = 0; i < P.parts.size(); i++) {
+        KgPart& part = P.parts[i];
+        const TermLattice& lat = m.terms[part.term]->lat;
+        const int V = lat.V, d = lat.d, d1 = d + 1;
+        DevBuf<int2> nF, nR, nX;  // the neighbour rows in each chain's numbering
+        nF.reserve((size_t)d1 * V);
+        nR.reserve((size_t)d1 * V);
+        nX.reserve((size_t)d1 * V);
+        launch_remap_nbr(lat.nbr.get(), nF.get(), d1 * V, part.F, P.Z2, s);
+        launch_remap_nbr(lat.nbr.get(), nR.get(), d1 * V, part.R, P.Z2, s);
+        launch_remap_nbr(lat.nbr.get(), nX.get(), d1 * V, part.X, P.Z1, s);
+        int first = 0;
+        for (int p = 0; p < part.passes; p++) {
+            const int q = part.steps[p];
+            int fa[3], ta[3];
+            for (int j = 0; j < q; j++) {
+                fa[j] = first + j;
+                ta[j] = d - first - j;
+            }
+            const size_t n = (size_t)V * (q == 1 ? 3 : 9);
+            std::vector<int>& off = P.tab_off[p];
+            off[3 * i] = (int)toff;
+            launch_pass_table(nF.get(), V, part.F, P.Z2, q, fa, P.tab.get() + toff, s);
+            toff += n;
+            off[3 * i + 1] = (int)toff;
+            if (p == 0)
+                launch_pass_table(nF.get(), V, part.F, P.Z2, q, ta, P.tab.get() + toff, s);
+            else
+                launch_pass_table(nR.get(), V, part.R, P.Z2, q, ta, P.tab.get() + toff, s);
+            toff += n;
+            off[3 * i + 2] = (int)toff;
+            launch_pass_table(nX.get(), V, part.X, P.Z1, q, fa, P.tab.get() + toff, s);
+            toff += n;
+            P.vp2[p] = part.R + V;  // the parts are in decreasing pass count
+            P.vp1[p] = part.X + V;
+            first += q;
+        }
+        DevBuf<int> gv;
+        gv.reserve((size_t)N * d1);
+        part.sw.reserve((size_t)N * d1);
+        launch_slice_weights(lat.vtx.get(), lat.bary.get(), nullptr, 0, slice_alpha(d), N, d1, 0, gv.get(),
+                             part.sw.get(), s);
+    }
+    // vertex rows: the zero rows stay zero (splats write vertex and partial rows, the
+    // blur passes rows below the zero row)
+    auto zeroed = [&](DevBuf<float>& b, size_t n) {
+        b.reserve(n);
+        launch_fill(b.get(), 0.f, n, s);
+    };
+    const size_t rows2 = (size_t)P.Z2 + 1 + parts_total;
+    zeroed(P.S2, rows2 * 2 * L);
+    zeroed(P.X2[0], (size_t)(P.Z2 + 1) * 2 * L);
+    zeroed(P.X2[1], (size_t)(P.Z2 + 1) * 2 * L);
+    zeroed(P.S1, ((size_t)P.Z1 + 1 + parts_total) * L);
+    zeroed(P.B1, (size_t)(P.Z1 + 1) * L);
+    // the all-ones input of every term (stacked, one value per pixel) through the plan's
+    // splat and both chains; the reverse rows are the terms' K^T 1
+    zeroed(P.ones[0], rows2);
+    zeroed(P.ones[1], (size_t)P.Z2 + 1);
+    zeroed(P.ones[2], (size_t)P.Z2 + 1);
+    DevBuf<float> in1;
+    in1.reserve((size_t)P.parts.size() * N);
+    launch_fill(in1.get(), 1.f, (size_t)P.parts.size() * N, s);
+    launch_splat(P.ch2.get(), P.nch, P.ent.get(), in1.get(), 1, P.ones[0].get(), 1, 1, s);
+    launch_combine(P.cb2.get(), P.ncb, P.ones[0].get(), 1, 1, s);
+    float* const obuf[2] = {P.ones[1].get(), P.ones[2].get()};
+    kg_blur(P, false, P.ones[0].get(), obuf, 1, s);
+    for (KgPart& part : P.parts) part.ones_rev = obuf[(part.passes - 1) & 1] + part.R;
+    P.gen = m.lat_gen;
+    DCRF_HIP_CHECK(hipStreamSynchronize(s));  // the uploads from host lists are complete
+}
+
+// The plans of the terms with kernel parameters, rebuilt (in place) when a lattice
+// changed.  A plan takes at most DCRF_MAX_TERMS / 2 terms (a pass launch holds two chains
+// per term), and its stacked input rows, entries and vertex rows stay within int.
+void ensure_kg_plans(dcrf_model& m) {
+    if (!m.kgplans.empty() && m.kgplans.front()->gen == m.lat_gen) return;
+    const long long N = m.N(), lim = std::numeric_limits<int>::max();
+    std::vector<std::vector<int>> groups;
+    std::vector<int> g;
+    long long e = 0, rows = 0;
+    for (int k = 0; k < (int)m.terms.size(); k++) {
+        const Term& t = *m.terms[k];
+        if (!t.num_kparams()) continue;
+        const long long ek = N * (t.lat.d + 1), rk = 2LL * t.lat.V + t.lat.n_partials;
+        if (!g.empty() && ((int)g.size() == DCRF_MAX_TERMS / 2 || e + ek > lim || rows + rk + 1 > lim ||
+                           (long long)(g.size() + 1) * N > lim)) {
+            groups.push_back(g);
+            g.clear();
+            e = rows = 0;
+        }
+        g.push_back(k);
+        e += ek;
+        rows += rk;
+    }
+    if (!g.empty()) groups.push_back(g);
+    m.kgplans.resize(groups.size());
+    for (size_t i = 0; i < groups.size(); i++) {
+        if (!m.kgplans[i]) m.kgplans[i] = std::make_unique<KgPlan>();
+        build_kg_plan(m, groups[i], *m.kgplans[i]);
     }
 }
 
@@ -766,7 +759,7 @@ void kernel_grad_group(dcrf_model& m, const KgPlan& P, const float* b, const flo
         const bool full = t.ktype == DCRF_FULL_KERNEL;
         double* partial = m.kpartial.get() + (size_t)k * pstride;
         if (kg_grad_fused(L, d)) {
-            launch_kg_grad(vtx, rank, fwd, rev, pki, xr, part.ones_rev.get(), x, t.feat0.get(), m.geod.pix_base.get(),
+            launch_kg_grad(vtx, rank, fwd, rev, pki, xr, part.ones_rev, x, t.feat0.get(), m.geod.pix_base.get(),
                            m.geod.pad_base.get(), m.geo.n_inst, N, L, d, full, partial, s);
             continue;
         }
@@ -775,7 +768,7 @@ void kernel_grad_group(dcrf_model& m, const KgPlan& P, const float* b, const flo
         float* dfg = dfr + (size_t)N * d;
         if (has_x) {
             launch_slice_grad(xr, L, 1, vtx, rank, nullptr, 0, N, L, d, 0, dfr, s);
-            launch_slice_grad(part.ones_rev.get(), 1, 0, vtx, rank, x, L, N, L, d, 1, dfr, s);
+            launch_slice_grad(part.ones_rev, 1, 0, vtx, rank, x, L, N, L, d, 1, dfr, s);
         }
         launch_slice_grad(fwd, L2, 1, vtx, rank, pki + L, L2, N, L, d, 0, dfg, s);
         launch_slice_grad(rev + L, L2, 1, vtx, rank, pki, L2, N, L, d, 1, dfg, s);

@@ -131,7 +131,8 @@ struct KgPart {
     int passes = 0;          // blur passes of the lattice
     std::vector<int> steps;  // lattice steps of each pass
     DevBuf<float> sw;        // [N][d+1] slice weights bary / (1 + 2^-d)
-    DevBuf<float> ones_rev;  // [V+1] splat of ones blurred in reverse order (row V zero)
+    const float* ones_rev = nullptr;  // [V] the all-ones input splatted and blurred in reverse
+                                      // order (rows of KgPlan::ones)
 };
 struct KgPlan {
     std::vector<KgPart> parts;
@@ -144,6 +145,7 @@ struct KgPlan {
     DevBuf<int> tab;            // the pass tables of all chains
     std::vector<std::vector<int>> tab_off;  // per pass: forward, reverse, x table of each part
     DevBuf<float> S2, X2[2], S1, B1;  // splat and ping-pong rows of the two layouts
+    DevBuf<float> ones[3];            // the all-ones input's splat and blur rows (one value each)
 };
 
 struct Term {
@@ -439,6 +441,16 @@ void launch_slice_grad(const float* vals, int ldv, int ks, const int* vtx, const
 size_t kparam_partial_doubles(int d);
 void launch_kparam_reduce(const float* r, const float* g, const float* f, const int* pix_base, const int* pad_base,
                           int n_inst, int N, int d, bool full, double* partial, double* acc, hipStream_t s);
+// The row numbering of a kernel-gradient plan's x layout (KgPlan): part i's vertex rows
+// start at F[i] in the paired layout and at X[i] in the x layout; partial rows follow
+// the zero rows Z2 and Z1.
+struct KgRowMap {
+    int n, Z2, Z1;
+    int F[DCRF_MAX_TERMS / 2], X[DCRF_MAX_TERMS / 2];
+};
+// ch1 / cb1 = ch / cb with their rows in the x layout.
+void launch_kg_relayout(const Chunk* ch, int nch, const Combine* cb, int ncb, const KgRowMap& m, Chunk* ch1,
+                        Combine* cb1, hipStream_t s);
 // out[i] = (float)acc[i].
 void launch_kparam_store(const double* acc, int n, float* out, hipStream_t s);
 // Scaled features diag(P) f or P f (P column-major) over the padded table.

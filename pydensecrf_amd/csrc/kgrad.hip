@@ -481,6 +481,32 @@ __global__ void k_scale_features(const float* __restrict__ f, int npad, int d, i
     out[i] = v;
 }
 
+// The x layout's segment and recombination lists from the paired layout's: a vertex row
+// F_i + j of part i becomes X_i + j, a partial row Z2 + q becomes Z1 + q.
+__device__ __forceinline__ int kg_map_row(const KgRowMap& m, int r) {
+    if (r > m.Z2) return m.Z1 + (r - m.Z2);
+    int i = 0;
+    for (int k = 1; k < m.n; k++)
+        if (r >= m.F[k]) i = k;  // parts in increasing F
+    return m.X[i] + (r - m.F[i]);
+}
+
+__global__ void k_kg_relayout(const Chunk* __restrict__ ch, int nch, const Combine* __restrict__ cb, int ncb,
+                              KgRowMap m, Chunk* __restrict__ ch1, Combine* __restrict__ cb1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nch) {
+        Chunk c = ch[i];
+        c.v = kg_map_row(m, c.v);
+        c.out = kg_map_row(m, c.out);
+        ch1[i] = c;
+    } else if (i < nch + ncb) {
+        Combine c = cb[i - nch];
+        c.v = kg_map_row(m, c.v);
+        c.first = kg_map_row(m, c.first);
+        cb1[i - nch] = c;
+    }
+}
+
 }  // namespace
 
 constexpr int kKparamBlocks = 512;   // k_kparam_partial's grid
@@ -584,6 +610,14 @@ void launch_kparam_reduce(const float* r, const float* g, const float* f, const 
     k_kparam_partial<<<nb, kThreads, 0, s>>>(r, g, f, pix_base, pad_base, n_inst, N, d, full ? 1 : 0, partial);
     DCRF_HIP_CHECK(hipGetLastError());
     k_kparam_final<<<np, kThreads, 0, s>>>(partial, nb, np, acc);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_kg_relayout(const Chunk* ch, int nch, const Combine* cb, int ncb, const KgRowMap& m, Chunk* ch1,
+                        Combine* cb1, hipStream_t s) {
+    const int n = nch + ncb;
+    if (n <= 0) return;
+    k_kg_relayout<<<blocks_for(n), kThreads, 0, s>>>(ch, nch, cb, ncb, m, ch1, cb1);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

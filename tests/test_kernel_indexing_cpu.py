@@ -21,7 +21,8 @@ per-block statistic rows and the matrix-core statistic's tiles: every entry is r
 exactly once and every write stays inside its buffer; and the one-pass kernel gradient
 k_kg_grad (kgrad.hip): its capped grid with several rounds per block takes every point
 exactly once, each parameter lane adds every point's product exactly once from the LDS
-row its group's leader wrote, and its partial rows fit the buffer the backward reserves.
+row its group's leader wrote, and its partial rows fit the buffer the backward reserves;
+and k_kg_relayout's row map from the plan's paired layout to its x layout.
 """
 import numpy as np
 import pytest
@@ -376,3 +377,39 @@ def kg_grad_launch(N, L, d, full, kparam_blocks=512):
 def test_kg_grad_points_once(N, L, d, full):
     taken, added = kg_grad_launch(N, L, d, full)
     assert np.all(taken == 1) and np.all(added == 1)
+
+
+def kg_map_row(F, X, Z2, Z1, r):
+    """kgrad.hip kg_map_row: a row of the kernel-gradient plan's paired layout in its x
+    layout."""
+    if r > Z2:
+        return Z1 + (r - Z2)
+    i = 0
+    for k in range(1, len(F)):
+        if r >= F[k]:
+            i = k
+    return X[i] + (r - F[i])
+
+
+@pytest.mark.parametrize("V,partials", [([36254, 101712], [900, 5000]), ([5], [0]), ([7, 3, 11, 2], [1, 0, 4, 2])])
+def test_kg_relayout_rows(V, partials):
+    """The x layout's rows of every vertex row and partial row of the paired layout
+    (forward rows F_i, reverse rows R_i = F_i + V_i side by side per term, the zero row,
+    then the partial rows): vertex rows land on [X_i, X_i + V_i), partial rows after the
+    x layout's zero row, one to one."""
+    F, X, row, xrow = [], [], 0, 0
+    for v in V:
+        F.append(row)
+        X.append(xrow)
+        row += 2 * v
+        xrow += v
+    Z2, Z1 = row, xrow
+    got = []
+    for i, v in enumerate(V):
+        rows = [kg_map_row(F, X, Z2, Z1, F[i] + j) for j in range(v)]
+        assert rows == list(range(X[i], X[i] + v))
+        got += rows
+    P = sum(partials)
+    prow = [kg_map_row(F, X, Z2, Z1, Z2 + 1 + q) for q in range(P)]
+    assert prow == list(range(Z1 + 1, Z1 + 1 + P))
+    assert len(set(got + prow)) == len(got) + len(prow) and max(got + prow, default=0) < Z1 + 1 + P
