@@ -576,9 +576,19 @@ def main():
     for _ in range(3):
         m.inference(iters)
     ms_host = (time.perf_counter() - h0) * 1e3 / 3
+    # and with a new unary for every call (setUnaryEnergy of a host (L, N) array first,
+    # as a drop-in user does per image on a model they keep)
+    h0 = time.perf_counter()
+    for _ in range(3):
+        m.set_unary(U)
+        m.inference(iters)
+    ms_host_u = (time.perf_counter() - h0) * 1e3 / 3
     out["host_buffers"] = {"ms_per_step": round(ms_host, 4),
                            "value": round(N * iters / (ms_host / 1e3) / 1e6, 3),
-                           "note": "dcrf_inference into a host (L, N) buffer: device work plus the Q download"}
+                           "with_unary_ms_per_step": round(ms_host_u, 4),
+                           "note": "dcrf_inference into a host (L, N) buffer: device work plus the Q download; "
+                                   "with_unary: dcrf_set_unary of a host array before each call (its upload "
+                                   "included)"}
     if not args.no_batched:
         out["batched"] = bench_batched(torch, capi, syn, 64, iters, 3, log)
         if not args.no_cpu_baseline:

@@ -62,7 +62,8 @@ const char *dcrf_last_error(void);
 /* Number of visible HIP devices (0 without a GPU); never fails. */
 int dcrf_device_count(void);
 /* No counterpart in the reference: the library keeps freed device buffers for reuse
- * (up to 4 GiB per process); this returns them to the runtime.  bytes may be NULL. */
+ * (up to 4 GiB per process) and one page-locked host staging area for large transfers
+ * (as large as the largest one); this returns both to the runtime.  bytes may be NULL. */
 int dcrf_release_cached_memory(size_t *bytes);
 
 /* ---- construction --------------------------------------------------------------- */

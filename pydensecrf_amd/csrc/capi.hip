@@ -15,6 +15,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <numeric>
 
@@ -57,13 +58,168 @@ struct PendingTiming {
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------- host transfers
+// Large copies between pageable host memory (the caller's arrays) and the device go
+// through one page-locked staging area per process (hipHostMalloc, grown on demand,
+// shared by all models under a lock, kept until dcrf_release_cached_memory) in chunks:
+// the DMA engine moves some chunks while host threads copy others in or out, where the
+// runtime would stage pageable memory through one host thread.
+constexpr size_t kStagedMin = (size_t)8 << 20;  // smaller copies: one plain copy
+constexpr int kCopyThreads = 8;
+
+struct HostStage {
+    std::mutex mu;
+    char* p = nullptr;
+    size_t cap = 0;
+};
+HostStage& host_stage() {
+    static HostStage* h = new HostStage;  // never destroyed: no runtime call at process exit
+    return *h;
+}
+
+// Threads for a host copy of n bytes: about one per 2 MiB, at most kCopyThreads and the
+// hardware threads.
+int copy_threads(size_t n) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max<size_t>(1, std::min({(size_t)kCopyThreads, hw, n >> 21}));
+}
+
+// Chunk t of n bytes split T ways, 4 KiB aligned: [off(t), off(t + 1)).
+inline size_t chunk_off(size_t n, int T, int t) { return t >= T ? n : (n / T * t) & ~(size_t)4095; }
+
+// work(t) for t < T: t = 0 on this thread, the others on threads of their own where the
+// system starts them (else here as well).  work must not throw.
+template <typename F>
+void run_parallel(int T, const F& work) {
+    std::vector<std::thread> th;
+    int t = 1;
+    try {
+        for (; t < T; t++) th.emplace_back(work, t);
+    } catch (...) {
+    }
+    work(0);
+    for (int u = t; u < T; u++) work(u);
+    for (auto& x : th) x.join();
+}
+
+// dst = src (host memory, n bytes) with copy_threads(n) threads.
+void host_copy(void* dst, const void* src, size_t n) {
+    const int T = copy_threads(n);
+    run_parallel(T, [&](int t) {
+        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
+        std::memcpy((char*)dst + a, (const char*)src + a, b - a);
+    });
+}
+
+char* stage_reserve(HostStage& H, size_t n) {
+    if (n > H.cap) {
+        if (H.p) DCRF_HIP_CHECK(hipHostFree(H.p));
+        H.p = nullptr;
+        H.cap = 0;
+        DCRF_HIP_CHECK(hipHostMalloc((void**)&H.p, n, hipHostMallocDefault));
+        H.cap = n;
+    }
+    return H.p;
+}
+
+// Events of one staged copy, destroyed on every exit.
+struct ChunkEvents {
+    std::vector<hipEvent_t> ev;
+    explicit ChunkEvents(int T) : ev(T, nullptr) {
+        for (auto& e : ev) DCRF_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~ChunkEvents() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// dst (host) = src (device), n bytes, after the work queued on s; returns when done.
+void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s) {
+    if (n < kStagedMin) {
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    HostStage& H = host_stage();
+    std::lock_guard<std::mutex> lk(H.mu);
+    char* stage = stage_reserve(H, n);
+    const int T = copy_threads(n);
+    ChunkEvents E(T);
+    for (int t = 0; t < T; t++) {  // all chunks queued before any thread starts
+        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
+        DCRF_HIP_CHECK(hipMemcpyAsync(stage + a, (const char*)src + a, b - a, hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipEventRecord(E.ev[t], s));
+    }
+    std::vector<hipError_t> err(T, hipSuccess);
+    run_parallel(T, [&](int t) {  // each chunk leaves the staging area as soon as it arrived
+        err[t] = hipEventSynchronize(E.ev[t]);
+        if (err[t] != hipSuccess) return;
+        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
+        std::memcpy((char*)dst + a, stage + a, b - a);
+    });
+    for (hipError_t e : err) DCRF_HIP_CHECK(e);
+}
+
+// dst (device) = src (host), n bytes, queued on s; returns when the copy is done.
+void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s) {
+    if (n < kStagedMin) {
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    HostStage& H = host_stage();
+    std::lock_guard<std::mutex> lk(H.mu);
+    char* stage = stage_reserve(H, n);
+    const int T = copy_threads(n);
+    std::vector<hipError_t> err(T, hipSuccess);
+    run_parallel(T, [&](int t) {  // each chunk is queued for the DMA as soon as it is staged
+        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
+        std::memcpy(stage + a, (const char*)src + a, b - a);
+        err[t] = hipMemcpyAsync((char*)dst + a, stage + a, b - a, hipMemcpyHostToDevice, s);
+    });
+    const hipError_t sync = hipStreamSynchronize(s);  // the staging area is free again
+    for (hipError_t e : err) DCRF_HIP_CHECK(e);
+    DCRF_HIP_CHECK(sync);
+}
+
+// Frees the staging area; returns its bytes.
+size_t release_host_stage() {
+    HostStage& H = host_stage();
+    std::lock_guard<std::mutex> lk(H.mu);
+    const size_t b = H.cap;
+    if (H.p) DCRF_HIP_CHECK(hipHostFree(H.p));
+    H.p = nullptr;
+    H.cap = 0;
+    return b;
+}
+
+}  // namespace
+
+// Host floats left uninitialised when allocated (a copy fills them).
+struct HostFloats {
+    std::unique_ptr<float[]> p;
+    size_t n = 0;
+    void resize(size_t count) {
+        if (count != n) {
+            p.reset(count ? new float[count] : nullptr);
+            n = count;
+        }
+    }
+    float* data() { return p.get(); }
+    const float* data() const { return p.get(); }
+    size_t size() const { return n; }
+};
+
 struct dcrf_model {
     Geometry geo;
     GeometryDev geod;
     int L = 0;
     bool dev_ready = false;
     hipStream_t stream = nullptr;
-    std::vector<float> unary_host;  // (L, N) row-major as given
+    HostFloats unary_host;  // (L, N) row-major as given
     bool has_unary = false, unary_dirty = false;
     DevBuf<float> unary;  // [N][L]
     // LogisticUnaryEnergy (U = L f): parameters L [L][F] row-major, features [N][F]
@@ -300,8 +456,7 @@ void ensure_unary(dcrf_model& m) {
     }
     m.stage.reserve(n);
     m.unary.reserve(n);
-    DCRF_HIP_CHECK(hipMemcpyAsync(m.stage.get(), m.unary_host.data(), sizeof(float) * n, hipMemcpyHostToDevice,
-                                  m.stream));
+    copy_to_device(m.stage.get(), m.unary_host.data(), sizeof(float) * n, m.stream);
     launch_transpose(m.stage.get(), m.unary.get(), m.L, m.N(), m.stream);
     DCRF_HIP_CHECK(hipStreamSynchronize(m.stream));
     m.unary_dirty = false;
@@ -1610,7 +1765,7 @@ const char* dcrf_last_error(void) { return g_last_error.c_str(); }
 
 int dcrf_release_cached_memory(size_t* bytes) {
     return guard([&] {
-        const size_t b = pool_release_all();
+        const size_t b = pool_release_all() + release_host_stage();
         if (bytes) *bytes = b;
     });
 }
@@ -1667,7 +1822,8 @@ int dcrf_set_unary(dcrf_model* m, const float* U) {
     return guard([&] {
         require(m && U, "null argument");
         const size_t n = (size_t)m->N() * m->L;
-        m->unary_host.assign(U, U + n);
+        m->unary_host.resize(n);
+        host_copy(m->unary_host.data(), U, sizeof(float) * n);
         m->has_unary = true;
         m->unary_dirty = true;
         m->logistic = false;
@@ -1737,7 +1893,10 @@ int dcrf_set_unary_instance(dcrf_model* m, int i, const float* U) {
         require(!m->logistic, "the model has a logistic unary: set a full unary with dcrf_set_unary first");
         require(i >= 0 && i < m->geo.n_inst, "instance index out of range");
         const int N = m->N(), L = m->L, b = m->geo.pix_base[i], Ni = m->geo.N[i];
-        if (m->unary_host.size() != (size_t)N * L) m->unary_host.assign((size_t)N * L, 0.f);
+        if (m->unary_host.size() != (size_t)N * L) {  // instances not set yet stay zero
+            m->unary_host.resize((size_t)N * L);
+            std::fill(m->unary_host.data(), m->unary_host.data() + (size_t)N * L, 0.f);
+        }
         for (int l = 0; l < L; l++)
             std::memcpy(m->unary_host.data() + (size_t)l * N + b, U + (size_t)l * Ni, sizeof(float) * Ni);
         m->has_unary = true;
@@ -1793,9 +1952,7 @@ int dcrf_inference(dcrf_model* m, int n_iter, float* Q) {
         prepare(*m);
         const float* U = m->has_unary ? m->unary.get() : nullptr;
         run_inference(*m, n_iter, U, m->Q.get(), m->stream);
-        DCRF_HIP_CHECK(hipMemcpyAsync(Q, m->Q.get(), sizeof(float) * (size_t)m->N() * m->L, hipMemcpyDeviceToHost,
-                                      m->stream));
-        DCRF_HIP_CHECK(hipStreamSynchronize(m->stream));
+        copy_to_host(Q, m->Q.get(), sizeof(float) * (size_t)m->N() * m->L, m->stream);
         if (m->timing) collect_timing(*m);
     });
 }
@@ -1904,8 +2061,7 @@ int dcrf_start_inference(dcrf_model* m, float* Q) {
         ensure_unary(*m);
         m->Q.reserve(n);
         launch_softmax_neg(m->unary.get(), m->Q.get(), m->N(), m->L, m->stream);
-        DCRF_HIP_CHECK(hipMemcpyAsync(Q, m->Q.get(), sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
-        DCRF_HIP_CHECK(hipStreamSynchronize(m->stream));
+        copy_to_host(Q, m->Q.get(), sizeof(float) * n, m->stream);
     });
 }
 
@@ -1917,15 +2073,14 @@ int dcrf_step_inference(dcrf_model* m, float* Q, float* tmp1, float* tmp2) {
         m->tmp1.reserve(n);
         m->tmp2.reserve(n);
         hipStream_t s = m->stream;
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->Q.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        copy_to_device(m->Q.get(), Q, sizeof(float) * n, s);
         if (m->terms.empty()) launch_fill(m->tmp2.get(), 0.f, n, s);
         run_filter(*m, m->Q.get(), m->L, false, s);
         launch_meanfield(m->has_unary ? m->unary.get() : nullptr, m->Q.get(), m->N(), m->L,
                          slice_table(*m, false, m->kw_set ? m->kw.get() : nullptr), m->tmp1.get(), m->terms.empty() ? nullptr : m->tmp2.get(), s);
-        DCRF_HIP_CHECK(hipMemcpyAsync(Q, m->Q.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
-        if (tmp1) DCRF_HIP_CHECK(hipMemcpyAsync(tmp1, m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
-        if (tmp2) DCRF_HIP_CHECK(hipMemcpyAsync(tmp2, m->tmp2.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        copy_to_host(Q, m->Q.get(), sizeof(float) * n, s);
+        if (tmp1) copy_to_host(tmp1, m->tmp1.get(), sizeof(float) * n, s);
+        if (tmp2) copy_to_host(tmp2, m->tmp2.get(), sizeof(float) * n, s);
         if (m->timing) collect_timing(*m);
     });
 }
@@ -1937,7 +2092,7 @@ int dcrf_kl_divergence(dcrf_model* m, const float* Q, double* kl) {
         const size_t n = (size_t)m->N() * m->L;
         m->work.reserve(n);
         hipStream_t s = m->stream;
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        copy_to_device(m->work.get(), Q, sizeof(float) * n, s);
         run_filter(*m, m->work.get(), m->L, false, s);
         const LaunchShape sh = shape_for(m->L);
         const int nb = (m->N() + sh.items - 1) / sh.items;
@@ -1964,12 +2119,11 @@ int dcrf_apply_pairwise(dcrf_model* m, int term, const float* Q, float* out, int
         m->work.reserve(n);
         m->tmp1.reserve(n);
         hipStream_t s = m->stream;
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        copy_to_device(m->work.get(), Q, sizeof(float) * n, s);
         run_filter(*m, m->work.get(), m->L, transpose != 0, s);
         launch_slice_term(m->tmp1.get(), m->N(), m->L,
                           slice_table(*m, transpose != 0, m->kw_set ? m->kw.get() : nullptr), term, s);
-        DCRF_HIP_CHECK(hipMemcpyAsync(out, m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        copy_to_host(out, m->tmp1.get(), sizeof(float) * n, s);
         if (m->timing) collect_timing(*m);
     });
 }
@@ -2033,13 +2187,12 @@ int dcrf_pairwise_energy(dcrf_model* m, const int16_t* labels, int term, float* 
         m->work.reserve(n);
         m->tmp1.reserve(n);
         hipStream_t s = m->stream;
-        DCRF_HIP_CHECK(hipMemcpyAsync(m->work.get(), Q.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+        copy_to_device(m->work.get(), Q.data(), sizeof(float) * n, s);
         run_filter(*m, m->work.get(), L, false, s);
         const SliceTable st = slice_table(*m, false, m->kw_set ? m->kw.get() : nullptr);
         for (int k = term < 0 ? 0 : term; k < (term < 0 ? T : term + 1); k++) {
             launch_slice_term(m->tmp1.get(), N, L, st, k, s);
-            DCRF_HIP_CHECK(hipMemcpyAsync(x.data(), m->tmp1.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
-            DCRF_HIP_CHECK(hipStreamSynchronize(s));
+            copy_to_host(x.data(), m->tmp1.get(), sizeof(float) * n, s);
             for (int i = 0; i < N; i++) {
                 const float e = labels[i] >= 0 && labels[i] < L ? -0.5f * x[(size_t)i * L + labels[i]] : 0.0f;
                 out[i] = out[i] + e;
@@ -2249,7 +2402,7 @@ int dcrf_gradient_objective(dcrf_model* m, int n_iter, int objective_kind, const
         if (kernel_grad && nkp && n_iter > 0)
             DCRF_HIP_CHECK(hipMemcpyAsync(KG.data(), m->kacc.get(), sizeof(double) * nkp, hipMemcpyDeviceToHost, s));
         if (unary_grad)
-            DCRF_HIP_CHECK(hipMemcpyAsync(unary_grad, m->tmp1.get(), sizeof(float) * NL, hipMemcpyDeviceToHost, s));
+            copy_to_host(unary_grad, m->tmp1.get(), sizeof(float) * NL, s);
         std::vector<double> UG;
         if (m->logistic) {
             // LogisticUnaryEnergy::gradient summed over the iterations (densecrf.cpp:261-262,

@@ -130,6 +130,47 @@ def test_start_step_kl_apply():
             np.testing.assert_allclose(g.apply(k, Qo, tr), o.apply(k, Qo, tr), rtol=1e-5, atol=1e-5)
 
 
+def test_large_host_transfers_bitwise():
+    """Host arrays of 22 MB (512x512x21), above the 8 MiB from which the library moves
+    them through its page-locked staging area in chunks with several host threads: the
+    unary upload, inference's download, start / step (Q up, Q, tmp1 and tmp2 down), KL
+    and apply (Q up, the messages down).  Each agrees bit for bit with a path whose
+    arrays never leave the device or with the other host path, and the staged unary
+    gives the same Q as the unary passed as a device buffer."""
+    import torch
+
+    from pydensecrf_amd import capi
+
+    W, H, L = 512, 512, 21
+    N = W * H
+    img, U, _ = image_case(W, H, L, seed=23)
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(U)
+    g.add_gaussian(3.0, 3.0, 3.0)
+    g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+    Qi = g.inference(2)
+    assert Qi.shape == (L, N)
+    Qd = torch.empty((N, L), dtype=torch.float32, device="cuda")
+    Ud = torch.tensor(np.ascontiguousarray(U.T), device="cuda")
+    g.inference_device(2, Qd.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(Qi.T.view(np.int32), Qd.cpu().numpy().view(np.int32))
+    g.inference_device(2, Qd.data_ptr(), Ud.data_ptr())  # the unary from the device
+    torch.cuda.synchronize()
+    assert np.array_equal(Qi.T.view(np.int32), Qd.cpu().numpy().view(np.int32))
+    Q = g.start()
+    for _ in range(2):
+        Q, t1, t2 = g.step(Q)
+    assert np.array_equal(Qi.view(np.int32), Q.view(np.int32))
+    assert np.isfinite(t1).all() and np.isfinite(t2).all() and np.abs(t2).max() > 0
+    kl1, kl2 = g.kl(Q), g.kl(np.ascontiguousarray(Q))
+    assert kl1 == kl2 and np.isfinite(kl1)
+    for k in range(2):
+        a = g.apply(k, Q)
+        assert np.array_equal(a.view(np.int32), g.apply(k, np.ascontiguousarray(Q)).view(np.int32))
+        assert np.abs(a).max() > 0
+
+
 @pytest.mark.parametrize("terms", ["g", "b", "gb", "bg"])
 @pytest.mark.parametrize("L", [4, 21])
 @pytest.mark.parametrize("W,H", [(45, 31), (64, 48), (37, 23)])
