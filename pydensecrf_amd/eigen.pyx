@@ -10,6 +10,32 @@ temporaries of ``DenseCRF`` (memory [N][L], the device layout of the library).
 from cpython.buffer cimport PyBUF_FORMAT
 from libc.stdlib cimport malloc, free, realloc
 from libc.string cimport memcpy
+from posix.mman cimport madvise, MADV_HUGEPAGE
+from posix.stdlib cimport posix_memalign
+
+# Storage of 4 MiB and more is 2 MiB aligned and advised for transparent huge pages (as
+# numpy does for its large arrays): a fresh 66 MB Q then takes ~33 page faults on first
+# touch instead of ~16,000.
+cdef size_t _HUGE = 4 << 20
+cdef size_t _HUGE_ALIGN = 2 << 20
+
+
+cdef float* _resize(float* old, size_t old_bytes, size_t nbytes) except NULL:
+    """realloc's contract (contents kept up to the smaller size; old freed), with large
+    blocks huge-page aligned and advised."""
+    cdef void* p = NULL
+    if nbytes < _HUGE:
+        p = realloc(old, nbytes)
+        if p == NULL:
+            raise MemoryError()
+        return <float*>p
+    if posix_memalign(&p, _HUGE_ALIGN, nbytes) != 0:
+        raise MemoryError()
+    madvise(p, nbytes, MADV_HUGEPAGE)  # advice only: its result does not matter
+    if old != NULL:
+        memcpy(p, old, min(old_bytes, nbytes))
+        free(old)
+    return <float*>p
 
 
 cdef class VectorXf:
@@ -29,9 +55,7 @@ cdef class VectorXf:
             return
         if self.exports:
             raise BufferError("cannot resize a VectorXf while its buffer is exported")
-        p = <float*>realloc(self.data, max(n, 1) * sizeof(float))
-        if p == NULL:
-            raise MemoryError()
+        p = _resize(self.data, self.shape[0] * sizeof(float), max(n, 1) * sizeof(float))
         self.data = p
         self.shape[0] = n
 
@@ -72,9 +96,7 @@ cdef class MatrixXf:
             return
         if self.exports:
             raise BufferError("cannot resize a MatrixXf while its buffer is exported")
-        p = <float*>realloc(self.data, max(rows * cols, 1) * sizeof(float))
-        if p == NULL:
-            raise MemoryError()
+        p = _resize(self.data, self.shape[0] * self.shape[1] * sizeof(float), max(rows * cols, 1) * sizeof(float))
         self.data = p
         self.shape[0] = rows
         self.shape[1] = cols
