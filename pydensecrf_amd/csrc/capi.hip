@@ -8,16 +8,19 @@
 // boundary: every entry point returns a status and records a message.
 #include "dcrf_internal.h"
 
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
-#include <thread>
 #include <new>
 #include <numeric>
+#include <thread>
 
 using namespace dcrf;
 
@@ -198,15 +201,31 @@ size_t release_host_stage() {
 
 }  // namespace
 
-// Host floats left uninitialised when allocated (a copy fills them).
+// Host floats left uninitialised when allocated (a copy fills them).  Blocks of 4 MiB
+// and more are 2 MiB aligned and advised for transparent huge pages: a 66 MB unary then
+// faults in ~33 pages on its first copy instead of ~16,000.
 struct HostFloats {
-    std::unique_ptr<float[]> p;
+    struct Free {
+        void operator()(float* q) const { std::free(q); }
+    };
+    std::unique_ptr<float, Free> p;
     size_t n = 0;
     void resize(size_t count) {
-        if (count != n) {
-            p.reset(count ? new float[count] : nullptr);
-            n = count;
+        if (count == n) return;
+        p.reset();
+        n = 0;
+        if (!count) return;
+        const size_t bytes = sizeof(float) * count;
+        void* q = nullptr;
+        if (bytes >= ((size_t)4 << 20)) {
+            if (posix_memalign(&q, (size_t)2 << 20, bytes) != 0) q = nullptr;
+            if (q) (void)madvise(q, bytes, MADV_HUGEPAGE);  // advice only
+        } else {
+            q = std::malloc(bytes);
         }
+        if (!q) throw std::bad_alloc();
+        p.reset(static_cast<float*>(q));
+        n = count;
     }
     float* data() { return p.get(); }
     const float* data() const { return p.get(); }
@@ -484,7 +503,6 @@ int max_run(const std::vector<int>& first) {
     for (size_t t = 0; t + 1 < first.size(); t++) mx = std::max(mx, first[t + 1] - first[t]);
     return mx;
 }
-
 
 // The splat's wave mode loads a wave's entries as one range: consecutive segments must
 // adjoin (segment c+1 begins where segment c ends), which the tile-major layout gives.

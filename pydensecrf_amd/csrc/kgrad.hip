@@ -278,18 +278,17 @@ struct KgGrad {
 // and R labels per lane (S <= 64: lane groups inside one wave), d <= 6:
 //   g = G(a, c) = dir 0 (K a rows, paired with c) + dir 1 (K^T c rows, paired with a),
 //   r = G(x, 1) = dir 0 (K x rows, paired with ones) + dir 1 (K^T 1, paired with x),
-//   fg = -r + g (g alone without a normaliser term),
+//   fg = -r + g (g alone without a normaliser term: x and its rows are not read),
 // each direction summed over the group's lanes (a pairwise tree towards the group's
 // first lane) before the two are added; the 4d lane shares go through one tree together.
 // The parameter products (pairwise.cpp:157-161), DIAG f[q(p)][j] fg[j], FULL fg[r]
 // f[q(p)][c] at c*d + r (column-major, like the reference), are added in double by one
-// lane per parameter: per round the group leaders leave fg and the point's feature row
-// in LDS and lane q of the wave adds its product for the wave's points in point order,
-// over the block's `rounds`
-// consecutive blocks' worth of points; at the end the waves' sums are added in wave
-// order and the block adds them to partial[blockIdx.x][q] (so one row per block sums
-// all reverse iterations, in iteration order).  The label order of each direction's sum differs
-// from the reference's sequential one (float tolerance).
+// lane per parameter.  Per round the group leaders leave fg and the point's feature row
+// in LDS, and lane q of the wave adds its product for the wave's points in point order,
+// over the block's `rounds` consecutive blocks' worth of points.  At the end the waves'
+// sums are added in wave order, and the block adds them to partial[blockIdx.x][q]: one
+// row per block sums all reverse iterations, in iteration order.  The label order of
+// each direction's sum differs from the reference's sequential one (float tolerance).
 template <int R, int D>
 __global__ void __launch_bounds__(kWaveBlock) k_kg_grad(const int* __restrict__ vtx, const int* __restrict__ rank,
                                                         KgGrad a, int N, int L, int S, int per, int rounds,
