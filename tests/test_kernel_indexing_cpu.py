@@ -22,7 +22,8 @@ exactly once and every write stays inside its buffer; and the one-pass kernel gr
 k_kg_grad (kgrad.hip): its capped grid with several rounds per block takes every point
 exactly once, each parameter lane adds every point's product exactly once from the LDS
 row its group's leader wrote, and its partial rows fit the buffer the backward reserves;
-and k_kg_relayout's row map from the plan's paired layout to its x layout.
+and k_kg_relayout's row map from the plan's paired layout to its x layout.  Last, the
+chunking of the staged host transfers (capi.hip), which is host arithmetic.
 """
 import numpy as np
 import pytest
@@ -413,3 +414,20 @@ def test_kg_relayout_rows(V, partials):
     prow = [kg_map_row(F, X, Z2, Z1, Z2 + 1 + q) for q in range(P)]
     assert prow == list(range(Z1 + 1, Z1 + 1 + P))
     assert len(set(got + prow)) == len(got) + len(prow) and max(got + prow, default=0) < Z1 + 1 + P
+
+
+def chunk_off(n, T, t):
+    """capi.hip chunk_off: chunk t of n bytes split T ways, 4 KiB aligned."""
+    return n if t >= T else (n // T * t) & ~4095
+
+
+@pytest.mark.parametrize("n", [8 << 20, 22020096, 36488452, (8 << 20) + 3, 66060288, 123456789])
+def test_staged_copy_chunks(n):
+    """The staged host transfers' chunks (capi.hip copy_to_host / copy_to_device /
+    host_copy): for every thread count they tile [0, n) in order, without overlap, and
+    all but the last start on a 4 KiB boundary."""
+    for T in range(1, 9):
+        offs = [chunk_off(n, T, t) for t in range(T + 1)]
+        assert offs[0] == 0 and offs[-1] == n
+        assert all(a <= b for a, b in zip(offs, offs[1:]))
+        assert all(o % 4096 == 0 for o in offs[:-1])
