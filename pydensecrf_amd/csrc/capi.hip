@@ -61,7 +61,7 @@ struct PendingTiming {
 
 }  // namespace
 
-namespace {
+namespace dcrf {
 
 // ---------------------------------------------------------------- host transfers
 // Large copies between pageable host memory (the caller's arrays) and the device go
@@ -69,7 +69,7 @@ namespace {
 // shared by all models under a lock, kept until dcrf_release_cached_memory) in chunks:
 // the DMA engine moves some chunks while host threads copy others in or out, where the
 // runtime would stage pageable memory through one host thread.
-constexpr size_t kStagedMin = (size_t)8 << 20;  // smaller copies: one plain copy
+constexpr size_t kStageMax = (size_t)256 << 20;  // larger copies: windows of this size
 constexpr int kCopyThreads = 8;
 
 struct HostStage {
@@ -116,6 +116,15 @@ void host_copy(void* dst, const void* src, size_t n) {
     });
 }
 
+// rows rows of cols floats from src (row stride ss) to dst (row stride ds), rows shared
+// out among copy_threads threads (row t, t + T, ...).
+void copy_rows(float* dst, size_t ds, const float* src, size_t ss, int rows, size_t cols) {
+    const int T = std::min(copy_threads((size_t)rows * cols * sizeof(float)), std::max(rows, 1));
+    run_parallel(T, [&](int t) {
+        for (int r = t; r < rows; r += T) std::memcpy(dst + r * ds, src + r * ss, sizeof(float) * cols);
+    });
+}
+
 char* stage_reserve(HostStage& H, size_t n) {
     if (n > H.cap) {
         if (H.p) DCRF_HIP_CHECK(hipHostFree(H.p));
@@ -139,7 +148,11 @@ struct ChunkEvents {
     }
 };
 
+void copy_to_host_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s);
+void copy_to_device_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s);
+
 // dst (host) = src (device), n bytes, after the work queued on s; returns when done.
+// Copies above kStageMax go through the staging area one window at a time.
 void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s) {
     if (n < kStagedMin) {
         DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
@@ -148,7 +161,28 @@ void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s) {
     }
     HostStage& H = host_stage();
     std::lock_guard<std::mutex> lk(H.mu);
-    char* stage = stage_reserve(H, n);
+    char* stage = stage_reserve(H, std::min(n, kStageMax));
+    for (size_t o = 0; o < n; o += kStageMax)
+        copy_to_host_window(stage, (char*)dst + o, (const char*)src + o, std::min(kStageMax, n - o), s);
+}
+
+// dst (device) = src (host), n bytes, queued on s; returns when the copy is done.
+void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s) {
+    if (n < kStagedMin) {
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    HostStage& H = host_stage();
+    std::lock_guard<std::mutex> lk(H.mu);
+    char* stage = stage_reserve(H, std::min(n, kStageMax));
+    for (size_t o = 0; o < n; o += kStageMax)
+        copy_to_device_window(stage, (char*)dst + o, (const char*)src + o, std::min(kStageMax, n - o), s);
+}
+
+// One window (n <= the staging area's size) of copy_to_host: every chunk's DMA is
+// queued at once, and each thread copies its chunk out when its event fires.
+void copy_to_host_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s) {
     const int T = copy_threads(n);
     ChunkEvents E(T);
     for (int t = 0; t < T; t++) {  // all chunks queued before any thread starts
@@ -166,16 +200,9 @@ void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s) {
     for (hipError_t e : err) DCRF_HIP_CHECK(e);
 }
 
-// dst (device) = src (host), n bytes, queued on s; returns when the copy is done.
-void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s) {
-    if (n < kStagedMin) {
-        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
-        return;
-    }
-    HostStage& H = host_stage();
-    std::lock_guard<std::mutex> lk(H.mu);
-    char* stage = stage_reserve(H, n);
+// One window of copy_to_device: each thread stages its chunk and queues that chunk's
+// DMA; returns when the window's copies are done (the staging area is free again).
+void copy_to_device_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s) {
     const int T = copy_threads(n);
     std::vector<hipError_t> err(T, hipSuccess);
     run_parallel(T, [&](int t) {  // each chunk is queued for the DMA as soon as it is staged
@@ -199,7 +226,7 @@ size_t release_host_stage() {
     return b;
 }
 
-}  // namespace
+}  // namespace dcrf
 
 // Host floats left uninitialised when allocated (a copy fills them).  Blocks of 4 MiB
 // and more are 2 MiB aligned and advised for transparent huge pages: a 66 MB unary then
@@ -239,6 +266,7 @@ struct dcrf_model {
     bool dev_ready = false;
     hipStream_t stream = nullptr;
     HostFloats unary_host;  // (L, N) row-major as given
+    std::vector<char> unary_unset;  // per instance: its columns of unary_host hold no data yet
     bool has_unary = false, unary_dirty = false;
     DevBuf<float> unary;  // [N][L]
     // LogisticUnaryEnergy (U = L f): parameters L [L][F] row-major, features [N][F]
@@ -454,11 +482,17 @@ void ensure_device(dcrf_model& m) {
     m.dev_ready = true;
 }
 
+// dst = src on the device (large lists through the staging area; then the copy is
+// complete on return, else it is queued on s).
 template <typename T>
 void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t s) {
     dst.reserve(src.size());
-    if (!src.empty())
-        DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
+    if (src.empty()) return;
+    const size_t bytes = sizeof(T) * src.size();
+    if (bytes >= kStagedMin)
+        copy_to_device(dst.get(), src.data(), bytes, s);
+    else
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst.get(), src.data(), bytes, hipMemcpyHostToDevice, s));
 }
 
 void ensure_unary(dcrf_model& m) {
@@ -475,6 +509,12 @@ void ensure_unary(dcrf_model& m) {
     }
     m.stage.reserve(n);
     m.unary.reserve(n);
+    for (int i = 0; i < (int)m.unary_unset.size(); i++)
+        if (m.unary_unset[i]) {  // an instance never given a unary: zeros, as the first set left them
+            const int b = m.geo.pix_base[i], Ni = m.geo.N[i];
+            for (int l = 0; l < m.L; l++) std::fill_n(m.unary_host.data() + (size_t)l * m.N() + b, Ni, 0.f);
+            m.unary_unset[i] = 0;
+        }
     copy_to_device(m.stage.get(), m.unary_host.data(), sizeof(float) * n, m.stream);
     launch_transpose(m.stage.get(), m.unary.get(), m.L, m.N(), m.stream);
     DCRF_HIP_CHECK(hipStreamSynchronize(m.stream));
@@ -1842,6 +1882,7 @@ int dcrf_set_unary(dcrf_model* m, const float* U) {
         const size_t n = (size_t)m->N() * m->L;
         m->unary_host.resize(n);
         host_copy(m->unary_host.data(), U, sizeof(float) * n);
+        m->unary_unset.clear();
         m->has_unary = true;
         m->unary_dirty = true;
         m->logistic = false;
@@ -1911,12 +1952,12 @@ int dcrf_set_unary_instance(dcrf_model* m, int i, const float* U) {
         require(!m->logistic, "the model has a logistic unary: set a full unary with dcrf_set_unary first");
         require(i >= 0 && i < m->geo.n_inst, "instance index out of range");
         const int N = m->N(), L = m->L, b = m->geo.pix_base[i], Ni = m->geo.N[i];
-        if (m->unary_host.size() != (size_t)N * L) {  // instances not set yet stay zero
+        if (m->unary_host.size() != (size_t)N * L) {  // instances not set are zero at the upload
             m->unary_host.resize((size_t)N * L);
-            std::fill(m->unary_host.data(), m->unary_host.data() + (size_t)N * L, 0.f);
+            m->unary_unset.assign(m->geo.n_inst, 1);
         }
-        for (int l = 0; l < L; l++)
-            std::memcpy(m->unary_host.data() + (size_t)l * N + b, U + (size_t)l * Ni, sizeof(float) * Ni);
+        copy_rows(m->unary_host.data() + b, N, U, Ni, L, Ni);
+        if (!m->unary_unset.empty()) m->unary_unset[i] = 0;
         m->has_unary = true;
         m->unary_dirty = true;
     });

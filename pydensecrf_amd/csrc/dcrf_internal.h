@@ -76,6 +76,14 @@ struct DevBuf {
     T* get() const { return p; }
 };
 
+// ---------------------------------------------------------------- host transfers
+// Copies between pageable host memory and the device; those of kStagedMin bytes and
+// more go through the process's page-locked staging area in chunks moved by several
+// host threads.  Both return when the copy is done (after the work queued on s).
+constexpr size_t kStagedMin = (size_t)8 << 20;
+void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s);
+void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s);
+
 // ---------------------------------------------------------------- lattice of one term
 struct Chunk {
     int v;    // vertex (term-local, later global)

@@ -548,10 +548,8 @@ void build_lattice_d(const Geometry& g, const GeometryDev& gd, const TileMap& tm
         DCRF_HIP_CHECK(hipGetLastError());
         lat.run_beg.resize(n_runs + 1);
         std::vector<unsigned long long> hk(n_runs);
-        DCRF_HIP_CHECK(hipMemcpyAsync(lat.run_beg.data(), rbeg.get(), sizeof(int) * n_runs, hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipMemcpyAsync(hk.data(), rkey.get(), sizeof(unsigned long long) * n_runs,
-                                      hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        copy_to_host(lat.run_beg.data(), rbeg.get(), sizeof(int) * n_runs, s);
+        copy_to_host(hk.data(), rkey.get(), sizeof(unsigned long long) * n_runs, s);
         lat.run_beg[n_runs] = Er;
         lat.run_v.resize(n_runs);
         lat.run_t.resize(n_runs);

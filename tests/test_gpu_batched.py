@@ -48,6 +48,42 @@ def test_batched_2d_matches_single():
         assert np.abs(Qb[i] - o.inference(n)).max() <= 1e-5
 
 
+def test_batched_instance_unaries_partial_and_reset():
+    """Unaries given per instance: an instance never given one computes with a zero
+    unary (as a plain model given zeros), a later full unary replaces them all, and a
+    large instance (its label rows copied by several threads) is bit-identical to its
+    image run alone."""
+    L, n = 21, 3
+    shapes = [(37, 23), (640, 480), (31, 17)]
+    cases = [image_case(w, h, L, seed=700 + i) for i, (w, h) in enumerate(shapes)]
+    b = BatchedDenseCRF2D(shapes, L)
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=30, srgb=13, rgbims=[img for img, _, _ in cases], compat=10)
+    for i in (0, 1):
+        b.model.set_unary_instance(i, cases[i][1])
+    Qb = b.inference(n)
+
+    def alone(i, U):
+        (w, h), img = shapes[i], cases[i][0]
+        s = capi.Model.create_2d(w, h, L)
+        s.set_unary(U)
+        s.add_gaussian(3.0, 3.0, 3.0)
+        s.add_bilateral(30.0, 30.0, 13.0, 13.0, 13.0, img, 10.0)
+        return np.ascontiguousarray(s.inference(n))
+
+    for i in (0, 1):
+        assert np.array_equal(Qb[i].view(np.int32), alone(i, cases[i][1]).view(np.int32)), "instance %d" % i
+    Z = np.zeros((L, shapes[2][0] * shapes[2][1]), np.float32)
+    assert np.array_equal(Qb[2].view(np.int32), alone(2, Z).view(np.int32))
+    # the full unary at once, then the third instance's own: all three as given
+    full = np.concatenate([U for _, U, _ in cases], axis=1)
+    b.model.set_unary(full)
+    b.model.set_unary_instance(2, cases[2][1])
+    Qb = b.inference(n)
+    for i in range(3):
+        assert np.array_equal(Qb[i].view(np.int32), alone(i, cases[i][1]).view(np.int32)), "instance %d" % i
+
+
 @pytest.mark.parametrize("L", [3, 130])
 def test_batched_tiny_and_wide_instances(L):
     """A batch mixing a one-pixel image, a one-row image and ordinary ones, at a label
