@@ -172,6 +172,29 @@ def test_large_host_transfers_bitwise(W, H, L):
         assert np.abs(a).max() > 0
 
 
+def test_release_cached_memory_keeps_models_working():
+    """dcrf_release_cached_memory returns the cached device blocks and the page-locked
+    staging area (which a 22 MB host transfer has just grown); a live model then gives
+    the same bits again, its staged transfers taking a new staging area."""
+    import ctypes
+
+    from pydensecrf_amd import capi
+
+    W, H, L = 512, 512, 21
+    img, U, _ = image_case(W, H, L, seed=31)
+    g = capi.Model.create_2d(W, H, L)
+    g.set_unary(U)
+    g.add_gaussian(3.0, 3.0, 3.0)
+    g.add_bilateral(80.0, 80.0, 13.0, 13.0, 13.0, img, 10.0)
+    Q0 = g.inference(2).copy()
+    freed = ctypes.c_size_t(0)
+    capi.check(capi.lib().dcrf_release_cached_memory(ctypes.byref(freed)))
+    assert freed.value >= W * H * L * 4  # at least the staging area of Q's download
+    g.set_unary(U)  # staged again at the next computation
+    assert np.array_equal(g.inference(2).view(np.int32), Q0.view(np.int32))
+    capi.check(capi.lib().dcrf_release_cached_memory(None))
+
+
 @pytest.mark.parametrize("terms", ["g", "b", "gb", "bg"])
 @pytest.mark.parametrize("L", [4, 21])
 @pytest.mark.parametrize("W,H", [(45, 31), (64, 48), (37, 23)])
