@@ -125,12 +125,18 @@ void copy_rows(float* dst, size_t ds, const float* src, size_t ss, int rows, siz
     });
 }
 
+// The staging area of at least n bytes, or null where the runtime cannot give that much
+// page-locked memory (the caller then copies without staging).
 char* stage_reserve(HostStage& H, size_t n) {
     if (n > H.cap) {
         if (H.p) DCRF_HIP_CHECK(hipHostFree(H.p));
         H.p = nullptr;
         H.cap = 0;
-        DCRF_HIP_CHECK(hipHostMalloc((void**)&H.p, n, hipHostMallocDefault));
+        if (hipHostMalloc((void**)&H.p, n, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();  // clear the failure: the copy goes unstaged
+            H.p = nullptr;
+            return nullptr;
+        }
         H.cap = n;
     }
     return H.p;
@@ -162,6 +168,11 @@ void copy_to_host(void* dst, const void* src, size_t n, hipStream_t s) {
     HostStage& H = host_stage();
     std::lock_guard<std::mutex> lk(H.mu);
     char* stage = stage_reserve(H, std::min(n, kStageMax));
+    if (!stage) {
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
     for (size_t o = 0; o < n; o += kStageMax)
         copy_to_host_window(stage, (char*)dst + o, (const char*)src + o, std::min(kStageMax, n - o), s);
 }
@@ -176,6 +187,11 @@ void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s) {
     HostStage& H = host_stage();
     std::lock_guard<std::mutex> lk(H.mu);
     char* stage = stage_reserve(H, std::min(n, kStageMax));
+    if (!stage) {
+        DCRF_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+        DCRF_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
     for (size_t o = 0; o < n; o += kStageMax)
         copy_to_device_window(stage, (char*)dst + o, (const char*)src + o, std::min(kStageMax, n - o), s);
 }
