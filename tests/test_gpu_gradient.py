@@ -359,6 +359,29 @@ def test_kernel_gradient_vs_oracle_l21(kt, nt):
     assert err <= KG_TOL_L21
 
 
+@pytest.mark.parametrize("nt", [po.NORMALIZE_SYMMETRIC, po.NORMALIZE_BEFORE])
+def test_kernel_gradient_multi_round(nt):
+    """320x240 pixels at 21 labels need 2134 blocks of 36 pixels, more than the one-pass
+    kernel gradient's 2048: each block then sums two point ranges (and every reverse
+    iteration) into its partial row.  DIAG and FULL kernels against the oracle."""
+    W, H, L, n = 320, 240, 21, 3
+    img, U, gt = image_case(W, H, L, seed=41 + nt)
+    g = capi.Model.create_2d(W, H, L)
+    o = po.OracleCRF(W=W, H=H, L=L)
+    for m in (g, o):
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0, ktype=capi.DIAG_KERNEL, ntype=nt)
+        m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.FULL_KERNEL, ntype=nt)
+    obj, ug, cg, _, kg = g.gradient(n, gt, kernel=True)
+    r, ugo, cgo, kgo = o.gradient(n, gt, kernel=True)
+    assert abs(obj - r) <= 1e-5 * abs(r)
+    assert _rel_close(ug, ugo, 1e-4)
+    assert kg.shape == kgo.shape == (2 + 25,)
+    err = np.abs(kg - kgo).max() / np.abs(kgo).max()
+    print("kernel gradient 320x240 nt=%d: max error / max entry %.3g" % (nt, err))
+    assert err <= KG_TOL_L21, (err, kg, kgo)
+
+
 @pytest.mark.parametrize("case", ["mixed_normalisation", "five_terms", "high_dim"])
 def test_kernel_gradient_term_mixes(case):
     """The kernel-parameter gradient's shared filter plans: an unnormalised term beside
