@@ -130,11 +130,12 @@ def test_start_step_kl_apply():
             np.testing.assert_allclose(g.apply(k, Qo, tr), o.apply(k, Qo, tr), rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("W,H,L", [(512, 512, 21), (1001, 701, 13)])
+@pytest.mark.parametrize("W,H,L", [(512, 512, 21), (1001, 701, 13), (2048, 1600, 21)])
 def test_large_host_transfers_bitwise(W, H, L):
-    """Host arrays of 22 MB (512x512x21) and 36.5 MB (1001x701x13, not a multiple of the
-    chunks' 4 KiB alignment), above the 8 MiB from which the library moves them through
-    its page-locked staging area in chunks with several host threads: the unary upload,
+    """Host arrays of 22 MB (512x512x21), 36.5 MB (1001x701x13, not a multiple of the
+    chunks' 4 KiB alignment) and 275 MB (2048x1600x21: two windows of the 256 MiB staging
+    area), above the 8 MiB from which the library moves them through its page-locked
+    staging area in chunks with several host threads: the unary upload,
     inference's download, start / step (Q up, Q, tmp1 and tmp2 down), KL and apply (Q
     up, the messages down).  Each agrees bit for bit with a path whose arrays never
     leave the device or with the other host path, and the staged unary gives the same Q
