@@ -359,6 +359,35 @@ def test_kernel_gradient_vs_oracle_l21(kt, nt):
     assert err <= KG_TOL_L21
 
 
+def test_kernel_gradient_plan_rebuilt_in_place():
+    """After a kernel-parameter change the kernel gradient's filter plan is rebuilt in
+    place (its buffers reused): the gradient is bit-identical to a fresh model's with
+    the new parameters, also when the lattices shrink or grow."""
+    W, H, L, n = 64, 48, 21, 3
+    img, U, gt = image_case(W, H, L, seed=77)
+
+    def model(kp=None):
+        m = capi.Model.create_2d(W, H, L)
+        m.set_unary(U)
+        m.add_gaussian(3.0, 3.0, 3.0, ktype=capi.DIAG_KERNEL)
+        m.add_bilateral(40.0, 40.0, 13.0, 13.0, 13.0, img, 10.0, ktype=capi.FULL_KERNEL)
+        if kp is not None:
+            m.set_kernel_params(kp)
+        return m
+
+    g = model()
+    p0 = g.kernel_params()
+    g.gradient(n, gt, kernel=True)
+    for f in (np.float32(1.3), np.float32(0.6)):  # coarser, then finer lattices
+        p = (p0 * f).astype(np.float32)
+        g.set_kernel_params(p)
+        got = g.gradient(n, gt, kernel=True)
+        want = model(p).gradient(n, gt, kernel=True)
+        assert got[0] == want[0]
+        for a, b in zip(got[1:], want[1:]):
+            assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))
+
+
 @pytest.mark.parametrize("nt", [po.NORMALIZE_SYMMETRIC, po.NORMALIZE_BEFORE])
 def test_kernel_gradient_multi_round(nt):
     """320x240 pixels at 21 labels need 2134 blocks of 36 pixels, more than the one-pass
