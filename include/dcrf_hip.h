@@ -114,6 +114,15 @@ int dcrf_start_inference(dcrf_model *m, float *Q);
 int dcrf_step_inference(dcrf_model *m, float *Q, float *tmp1, float *tmp2);
 /* klDivergence(Q): densecrf.cpp:214-235, densecrf.pyx:100-101. */
 int dcrf_kl_divergence(dcrf_model *m, const float *Q, double *kl);
+/* DenseCRF::map(n) (densecrf.cpp:132-137, densecrf.h:75; the reference's C++ interface,
+ * not its Python binding): n iterations of inference, then labels[p] = the first label
+ * of the largest Q[p][l] (currentMap).  labels out [N]. */
+int dcrf_map(dcrf_model *m, int n_iterations, int16_t *labels);
+/* DenseCRF::currentMap(Q) (densecrf.cpp:202-211, densecrf.h:80): labels[p] = the first
+ * label of the largest Q[p][l], Q [N][L]; labels out [N]. */
+int dcrf_current_map(dcrf_model *m, const float *Q, int16_t *labels);
+/* currentMap on device buffers: Q_dev [N][L] float32 -> labels_dev [N] int16, on stream. */
+int dcrf_current_map_device(dcrf_model *m, const float *Q_dev, int16_t *labels_dev, void *stream);
 /* PairwisePotential::apply / applyTranspose of one term (pairwise.cpp:173-183):
  * out = compat(K~ Q) or its transpose.  Q, out [N][L]. */
 int dcrf_apply_pairwise(dcrf_model *m, int term, const float *Q, float *out, int transpose);

@@ -49,6 +49,9 @@ SIGNATURES = {
     "dcrf_start_inference": (_c, [_P, _F]),
     "dcrf_step_inference": (_c, [_P, _F, _F, _F]),
     "dcrf_kl_divergence": (_c, [_P, _F, _D]),
+    "dcrf_map": (_c, [_P, _c, _S]),
+    "dcrf_current_map": (_c, [_P, _F, _S]),
+    "dcrf_current_map_device": (_c, [_P, _P, _P, _P]),
     "dcrf_apply_pairwise": (_c, [_P, _c, _F, _F, _c]),
     "dcrf_inference_device": (_c, [_P, _c, _P, _P, _P]),
     "dcrf_capture_inference": (_c, [_P, _c, _P, _P]),
@@ -294,6 +297,25 @@ class Model:
         t1, t2 = self._buf(), self._buf()
         check(lib().dcrf_step_inference(self.h, fptr(Q), fptr(t1), fptr(t2)))
         return Q.T, t1.T, t2.T
+
+    def map(self, n):
+        """DenseCRF::map(n) (densecrf.cpp:132-137): labels (N,) int16 after n iterations."""
+        self.tuned()
+        out = np.empty(self.N, np.int16)
+        check(lib().dcrf_map(self.h, n, out.ctypes.data_as(_S)))
+        return out
+
+    def current_map(self, Q_LxN):
+        """DenseCRF::currentMap(Q) (densecrf.cpp:202-211): the first label of each pixel's
+        largest Q, (N,) int16."""
+        Q = np.ascontiguousarray(f32(Q_LxN).T)
+        out = np.empty(self.N, np.int16)
+        check(lib().dcrf_current_map(self.h, fptr(Q), out.ctypes.data_as(_S)))
+        return out
+
+    def current_map_device(self, Q_ptr, labels_ptr, stream=None):
+        """currentMap of a device Q [N][L] float32 into device labels [N] int16."""
+        check(lib().dcrf_current_map_device(self.h, Q_ptr, labels_ptr, stream))
 
     def kl(self, Q_LxN):
         Q = np.ascontiguousarray(f32(Q_LxN).T)

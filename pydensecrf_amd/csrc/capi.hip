@@ -362,6 +362,7 @@ struct dcrf_model {
     std::vector<std::unique_ptr<KgPlan>> kgplans;
     int lat_gen = 0;
     DevBuf<float> kgbuf, kgdf;
+    DevBuf<int16_t> labels;  // dcrf_map / dcrf_current_map
     DevBuf<double> kpartial, kacc;
 
     // captured inference (dcrf_capture_inference)
@@ -2029,6 +2030,41 @@ int dcrf_inference(dcrf_model* m, int n_iter, float* Q) {
         run_inference(*m, n_iter, U, m->Q.get(), m->stream);
         copy_to_host(Q, m->Q.get(), sizeof(float) * (size_t)m->N() * m->L, m->stream);
         if (m->timing) collect_timing(*m);
+    });
+}
+
+int dcrf_map(dcrf_model* m, int n_iter, int16_t* labels) {
+    return guard([&] {
+        require(m && labels, "null argument");
+        require(n_iter >= 0, "number of iterations must be non-negative");
+        prepare(*m);
+        const float* U = m->has_unary ? m->unary.get() : nullptr;
+        run_inference(*m, n_iter, U, m->Q.get(), m->stream);
+        m->labels.reserve(m->N());
+        launch_argmax(m->Q.get(), m->L, m->N(), m->L, m->labels.get(), m->stream);
+        copy_to_host(labels, m->labels.get(), sizeof(int16_t) * m->N(), m->stream);
+        if (m->timing) collect_timing(*m);
+    });
+}
+
+int dcrf_current_map(dcrf_model* m, const float* Q, int16_t* labels) {
+    return guard([&] {
+        require(m && Q && labels, "null argument");
+        ensure_device(*m);
+        const size_t n = (size_t)m->N() * m->L;
+        m->work.reserve(n);
+        m->labels.reserve(m->N());
+        copy_to_device(m->work.get(), Q, sizeof(float) * n, m->stream);
+        launch_argmax(m->work.get(), m->L, m->N(), m->L, m->labels.get(), m->stream);
+        copy_to_host(labels, m->labels.get(), sizeof(int16_t) * m->N(), m->stream);
+    });
+}
+
+int dcrf_current_map_device(dcrf_model* m, const float* Q_dev, int16_t* labels_dev, void* stream) {
+    return guard([&] {
+        require(m && Q_dev && labels_dev, "null argument");
+        ensure_device(*m);
+        launch_argmax(Q_dev, m->L, m->N(), m->L, labels_dev, (hipStream_t)stream);
     });
 }
 

@@ -387,6 +387,8 @@ void launch_norm_finish(float* norm, int N, int ntype, hipStream_t s);
 void launch_fill(float* p, float v, size_t n, hipStream_t s);
 // out[c][r] = in[r][c] (in is rows x cols row-major).
 void launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s);
+// out[p] = the first label of the largest Q[p][l] (DenseCRF::currentMap), Q row stride ldq.
+void launch_argmax(const float* Q, int ldq, int N, int L, int16_t* out, hipStream_t s);
 // ---------------------------------------------------------------- adjoint kernels
 // b = sumAndNormalize(Q .* dQ, Q); dU = b.
 void launch_sn_first(const float* Q, const float* dQ, float* b, float* dU, int N, int L, hipStream_t s);

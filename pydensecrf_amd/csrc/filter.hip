@@ -1074,6 +1074,25 @@ __global__ void k_slice_weights(const int* __restrict__ vtx, const float* __rest
     sw[i] = use_norm ? wa * norm[p] : wa;
 }
 
+// DenseCRF::currentMap (densecrf.cpp:202-211): out[p] = the first label of the largest
+// Q[p][l] (a later label replaces it only when strictly larger, as Eigen's maxCoeff
+// visitor does), one thread per pixel.
+__global__ void k_argmax(const float* __restrict__ Q, int ldq, int N, int L, int16_t* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const float* row = Q + (size_t)p * ldq;
+    float best = row[0];
+    int arg = 0;
+    for (int l = 1; l < L; l++) {
+        const float v = row[l];
+        if (v > best) {
+            best = v;
+            arg = l;
+        }
+    }
+    out[p] = (int16_t)arg;
+}
+
 // out[c][r] = in[r][c] for an rows x cols row-major input, 32x32 tiles through LDS.
 __global__ void k_transpose(const float* __restrict__ in, float* __restrict__ out, int rows, int cols) {
     __shared__ float tile[32][33];
@@ -1095,6 +1114,12 @@ __global__ void k_transpose(const float* __restrict__ in, float* __restrict__ ou
 void launch_fill(float* p, float v, size_t n, hipStream_t s) {
     if (!n) return;
     k_fill<<<blocks_for(n), kThreads, 0, s>>>(p, v, n);
+    DCRF_HIP_CHECK(hipGetLastError());
+}
+
+void launch_argmax(const float* Q, int ldq, int N, int L, int16_t* out, hipStream_t s) {
+    if (N <= 0) return;
+    k_argmax<<<blocks_for(N), kThreads, 0, s>>>(Q, ldq, N, L, out);
     DCRF_HIP_CHECK(hipGetLastError());
 }
 

@@ -10,7 +10,7 @@ and run mean-field inference on the GPU.  Q and the step temporaries are
 import array
 from numbers import Number
 
-from libc.stdint cimport uint8_t
+from libc.stdint cimport int16_t, uint8_t
 
 from pydensecrf_amd.eigen cimport MatrixXf, VectorXf, c_matrixXf, c_vectorXf
 
@@ -36,6 +36,8 @@ cdef extern from "dcrf_hip.h" nogil:
     int dcrf_start_inference(dcrf_model* m, float* Q)
     int dcrf_step_inference(dcrf_model* m, float* Q, float* tmp1, float* tmp2)
     int dcrf_kl_divergence(dcrf_model* m, const float* Q, double* kl)
+    int dcrf_map(dcrf_model* m, int n, int16_t* labels)
+    int dcrf_current_map(dcrf_model* m, const float* Q, int16_t* labels)
     int dcrf_set_logistic_unary(dcrf_model* m, const float* Lmat, int F, const float* feat)
 
 
@@ -236,7 +238,7 @@ cdef class DenseCRF:
         _check(st)
         return Q, MatrixXf(), MatrixXf()
 
-    def stepInference(self, MatrixXf Q, MatrixXf tmp1, MatrixXf tmp2):
+    def stepInference(self, MatrixXf Q not None, MatrixXf tmp1 not None, MatrixXf tmp2 not None):
         if Q.shape[0] != self._nlabel or Q.shape[1] != self._nvar:
             raise ValueError("Bad shape for Q (Need {}, got {})".format(
                 (self._nlabel, self._nvar), (Q.shape[0], Q.shape[1])))
@@ -247,7 +249,7 @@ cdef class DenseCRF:
             st = dcrf_step_inference(self._this, Q.data, tmp1.data, tmp2.data)
         _check(st)
 
-    def klDivergence(self, MatrixXf Q):
+    def klDivergence(self, MatrixXf Q not None):
         if Q.shape[0] != self._nlabel or Q.shape[1] != self._nvar:
             raise ValueError("Bad shape for Q (Need {}, got {})".format(
                 (self._nlabel, self._nvar), (Q.shape[0], Q.shape[1])))
@@ -257,6 +259,35 @@ cdef class DenseCRF:
             st = dcrf_kl_divergence(self._this, Q.data, &kl)
         _check(st)
         return kl
+
+    def map(self, int niter):
+        """DenseCRF::map(n) of the reference's C++ interface (densecrf.cpp:132-137), which
+        its Python binding does not wrap: the MAP label of every variable, an (N,) int16
+        array, after niter iterations; the same labels as np.argmax(inference(niter), 0)."""
+        import numpy as np
+        self._tune()
+        out = np.empty(self._nvar, np.int16)
+        cdef int16_t[::1] o = out
+        cdef int st
+        with nogil:
+            st = dcrf_map(self._this, niter, &o[0])
+        _check(st)
+        return out
+
+    def currentMap(self, MatrixXf Q not None):
+        """DenseCRF::currentMap(Q) of the reference's C++ interface (densecrf.cpp:202-211):
+        the first label of each variable's largest Q, an (N,) int16 array."""
+        import numpy as np
+        if Q.shape[0] != self._nlabel or Q.shape[1] != self._nvar:
+            raise ValueError("Bad shape for Q (Need {}, got {})".format(
+                (self._nlabel, self._nvar), (Q.shape[0], Q.shape[1])))
+        out = np.empty(self._nvar, np.int16)
+        cdef int16_t[::1] o = out
+        cdef int st
+        with nogil:
+            st = dcrf_current_map(self._this, Q.data, &o[0])
+        _check(st)
+        return out
 
     @property
     def _handle(self):

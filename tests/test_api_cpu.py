@@ -212,6 +212,13 @@ def test_none_arguments():
         d.setUnaryEnergy(None)
     with pytest.raises(TypeError):
         d.addPairwiseEnergy(None, compat=3)
+    # Q buffers: a TypeError before any device work (not a read through None)
+    from pydensecrf_amd import eigen
+    q = eigen.MatrixXf()
+    for call in (lambda: d.stepInference(None, q, q), lambda: d.stepInference(q, None, q),
+                 lambda: d.klDivergence(None), lambda: d.currentMap(None)):
+        with pytest.raises(TypeError):
+            call()
 
 
 def test_enums_match_reference_values():

@@ -92,3 +92,11 @@ def test_step_inference_in_place(generic):
     assert np.array_equal(np.argmax(Q, axis=0), np.argmax(Qo, axis=0))
     Qi = np.array(d.inference(4))
     assert np.abs(Qi - view).max() <= 1e-6
+    # the reference C++ interface's map / currentMap (not in its Python binding)
+    assert np.array_equal(d.currentMap(Q), np.argmax(Q, axis=0).astype(np.int16))
+    assert np.array_equal(d.map(4), Qi.argmax(0).astype(np.int16))
+    from pydensecrf_amd import eigen
+    with pytest.raises(ValueError):
+        d.currentMap(eigen.MatrixXf())  # shape (0, 0)
+    with pytest.raises(TypeError):
+        d.currentMap(None)

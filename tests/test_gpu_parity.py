@@ -173,6 +173,48 @@ def test_large_host_transfers_bitwise(W, H, L):
         assert np.abs(a).max() > 0
 
 
+def test_map_and_current_map():
+    """DenseCRF::map / currentMap (densecrf.cpp:132-137, 202-211): the first label of each
+    pixel's largest Q.  Exact ties take the first label (Eigen's maxCoeff); one label
+    gives zeros; a batched model's labels are its instances' in order; the device form
+    gives the host form's labels."""
+    import torch
+
+    from pydensecrf_amd import capi
+
+    W, H, L, n = 45, 31, 6, 3
+    img, U, _ = image_case(W, H, L, seed=13)
+    g, o = build_both(W, H, L, U, _std_terms(img))
+    Q = g.inference(n)
+    assert np.array_equal(g.map(n), Q.argmax(0).astype(np.int16))
+    assert np.array_equal(g.map(n), o.inference(n).argmax(0).astype(np.int16))
+    # ties and equal rows: the first of the largest
+    N = W * H
+    rng = np.random.default_rng(5)
+    T = rng.integers(0, 3, size=(L, N)).astype(np.float32)  # many exact ties
+    T[:, :7] = 1.0  # whole columns equal
+    want = np.array([next(l for l in range(L) if T[l, p] == T[:, p].max()) for p in range(N)], np.int16)
+    assert np.array_equal(g.current_map(T), want) and np.array_equal(g.current_map(T)[:7], np.zeros(7, np.int16))
+    Td = torch.tensor(np.ascontiguousarray(T.T), device="cuda")
+    lab = torch.empty(N, dtype=torch.int16, device="cuda")
+    g.current_map_device(Td.data_ptr(), lab.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(lab.cpu().numpy(), want)
+    # one label: every pixel takes label 0
+    g1, _ = build_both(W, H, 1, U[:1], _std_terms(img))
+    assert not g1.map(2).any()
+    # a batch: labels of the instances in order
+    from pydensecrf_amd.batched import BatchedDenseCRF2D
+    shapes = [(20, 15), (33, 9)]
+    cases = [image_case(w, h, L, seed=60 + i) for i, (w, h) in enumerate(shapes)]
+    b = BatchedDenseCRF2D(shapes, L)
+    b.setUnaryEnergy([c[1] for c in cases])
+    b.addPairwiseGaussian(sxy=3, compat=3)
+    b.addPairwiseBilateral(sxy=30, srgb=13, rgbims=[c[0] for c in cases], compat=10)
+    Qb = b.inference(n)
+    assert np.array_equal(b.model.map(n), np.concatenate([q.argmax(0) for q in Qb]).astype(np.int16))
+
+
 def test_release_cached_memory_keeps_models_working():
     """dcrf_release_cached_memory returns the cached device blocks and the page-locked
     staging area (which a 22 MB host transfer has just grown); a live model then gives

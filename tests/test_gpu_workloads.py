@@ -111,6 +111,9 @@ def test_image_workload(name, cfg):
     assert _sha(lat["offset"]) == str(z["offset_gaussian_sha"]), "Gaussian lattice offsets differ"
     Q = np.ascontiguousarray(g.inference(n))
     _check_image_fixture(z, Q, label)
+    labels = g.map(n)  # DenseCRF::map: the reference's MAP, from the device
+    assert labels.dtype == np.int16 and _sha(labels) == str(z["map_sha"]), "%s: map() differs" % label
+    assert np.array_equal(labels, Q.argmax(0).astype(np.int16))
     o = _oracle_2d(W, H, L, U, img)
     Qo = o.inference(n)
     err = float(np.abs(Q - Qo).max())
