@@ -1036,6 +1036,12 @@ double orc_crf_gradient_obj(orc_crf *c, int n_iter, int objective, const short *
 {
     const int N = c->N, L = c->L;
     const size_t n = (size_t)N * L;
+    /* Hamming with given class weights: a labelled pixel whose label has no weight has no
+     * defined objective (the reference reads past its weight vector, objective.cpp:73):
+     * NaN, outputs untouched (the library rejects the call the same way) */
+    if (objective == ORC_HAMMING && class_weight)
+        for (int i = 0; i < N; i++)
+            if (gt[i] >= 0 && gt[i] < L && gt[i] >= n_class_weight) return NAN;
     float **Q = (float **)malloc(sizeof(float *) * (n_iter + 1));
     float *tmp1 = (float *)malloc(sizeof(float) * n), *tmp2 = (float *)malloc(sizeof(float) * n);
     float *b = (float *)malloc(sizeof(float) * n), *kq = (float *)malloc(sizeof(float) * n);

@@ -83,7 +83,8 @@ double orc_crf_gradient_k(orc_crf *c, int n_iter, const short *gt, float robust,
 int orc_crf_num_kernel_params(const orc_crf *c);
 /* The same with another objective (objective.cpp:37-108): ORC_LOG_LIKELIHOOD (param =
  * robust), ORC_HAMMING (class_weight[n_class_weight], or NULL and param =
- * class_weight_pow), ORC_IOU. */
+ * class_weight_pow), ORC_IOU.  Hamming with class weights and a labelled pixel whose
+ * label is n_class_weight or more: NaN, outputs untouched. */
 enum { ORC_LOG_LIKELIHOOD = 0, ORC_HAMMING = 1, ORC_IOU = 2 };
 double orc_crf_gradient_obj(orc_crf *c, int n_iter, int objective, const short *gt, float param,
                             const float *class_weight, int n_class_weight, float *unary_grad, float *compat_grad,

@@ -340,6 +340,8 @@ class OracleCRF(_CRFBase):
         r = self._lib.orc_crf_gradient_obj(self._h, n, objective, gt.ctypes.data_as(_S), param,
                                            None if cw is None else _fp(cw), 0 if cw is None else cw.size,
                                            _fp(ug), _fp(cg), _fp(kg) if kernel else None)
+        if np.isnan(r) and cw is not None:
+            raise ValueError("a ground-truth label has no class weight")
         return (r, ug.T, cg[:self._lib.orc_crf_num_compat_params(self._h)],
                 kg[:self._lib.orc_crf_num_kernel_params(self._h)] if kernel else None)
 
