@@ -38,6 +38,8 @@ cdef extern from "dcrf_hip.h" nogil:
     int dcrf_kl_divergence(dcrf_model* m, const float* Q, double* kl)
     int dcrf_map(dcrf_model* m, int n, int16_t* labels)
     int dcrf_current_map(dcrf_model* m, const float* Q, int16_t* labels)
+    int dcrf_unary_energy(dcrf_model* m, const int16_t* labels, float* out)
+    int dcrf_pairwise_energy(dcrf_model* m, const int16_t* labels, int term, float* out)
     int dcrf_set_logistic_unary(dcrf_model* m, const float* Lmat, int F, const float* feat)
 
 
@@ -286,6 +288,39 @@ cdef class DenseCRF:
         cdef int st
         with nogil:
             st = dcrf_current_map(self._this, Q.data, &o[0])
+        _check(st)
+        return out
+
+    def unaryEnergy(self, labels):
+        """DenseCRF::unaryEnergy(l) of the reference's C++ interface (densecrf.cpp:141-153):
+        U(l_i, i) per variable for a labelling of N labels, an (N,) float32 array."""
+        import numpy as np
+        lab = np.ascontiguousarray(labels, dtype=np.int16).ravel()
+        if lab.shape[0] != self._nvar:
+            raise ValueError("Bad shape for labels (Need ({},), got {})".format(self._nvar, lab.shape))
+        out = np.empty(self._nvar, np.float32)
+        cdef int16_t[::1] l = lab
+        cdef float[::1] o = out
+        cdef int st
+        with nogil:
+            st = dcrf_unary_energy(self._this, &l[0], &o[0])
+        _check(st)
+        return out
+
+    def pairwiseEnergy(self, labels, int term=-1):
+        """DenseCRF::pairwiseEnergy(l, term) of the reference's C++ interface
+        (densecrf.cpp:155-177): -0.5 (C K onehot(l))(l_i, i) per variable for pairwise
+        term `term` (-1: the sum over all terms), an (N,) float32 array."""
+        import numpy as np
+        lab = np.ascontiguousarray(labels, dtype=np.int16).ravel()
+        if lab.shape[0] != self._nvar:
+            raise ValueError("Bad shape for labels (Need ({},), got {})".format(self._nvar, lab.shape))
+        out = np.empty(self._nvar, np.float32)
+        cdef int16_t[::1] l = lab
+        cdef float[::1] o = out
+        cdef int st
+        with nogil:
+            st = dcrf_pairwise_energy(self._this, &l[0], term, &o[0])
         _check(st)
         return out
 

@@ -100,3 +100,24 @@ def test_step_inference_in_place(generic):
         d.currentMap(eigen.MatrixXf())  # shape (0, 0)
     with pytest.raises(TypeError):
         d.currentMap(None)
+
+
+def test_debug_energies_reference_fixture():
+    """DenseCRF::unaryEnergy / pairwiseEnergy (densecrf.cpp:141-177; the reference's Python
+    binding wraps neither) through the drop-in classes, against the reference's outputs in
+    tests/golden/learning_small.npz: unary bit-exact, pairwise within 1e-5."""
+    dcrf = _dcrf()
+    z = np.load(os.path.join(GOLDEN, "learning_small.npz"), allow_pickle=False)
+    W, H, L = [int(v) for v in z["e_meta"]]
+    d = dcrf.DenseCRF2D(W, H, L)
+    d.setUnaryEnergy(z["e_U"])
+    d.addPairwiseGaussian(sxy=(3, 2), compat=3, normalization=dcrf.NORMALIZE_SYMMETRIC)
+    d.addPairwiseBilateral(sxy=(20, 30), srgb=(13, 17, 11), rgbim=np.ascontiguousarray(z["e_img"]),
+                           compat=np.arange(1, L + 1, dtype=np.float32), normalization=dcrf.NORMALIZE_BEFORE)
+    lab = z["e_labels"]
+    np.testing.assert_array_equal(d.unaryEnergy(lab), z["e_unary"])
+    for term, key in ((0, "e_pair0"), (1, "e_pair1"), (-1, "e_pair")):
+        np.testing.assert_allclose(d.pairwiseEnergy(lab, term), z[key], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(d.pairwiseEnergy(lab), z["e_pair"], rtol=1e-5, atol=1e-5)
+    with pytest.raises(ValueError):
+        d.unaryEnergy(lab[:-1])
