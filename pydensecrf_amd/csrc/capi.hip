@@ -92,6 +92,12 @@ int copy_threads(size_t n) {
 // Chunk t of n bytes split T ways, 4 KiB aligned: [off(t), off(t + 1)).
 inline size_t chunk_off(size_t n, int T, int t) { return t >= T ? n : (n / T * t) & ~(size_t)4095; }
 
+// Chunks of one staged window of n bytes moved by T threads: about 2 MiB each, at least
+// one and at most 8 per thread (thread t takes chunks t, t + T, ...).  The host copy
+// left after the window's last DMA (downloads), or made before its first (uploads), is
+// then one small chunk rather than a thread's whole share.
+inline int window_chunks(size_t n, int T) { return (int)std::clamp<size_t>(n >> 21, (size_t)T, (size_t)8 * T); }
+
 // work(t) for t < T: t = 0 on this thread, the others on threads of their own where the
 // system starts them (else here as well).  work must not throw.
 template <typename F>
@@ -197,34 +203,40 @@ void copy_to_device(void* dst, const void* src, size_t n, hipStream_t s) {
 }
 
 // One window (n <= the staging area's size) of copy_to_host: every chunk's DMA is
-// queued at once, and each thread copies its chunk out when its event fires.
+// queued at once, in order, and each chunk is copied out as soon as its event fires.
 void copy_to_host_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s) {
-    const int T = copy_threads(n);
-    ChunkEvents E(T);
-    for (int t = 0; t < T; t++) {  // all chunks queued before any thread starts
-        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
+    const int T = copy_threads(n), C = window_chunks(n, T);
+    ChunkEvents E(C);
+    for (int c = 0; c < C; c++) {  // all chunks queued before any thread starts
+        const size_t a = chunk_off(n, C, c), b = chunk_off(n, C, c + 1);
         DCRF_HIP_CHECK(hipMemcpyAsync(stage + a, (const char*)src + a, b - a, hipMemcpyDeviceToHost, s));
-        DCRF_HIP_CHECK(hipEventRecord(E.ev[t], s));
+        DCRF_HIP_CHECK(hipEventRecord(E.ev[c], s));
     }
-    std::vector<hipError_t> err(T, hipSuccess);
+    std::vector<hipError_t> err(C, hipSuccess);
     run_parallel(T, [&](int t) {  // each chunk leaves the staging area as soon as it arrived
-        err[t] = hipEventSynchronize(E.ev[t]);
-        if (err[t] != hipSuccess) return;
-        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
-        std::memcpy((char*)dst + a, stage + a, b - a);
+        for (int c = t; c < C; c += T) {
+            err[c] = hipEventSynchronize(E.ev[c]);
+            if (err[c] != hipSuccess) return;
+            const size_t a = chunk_off(n, C, c), b = chunk_off(n, C, c + 1);
+            std::memcpy((char*)dst + a, stage + a, b - a);
+        }
     });
     for (hipError_t e : err) DCRF_HIP_CHECK(e);
 }
 
-// One window of copy_to_device: each thread stages its chunk and queues that chunk's
-// DMA; returns when the window's copies are done (the staging area is free again).
+// One window of copy_to_device: each thread stages its chunks in turn and queues each
+// chunk's DMA as soon as it is staged; returns when the window's copies are done (the
+// staging area is free again).
 void copy_to_device_window(char* stage, void* dst, const void* src, size_t n, hipStream_t s) {
-    const int T = copy_threads(n);
-    std::vector<hipError_t> err(T, hipSuccess);
-    run_parallel(T, [&](int t) {  // each chunk is queued for the DMA as soon as it is staged
-        const size_t a = chunk_off(n, T, t), b = chunk_off(n, T, t + 1);
-        std::memcpy(stage + a, (const char*)src + a, b - a);
-        err[t] = hipMemcpyAsync((char*)dst + a, stage + a, b - a, hipMemcpyHostToDevice, s);
+    const int T = copy_threads(n), C = window_chunks(n, T);
+    std::vector<hipError_t> err(C, hipSuccess);
+    run_parallel(T, [&](int t) {
+        for (int c = t; c < C; c += T) {
+            const size_t a = chunk_off(n, C, c), b = chunk_off(n, C, c + 1);
+            std::memcpy(stage + a, (const char*)src + a, b - a);
+            err[c] = hipMemcpyAsync((char*)dst + a, stage + a, b - a, hipMemcpyHostToDevice, s);
+            if (err[c] != hipSuccess) return;
+        }
     });
     const hipError_t sync = hipStreamSynchronize(s);  // the staging area is free again
     for (hipError_t e : err) DCRF_HIP_CHECK(e);

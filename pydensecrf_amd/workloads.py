@@ -23,6 +23,13 @@
         kernel parameters: R changes of the parameter vector, each rebuilding both
         lattices (dcrf_set_kernel_params) and the model's layout.
 
+    python -m pydensecrf_amd.workloads dropin [--reps R]
+        The drop-in Cython API with host arrays, DenseCRF2D 1024x768, 21 labels,
+        Gaussian (sxy=3, compat=3) + bilateral (sxy=80, srgb=13, compat=10): R images,
+        each a new model (DenseCRF2D + setUnaryEnergy, both pairwise terms,
+        inference(10)), after one warm-up image; then, on the last model, R calls each of
+        inference(10), setUnaryEnergy + inference(10) and map(10).
+
 Each prints one line with its host-clock time per repetition; run under rocprofv3 for
 the per-kernel figures.
 """
@@ -217,10 +224,53 @@ def kernel_rebuild(reps):
           % (ms, split[0] * 1e3 / reps, split[1] * 1e3 / reps), flush=True)
 
 
+def dropin(reps):
+    from . import densecrf as dcrf, synthetic as syn
+
+    W, H, L = 1024, 768, 21
+    img, U, _ = syn.make_image(W, H, L=L, seed=0)
+    U = np.ascontiguousarray(U, dtype=np.float32)
+    img = np.ascontiguousarray(img)
+    phases = {"create+unary": [], "gaussian": [], "bilateral": [], "inference": []}
+    d = None
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        d = dcrf.DenseCRF2D(W, H, L)
+        d.setUnaryEnergy(U)
+        t1 = time.perf_counter()
+        d.addPairwiseGaussian(sxy=3, compat=3)
+        t2 = time.perf_counter()
+        d.addPairwiseBilateral(sxy=80, srgb=13, rgbim=img, compat=10)
+        t3 = time.perf_counter()
+        Q = d.inference(10)
+        t4 = time.perf_counter()
+        if r:  # the first image warms the runtime up
+            for k, v in zip(phases, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+                phases[k].append(v)
+    del Q
+    per_image = sum(np.mean(v) for v in phases.values())
+    print("dropin 1024x768x21, new model per image: %.2f ms (%s)" % (
+        1e3 * per_image, ", ".join("%s %.2f" % (k, 1e3 * np.mean(v)) for k, v in phases.items())))
+    reused = {"inference(10)": [], "setUnaryEnergy+inference(10)": [], "map(10)": []}
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        Q = d.inference(10)
+        t1 = time.perf_counter()
+        d.setUnaryEnergy(U)
+        Q = d.inference(10)
+        t2 = time.perf_counter()
+        d.map(10)
+        t3 = time.perf_counter()
+        for k, v in zip(reused, (t1 - t0, t2 - t1, t3 - t2)):
+            reused[k].append(v)
+    print("dropin 1024x768x21, model reused: " + ", ".join("%s %.2f ms" % (k, 1e3 * np.mean(v))
+                                                           for k, v in reused.items()))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("workload", choices=["kernel-gradient", "kernel-learning", "logistic-unary", "kernel-rebuild",
-                                         "fwd-bwd"])
+                                         "fwd-bwd", "dropin"])
     ap.add_argument("--size", type=int, default=512, choices=[512, 1024])
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--reps", type=int, default=5)
@@ -234,6 +284,8 @@ def main():
         fwd_bwd(a.steps, a.size)
     elif a.workload == "kernel-rebuild":
         kernel_rebuild(a.reps)
+    elif a.workload == "dropin":
+        dropin(a.reps)
     else:
         logistic_unary(a.reps, a.features)
 

@@ -421,13 +421,25 @@ def chunk_off(n, T, t):
     return n if t >= T else (n // T * t) & ~4095
 
 
-@pytest.mark.parametrize("n", [8 << 20, 22020096, 36488452, (8 << 20) + 3, 66060288, 123456789])
+def window_chunks(n, T):
+    """capi.hip window_chunks: ~2 MiB chunks, between T and 8 T of them."""
+    return min(max(n >> 21, T), 8 * T)
+
+
+@pytest.mark.parametrize("n", [8 << 20, 22020096, 36488452, (8 << 20) + 3, 66060288, 123456789, 256 << 20])
 def test_staged_copy_chunks(n):
     """The staged host transfers' chunks (capi.hip copy_to_host / copy_to_device /
-    host_copy): for every thread count they tile [0, n) in order, without overlap, and
-    all but the last start on a 4 KiB boundary."""
+    host_copy): for every thread and chunk count they tile [0, n) in order, without
+    overlap, and all but the last start on a 4 KiB boundary; in a staged window
+    (n >= 8 MiB) every chunk is non-empty and thread t takes chunks t, t + T, ..., so
+    each chunk has exactly one thread."""
     for T in range(1, 9):
-        offs = [chunk_off(n, T, t) for t in range(T + 1)]
-        assert offs[0] == 0 and offs[-1] == n
-        assert all(a <= b for a, b in zip(offs, offs[1:]))
-        assert all(o % 4096 == 0 for o in offs[:-1])
+        for C in (T, window_chunks(n, T)):
+            offs = [chunk_off(n, C, c) for c in range(C + 1)]
+            assert offs[0] == 0 and offs[-1] == n
+            assert all(a < b for a, b in zip(offs, offs[1:]))
+            assert all(o % 4096 == 0 for o in offs[:-1])
+        C = window_chunks(n, T)
+        assert T <= C <= 8 * T
+        owned = sorted(c for t in range(T) for c in range(t, C, T))
+        assert owned == list(range(C))
